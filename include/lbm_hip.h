@@ -1,0 +1,153 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X (gfx950) time-stepping core of the
+ * PyLBM-Isca linear baroclinic model.
+ *
+ * The reference's only plugin point is its array-module protocol
+ * (pylbm_isca/backend.py:53-77, every hot object keeps ``self.xp``); its hot
+ * path is Spectral (pylbm_isca/spectral.py:43-289), Dynamics.tendencies
+ * (pylbm_isca/dynamics.py:156-271), Integrator.step
+ * (pylbm_isca/integrator.py:264-315), State (pylbm_isca/state.py:26-148) and
+ * the LBM.run loop (pylbm_isca/model.py:168-290), batched over ensemble
+ * members by pylbm_isca/ensemble.py:41-165.  A device="hip" backend binds the
+ * entry points below (ctypes).
+ *
+ * Conventions
+ * - Every entry point returns 0 on success or a negative lbm_status code;
+ *   lbm_last_error() gives the message.  No C++ exception crosses the ABI.
+ * - Array arguments named d_* are DEVICE pointers (e.g. torch tensors'
+ *   data_ptr()) to float64 data in the reference's layouts:
+ *     grid 3-D      (B, nlat, nlon, nlev)  C order, latitudes north->south
+ *     grid 2-D      (B, nlat, nlon)
+ *     spectral 3-D  (B, nmax+1, nmax+1, nlev) complex128, [m, n], zero n<m
+ *     spectral 2-D  (B, nmax+1, nmax+1)      complex128
+ *   B is a leading member (or batch) axis.
+ * - h_* arguments are HOST pointers.
+ * - stream is a hipStream_t passed as void* (NULL = the default stream).
+ * - The plan owns every table and work buffer; calls on one plan must not
+ *   overlap (one plan per stream/thread).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LBM_OK = 0,
+  LBM_ERR_VALUE = -1,     /* bad argument / shape: Python ValueError           */
+  LBM_ERR_RUNTIME = -2,   /* HIP / device failure: Python RuntimeError          */
+  LBM_ERR_NONFINITE = -3, /* model blew up: Python FloatingPointError           */
+  LBM_ERR_STATE = -4      /* call order (e.g. run before basic state): RuntimeError */
+} lbm_status;
+
+typedef enum { LBM_FP64 = 0, LBM_FP32 = 1 } lbm_precision;
+
+/* Model description.  Mirrors Spectral(nlon, nlat, nmax, radius)
+ * (spectral.py:49-50), VerticalGrid(sigm) (vertical.py:59) and Config
+ * (model.py:34-74); unit conversions (hours/days -> s^-1) are done by the
+ * caller as in integrator.py:41-118, 243-253. */
+typedef struct {
+  int32_t nlon, nlat, nmax, nlev;   /* 128, 64, 42, 20 at T42 L20           */
+  int32_t nmem;                     /* ensemble members E >= 1               */
+  int32_t nbasic;                   /* 1 = one shared basic state, or nmem   */
+  int32_t precision;                /* lbm_precision                         */
+  int32_t strictly_linear;          /* Dynamics(strictly_linear=...)         */
+  int32_t inistp;                   /* start-up half steps (model.py:40)     */
+  int32_t skip_low_n_vdiff;         /* 1: vor/div vdiff skips n=0,1          */
+  double radius;                    /* m                                      */
+  double delt;                      /* s                                      */
+  double robert;                    /* Robert-Asselin gamma                   */
+  const double *h_sigm;             /* nlev+1 half levels, sigm[0]=1 ... 0    */
+  /* damping, s^-1 (integrator.py:41-98): */
+  const double *h_kv, *h_kd, *h_ktt, *h_kq; /* each (nmax+1) x nlev          */
+  /* vertical diffusion nu per level in s^-1, NULL = off (integrator.py:243) */
+  const double *h_vdif_vor, *h_vdif_div, *h_vdif_t, *h_vdif_q;
+} lbm_desc;
+
+typedef struct lbm_plan lbm_plan;
+
+/* Builds the Gaussian grid, FP64 associated-Legendre tables
+ * (spectral.py:57-154), the Simmons-Burridge operators (vertical.py:59-129),
+ * the vertical-diffusion operator (integrator.py:182-206) and allocates
+ * every device buffer. */
+int lbm_plan_create(const lbm_desc *desc, lbm_plan **out);
+int lbm_plan_destroy(lbm_plan *plan);
+
+/* Host copies of setup arrays: "mu", "gw", "coslat", "lat", "lon", "sig",
+ * "dsig", "alpha", "cpg", "w3", "wP", "dsigm", "ww" (nlev*nlev), "dlog3",
+ * "tbar" (nbasic*nlev, after lbm_set_basic_state), "minv" (nbasic*(nmax+1)*
+ * nlev*nlev for dt=delt), "cond" (nbasic*(nmax+1)), "lop" (nlev*nlev).
+ * n is the capacity of h_out in doubles; returns the count written. */
+int lbm_plan_get(const lbm_plan *plan, const char *name, double *h_out, int64_t n);
+
+/* Spectral transforms on caller buffers (spectral.py:186-276).  B members
+ * of nk columns each; complex buffers are interleaved re/im float64.
+ *   lbm_grid_to_wave   : d_g (B,J,I,nk)            -> d_w (B,M,M,nk)
+ *   lbm_wave_to_grid   : d_w (B,M,M,nk)            -> d_g (B,J,I,nk)
+ *   lbm_uv_to_vordiv   : d_u, d_v (B,J,I,nk)       -> d_vor, d_div
+ *   lbm_psichi_to_uv   : d_psi, d_chi (B,M,M,nk)   -> d_u, d_v (grid)
+ *   lbm_hgrad          : d_w (B,M,M,nk)            -> d_gx, d_gy (grid) */
+int lbm_grid_to_wave(lbm_plan *plan, const double *d_g, double *d_w, int B, int nk, void *stream);
+int lbm_wave_to_grid(lbm_plan *plan, const double *d_w, double *d_g, int B, int nk, void *stream);
+int lbm_uv_to_vordiv(lbm_plan *plan, const double *d_u, const double *d_v, double *d_vor,
+                     double *d_div, int B, int nk, void *stream);
+int lbm_psichi_to_uv(lbm_plan *plan, const double *d_psi, const double *d_chi, double *d_u,
+                     double *d_v, int B, int nk, void *stream);
+int lbm_hgrad(lbm_plan *plan, const double *d_w, double *d_gx, double *d_gy, int B, int nk,
+              void *stream);
+
+/* Basic state(s): nbasic states, u,v,t,q (nbasic,J,I,L), ps (nbasic,J,I) in
+ * hPa.  Computes the BasicStateTerms of dynamics.py:54-93 on the device,
+ * the reference temperature, tmtx/gg (vertical.py:139-193) and the FP64
+ * semi-implicit matrices M(n) and their inverses for every start-up dt and
+ * delt (integrator.py:130-163), plus the vertical-diffusion inverses
+ * (integrator.py:216-228). */
+int lbm_set_basic_state(lbm_plan *plan, const double *d_u, const double *d_v,
+                        const double *d_t, const double *d_q, const double *d_ps,
+                        void *stream);
+
+/* Temperature forcing frc_t in K/s, (nfrc,J,I,L) with nfrc = 1 (shared) or
+ * nmem; NULL clears it.  The forcing is time constant (ensemble.py:109-121,
+ * forcing.py:71-96).  The buffer is copied. */
+int lbm_set_forcing(lbm_plan *plan, const double *d_frc_t, int nfrc, void *stream);
+
+/* Initial perturbation, set_initial_perturbation (model.py:142-148): grid
+ * fields (nmem,J,I,L) and ps' (nmem,J,I) in hPa for BOTH time levels; NULL
+ * pointers mean zero.  Resets the clock. */
+int lbm_set_state(lbm_plan *plan, const double *d_u, const double *d_v, const double *d_t,
+                  const double *d_q, const double *d_ps, void *stream);
+
+/* Per-member number of full steps (excluding start-up), NULL = all equal to
+ * nsteps.  Members with fewer steps stop advancing when they reach theirs. */
+int lbm_set_lengths(lbm_plan *plan, const int32_t *h_nsteps, void *stream);
+
+/* Runs the LBM.run loop (model.py:211-253) for the longest member:
+ * start-up half steps then full steps; use_graph = 1 captures the step
+ * sequence as a hipGraph and replays it.  check_every > 0 polls an on-device
+ * finiteness flag every check_every steps (0 = once at the end) and
+ * returns LBM_ERR_NONFINITE naming the step, as model.py:245-247 does.
+ * nsteps <= 0 uses the lengths of lbm_set_lengths. */
+int lbm_run(lbm_plan *plan, int nsteps, int use_graph, int check_every, void *stream);
+
+/* Current time level ga in grid space (state.py:94-99 of the last step). */
+int lbm_get_state(lbm_plan *plan, double *d_u, double *d_v, double *d_t, double *d_q,
+                  double *d_ps, void *stream);
+
+/* Spectral prognostic state (vor, div, t, q (nmem,M,M,L), pi (nmem,M,M))
+ * at the current (level 0) or previous filtered (level 1) time level. */
+int lbm_get_spectral(lbm_plan *plan, int level, double *d_vor, double *d_div, double *d_t,
+                     double *d_q, double *d_pi, void *stream);
+
+/* Number of steps taken since lbm_set_state (start-up included). */
+int lbm_steps_taken(const lbm_plan *plan, int64_t *out);
+
+/* Copies the last error message into buf (always NUL terminated). */
+int lbm_last_error(char *buf, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_HIP_H */

@@ -1,0 +1,122 @@
+"""ctypes binding of ``liblbm_hip.so`` (the C ABI of ``include/lbm_hip.h``).
+
+The library is built in-tree by ``lbm_isca_amd.build`` (``hipcc
+--offload-arch=gfx950``).  There is no fallback: if the library is missing
+or no HIP device is visible, every entry point raises.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+
+LBM_OK = 0
+LBM_ERR_VALUE = -1
+LBM_ERR_RUNTIME = -2
+LBM_ERR_NONFINITE = -3
+LBM_ERR_STATE = -4
+
+LBM_FP64 = 0
+LBM_FP32 = 1
+
+# Every symbol include/lbm_hip.h declares.
+EXPORTS = (
+    "lbm_plan_create", "lbm_plan_destroy", "lbm_plan_get",
+    "lbm_grid_to_wave", "lbm_wave_to_grid", "lbm_uv_to_vordiv", "lbm_psichi_to_uv",
+    "lbm_hgrad", "lbm_set_basic_state", "lbm_set_forcing", "lbm_set_state",
+    "lbm_set_lengths", "lbm_run", "lbm_get_state", "lbm_get_spectral",
+    "lbm_steps_taken", "lbm_last_error",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class LbmDesc(ctypes.Structure):
+    """Mirror of ``lbm_desc``."""
+    _fields_ = [
+        ("nlon", ctypes.c_int32), ("nlat", ctypes.c_int32), ("nmax", ctypes.c_int32),
+        ("nlev", ctypes.c_int32), ("nmem", ctypes.c_int32), ("nbasic", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("strictly_linear", ctypes.c_int32),
+        ("inistp", ctypes.c_int32), ("skip_low_n_vdiff", ctypes.c_int32),
+        ("radius", ctypes.c_double), ("delt", ctypes.c_double), ("robert", ctypes.c_double),
+        ("h_sigm", _dp),
+        ("h_kv", _dp), ("h_kd", _dp), ("h_ktt", _dp), ("h_kq", _dp),
+        ("h_vdif_vor", _dp), ("h_vdif_div", _dp), ("h_vdif_t", _dp), ("h_vdif_q", _dp),
+    ]
+
+
+_LIB = None
+
+
+def load():
+    """Load the library once; raise RuntimeError if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = _vp
+    sig = {
+        "lbm_plan_create": ([ctypes.POINTER(LbmDesc), ctypes.POINTER(P)], ctypes.c_int),
+        "lbm_plan_destroy": ([P], ctypes.c_int),
+        "lbm_plan_get": ([P, ctypes.c_char_p, _dp, ctypes.c_int64], ctypes.c_int),
+        "lbm_grid_to_wave": ([P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_wave_to_grid": ([P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_uv_to_vordiv": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_psichi_to_uv": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_hgrad": ([P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_set_basic_state": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_set_forcing": ([P, P, ctypes.c_int, P], ctypes.c_int),
+        "lbm_set_state": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_set_lengths": ([P, ctypes.POINTER(ctypes.c_int32), P], ctypes.c_int),
+        "lbm_run": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_get_state": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_get_spectral": ([P, ctypes.c_int, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_steps_taken": ([P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "lbm_last_error": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def last_error():
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    lib.lbm_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    """Map a status code to the reference's exception types (backend.py:71-77,
+    model.py:245-247)."""
+    if code >= 0:
+        return code
+    msg = last_error()
+    if code == LBM_ERR_VALUE:
+        raise ValueError(msg)
+    if code == LBM_ERR_NONFINITE:
+        raise FloatingPointError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t):
+    """Device pointer of a contiguous torch tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def dptr(a):
+    """ctypes double* of a contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)

@@ -1,0 +1,1285 @@
+// Host side of the gfx950 time-stepping core: FP64 setup (Gaussian grid,
+// associated Legendre tables, Simmons-Burridge operators, semi-implicit and
+// vertical-diffusion inverses), device buffers, the C ABI of
+// include/lbm_hip.h and the step loop (eager or hipGraph replay).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_model.h"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(LBM_ERR_RUNTIME, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int _r = (expr);         \
+    if (_r != LBM_OK) return _r; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// FP64 host setup
+// ---------------------------------------------------------------------------
+// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_J, sorted
+// north first (descending mu), weights halved to sum to one
+// (spectral.py:57-61).
+void gauss_legendre(int J, std::vector<double> &mu, std::vector<double> &gw) {
+  mu.assign(J, 0.0);
+  gw.assign(J, 0.0);
+  for (int i = 0; i < J; ++i) {
+    double x = std::cos(M_PI * (i + 0.75) / (J + 0.5));
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int n = 2; n <= J; ++n) {
+        const double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = J * (x * p1 - p0) / (x * x - 1.0);
+      const double dx = p1 / dp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-16 && it > 2) break;
+    }
+    {
+      double p0 = 1.0, p1 = x;
+      for (int n = 2; n <= J; ++n) {
+        const double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = J * (x * p1 - p0) / (x * x - 1.0);
+    }
+    mu[i] = x;  // descending already: i=0 is the northernmost root
+    gw[i] = 1.0 / ((1.0 - x * x) * dp * dp);  // = w/2
+  }
+}
+
+// P[m][n][j] (n <= nmax+1) and H[m][n][j] (n <= nmax) of spectral.py:93-138.
+void legendre(int nmax, const std::vector<double> &mu, std::vector<double> &P,
+              std::vector<double> &H) {
+  const int J = (int)mu.size(), M = nmax + 1, NX = nmax + 2;
+  std::vector<double> eps((size_t)M * NX, 0.0), p((size_t)M * NX * J, 0.0);
+  auto E = [&](int m, int n) -> double & { return eps[(size_t)m * NX + n]; };
+  auto Pp = [&](int m, int n, int j) -> double & { return p[((size_t)m * NX + n) * J + j]; };
+  for (int m = 0; m < M; ++m)
+    for (int n = m + 1; n < NX; ++n)
+      E(m, n) = std::sqrt((double(n) * n - double(m) * m) / (4.0 * n * n - 1.0));
+  for (int j = 0; j < J; ++j) {
+    const double x = mu[j], c = std::sqrt(1.0 - x * x);
+    Pp(0, 0, j) = 1.0;
+    for (int m = 1; m < M; ++m) Pp(m, m, j) = Pp(m - 1, m - 1, j) * std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * c;
+    for (int m = 0; m < M; ++m) {
+      Pp(m, m + 1, j) = Pp(m, m, j) * std::sqrt(2.0 * m + 3.0) * x;
+      for (int n = m + 2; n < NX; ++n)
+        Pp(m, n, j) = (x * Pp(m, n - 1, j) - E(m, n - 1) * Pp(m, n - 2, j)) / E(m, n);
+    }
+  }
+  P.assign((size_t)M * M * J, 0.0);
+  H.assign((size_t)M * M * J, 0.0);
+  for (int m = 0; m < M; ++m)
+    for (int n = m; n < M; ++n)
+      for (int j = 0; j < J; ++j) {
+        P[((size_t)m * M + n) * J + j] = Pp(m, n, j);
+        double h = -n * E(m, n + 1) * Pp(m, n + 1, j);
+        if (n - 1 >= m) h += (n + 1.0) * E(m, n) * Pp(m, n - 1, j);
+        H[((size_t)m * M + n) * J + j] = h;
+      }
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting (FP64).
+bool invert(std::vector<double> &a, int n) {
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+    if (a[(size_t)piv * n + c] == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const double d = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
+template <class T>
+struct DevTabs {
+  T *Ps = nullptr, *Hs = nullptr, *Pa = nullptr, *Ha = nullptr;  // [s][Jh]
+  T *inv_lap = nullptr, *nn1a2 = nullptr, *lap = nullptr;        // [M]
+  T *rcos = nullptr;                                             // [J]
+  T *vc = nullptr;                                               // VOff
+  T *kv = nullptr, *kd = nullptr, *ktt = nullptr, *kq = nullptr; // [M][L]
+  T *tref = nullptr;                                             // [nb][L]
+  T *minv = nullptr;                                             // [nset][nb][M][L][L]
+  T *vinv = nullptr;                                             // [nset][4][L][L]
+};
+
+template <class T>
+int upload(T **dst, const std::vector<double> &src) {
+  std::vector<T> tmp(src.begin(), src.end());
+  if (*dst == nullptr) HIPCHK(hipMalloc((void **)dst, std::max<size_t>(tmp.size(), 1) * sizeof(T)));
+  HIPCHK(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+template <class T>
+void free_tabs(DevTabs<T> &t) {
+  T *ps[] = {t.Ps, t.Hs, t.Pa, t.Ha, t.inv_lap, t.nn1a2, t.lap, t.rcos, t.vc,
+             t.kv, t.kd, t.ktt, t.kq, t.tref, t.minv, t.vinv};
+  for (T *p : ps)
+    if (p) (void)hipFree(p);
+  t = DevTabs<T>();
+}
+
+inline unsigned nblk(size_t n, unsigned bs = 256) {
+  size_t b = (n + bs - 1) / bs;
+  return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 65535u * 16u);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct lbm_plan {
+  // sizes
+  int J = 0, I = 0, N = 0, M = 0, L = 0, E = 0, nb = 0, prec = 0, nspec = 0, Jh = 0, nset = 0;
+  int inistp = 2, strictly_linear = 0, skip_low_n = 1, vmask = 0;
+  double a = 0, delt = 0, robert = 0;
+  // host FP64 setup
+  std::vector<double> mu, gw, coslat, lat, lon;
+  std::vector<double> sigm, dsig, alpha, sig, dlog2, dlog3, cpg, w3, wP, dsigm, ww, lop;
+  std::vector<double> kv, kd, ktt, kq;
+  std::vector<double> vdif[4];
+  std::vector<double> tbar, minv, vinv;
+  std::vector<short> n_of_s;
+  // device
+  DevTabs<double> d64;
+  DevTabs<float> d32;
+  short *d_n_of_s = nullptr;
+  double *d_mu = nullptr;
+  double *d_psbar = nullptr;  // [nb][J][I] FP64 basic surface pressure
+  void *sa3 = nullptr, *sa2 = nullptr, *sb3 = nullptr, *tend3 = nullptr, *tend2 = nullptr;
+  void *fin = nullptr, *gin = nullptr, *gout = nullptr, *fout = nullptr;
+  void *raw = nullptr, *psa = nullptr, *psb = nullptr, *bsg = nullptr, *frc = nullptr;
+  int nfrc = 0;
+  Ctl *ctl = nullptr;
+  int *pend = nullptr, *d_nsteps = nullptr;
+  std::vector<int> nsteps_h;
+  bool basic_set = false, have_raw = false;
+  long long steps_total = 0;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  hipStream_t own = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int gsteps[2] = {0, 0};
+
+  template <class T>
+  DevTabs<T> &tabs();
+  size_t rsize() const { return prec == LBM_FP64 ? 8 : 4; }
+  hipStream_t strm(void *s) const { return s ? (hipStream_t)s : own; }
+};
+template <>
+DevTabs<double> &lbm_plan::tabs<double>() { return d64; }
+template <>
+DevTabs<float> &lbm_plan::tabs<float>() { return d32; }
+
+namespace {
+
+int scratch_get(lbm_plan *p, size_t bytes, void **out, hipStream_t st) {
+  if (bytes > p->scratch_bytes) {
+    if (p->scratch) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(p->scratch));
+      p->scratch = nullptr;
+    }
+    HIPCHK(hipMalloc(&p->scratch, bytes));
+    p->scratch_bytes = bytes;
+  }
+  *out = p->scratch;
+  return LBM_OK;
+}
+
+void destroy_graphs(lbm_plan *p) {
+  for (int g = 0; g < 2; ++g)
+    if (p->gexec[g]) {
+      (void)hipGraphExecDestroy(p->gexec[g]);
+      p->gexec[g] = nullptr;
+      p->gsteps[g] = 0;
+    }
+}
+
+// Tables of one precision from the FP64 host setup.
+template <class T>
+int build_tabs(lbm_plan *p) {
+  DevTabs<T> &t = p->tabs<T>();
+  const int M = p->M, J = p->J, Jh = p->Jh, L = p->L;
+  std::vector<double> P, H;
+  legendre(p->N, p->mu, P, H);
+  std::vector<double> Ps((size_t)p->nspec * Jh), Hs(Ps.size()), Pa(Ps.size()), Ha(Ps.size());
+  for (int m = 0; m < M; ++m)
+    for (int n = m; n < M; ++n) {
+      const int s = spec_off(m, M) + n - m;
+      for (int j = 0; j < Jh; ++j) {
+        const double pv = P[((size_t)m * M + n) * J + j], hv = H[((size_t)m * M + n) * J + j];
+        Ps[(size_t)s * Jh + j] = pv;
+        Hs[(size_t)s * Jh + j] = hv;
+        Pa[(size_t)s * Jh + j] = pv * p->gw[j];
+        Ha[(size_t)s * Jh + j] = hv * p->gw[j];
+      }
+    }
+  CHK(upload(&t.Ps, Ps));
+  CHK(upload(&t.Hs, Hs));
+  CHK(upload(&t.Pa, Pa));
+  CHK(upload(&t.Ha, Ha));
+  std::vector<double> il(M), nn(M), lp(M), rc(J);
+  for (int n = 0; n < M; ++n) {
+    const double nn1 = double(n) * (n + 1.0);
+    il[n] = n > 0 ? -p->a * p->a / nn1 : 0.0;
+    nn[n] = nn1 / (p->a * p->a);
+    lp[n] = -nn1 / (p->a * p->a);
+  }
+  for (int j = 0; j < J; ++j) rc[j] = 1.0 / (p->a * p->coslat[j]);
+  CHK(upload(&t.inv_lap, il));
+  CHK(upload(&t.nn1a2, nn));
+  CHK(upload(&t.lap, lp));
+  CHK(upload(&t.rcos, rc));
+  VOff vo{L};
+  std::vector<double> vc(vo.size());
+  for (int k = 0; k < L; ++k) {
+    vc[vo.dsig() + k] = p->dsig[k];
+    vc[vo.alpha() + k] = p->alpha[k];
+    vc[vo.w3() + k] = p->w3[k];
+    vc[vo.wP() + k] = p->wP[k];
+    vc[vo.cpg() + k] = p->cpg[k];
+    vc[vo.dlog3() + k] = p->dlog3[k];
+  }
+  for (int k = 0; k <= L; ++k) vc[vo.sigm() + k] = p->sigm[k];
+  CHK(upload(&t.vc, vc));
+  CHK(upload(&t.kv, p->kv));
+  CHK(upload(&t.kd, p->kd));
+  CHK(upload(&t.ktt, p->ktt));
+  CHK(upload(&t.kq, p->kq));
+  return LBM_OK;
+}
+
+template <class T>
+int set_twiddles() {
+  cx<T> tw[FFT_I];
+  for (int k = 0; k < FFT_I; ++k) {
+    const double ang = -2.0 * M_PI * k / FFT_I;
+    tw[k] = mk((T)std::cos(ang), (T)std::sin(ang));
+  }
+  if (sizeof(T) == 8)
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_d), tw, sizeof(tw)));
+  else
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_f), tw, sizeof(tw)));
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Transform building blocks on internal buffers.
+// ---------------------------------------------------------------------------
+template <class T, class R>
+int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
+               hipStream_t st) {
+  DevTabs<T> &t = p->tabs<T>();
+  dim3 grid((ncols + 7) / 8, p->M);
+  hipLaunchKernelGGL((k_syn<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
+                     t.Ps, t.Hs, t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T, class R>
+int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
+               hipStream_t st) {
+  DevTabs<T> &t = p->tabs<T>();
+  dim3 grid((ncols + 15) / 16, p->M);
+  hipLaunchKernelGGL((k_ana<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
+                     t.Pa, t.Ha, t.nn1a2, ncols, C, sl, p->J, p->M);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st) {
+  const int C = cm.ncols();
+  const long rows = (long)p->J * C;
+  hipLaunchKernelGGL((k_ifft128<T>), dim3((unsigned)((rows + FFT_ROWS - 1) / FFT_ROWS)), dim3(256),
+                     0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st) {
+  const int C = cm.ncols();
+  const long rows = (long)p->J * C;
+  hipLaunchKernelGGL((k_fft128<T>), dim3((unsigned)((rows + FFT_ROWS - 1) / FFT_ROWS)), dim3(256),
+                     0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Spectral API: (B, nk) transforms on caller FP64 buffers.
+// ---------------------------------------------------------------------------
+enum SpecOp { OP_G2W, OP_W2G, OP_UV2VD, OP_PC2UV, OP_HGRAD };
+
+template <class T>
+int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, double *out0,
+                double *out1, int B, int nk, hipStream_t st) {
+  if (B <= 0 || nk <= 0) return fail(LBM_ERR_VALUE, "batch and column counts must be positive");
+  const int J = p->J, I = p->I, M = p->M;
+  const int ncols = B * nk;
+  const int nf = (op == OP_G2W || op == OP_W2G) ? 1 : 2;
+  const size_t gbytes = (size_t)B * J * nf * nk * I * sizeof(T);
+  const size_t fbytes = (size_t)M * J * nf * ncols * sizeof(cx<T>);
+  const size_t sbytes = (size_t)p->nspec * nf * ncols * sizeof(cx<T>);
+  const size_t al = 256;
+  auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+  void *base;
+  CHK(scratch_get(p, up(gbytes) + up(fbytes) + up(sbytes), &base, st));
+  char *cb = (char *)base;
+  T *g = (T *)cb;
+  cx<T> *f = (cx<T> *)(cb + up(gbytes));
+  cx<T> *s = (cx<T> *)(cb + up(gbytes) + up(fbytes));
+  ColMap cm{B, nk, nf, 0};
+  const size_t gtot = (size_t)B * J * I * nk;
+  const double *rc64 = p->d64.rcos;
+  const size_t stot = (size_t)p->nspec * ncols;
+  if (op == OP_G2W || op == OP_UV2VD) {
+    hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, in0, g, B, J, I, nk,
+                       nf * nk, 0, op == OP_UV2VD ? rc64 : (const double *)nullptr);
+    if (op == OP_UV2VD)
+      hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, in1, g, B, J, I, nk,
+                         nf * nk, nk, rc64);
+    HIPCHK(hipGetLastError());
+    CHK(launch_fft<T>(p, g, f, cm, st));
+    if (op == OP_G2W) {
+      InSlots sl{{0}};
+      CHK((launch_ana<T, AnaScalar>(p, f, s, ncols, ncols, sl, st)));
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
+                         out0, B, M, nk, 1, 0, ncols);
+    } else {
+      InSlots sl{{0, ncols}};
+      CHK((launch_ana<T, AnaVorDiv>(p, f, s, ncols, 2 * ncols, sl, st)));
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
+                         out0, B, M, nk, 2, 0, ncols);
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
+                         out1, B, M, nk, 2, 1, ncols);
+    }
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+  }
+  // synthesis family
+  const int nin = (op == OP_PC2UV) ? 2 : 1;
+  hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(stot)), dim3(256), 0, st, in0, s, B, M, nk, nin, 0,
+                     ncols);
+  if (op == OP_PC2UV)
+    hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(stot)), dim3(256), 0, st, in1, s, B, M, nk, nin,
+                       1, ncols);
+  HIPCHK(hipGetLastError());
+  if (op == OP_W2G) {
+    OutSlots sl{{0}};
+    CHK((launch_syn<T, SynScalar>(p, s, f, ncols, ncols, sl, st)));
+  } else if (op == OP_PC2UV) {
+    OutSlots sl{{0, ncols}};
+    CHK((launch_syn<T, SynPsiChi>(p, s, f, ncols, 2 * ncols, sl, st)));
+  } else {
+    OutSlots sl{{0, ncols}};
+    CHK((launch_syn<T, SynGrad>(p, s, f, ncols, 2 * ncols, sl, st)));
+  }
+  CHK(launch_ifft<T>(p, f, g, cm, st));
+  hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, g, out0, B, J, I, nk,
+                     nf * nk, 0, 0, (const T *)nullptr, 0, 0, 1);
+  if (nf == 2)
+    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, g, out1, B, J, I, nk,
+                       nf * nk, nk, 0, (const T *)nullptr, 0, 0, 1);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int spectral_dispatch(lbm_plan *p, SpecOp op, const double *a0, const double *a1, double *o0,
+                      double *o1, int B, int nk, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  hipStream_t st = p->strm(stream);
+  if (p->prec == LBM_FP64) return spectral_op<double>(p, op, a0, a1, o0, o1, B, nk, st);
+  return spectral_op<float>(p, op, a0, a1, o0, o1, B, nk, st);
+}
+
+// ---------------------------------------------------------------------------
+// Vertical operators (vertical.py:59-129) and the vertical-diffusion
+// Laplacian (integrator.py:182-206), FP64 on the host.
+// ---------------------------------------------------------------------------
+int build_vertical(lbm_plan *p, const double *sigm) {
+  const int L = p->L;
+  p->sigm.assign(sigm, sigm + L + 1);
+  p->dsig.resize(L);
+  p->alpha.resize(L);
+  p->sig.resize(L);
+  p->dlog2.resize(L);
+  p->dlog3.resize(L);
+  p->cpg.resize(L);
+  p->w3.resize(L);
+  p->wP.resize(L);
+  for (int k = 0; k < L; ++k) {
+    const double bot = sigm[k], top = sigm[k + 1];
+    if (!(bot > top) || top < 0.0) return fail(LBM_ERR_VALUE, "sigm must decrease from 1 to 0");
+    p->dsig[k] = bot - top;
+    const double ratio = top > 0.0 ? top * (std::log(bot) - std::log(top)) / (bot - top) : 0.0;
+    p->alpha[k] = 1.0 - ratio;
+    p->sig[k] = bot * std::exp(-p->alpha[k]);
+    p->dlog2[k] = top > 0.0 ? std::log(p->sig[k]) - std::log(top) : 0.0;
+    p->dlog3[k] = top > 0.0 ? std::log(bot) - std::log(top) : 0.0;
+    p->cpg[k] = (bot * p->alpha[k] + top * p->dlog2[k]) / p->dsig[k];
+    p->w3[k] = p->dlog3[k] / p->dsig[k];
+    p->wP[k] = p->alpha[k] - p->cpg[k];
+  }
+  p->dsigm.assign(L + 1, 0.0);
+  p->dsigm[0] = sigm[0] - p->sig[0];
+  for (int k = 1; k < L; ++k) p->dsigm[k] = p->sig[k - 1] - p->sig[k];
+  p->dsigm[L] = p->sig[L - 1] - sigm[L];
+  p->ww.assign((size_t)L * L, 0.0);
+  for (int k = 0; k < L; ++k) {
+    for (int l = 0; l < k; ++l) p->ww[(size_t)k * L + l] = R_DRY * p->dlog3[l];
+    p->ww[(size_t)k * L + k] = R_DRY * p->alpha[k];
+  }
+  // tridiagonal sigma Laplacian with zero-flux ends
+  std::vector<double> lo(L, 0.0), up(L, 0.0), di(L, 0.0);
+  const std::vector<double> &ds = p->dsig, &dm = p->dsigm;
+  for (int k = 1; k < L; ++k) lo[k] = 1.0 / (ds[k] * dm[k]);
+  for (int k = 0; k < L - 1; ++k) up[k] = 1.0 / (ds[k] * dm[k + 1]);
+  for (int k = 0; k < L; ++k) di[k] = -(lo[k] + up[k]);
+  up[0] = (dm[0] + dm[1]) / (ds[0] * dm[0] * dm[1]);
+  lo[0] = 0.0;
+  di[0] = -up[0];
+  lo[L - 1] = (dm[L - 1] + dm[L]) / (ds[L - 1] * dm[L - 1] * dm[L]);
+  up[L - 1] = 0.0;
+  di[L - 1] = -lo[L - 1];
+  p->lop.assign((size_t)L * L, 0.0);
+  for (int k = 0; k < L; ++k) {
+    p->lop[(size_t)k * L + k] = di[k];
+    if (k > 0) p->lop[(size_t)k * L + k - 1] = lo[k];
+    if (k < L - 1) p->lop[(size_t)k * L + k + 1] = up[k];
+  }
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Basic state: FP64 transform pipeline, then the reference-temperature
+// operators and the FP64 semi-implicit inverses on the host.
+// ---------------------------------------------------------------------------
+double step_dt(const lbm_plan *p, int set) {
+  return set < p->inistp ? p->delt / std::ldexp(1.0, p->inistp - set) : p->delt;
+}
+
+int build_matrices(lbm_plan *p) {
+  const int M = p->M, L = p->L, nb = p->nb, nset = p->nset;
+  p->minv.assign((size_t)nset * nb * M * L * L, 0.0);
+  for (int set = 0; set < nset; ++set) {
+    const double dt = step_dt(p, set), dt2 = 2.0 * dt;
+    for (int b = 0; b < nb; ++b) {
+      const double *tb = &p->tbar[(size_t)b * L];
+      // tmtx (vertical.py:183-188), whmtx = WW tmtx, gcmtx = gg (x) dsig
+      std::vector<double> tm((size_t)L * L, 0.0), wh((size_t)L * L, 0.0), gg(L);
+      for (int k = 0; k < L; ++k) {
+        tm[(size_t)k * L + k] = KAPPA * tb[k] * p->alpha[k];
+        for (int l = k + 1; l < L; ++l) tm[(size_t)k * L + l] = KAPPA * tb[k] * p->w3[k] * p->dsig[l];
+        gg[k] = R_DRY * tb[k] * p->cpg[k];
+      }
+      for (int r = 0; r < L; ++r)
+        for (int c = 0; c < L; ++c) {
+          double acc = 0;
+          for (int q = 0; q < L; ++q) acc += p->ww[(size_t)r * L + q] * tm[(size_t)q * L + c];
+          wh[(size_t)r * L + c] = acc;
+        }
+      // M(n) = diag(fd ft) - dt^2 lap_n (whmtx + gcmtx ft)  (integrator.py:149-158)
+      for (int n = 0; n < M; ++n) {
+        const double lap = -double(n) * (n + 1.0) / (p->a * p->a);
+        std::vector<double> mat((size_t)L * L);
+        for (int r = 0; r < L; ++r) {
+          const double ft = 1.0 + dt2 * p->ktt[(size_t)n * L + r];
+          const double fd = 1.0 + dt2 * p->kd[(size_t)n * L + r];
+          for (int c = 0; c < L; ++c)
+            mat[(size_t)r * L + c] = -dt * dt * lap * (wh[(size_t)r * L + c] + gg[r] * p->dsig[c] * ft);
+          mat[(size_t)r * L + r] += fd * ft;
+        }
+        if (!invert(mat, L))
+          return fail(LBM_ERR_VALUE, "singular semi-implicit matrix (n=%d, basic state %d)", n, b);
+        std::copy(mat.begin(), mat.end(), p->minv.begin() + (((size_t)set * nb + b) * M + n) * L * L);
+      }
+    }
+  }
+  return LBM_OK;
+}
+
+// (I - dt nu lop)^-1 per variable and dt (integrator.py:216-228)
+int build_vdiff(lbm_plan *p) {
+  const int L = p->L, nset = p->nset;
+  p->vinv.assign((size_t)nset * 4 * L * L, 0.0);
+  p->vmask = 0;
+  for (int f = 0; f < 4; ++f)
+    if (!p->vdif[f].empty()) p->vmask |= 1 << f;
+  for (int set = 0; set < nset; ++set) {
+    const double dt = step_dt(p, set);
+    for (int f = 0; f < 4; ++f) {
+      std::vector<double> mat((size_t)L * L, 0.0);
+      for (int r = 0; r < L; ++r) {
+        const double nu = p->vdif[f].empty() ? 0.0 : p->vdif[f][r];
+        for (int c = 0; c < L; ++c)
+          mat[(size_t)r * L + c] = (r == c ? 1.0 : 0.0) - dt * nu * p->lop[(size_t)r * L + c];
+      }
+      if (!invert(mat, L)) return fail(LBM_ERR_VALUE, "singular vertical-diffusion matrix");
+      std::copy(mat.begin(), mat.end(), p->vinv.begin() + ((size_t)set * 4 + f) * L * L);
+    }
+  }
+  return LBM_OK;
+}
+
+template <class T>
+int upload_operators(lbm_plan *p) {
+  DevTabs<T> &t = p->tabs<T>();
+  CHK(upload(&t.tref, p->tbar));
+  CHK(upload(&t.minv, p->minv));
+  CHK(upload(&t.vinv, p->vinv));
+  return LBM_OK;
+}
+
+int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, const double *q,
+              const double *ps, hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, nb = p->nb, M = p->M;
+  const BRow br{L};
+  const int ncl = nb * L;
+  const ColMap cm{nb, L, 2, 1};  // u/(a cos), v/(a cos), ln ps
+  const int C = cm.ncols();
+  const size_t g1 = (size_t)nb * J * cm.rows() * I;
+  const size_t f1 = (size_t)M * J * C;              // complex
+  const size_t s1 = (size_t)p->nspec * C;           // complex
+  const size_t g2 = (size_t)nb * J * 2 * L * I, g3 = (size_t)nb * J * 2 * I;
+  const size_t bs = (size_t)nb * J * br.rows() * I;
+  double *w = nullptr;
+  const size_t total = g1 + 2 * f1 + 2 * s1 + g2 + g3 + bs + (size_t)nb * L;
+  HIPCHK(hipMalloc(&w, total * sizeof(double)));
+  double *cur = w;
+  double *gtmp = cur; cur += g1;
+  cx<double> *F = (cx<double> *)cur; cur += 2 * f1;
+  cx<double> *S = (cx<double> *)cur; cur += 2 * s1;
+  double *gs = cur; cur += g2;
+  double *gg = cur; cur += g3;
+  double *bsg64 = cur; cur += bs;
+  double *tb = cur;
+  const double *rc = p->d64.rcos;
+  const size_t n3 = (size_t)nb * J * I * L, n2 = (size_t)nb * J * I;
+  hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk(n3)), dim3(256), 0, st, u, gtmp, nb, J, I, L,
+                     cm.rows(), 0, rc);
+  hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk(n3)), dim3(256), 0, st, v, gtmp, nb, J, I, L,
+                     cm.rows(), L, rc);
+  hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk(n2)), dim3(256), 0, st, ps, gtmp, nb, J, I, 1,
+                     cm.rows(), 2 * L, (const double *)nullptr);
+  hipLaunchKernelGGL((k_log_row<double>), dim3(nblk(n2)), dim3(256), 0, st, gtmp, nb, J, I,
+                     cm.rows(), 2 * L);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(p->d_psbar, ps, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  CHK(launch_fft<double>(p, gtmp, F, cm, st));
+  cx<double> *Sln = S + (size_t)p->nspec * 2 * ncl;
+  {
+    InSlots sl{{0, ncl}};
+    CHK((launch_ana<double, AnaVorDiv>(p, F, S, ncl, C, sl, st)));
+    InSlots s2{{2 * ncl}};
+    CHK((launch_ana<double, AnaScalar>(p, F, Sln, nb, C, s2, st)));
+  }
+  // grid vor, div (two fields as 2*ncl columns of one) and grad ln ps
+  {
+    OutSlots sl{{0}};
+    CHK((launch_syn<double, SynScalar>(p, S, F, 2 * ncl, 2 * ncl, sl, st)));
+    CHK(launch_ifft<double>(p, F, gs, ColMap{nb, L, 2, 0}, st));
+    OutSlots s2{{0, nb}};
+    CHK((launch_syn<double, SynGrad>(p, Sln, F, nb, 2 * nb, s2, st)));
+    CHK(launch_ifft<double>(p, F, gg, ColMap{nb, 1, 0, 2}, st));
+  }
+  hipLaunchKernelGGL(k_basic_terms, dim3(nb * J), dim3(I), 0, st, u, v, t, q, ps, gs, gg,
+                     p->d64.vc, p->d_mu, bsg64, J, I, L);
+  hipLaunchKernelGGL(k_tbar, dim3(nb * L), dim3(256), 0, st, bsg64, tb, J, I, L);
+  HIPCHK(hipGetLastError());
+  p->tbar.assign((size_t)nb * L, 0.0);
+  HIPCHK(hipMemcpyAsync(p->tbar.data(), tb, (size_t)nb * L * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (p->prec == LBM_FP64) {
+    HIPCHK(hipMemcpyAsync(p->bsg, bsg64, bs * sizeof(double), hipMemcpyDeviceToDevice, st));
+  } else {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(bs)), dim3(256), 0, st, bsg64,
+                       (float *)p->bsg, bs);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipFree(w));
+  CHK(build_matrices(p));
+  CHK(upload_operators<double>(p));
+  if (p->prec == LBM_FP32) CHK(upload_operators<float>(p));
+  p->basic_set = true;
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// State I/O
+// ---------------------------------------------------------------------------
+template <class T>
+int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, const double *q,
+                const double *ps, hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, E = p->E, M = p->M;
+  const int EL = E * L;
+  const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
+  const size_t s3 = (size_t)p->nspec * 4 * EL, s2 = (size_t)p->nspec * E;
+  HIPCHK(hipMemsetAsync(p->pend, 0, E * sizeof(int), st));
+  if (ps) {
+    hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, ps, (T *)p->psa, E, J, I,
+                       1, 1, 0, (const double *)nullptr);
+    HIPCHK(hipGetLastError());
+  } else {
+    HIPCHK(hipMemsetAsync(p->psa, 0, n2 * sizeof(T), st));
+  }
+  HIPCHK(hipMemcpyAsync(p->psb, p->psa, n2 * sizeof(T), hipMemcpyDeviceToDevice, st));
+  const double *src[4] = {u, v, t, q};
+  if (!(u || v || t || q || ps)) {
+    HIPCHK(hipMemsetAsync(p->sa3, 0, s3 * sizeof(cx<T>), st));
+    HIPCHK(hipMemsetAsync(p->sb3, 0, s3 * sizeof(cx<T>), st));
+    HIPCHK(hipMemsetAsync(p->sa2, 0, s2 * sizeof(cx<T>), st));
+    p->have_raw = false;
+    return LBM_OK;
+  }
+  // raw grid (u, v, t, q) for the first tendency call
+  for (int f = 0; f < 4; ++f) {
+    if (src[f])
+      hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, src[f], (T *)p->raw, E,
+                         J, I, L, 4 * L, f * L, (const double *)nullptr);
+    else
+      hipLaunchKernelGGL((k_zero_rows<T>), dim3(nblk(n3)), dim3(256), 0, st, (T *)p->raw, E, J, I,
+                         4 * L, f * L, L);
+  }
+  HIPCHK(hipGetLastError());
+  // analysis (state.py:86-92): rows u/(a cos), v/(a cos), t, q; surface pi'
+  const ColMap cm{E, L, 4, 1};
+  const int C = cm.ncols();
+  const size_t gb = ((size_t)E * J * cm.rows() * I * sizeof(T) + 255) / 256 * 256;
+  const size_t fb = (size_t)M * J * C * sizeof(cx<T>);
+  void *base;
+  CHK(scratch_get(p, gb + fb, &base, st));
+  T *g = (T *)base;
+  cx<T> *F = (cx<T> *)((char *)base + gb);
+  const double *rc = p->d64.rcos;
+  for (int f = 0; f < 4; ++f) {
+    if (src[f])
+      hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, src[f], g, E, J, I, L,
+                         cm.rows(), f * L, f < 2 ? rc : (const double *)nullptr);
+    else
+      hipLaunchKernelGGL((k_zero_rows<T>), dim3(nblk(n3)), dim3(256), 0, st, g, E, J, I, cm.rows(),
+                         f * L, L);
+  }
+  hipLaunchKernelGGL((k_pack_pi<T>), dim3(nblk(n2)), dim3(256), 0, st, ps, g, E, J, I, cm.rows(),
+                     4 * L, (const double *)p->d_psbar, p->nb);
+  HIPCHK(hipGetLastError());
+  CHK(launch_fft<T>(p, g, F, cm, st));
+  {
+    InSlots sl{{0, EL, 2 * EL, 3 * EL}};
+    CHK((launch_ana<T, AnaState>(p, F, p->sa3, EL, C, sl, st)));
+    InSlots s2{{4 * EL}};
+    CHK((launch_ana<T, AnaScalar>(p, F, p->sa2, E, C, s2, st)));
+  }
+  HIPCHK(hipMemcpyAsync(p->sb3, p->sa3, s3 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
+  p->have_raw = true;
+  return LBM_OK;
+}
+
+// Synthesis of the current spectral level into gin ([e][j][6L+3][I]).
+template <class T>
+int synthesize_state(lbm_plan *p, hipStream_t st) {
+  const int E = p->E, L = p->L, EL = E * L;
+  const ColMap cm{E, L, 6, 3};
+  const int C = cm.ncols();
+  OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
+  CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, C, sl, st)));
+  OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E}};
+  CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, C, s2, st)));
+  CHK(launch_ifft<T>(p, p->fin, p->gin, cm, st));
+  return LBM_OK;
+}
+
+template <class T>
+int get_state_t(lbm_plan *p, double *u, double *v, double *t, double *q, double *ps,
+                hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, E = p->E;
+  const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
+  double *dst[4] = {u, v, t, q};
+  if (p->have_raw && p->steps_total == 0) {
+    for (int f = 0; f < 4; ++f)
+      if (dst[f])
+        hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, (const T *)p->raw,
+                           dst[f], E, J, I, L, 4 * L, f * L, 0, (const T *)nullptr, 0, 0, 1);
+    if (ps)
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->psa, ps,
+                         E, J, I, 1, 1, 0, 0, (const T *)nullptr, 0, 0, 1);
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+  }
+  CHK(synthesize_state<T>(p, st));
+  const int rows = 6 * L + 3;
+  const int r0[4] = {0, L, 4 * L, 5 * L};
+  for (int f = 0; f < 4; ++f)
+    if (dst[f])
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, (const T *)p->gin,
+                         dst[f], E, J, I, L, rows, r0[f], 0, (const T *)nullptr, 0, 0, 1);
+  if (ps) {
+    const BRow br{L};
+    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->gin, ps, E,
+                       J, I, 1, rows, 6 * L + 2, 1, (const T *)p->bsg, br.rows(), br.psbar(), p->nb);
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One time step: synthesis, grid tendencies, analysis, spectral step.
+// ---------------------------------------------------------------------------
+template <class T, int L>
+int launch_step_kernel(lbm_plan *p, hipStream_t st) {
+  DevTabs<T> &t = p->tabs<T>();
+  StepArgs<T> a;
+  a.sa3 = (cx<T> *)p->sa3;
+  a.sa2 = (cx<T> *)p->sa2;
+  a.sb3 = (cx<T> *)p->sb3;
+  a.tend3 = (const cx<T> *)p->tend3;
+  a.tend2 = (const cx<T> *)p->tend2;
+  a.kv = t.kv;
+  a.kd = t.kd;
+  a.ktt = t.ktt;
+  a.kq = t.kq;
+  a.minv = t.minv;
+  a.vinv = t.vinv;
+  a.vc = t.vc;
+  a.tref = t.tref;
+  a.lap = t.lap;
+  a.n_of_s = p->d_n_of_s;
+  a.ctl = p->ctl;
+  a.pend = p->pend;
+  a.nsteps = p->d_nsteps;
+  a.E = p->E;
+  a.M = p->M;
+  a.nspec = p->nspec;
+  a.nbasic = p->nb;
+  a.inistp = p->inistp;
+  a.vmask = p->vmask;
+  a.skip_low_n = p->skip_low_n;
+  a.delt = (T)p->delt;
+  a.robert = (T)p->robert;
+  const size_t nthr = (size_t)p->nspec * p->E;
+  hipLaunchKernelGGL((k_step<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int one_step(lbm_plan *p, hipStream_t st) {
+  const int E = p->E, L = p->L, EL = E * L;
+  DevTabs<T> &t = p->tabs<T>();
+  CHK(synthesize_state<T>(p, st));
+  TendArgs<T> ta;
+  ta.gin = (const T *)p->gin;
+  ta.raw = p->have_raw ? (const T *)p->raw : nullptr;
+  ta.bsg = (const T *)p->bsg;
+  ta.frc = (const T *)p->frc;
+  ta.gout = (T *)p->gout;
+  ta.psa = (T *)p->psa;
+  ta.psb = (T *)p->psb;
+  ta.pend = p->pend;
+  ta.vc = t.vc;
+  ta.tref = t.tref;
+  ta.rcos = t.rcos;
+  ta.ctl = p->ctl;
+  ta.E = E;
+  ta.J = p->J;
+  ta.I = p->I;
+  ta.L = L;
+  ta.nbasic = p->nb;
+  ta.nfrc = p->nfrc;
+  ta.strictly_linear = p->strictly_linear;
+  ta.robert = (T)p->robert;
+  hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
+  HIPCHK(hipGetLastError());
+  const ColMap cmo{E, L, 9, 2};
+  const int Co = cmo.ncols();
+  CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st));
+  InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
+  CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st)));
+  InSlots s2{{9 * EL}};
+  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st)));
+  if (L == 20)
+    CHK((launch_step_kernel<T, 20>(p, st)));
+  else if (L == 40)
+    CHK((launch_step_kernel<T, 40>(p, st)));
+  else
+    return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, p->ctl);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int step_dispatch(lbm_plan *p, hipStream_t st) {
+  return p->prec == LBM_FP64 ? one_step<double>(p, st) : one_step<float>(p, st);
+}
+
+int capture_graph(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
+  if (p->gexec[slot] && p->gsteps[slot] == nsteps) return LBM_OK;
+  if (p->gexec[slot]) {
+    (void)hipGraphExecDestroy(p->gexec[slot]);
+    p->gexec[slot] = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = LBM_OK;
+  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = step_dispatch(p, st);
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (rc != LBM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e2 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph capture failed: %s", hipGetErrorString(e2));
+  hipError_t e3 = hipGraphInstantiate(&p->gexec[slot], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph instantiate failed: %s", hipGetErrorString(e3));
+  p->gsteps[slot] = nsteps;
+  return LBM_OK;
+}
+
+int check_bad(lbm_plan *p, hipStream_t st, long long base_step) {
+  Ctl h;
+  HIPCHK(hipMemcpyAsync(&h, p->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h.bad_step != INT_MAX)
+    return fail(LBM_ERR_NONFINITE, "model blew up at step %d (t=%.2f d)", h.bad_step,
+                (double)std::max(0LL, (long long)h.bad_step - p->inistp) * p->delt / 86400.0);
+  (void)base_step;
+  return LBM_OK;
+}
+
+template <class T>
+int final_check(lbm_plan *p, hipStream_t st) {
+  const size_t n3 = (size_t)p->nspec * 4 * p->E * p->L, n2 = (size_t)p->nspec * p->E;
+  hipLaunchKernelGGL((k_check_spec<T>), dim3(256), dim3(256), 0, st, (const cx<T> *)p->sa3, n3,
+                     (const cx<T> *)p->sa2, n2, p->ctl);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+size_t alloc_bytes(const lbm_plan *p) {
+  (void)p;
+  return sizeof(T);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int lbm_last_error(char *buf, int64_t len) {
+  if (!buf || len <= 0) return LBM_ERR_VALUE;
+  std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return LBM_OK;
+}
+
+int lbm_plan_destroy(lbm_plan *p) {
+  if (!p) return LBM_OK;
+  destroy_graphs(p);
+  free_tabs(p->d64);
+  free_tabs(p->d32);
+  void *bufs[] = {p->d_n_of_s, p->d_mu, p->d_psbar, p->sa3, p->sa2, p->sb3, p->tend3, p->tend2,
+                  p->fin, p->gin, p->gout, p->fout, p->raw, p->psa, p->psb, p->bsg, p->frc,
+                  p->ctl, p->pend, p->d_nsteps, p->scratch};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (p->own) (void)hipStreamDestroy(p->own);
+  delete p;
+  return LBM_OK;
+}
+
+int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
+  if (!d || !out) return fail(LBM_ERR_VALUE, "null descriptor or output");
+  *out = nullptr;
+  if (d->nlon != FFT_I) return fail(LBM_ERR_VALUE, "nlon=%d: only 128 longitudes are built", d->nlon);
+  if (d->nlat <= 0 || d->nlat % 2 || d->nlat > 64 * 32)
+    return fail(LBM_ERR_VALUE, "nlat must be even and positive");
+  if (d->nmax < 1 || d->nmax + 1 > 64 || d->nmax >= d->nlat)
+    return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=128", d->nmax);
+  if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
+  if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
+  if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");
+  if (d->precision != LBM_FP64 && d->precision != LBM_FP32) return fail(LBM_ERR_VALUE, "bad precision");
+  if (!d->h_sigm || !d->h_kv || !d->h_kd || !d->h_ktt || !d->h_kq)
+    return fail(LBM_ERR_VALUE, "sigm and damping arrays are required");
+  if (d->inistp < 0 || d->inistp > 8) return fail(LBM_ERR_VALUE, "inistp out of range");
+  lbm_plan *p = new lbm_plan();
+  p->J = d->nlat;
+  p->I = d->nlon;
+  p->N = d->nmax;
+  p->M = d->nmax + 1;
+  p->L = d->nlev;
+  p->E = d->nmem;
+  p->nb = d->nbasic;
+  p->prec = d->precision;
+  p->nspec = p->M * (p->M + 1) / 2;
+  p->Jh = p->J / 2;
+  p->inistp = d->inistp;
+  p->nset = p->inistp + 1;
+  p->strictly_linear = d->strictly_linear;
+  p->skip_low_n = d->skip_low_n_vdiff;
+  p->a = d->radius;
+  p->delt = d->delt;
+  p->robert = d->robert;
+  const int J = p->J, I = p->I, M = p->M, L = p->L, E = p->E;
+  int rc = LBM_OK;
+  auto bail = [&](int r) {
+    lbm_plan_destroy(p);
+    return r;
+  };
+  gauss_legendre(J, p->mu, p->gw);
+  p->coslat.resize(J);
+  p->lat.resize(J);
+  for (int j = 0; j < J; ++j) {
+    p->coslat[j] = std::sqrt(1.0 - p->mu[j] * p->mu[j]);
+    p->lat[j] = std::asin(p->mu[j]);
+  }
+  p->lon.resize(I);
+  for (int i = 0; i < I; ++i) p->lon[i] = i * (2.0 * M_PI / I);
+  if ((rc = build_vertical(p, d->h_sigm)) != LBM_OK) return bail(rc);
+  p->kv.assign(d->h_kv, d->h_kv + (size_t)M * L);
+  p->kd.assign(d->h_kd, d->h_kd + (size_t)M * L);
+  p->ktt.assign(d->h_ktt, d->h_ktt + (size_t)M * L);
+  p->kq.assign(d->h_kq, d->h_kq + (size_t)M * L);
+  const double *vd[4] = {d->h_vdif_vor, d->h_vdif_div, d->h_vdif_t, d->h_vdif_q};
+  for (int f = 0; f < 4; ++f)
+    if (vd[f]) p->vdif[f].assign(vd[f], vd[f] + L);
+  if ((rc = build_vdiff(p)) != LBM_OK) return bail(rc);
+  p->n_of_s.resize(p->nspec);
+  for (int m = 0; m < M; ++m)
+    for (int n = m; n < M; ++n) p->n_of_s[spec_off(m, M) + n - m] = (short)n;
+
+  auto dalloc = [&](void **ptr, size_t bytes) -> int {
+    HIPCHK(hipMalloc(ptr, std::max<size_t>(bytes, 16)));
+    return LBM_OK;
+  };
+  if (hipStreamCreate(&p->own) != hipSuccess) return bail(fail(LBM_ERR_RUNTIME, "stream create failed"));
+  if ((rc = build_tabs<double>(p)) != LBM_OK) return bail(rc);
+  if (p->prec == LBM_FP32 && (rc = build_tabs<float>(p)) != LBM_OK) return bail(rc);
+  if ((rc = set_twiddles<double>()) != LBM_OK) return bail(rc);
+  if ((rc = set_twiddles<float>()) != LBM_OK) return bail(rc);
+  // identity-free defaults for the reference temperature until a basic state
+  p->tbar.assign((size_t)p->nb * L, 0.0);
+  const size_t rs = p->rsize(), cs = 2 * rs;
+  const size_t EL = (size_t)E * L;
+  const ColMap cin{E, L, 6, 3}, cout{E, L, 9, 2};
+  struct A {
+    void **ptr;
+    size_t bytes;
+  } allocs[] = {
+      {(void **)&p->d_n_of_s, sizeof(short) * p->nspec},
+      {(void **)&p->d_mu, sizeof(double) * J},
+      {(void **)&p->d_psbar, sizeof(double) * p->nb * J * I},
+      {&p->sa3, cs * p->nspec * 4 * EL},
+      {&p->sb3, cs * p->nspec * 4 * EL},
+      {&p->sa2, cs * p->nspec * E},
+      {&p->tend3, cs * p->nspec * 4 * EL},
+      {&p->tend2, cs * p->nspec * 2 * E},
+      {&p->fin, cs * M * J * cin.ncols()},
+      {&p->gin, rs * (size_t)E * J * cin.rows() * I},
+      {&p->gout, rs * (size_t)E * J * cout.rows() * I},
+      {&p->fout, cs * M * J * cout.ncols()},
+      {&p->raw, rs * (size_t)E * J * 4 * L * I},
+      {&p->psa, rs * (size_t)E * J * I},
+      {&p->psb, rs * (size_t)E * J * I},
+      {&p->bsg, rs * (size_t)p->nb * J * BRow{L}.rows() * I},
+      {(void **)&p->ctl, sizeof(Ctl)},
+      {(void **)&p->pend, sizeof(int) * E},
+      {(void **)&p->d_nsteps, sizeof(int) * E},
+  };
+  for (auto &al : allocs)
+    if ((rc = dalloc(al.ptr, al.bytes)) != LBM_OK) return bail(rc);
+  if (hipMemcpy(p->d_n_of_s, p->n_of_s.data(), sizeof(short) * p->nspec, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->d_mu, p->mu.data(), sizeof(double) * J, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(LBM_ERR_RUNTIME, "table upload failed"));
+  Ctl c0{0, 0, INT_MAX, 0};
+  if (hipMemcpy(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(p->pend, 0, sizeof(int) * E) != hipSuccess ||
+      hipMemset(p->sa3, 0, cs * p->nspec * 4 * EL) != hipSuccess ||
+      hipMemset(p->sb3, 0, cs * p->nspec * 4 * EL) != hipSuccess ||
+      hipMemset(p->sa2, 0, cs * p->nspec * E) != hipSuccess ||
+      hipMemset(p->psa, 0, rs * (size_t)E * J * I) != hipSuccess ||
+      hipMemset(p->psb, 0, rs * (size_t)E * J * I) != hipSuccess)
+    return bail(fail(LBM_ERR_RUNTIME, "initialisation failed"));
+  p->nsteps_h.assign(E, 0);
+  *out = p;
+  return LBM_OK;
+}
+
+int lbm_plan_get(const lbm_plan *p, const char *name, double *h_out, int64_t n) {
+  if (!p || !name || !h_out) return fail(LBM_ERR_VALUE, "null argument");
+  const std::vector<double> *v = nullptr;
+  std::string s(name);
+  if (s == "mu") v = &p->mu;
+  else if (s == "gw") v = &p->gw;
+  else if (s == "coslat") v = &p->coslat;
+  else if (s == "lat") v = &p->lat;
+  else if (s == "lon") v = &p->lon;
+  else if (s == "sig") v = &p->sig;
+  else if (s == "sigm") v = &p->sigm;
+  else if (s == "dsig") v = &p->dsig;
+  else if (s == "alpha") v = &p->alpha;
+  else if (s == "cpg") v = &p->cpg;
+  else if (s == "w3") v = &p->w3;
+  else if (s == "wP") v = &p->wP;
+  else if (s == "dsigm") v = &p->dsigm;
+  else if (s == "dlog3") v = &p->dlog3;
+  else if (s == "ww") v = &p->ww;
+  else if (s == "lop") v = &p->lop;
+  else if (s == "tbar") v = &p->tbar;
+  else if (s == "minv") v = &p->minv;
+  else if (s == "vinv") v = &p->vinv;
+  if (!v) return fail(LBM_ERR_VALUE, "unknown plan array '%s'", name);
+  const int64_t cnt = (int64_t)v->size();
+  if (n < cnt) return fail(LBM_ERR_VALUE, "'%s' needs %lld doubles", name, (long long)cnt);
+  std::copy(v->begin(), v->end(), h_out);
+  return (int)cnt;
+}
+
+int lbm_grid_to_wave(lbm_plan *p, const double *g, double *w, int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_G2W, g, nullptr, w, nullptr, B, nk, s);
+}
+int lbm_wave_to_grid(lbm_plan *p, const double *w, double *g, int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_W2G, w, nullptr, g, nullptr, B, nk, s);
+}
+int lbm_uv_to_vordiv(lbm_plan *p, const double *u, const double *v, double *vor, double *div, int B,
+                     int nk, void *s) {
+  return spectral_dispatch(p, OP_UV2VD, u, v, vor, div, B, nk, s);
+}
+int lbm_psichi_to_uv(lbm_plan *p, const double *psi, const double *chi, double *u, double *v, int B,
+                     int nk, void *s) {
+  return spectral_dispatch(p, OP_PC2UV, psi, chi, u, v, B, nk, s);
+}
+int lbm_hgrad(lbm_plan *p, const double *w, double *gx, double *gy, int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_HGRAD, w, nullptr, gx, gy, B, nk, s);
+}
+
+int lbm_set_basic_state(lbm_plan *p, const double *u, const double *v, const double *t,
+                        const double *q, const double *ps, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!u || !v || !t || !q || !ps) return fail(LBM_ERR_VALUE, "basic state needs u, v, t, q, ps");
+  destroy_graphs(p);
+  return set_basic(p, u, v, t, q, ps, p->strm(stream));
+}
+
+int lbm_set_forcing(lbm_plan *p, const double *frc, int nfrc, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  hipStream_t st = p->strm(stream);
+  destroy_graphs(p);
+  if (!frc) {
+    if (p->frc) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(p->frc));
+    }
+    p->frc = nullptr;
+    p->nfrc = 0;
+    return LBM_OK;
+  }
+  if (nfrc != 1 && nfrc != p->E) return fail(LBM_ERR_VALUE, "forcing must have 1 or nmem members");
+  const int J = p->J, I = p->I, L = p->L;
+  const size_t n = (size_t)nfrc * J * I * L;
+  if (p->frc && p->nfrc != nfrc) {
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipFree(p->frc));
+    p->frc = nullptr;
+  }
+  if (!p->frc) HIPCHK(hipMalloc(&p->frc, n * p->rsize()));
+  p->nfrc = nfrc;
+  if (p->prec == LBM_FP64)
+    hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk(n)), dim3(256), 0, st, frc, (double *)p->frc,
+                       nfrc, J, I, L, L, 0, (const double *)nullptr);
+  else
+    hipLaunchKernelGGL((k_pack_grid<float>), dim3(nblk(n)), dim3(256), 0, st, frc, (float *)p->frc,
+                       nfrc, J, I, L, L, 0, (const double *)nullptr);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_set_state(lbm_plan *p, const double *u, const double *v, const double *t, const double *q,
+                  const double *ps, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before the perturbation");
+  hipStream_t st = p->strm(stream);
+  p->steps_total = 0;
+  Ctl c0{0, 0, INT_MAX, 0};
+  int rc = p->prec == LBM_FP64 ? set_state_t<double>(p, u, v, t, q, ps, st)
+                               : set_state_t<float>(p, u, v, t, q, ps, st);
+  if (rc != LBM_OK) return rc;
+  c0.raw = p->have_raw ? 1 : 0;
+  HIPCHK(hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_set_lengths(lbm_plan *p, const int32_t *h_nsteps, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  (void)stream;
+  if (!h_nsteps) {
+    std::fill(p->nsteps_h.begin(), p->nsteps_h.end(), 0);
+    return LBM_OK;
+  }
+  for (int e = 0; e < p->E; ++e) {
+    if (h_nsteps[e] < 0) return fail(LBM_ERR_VALUE, "negative length for member %d", e);
+    p->nsteps_h[e] = h_nsteps[e];
+  }
+  return LBM_OK;
+}
+
+int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_run before lbm_set_basic_state");
+  hipStream_t st = use_graph ? p->own : p->strm(stream);
+  if (use_graph && stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  std::vector<int> len(p->E);
+  for (int e = 0; e < p->E; ++e) len[e] = nsteps > 0 ? nsteps : p->nsteps_h[e];
+  const int maxlen = *std::max_element(len.begin(), len.end());
+  const int total = p->inistp + maxlen;
+  if (maxlen <= 0) return LBM_OK;
+  HIPCHK(hipMemcpyAsync(p->d_nsteps, len.data(), sizeof(int) * p->E, hipMemcpyHostToDevice, st));
+  // every run restarts the start-up half steps (model.py:211-219)
+  {
+    Ctl h;
+    HIPCHK(hipMemcpyAsync(&h, p->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    h.step = 0;
+    h.bad_step = INT_MAX;
+    HIPCHK(hipMemcpyAsync(p->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  }
+  const int chunk = 8;
+  int done = 0;
+  int rc = LBM_OK;
+  while (done < total) {
+    int todo = total - done;
+    if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
+    if (use_graph) {
+      if (todo >= chunk) {
+        CHK(capture_graph(p, 0, chunk, st));
+        HIPCHK(hipGraphLaunch(p->gexec[0], st));
+        todo = chunk;
+      } else {
+        CHK(capture_graph(p, 1, 1, st));
+        HIPCHK(hipGraphLaunch(p->gexec[1], st));
+        todo = 1;
+      }
+    } else {
+      for (int s = 0; s < todo; ++s) CHK(step_dispatch(p, st));
+    }
+    done += todo;
+    if (check_every > 0 && done % check_every == 0 && (rc = check_bad(p, st, done)) != LBM_OK) break;
+  }
+  p->steps_total += done;
+  if (rc != LBM_OK) return rc;
+  CHK(p->prec == LBM_FP64 ? final_check<double>(p, st) : final_check<float>(p, st));
+  CHK(check_bad(p, st, done));
+  if (use_graph && stream) HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_get_state(lbm_plan *p, double *u, double *v, double *t, double *q, double *ps,
+                  void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "no basic state");
+  hipStream_t st = p->strm(stream);
+  return p->prec == LBM_FP64 ? get_state_t<double>(p, u, v, t, q, ps, st)
+                             : get_state_t<float>(p, u, v, t, q, ps, st);
+}
+
+int lbm_get_spectral(lbm_plan *p, int level, double *vor, double *div, double *t, double *q,
+                     double *pi, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (level != 0 && level != 1) return fail(LBM_ERR_VALUE, "level must be 0 or 1");
+  hipStream_t st = p->strm(stream);
+  const int E = p->E, M = p->M, L = p->L;
+  double *dst[4] = {vor, div, t, q};
+  const void *src3 = level == 0 ? p->sa3 : p->sb3;
+  const size_t tot = (size_t)E * M * M * L;
+  for (int f = 0; f < 4; ++f) {
+    if (!dst[f]) continue;
+    if (p->prec == LBM_FP64)
+      hipLaunchKernelGGL((k_unpack_spec<double>), dim3(nblk(tot)), dim3(256), 0, st,
+                         (const cx<double> *)src3, dst[f], E, M, L, 4, f, E * L);
+    else
+      hipLaunchKernelGGL((k_unpack_spec<float>), dim3(nblk(tot)), dim3(256), 0, st,
+                         (const cx<float> *)src3, dst[f], E, M, L, 4, f, E * L);
+  }
+  if (pi && level == 0) {
+    if (p->prec == LBM_FP64)
+      hipLaunchKernelGGL((k_unpack_spec<double>), dim3(nblk((size_t)E * M * M)), dim3(256), 0, st,
+                         (const cx<double> *)p->sa2, pi, E, M, 1, 1, 0, E);
+    else
+      hipLaunchKernelGGL((k_unpack_spec<float>), dim3(nblk((size_t)E * M * M)), dim3(256), 0, st,
+                         (const cx<float> *)p->sa2, pi, E, M, 1, 1, 0, E);
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_steps_taken(const lbm_plan *p, int64_t *out) {
+  if (!p || !out) return fail(LBM_ERR_VALUE, "null argument");
+  *out = p->steps_total;
+  return LBM_OK;
+}
+
+}  // extern "C"

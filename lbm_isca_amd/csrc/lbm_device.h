@@ -1,0 +1,478 @@
+// Device code of the gfx950 time-stepping core: complex helpers, the
+// length-128 real FFT pair, the per-m Legendre synthesis/analysis with
+// north/south folding, the fused grid-point tendency kernel and the fused
+// spectral semi-implicit leapfrog + vertical diffusion + Robert-Asselin
+// kernel.  Everything is templated on the arithmetic type T (double/float).
+//
+// Internal layouts (E members, L levels, M = nmax+1, J lats, Jh = J/2):
+//   spectral  [s][f][ncols]  complex, s = packed (m, n>=m) index,
+//             ncols = E*L for level fields, E for surface fields
+//   Fourier   [m][j][C]      complex, C = nF3*E*L + nF2*E columns
+//   grid      [e][j][r][I]   real, rows r = f*L + l (level fields) then
+//             nF3*L + g (surface fields); longitude fastest
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+template <class T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <class T>
+__host__ __device__ inline cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <class T>
+__host__ __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <class T>
+__host__ __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <class T>
+__host__ __device__ inline cx<T> operator-(cx<T> a) { return mk(-a.x, -a.y); }
+template <class T>
+__host__ __device__ inline cx<T> operator*(T s, cx<T> a) { return mk(s * a.x, s * a.y); }
+template <class T>
+__host__ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class T>
+__host__ __device__ inline cx<T> conj(cx<T> a) { return mk(a.x, -a.y); }
+// (i*m) * a
+template <class T>
+__host__ __device__ inline cx<T> im_mul(T m, cx<T> a) { return mk(-m * a.y, m * a.x); }
+template <class T>
+__host__ __device__ inline void fma_acc(cx<T> &acc, T s, cx<T> a) {
+  acc.x = fma(s, a.x, acc.x);
+  acc.y = fma(s, a.y, acc.y);
+}
+template <class T>
+__host__ __device__ inline cx<T> zero() { return mk<T>(0, 0); }
+
+// Packed triangular index of (m, n=m).
+__host__ __device__ inline int spec_off(int m, int M) { return m * M - (m * (m - 1)) / 2; }
+
+// ---------------------------------------------------------------------------
+// Length-8 DFT in place, natural order in and out.  SIGN = -1 forward
+// (exp(-2 pi i nk/8)), +1 backward, unnormalised.  Radix-2 split into two
+// length-4 DFTs (even/odd samples).
+// ---------------------------------------------------------------------------
+template <int SIGN, class T>
+__host__ __device__ inline void dft4(cx<T> &a0, cx<T> &a1, cx<T> &a2, cx<T> &a3) {
+  // j_s = exp(SIGN * i pi/2) = SIGN * i
+  cx<T> s02 = a0 + a2, d02 = a0 - a2, s13 = a1 + a3, d13 = a1 - a3;
+  cx<T> jd13 = (SIGN > 0) ? mk(-d13.y, d13.x) : mk(d13.y, -d13.x);
+  a0 = s02 + s13;
+  a2 = s02 - s13;
+  a1 = d02 + jd13;
+  a3 = d02 - jd13;
+}
+
+template <int SIGN, class T>
+__host__ __device__ inline void dft8(cx<T> *v) {
+  cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<SIGN>(e0, e1, e2, e3);
+  dft4<SIGN>(o0, o1, o2, o3);
+  const T r = (T)0.70710678118654752440;
+  // w^k = exp(SIGN * 2 pi i k / 8)
+  cx<T> w1o1 = cmul(mk(r, (T)SIGN * r), o1);
+  cx<T> w2o2 = (SIGN > 0) ? mk(-o2.y, o2.x) : mk(o2.y, -o2.x);
+  cx<T> w3o3 = cmul(mk(-r, (T)SIGN * r), o3);
+  v[0] = e0 + o0;
+  v[4] = e0 - o0;
+  v[1] = e1 + w1o1;
+  v[5] = e1 - w1o1;
+  v[2] = e2 + w2o2;
+  v[6] = e2 - w2o2;
+  v[3] = e3 + w3o3;
+  v[7] = e3 - w3o3;
+}
+
+// Column-space map: Fourier/grid columns -> (member, grid row).
+struct ColMap {
+  int E, L, nF3, nF2;  // members, levels, level fields, surface fields
+  __host__ __device__ int ncols() const { return nF3 * E * L + nF2 * E; }
+  __host__ __device__ int rows() const { return nF3 * L + nF2; }
+  __host__ __device__ void map(int col, int &e, int &r) const {
+    const int EL = E * L;
+    if (col < nF3 * EL) {
+      const int f = col / EL, rem = col - f * EL;
+      e = rem / L;
+      r = f * L + (rem - e * L);
+    } else {
+      const int c2 = col - nF3 * EL;
+      const int g = c2 / E;
+      e = c2 - g * E;
+      r = nF3 * L + g;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Length-128 real FFTs, 8 threads per row (64 = 8 x 8 complex FFT of the
+// even/odd packed samples, then the real split).  32 rows per 256-thread
+// block.  Fourier columns are rows j*C + col.
+// ---------------------------------------------------------------------------
+constexpr int FFT_I = 128;
+constexpr int FFT_ROWS = 32;
+
+// Twiddles exp(-2 pi i k / 128), k = 0..127, filled by the host.
+template <class T>
+struct Tw;
+template <>
+struct Tw<double> {
+  static __device__ inline const cx<double> *get();
+};
+template <>
+struct Tw<float> {
+  static __device__ inline const cx<float> *get();
+};
+__constant__ cx<double> c_tw128_d[FFT_I];
+__constant__ cx<float> c_tw128_f[FFT_I];
+__device__ inline const cx<double> *Tw<double>::get() { return c_tw128_d; }
+__device__ inline const cx<float> *Tw<float>::get() { return c_tw128_f; }
+
+// Fourier (m <= M-1) -> grid.  g = X0.re + 2 sum_k Re(X_k e^{i k lam}),
+// i.e. irfft(zero-filled * nlon) of spectral.py:166-172.
+template <class T>
+__global__ __launch_bounds__(256) void k_ifft128(const cx<T> *__restrict__ F, T *__restrict__ G,
+                                                 int C, ColMap cm, int J, int M) {
+  __shared__ cx<T> sh[FFT_ROWS][8][9];
+  const cx<T> *tw = Tw<T>::get();
+  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
+  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+  const bool live = row < (long)J * C;
+  int j = 0, col = 0;
+  if (live) {
+    j = (int)(row / C);
+    col = (int)(row - (long)j * C);
+  }
+  cx<T> v[8];
+#pragma unroll
+  for (int n2 = 0; n2 < 8; ++n2) {
+    const int kk = tt + 8 * n2;
+    cx<T> A = zero<T>(), B = zero<T>();
+    if (live && kk < M) {
+      A = F[((size_t)kk * J + j) * C + col];
+      if (kk == 0) A.y = 0;
+    }
+    const int kb = 64 - kk;
+    if (live && kb < M) B = conj(F[((size_t)kb * J + j) * C + col]);
+    const cx<T> w = conj(tw[kk]);  // exp(+2 pi i kk/128)
+    v[n2] = (A + B) + cmul(mk((T)0, (T)1), cmul(w, A - B));
+  }
+  dft8<+1>(v);
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) sh[rl][tt][k2] = cmul(v[k2], conj(tw[(2 * tt * k2) & 127]));
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = sh[rl][n1][tt];
+  dft8<+1>(v);
+  if (!live) return;
+  int e, r;
+  cm.map(col, e, r);
+  cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I);
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) out[tt + 8 * k1] = v[k1];
+}
+
+// grid -> Fourier (m <= M-1): rfft / nlon of spectral.py:159-164.
+template <class T>
+__global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *__restrict__ F,
+                                                int C, ColMap cm, int J, int M) {
+  __shared__ cx<T> sh[FFT_ROWS][8][9];
+  const cx<T> *tw = Tw<T>::get();
+  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
+  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+  const bool live = row < (long)J * C;
+  int j = 0, col = 0;
+  cx<T> v[8];
+  if (live) {
+    j = (int)(row / C);
+    col = (int)(row - (long)j * C);
+    int e, r;
+    cm.map(col, e, r);
+    const cx<T> *in =
+        reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I);
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[tt + 8 * n2];
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) v[n2] = zero<T>();
+  }
+  dft8<-1>(v);
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) sh[rl][tt][k2] = cmul(v[k2], tw[(2 * tt * k2) & 127]);
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = sh[rl][n1][tt];
+  __syncthreads();
+  // Z[tt + 8 k1] = v[k1]; stage the 64 values of the row for the real split
+  cx<T> *z = &sh[rl][0][0];
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) z[tt + 8 * k1] = v[k1];
+  __syncthreads();
+  if (!live) return;
+  const T half = (T)0.5, inv_n = (T)1 / (T)FFT_I;
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) {
+    const int k = tt + 8 * k1;
+    if (k >= M) break;
+    const cx<T> zk = z[k], zc = conj(z[(64 - k) & 63]);
+    const cx<T> s = zk + zc, d = zk - zc;
+    // X = (s - i w d)/2, w = exp(-2 pi i k/128)
+    const cx<T> wd = cmul(tw[k], d);
+    cx<T> X = mk(half * (s.x + wd.y), half * (s.y - wd.x));
+    F[((size_t)k * J + j) * C + col] = inv_n * X;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Legendre synthesis recipes: for each output o, the Fourier value at
+// latitude j is  sum_n P_n^m(mu_j) A_o(n) + H_n^m(mu_j) B_o(n), optionally
+// times 1/(a cos lat_j).  A_o, B_o are linear in the NIN input fields.
+// ---------------------------------------------------------------------------
+struct SynLevel {  // (vor, div, t, q) -> (U, V, vor, div, t, q): state.py:94-99, dynamics.py:167-169
+  static constexpr int NIN = 4, NOUT = 6;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int o) { return o < 2; }
+  static constexpr bool scaled(int o) { return o < 2; }
+  template <class T>
+  __device__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
+    const cx<T> psi = il * x[0], chi = il * x[1];
+    A[0] = im_mul(fm, chi);  // U = P (im chi) - H psi   (spectral.py:271)
+    B[0] = -psi;
+    A[1] = im_mul(fm, psi);  // V = P (im psi) + H chi   (spectral.py:272)
+    B[1] = chi;
+    A[2] = x[0];
+    A[3] = x[1];
+    A[4] = x[2];
+    A[5] = x[3];
+  }
+};
+struct SynSurf {  // pi -> (d/dx, d/dy, pi): hgrad of spectral.py:210-225 + wave_to_grid
+  static constexpr int NIN = 1, NOUT = 3;
+  static constexpr bool hasA(int o) { return o != 1; }
+  static constexpr bool hasB(int o) { return o == 1; }
+  static constexpr bool scaled(int o) { return o < 2; }
+  template <class T>
+  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+    A[0] = im_mul(fm, x[0]);
+    B[1] = x[0];
+    A[2] = x[0];
+  }
+};
+struct SynScalar {  // wave_to_grid (spectral.py:193-197)
+  static constexpr int NIN = 1, NOUT = 1;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int) { return false; }
+  static constexpr bool scaled(int) { return false; }
+  template <class T>
+  __device__ static void terms(const cx<T> *x, T, T, cx<T> *A, cx<T> *) { A[0] = x[0]; }
+};
+struct SynPsiChi {  // psichi_to_uv (spectral.py:262-276)
+  static constexpr int NIN = 2, NOUT = 2;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int) { return true; }
+  static constexpr bool scaled(int) { return true; }
+  template <class T>
+  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+    A[0] = im_mul(fm, x[1]);
+    B[0] = -x[0];
+    A[1] = im_mul(fm, x[0]);
+    B[1] = x[1];
+  }
+};
+struct SynGrad {  // hgrad (spectral.py:210-225)
+  static constexpr int NIN = 1, NOUT = 2;
+  static constexpr bool hasA(int o) { return o == 0; }
+  static constexpr bool hasB(int o) { return o == 1; }
+  static constexpr bool scaled(int) { return true; }
+  template <class T>
+  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+    A[0] = im_mul(fm, x[0]);
+    B[1] = x[0];
+  }
+};
+
+struct OutSlots {
+  int off[8];  // column offset in the Fourier row of each output field
+};
+
+// Synthesis, one block per (m, 8 columns); threads (col-lane, lat-pair).
+// P(-mu) = (-1)^(n+m) P(mu), H(-mu) = -(-1)^(n+m) H(mu): the north and
+// south rows come from the even/odd partial sums.
+template <class T, class R>
+__global__ __launch_bounds__(256) void k_syn(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
+                                             const T *__restrict__ Ps, const T *__restrict__ Hs,
+                                             const T *__restrict__ inv_lap,
+                                             const T *__restrict__ rcos, int ncols, int C,
+                                             OutSlots slots, int J, int M) {
+  constexpr int NIN = R::NIN, NOUT = R::NOUT;
+  const int t = threadIdx.x, cl = t & 7;
+  const int col = blockIdx.x * 8 + cl, m = blockIdx.y;
+  const int Jh = J / 2;
+  if (col >= ncols) return;
+  const T fm = (T)m;
+  for (int jp = t >> 3; jp < Jh; jp += 32) {
+    cx<T> EP[NOUT], OP[NOUT], EH[NOUT], OH[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) EP[o] = OP[o] = EH[o] = OH[o] = zero<T>();
+    int s = spec_off(m, M);
+    for (int n = m; n < M; ++n, ++s) {
+      cx<T> x[NIN], A[NOUT], B[NOUT];
+#pragma unroll
+      for (int f = 0; f < NIN; ++f) x[f] = in[((size_t)s * NIN + f) * ncols + col];
+      R::template terms<T>(x, inv_lap[n], fm, A, B);
+      const T p = Ps[(size_t)s * Jh + jp], h = Hs[(size_t)s * Jh + jp];
+      if ((n - m) & 1) {
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+          if (R::hasA(o)) fma_acc(OP[o], p, A[o]);
+          if (R::hasB(o)) fma_acc(OH[o], h, B[o]);
+        }
+      } else {
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) {
+          if (R::hasA(o)) fma_acc(EP[o], p, A[o]);
+          if (R::hasB(o)) fma_acc(EH[o], h, B[o]);
+        }
+      }
+    }
+    const int jn = jp, js = J - 1 - jp;
+    const T sc = rcos[jp];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) {
+      cx<T> vn = (EP[o] + OP[o]) + (EH[o] + OH[o]);
+      cx<T> vs = (EP[o] - OP[o]) + (OH[o] - EH[o]);
+      if (R::scaled(o)) {
+        vn = sc * vn;
+        vs = sc * vs;
+      }
+      out[((size_t)m * J + jn) * C + slots.off[o] + col] = vn;
+      out[((size_t)m * J + js) * C + slots.off[o] + col] = vs;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Legendre analysis recipes.  For each input field f the kernel forms
+// AP_f = sum_j gw_j P_n^m(mu_j) F_f(j) and (where needed)
+// AH_f = sum_j gw_j H_n^m(mu_j) F_f(j); combine() maps them to outputs.
+// Vector (u-like) inputs are pre-scaled in grid space by 1/(a cos lat), so
+// gw/(a(1-mu^2)) * (u cos lat) of spectral.py:236-243 becomes gw * u/(a cos).
+// ---------------------------------------------------------------------------
+struct AnaTend {  // (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t, q): dynamics.py:219,246-262
+  static constexpr int NIN = 9, NOUT = 4;
+  static constexpr bool needP(int f) { return f != 4 && f != 7; }
+  static constexpr bool needH(int f) { return f == 0 || f == 1 || f == 4 || f == 7; }
+  template <class T>
+  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T nn1a2, cx<T> *o) {
+    o[0] = im_mul(fm, AP[1]) + AH[0];                                   // curl(fu, fv)
+    o[1] = im_mul(fm, AP[0]) - AH[1] + nn1a2 * AP[2];                   // div - lap(ke)
+    o[2] = AP[5] - im_mul(fm, AP[3]) + AH[4];                           // -div(xu,xv) + gt
+    o[3] = AP[8] - im_mul(fm, AP[6]) + AH[7];                           // -div(yu,yv) + g
+  }
+};
+struct AnaState {  // (u', v', t, q) -> (vor, div, t, q): state.py:86-92
+  static constexpr int NIN = 4, NOUT = 4;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int f) { return f < 2; }
+  template <class T>
+  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
+    o[0] = im_mul(fm, AP[1]) + AH[0];
+    o[1] = im_mul(fm, AP[0]) - AH[1];
+    o[2] = AP[2];
+    o[3] = AP[3];
+  }
+};
+struct AnaVorDiv {  // uv_to_vordiv (spectral.py:227-244)
+  static constexpr int NIN = 2, NOUT = 2;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int) { return true; }
+  template <class T>
+  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
+    o[0] = im_mul(fm, AP[1]) + AH[0];
+    o[1] = im_mul(fm, AP[0]) - AH[1];
+  }
+};
+struct AnaScalar {  // grid_to_wave (spectral.py:186-191)
+  static constexpr int NIN = 1, NOUT = 1;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int) { return false; }
+  template <class T>
+  __device__ static void combine(const cx<T> *AP, const cx<T> *, T, T, cx<T> *o) { o[0] = AP[0]; }
+};
+
+struct InSlots {
+  int off[12];  // column offset in the Fourier row of each input field
+};
+
+// Analysis, one block per (m, 16 columns); threads (col-lane, n-slot).  A
+// thread owns n = m + ns + 16k (k = 0..2), all of one parity, so it folds
+// the north/south rows once: symmetric sums for even n-m with P (odd with H).
+template <class T, class R>
+__global__ __launch_bounds__(256) void k_ana(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
+                                             const T *__restrict__ Pa, const T *__restrict__ Ha,
+                                             const T *__restrict__ nn1a2, int ncols, int C,
+                                             InSlots slots, int J, int M) {
+  constexpr int NIN = R::NIN, NOUT = R::NOUT, KN = 3;
+  const int t = threadIdx.x, cl = t & 15, ns = t >> 4;
+  const int col = blockIdx.x * 16 + cl, m = blockIdx.y;
+  const int Jh = J / 2;
+  if (col >= ncols) return;
+  const bool odd = (ns & 1) != 0;
+  const int s0 = spec_off(m, M) - m;  // s = s0 + n
+  cx<T> AP[NIN][KN], AH[NIN][KN];
+#pragma unroll
+  for (int f = 0; f < NIN; ++f)
+#pragma unroll
+    for (int k = 0; k < KN; ++k) AP[f][k] = AH[f][k] = zero<T>();
+  int nk[KN];
+  bool ok[KN];
+#pragma unroll
+  for (int k = 0; k < KN; ++k) {
+    nk[k] = m + ns + 16 * k;
+    ok[k] = nk[k] < M;
+  }
+  if (!ok[0]) return;
+  for (int jp = 0; jp < Jh; ++jp) {
+    const cx<T> *rn = in + ((size_t)m * J + jp) * C + col;
+    const cx<T> *rs = in + ((size_t)m * J + (J - 1 - jp)) * C + col;
+    T p[KN], h[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      const size_t ti = (size_t)(s0 + (ok[k] ? nk[k] : m)) * Jh + jp;
+      p[k] = ok[k] ? Pa[ti] : (T)0;
+      h[k] = ok[k] ? Ha[ti] : (T)0;
+    }
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      const cx<T> fn = rn[slots.off[f]], fs = rs[slots.off[f]];
+      const cx<T> sym = fn + fs, anti = fn - fs;
+      const cx<T> fp = odd ? anti : sym, fh = odd ? sym : anti;
+#pragma unroll
+      for (int k = 0; k < KN; ++k) {
+        if (R::needP(f)) fma_acc(AP[f][k], p[k], fp);
+        if (R::needH(f)) fma_acc(AH[f][k], h[k], fh);
+      }
+    }
+  }
+  const T fm = (T)m;
+#pragma unroll
+  for (int k = 0; k < KN; ++k) {
+    if (!ok[k]) continue;
+    cx<T> ap[NIN], ah[NIN], o[NOUT];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      ap[f] = AP[f][k];
+      ah[f] = AH[f][k];
+    }
+    R::template combine<T>(ap, ah, fm, nn1a2[nk[k]], o);
+    const int s = s0 + nk[k];
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) out[((size_t)s * NOUT + q) * ncols + col] = o[q];
+  }
+}
+
+}  // namespace lbm

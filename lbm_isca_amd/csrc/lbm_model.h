@@ -1,0 +1,639 @@
+// Device kernels of the linearised model: grid-point tendencies, the
+// spectral semi-implicit leapfrog step, basic-state terms and the layout
+// conversions between the reference's arrays and the internal ones.
+#pragma once
+#include "lbm_device.h"
+
+namespace lbm {
+
+constexpr double R_DRY = 287.04;
+constexpr double KAPPA = 2.0 / 7.0;
+constexpr double EPSVT = 461.50 / 287.04 - 1.0;
+constexpr double OMEGA = 7.2921150e-5;
+
+// Vertical constants, packed in one array (index helpers below).
+struct VOff {
+  int L;
+  __host__ __device__ int dsig() const { return 0; }
+  __host__ __device__ int sigm() const { return L; }          // L+1 entries
+  __host__ __device__ int alpha() const { return 2 * L + 1; }
+  __host__ __device__ int w3() const { return 3 * L + 1; }
+  __host__ __device__ int wP() const { return 4 * L + 1; }
+  __host__ __device__ int cpg() const { return 5 * L + 1; }
+  __host__ __device__ int dlog3() const { return 6 * L + 1; }
+  __host__ __device__ int size() const { return 7 * L + 1; }
+};
+
+// Rows of the basic-state grid block [eb][j][row][I].
+struct BRow {
+  int L;
+  __host__ __device__ int u() const { return 0; }
+  __host__ __device__ int v() const { return L; }
+  __host__ __device__ int t() const { return 2 * L; }
+  __host__ __device__ int q() const { return 3 * L; }
+  __host__ __device__ int tv() const { return 4 * L; }
+  __host__ __device__ int absvor() const { return 5 * L; }
+  __host__ __device__ int div() const { return 6 * L; }
+  __host__ __device__ int x5() const { return 7 * L; }
+  __host__ __device__ int sdot() const { return 8 * L; }  // L+1 half levels
+  __host__ __device__ int pix() const { return 9 * L + 1; }
+  __host__ __device__ int piy() const { return 9 * L + 2; }
+  __host__ __device__ int psbar() const { return 9 * L + 3; }
+  __host__ __device__ int rows() const { return 9 * L + 4; }
+};
+
+// Loop control read by the step kernels (graph-replayable: no per-step
+// kernel arguments).
+struct Ctl {
+  int step;      // 0-based step index within the current run
+  int raw;       // 1: the next tendency call reads the raw initial grid
+  int bad_step;  // first step whose input state was non-finite (INT_MAX = none)
+  int pad;
+};
+
+__global__ void k_tick(Ctl *ctl) {
+  ctl->step += 1;
+  ctl->raw = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Grid-point tendencies of dynamics.py:156-262 for one column per thread.
+// Inputs (synthesised state, rows of gin): U V vor div T Q (L each), then
+// d/dx pi, d/dy pi, pi.  Outputs (rows of gout): fu' fv' ke xu' xv' gt yu'
+// yv' g (L each; ' = times 1/(a cos lat), the analysis metric), then gt_pi
+// and pi of the previous (filtered) level.  Vertical sums are prefix/suffix
+// sums: S_k = sum_{l>=k} dsig (D+A), sdot_k = sigm_k S_0 - S_k,
+// (tmtx div)_k = kappa Tref_k (alpha_k D_k + w3_k sum_{l>k} dsig_l D_l),
+// phi_k = sum_{l<k} R dlog3_l x_l + R alpha_k x_k.
+// ---------------------------------------------------------------------------
+template <class T>
+struct TendArgs {
+  const T *gin;     // [e][j][6L+3][I]
+  const T *raw;     // [e][j][4L][I] raw initial (u,v,t,q) or null
+  const T *bsg;     // [eb][j][9L+4][I]
+  const T *frc;     // [ef][j][L][I] or null
+  T *gout;          // [e][j][9L+2][I]
+  T *psa, *psb;     // [e][j][I] ps' at t and at t-dt (filtered)
+  const int *pend;  // [e] deferred update of ps': 0 none, 1 assign, 2 RA filter
+  const T *vc;      // vertical constants (VOff)
+  const T *tref;    // [eb][L]
+  const T *rcos;    // [J] 1/(a cos lat)
+  Ctl *ctl;
+  int E, J, I, L, nbasic, nfrc, strictly_linear;
+  T robert;
+};
+
+template <class T>
+__device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig) {
+  return (flux_hi - flux_lo - f * (s_hi - s_lo)) * rdsig;
+}
+
+template <class T>
+__global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
+  const int i = threadIdx.x;
+  const int e = blockIdx.x / a.J, j = blockIdx.x - e * a.J;
+  const int L = a.L, I = a.I, J = a.J;
+  const int eb = a.nbasic > 1 ? e : 0;
+  const VOff vo{L};
+  const BRow br{L};
+  const T *vc = a.vc;
+  const T *tref = a.tref + (size_t)eb * L;
+  const int Rin = 6 * L + 3, Rout = 9 * L + 2;
+  const bool use_raw = a.raw != nullptr && a.ctl->raw;
+  const T *gi = a.gin + (size_t)(e * J + j) * Rin * I + i;
+  const T *rw = use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I + i : nullptr;
+  const T *bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
+  const T *fr = a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
+  T *go = a.gout + (size_t)(e * J + j) * Rout * I + i;
+  const T rc = a.rcos[j];
+
+  // perturbation inputs: raw grid on the first step after set_state
+  auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : gi[(size_t)k * I]; };
+  auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : gi[(size_t)(L + k) * I]; };
+  auto TP = [&](int k) { return use_raw ? rw[(size_t)(2 * L + k) * I] : gi[(size_t)(4 * L + k) * I]; };
+  auto QP = [&](int k) { return use_raw ? rw[(size_t)(3 * L + k) * I] : gi[(size_t)(5 * L + k) * I]; };
+  auto VOR = [&](int k) { return gi[(size_t)(2 * L + k) * I]; };
+  auto DIV = [&](int k) { return gi[(size_t)(3 * L + k) * I]; };
+  auto B = [&](int row) { return bs[(size_t)row * I]; };
+
+  const T pix = gi[(size_t)(6 * L) * I], piy = gi[(size_t)(6 * L + 1) * I];
+  const T pig = gi[(size_t)(6 * L + 2) * I];
+  const T bpix = B(br.pix()), bpiy = B(br.piy()), psbar = B(br.psbar());
+
+  // deferred ps' update of the previous step (state.py:94-128 on ps'):
+  // ps'_new = ps_bar (exp(pi) - 1); gb <- ga (start-up) or RA filter; ga <- new
+  const size_t pidx = (size_t)(e * J + j) * I + i;
+  T psa = a.psa[pidx], psb = a.psb[pidx];
+  const int mode = a.pend[e];
+  if (mode != 0) {
+    const T psn = psbar * expm1(pig);
+    psb = (mode == 1) ? psa : psa + a.robert * (psb - (T)2 * psa + psn);
+    psa = psn;
+    a.psa[pidx] = psa;
+    a.psb[pidx] = psb;
+  }
+  T chk = psa;
+
+  // pass 1: column totals S_0 and sum dsig*D
+  T S0 = 0, D0 = 0;
+  for (int k = 0; k < L; ++k) {
+    const T d = DIV(k);
+    const T padv = UP(k) * bpix + B(br.u() + k) * pix + VP(k) * bpiy + B(br.v() + k) * piy;
+    const T ds = vc[vo.dsig() + k];
+    S0 += ds * (d + padv);
+    D0 += ds * d;
+  }
+  // continuity: gt_pi = -S_0 + sum dsig D (dynamics.py:188)
+  go[(size_t)(9 * L) * I] = D0 - S0;
+  // previous-level pi' = ln(ps_bar + ps'_b) - ln ps_bar (state.py:77-79)
+  go[(size_t)(9 * L + 1) * I] = log1p(psb / psbar);
+
+  const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY;
+  const bool full_ke = !a.strictly_linear;
+  T P = 0, PD = 0, phacc = 0;
+  // vertical-advection fluxes at half level k for the 8 (field; sdot) pairs
+  T fl_up = 0, fl_bu = 0, fl_vp = 0, fl_bv = 0, fl_tp = 0, fl_bt = 0, fl_qp = 0, fl_bq = 0;
+  T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
+  T bu = B(br.u()), bv = B(br.v()), bt = B(br.t()), bq = B(br.q());
+  T sd_lo = 0;               // perturbation sdot at half level k
+  T bsd_lo = B(br.sdot());   // basic sdot at half level k
+  for (int k = 0; k < L; ++k) {
+    const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
+    const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
+    const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
+    const T tr = tref[k];
+    const T d = DIV(k), vo_ = VOR(k);
+    const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
+    const T bx5 = B(br.x5() + k);
+    const T padv = up * bpix + bu * pix + vp * bpiy + bv * piy;
+    chk += up + vp + tp + qp;
+
+    const T Pk1 = P + ds * (d + padv), PDk1 = PD + ds * d;
+    const T Sk1 = S0 - Pk1;
+    const bool top = (k + 1 == L);
+    const T sd_hi = top ? (T)0 : Pk1 - ((T)1 - vc[vo.sigm() + k + 1]) * S0;
+    const T bsd_hi = B(br.sdot() + k + 1);
+
+    T upn = 0, vpn = 0, tpn = 0, qpn = 0, bun = 0, bvn = 0, btn = 0, bqn = 0;
+    if (!top) {
+      upn = UP(k + 1); vpn = VP(k + 1); tpn = TP(k + 1); qpn = QP(k + 1);
+      bun = B(br.u() + k + 1); bvn = B(br.v() + k + 1);
+      btn = B(br.t() + k + 1); bqn = B(br.q() + k + 1);
+    }
+    const T h = (T)0.5;
+    // flux at half level k+1: s_{k+1} (f_{k+1} + f_k)/2 (0 at the top)
+    const T fh_up = top ? 0 : bsd_hi * h * (upn + up);
+    const T fh_bu = top ? 0 : sd_hi * h * (bun + bu);
+    const T fh_vp = top ? 0 : bsd_hi * h * (vpn + vp);
+    const T fh_bv = top ? 0 : sd_hi * h * (bvn + bv);
+    const T fh_tp = top ? 0 : bsd_hi * h * (tpn + tp);
+    const T fh_bt = top ? 0 : sd_hi * h * (btn + bt);
+    const T fh_qp = top ? 0 : bsd_hi * h * (qpn + qp);
+    const T fh_bq = top ? 0 : sd_hi * h * (bqn + bq);
+    const T va_up = vadv_term(fh_up, fl_up, up, bsd_hi, bsd_lo, rds);
+    const T va_bu = vadv_term(fh_bu, fl_bu, bu, sd_hi, sd_lo, rds);
+    const T va_vp = vadv_term(fh_vp, fl_vp, vp, bsd_hi, bsd_lo, rds);
+    const T va_bv = vadv_term(fh_bv, fl_bv, bv, sd_hi, sd_lo, rds);
+    const T va_tp = vadv_term(fh_tp, fl_tp, tp, bsd_hi, bsd_lo, rds);
+    const T va_bt = vadv_term(fh_bt, fl_bt, bt, sd_hi, sd_lo, rds);
+    const T va_qp = vadv_term(fh_qp, fl_qp, qp, bsd_hi, bsd_lo, rds);
+    const T va_bq = vadv_term(fh_bq, fl_bq, bq, sd_hi, sd_lo, rds);
+
+    // temperature (dynamics.py:183, 197-218)
+    const T tv = tp + eps * (tp * bq + bt * qp);
+    const T x5p = alpha * d + wP * padv + w3 * Sk1;
+    const T adiab = -kap * (tv * bx5 + btv * x5p);
+    const T tdev = bt - tr, tvdev = btv - tr;
+    const T xu = up * tdev + bu * tp, xv = vp * tdev + bv * tp;
+    const T hadv = tp * bdiv + tdev * d;
+    const T tm = kap * tr * (alpha * d + w3 * (D0 - PDk1));
+    T gt = adiab + hadv + va_tp + va_bt + tm;
+    if (fr) gt += fr[(size_t)k * I];
+    // momentum (dynamics.py:224-244)
+    const T fu = vp * babsv + bv * vo_ - rcpg * (tv * bpix + tvdev * pix) + va_up + va_bu;
+    const T fv = -up * babsv - bu * vo_ - rcpg * (tv * bpiy + tvdev * piy) + va_vp + va_bv;
+    const T xv_ = tv - tp;
+    const T phiv = phacc + rgas * alpha * xv_;
+    phacc += rgas * dlog3 * xv_;
+    T ke = phiv + up * bu + vp * bv;
+    if (full_ke) ke += h * (up * up + vp * vp);
+    // humidity (dynamics.py:256-259)
+    const T yu = up * bq + bu * qp, yv = vp * bq + bv * qp;
+    const T g = qp * bdiv + bq * d + va_qp + va_bq;
+
+    go[(size_t)(0 * L + k) * I] = fu * rc;
+    go[(size_t)(1 * L + k) * I] = fv * rc;
+    go[(size_t)(2 * L + k) * I] = ke;
+    go[(size_t)(3 * L + k) * I] = xu * rc;
+    go[(size_t)(4 * L + k) * I] = xv * rc;
+    go[(size_t)(5 * L + k) * I] = gt;
+    go[(size_t)(6 * L + k) * I] = yu * rc;
+    go[(size_t)(7 * L + k) * I] = yv * rc;
+    go[(size_t)(8 * L + k) * I] = g;
+
+    P = Pk1;
+    PD = PDk1;
+    fl_up = fh_up; fl_bu = fh_bu; fl_vp = fh_vp; fl_bv = fh_bv;
+    fl_tp = fh_tp; fl_bt = fh_bt; fl_qp = fh_qp; fl_bq = fh_bq;
+    sd_lo = sd_hi;
+    bsd_lo = bsd_hi;
+    up = upn; vp = vpn; tp = tpn; qp = qpn;
+    bu = bun; bv = bvn; bt = btn; bq = bqn;
+  }
+  if (!isfinite(chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
+}
+
+// ---------------------------------------------------------------------------
+// Spectral step, integrator.py:264-315 fused with the Robert-Asselin filter
+// (state.py:110-128, applied to the spectral fields it is linear in) and
+// the start-up schedule of model.py:211-219.  One thread per (s, member),
+// all levels in registers.
+// ---------------------------------------------------------------------------
+template <class T>
+struct StepArgs {
+  cx<T> *sa3, *sa2, *sb3;        // [s][4][E*L], [s][E], [s][4][E*L]
+  const cx<T> *tend3, *tend2;    // [s][4][E*L], [s][2][E] (wt_pi, pi_prev)
+  const T *kv, *kd, *ktt, *kq;   // [M][L]
+  const T *minv;                 // [nset][nbasic][M][L][L]
+  const T *vinv;                 // [nset][4][L][L]
+  const T *vc, *tref, *lap;      // VOff, [nbasic][L], [M]
+  const short *n_of_s;           // [nspec]
+  Ctl *ctl;
+  int *pend;                     // [E]
+  const int *nsteps;             // [E] full steps per member
+  int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  T delt, robert;
+};
+
+template <class T, int L>
+__device__ inline void matvec(const T *__restrict__ A, const cx<T> *x, cx<T> *y) {
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    cx<T> acc = zero<T>();
+#pragma unroll
+    for (int c = 0; c < L; ++c) fma_acc(acc, A[r * L + c], x[c]);
+    y[r] = acc;
+  }
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  const int E = a.E;
+  const int s = idx / E, e = idx - s * E;
+  if (s >= a.nspec) return;
+  const int st = a.ctl->step;
+  const bool active = st < a.inistp + a.nsteps[e];
+  const bool oinit = st < a.inistp;
+  if (s == 0) a.pend[e] = active ? (oinit ? 1 : 2) : 0;
+  if (!active) return;
+  const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
+  const int set = oinit ? st : a.inistp;
+  const T dt2 = (T)2 * dt;
+  const int n = a.n_of_s[s];
+  const int eb = a.nbasic > 1 ? e : 0;
+  const VOff vo{L};
+  const T *vc = a.vc;
+  const T *tref = a.tref + (size_t)eb * L;
+  const T *ktt = a.ktt + (size_t)n * L, *kd = a.kd + (size_t)n * L;
+  const T *kv = a.kv + (size_t)n * L, *kq = a.kq + (size_t)n * L;
+  const size_t EL = (size_t)E * L;
+  const size_t b3 = (size_t)s * 4 * EL + (size_t)e * L;
+  cx<T> *sa = a.sa3 + b3, *sb = a.sb3 + b3;
+  const cx<T> *tn = a.tend3 + b3;
+  const cx<T> wt_pi = a.tend2[(size_t)s * 2 * E + e];
+  const cx<T> pi_prev = a.tend2[(size_t)s * 2 * E + E + e];
+  const T lapn = a.lap[n];
+  const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
+
+  cx<T> w[L], dbar[L], x[L];
+  // temperature estimate and its geopotential (integrator.py:287-292)
+#pragma unroll
+  for (int l = 0; l < L; ++l) w[l] = ((T)1 + dt * ktt[l]) * sb[2 * EL + l] + dt * tn[2 * EL + l];
+  {
+    cx<T> acc = zero<T>();
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const cx<T> wl = w[l];
+      w[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
+      acc = acc + (rg * vc[vo.dlog3() + l]) * wl;
+    }
+  }
+  const cx<T> wdpix = pi_prev + dt * wt_pi;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const T ft = (T)1 + dt2 * ktt[l];
+    const T gg = rg * tref[l] * vc[vo.cpg() + l];
+    const cx<T> phy = w[l] + (ft * gg) * wdpix;
+    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[EL + l] + dt * (ft * tn[EL + l] - lapn * phy);
+  }
+  // dbar = M(n)^-1 rhs (integrator.py:165-172)
+  matvec<T, L>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, x, dbar);
+  const T *vinv = a.vinv + (size_t)set * 4 * L * L;
+  const bool vd_low = !(a.skip_low_n && n <= 1);
+
+  // surface pressure (integrator.py:301, 308)
+  {
+    cx<T> sdb = zero<T>();
+#pragma unroll
+    for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
+    a.sa2[(size_t)s * E + e] = pi_prev + dt2 * (wt_pi - sdb);
+  }
+  // divergence
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[EL + l];
+  if ((a.vmask & 2) && vd_low) {
+    matvec<T, L>(vinv + 1 * L * L, x, w);
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = w[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> cur = sa[EL + l];
+    sb[EL + l] = oinit ? cur : cur + gam * (sb[EL + l] - (T)2 * cur + x[l]);
+    sa[EL + l] = x[l];
+  }
+  // temperature: restore -tmtx dbar (integrator.py:300), damped update
+  {
+    cx<T> suf = zero<T>();
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) {
+      const T ft = (T)1 + dt2 * ktt[l];
+      const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * dbar[l] + vc[vo.w3() + l] * suf);
+      suf = suf + vc[vo.dsig() + l] * dbar[l];
+      x[l] = (T)(1) / ft * (sb[2 * EL + l] + dt2 * (tn[2 * EL + l] - tmd));
+    }
+  }
+  if (a.vmask & 4) {
+    matvec<T, L>(vinv + 2 * L * L, x, w);
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = w[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> cur = sa[2 * EL + l];
+    sb[2 * EL + l] = oinit ? cur : cur + gam * (sb[2 * EL + l] - (T)2 * cur + x[l]);
+    sa[2 * EL + l] = x[l];
+  }
+  // vorticity
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[l] + dt2 * tn[l]);
+  if ((a.vmask & 1) && vd_low) {
+    matvec<T, L>(vinv, x, w);
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = w[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> cur = sa[l];
+    sb[l] = oinit ? cur : cur + gam * (sb[l] - (T)2 * cur + x[l]);
+    sa[l] = x[l];
+  }
+  // humidity
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+    x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[3 * EL + l] + dt2 * tn[3 * EL + l]);
+  if (a.vmask & 8) {
+    matvec<T, L>(vinv + 3 * L * L, x, w);
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = w[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> cur = sa[3 * EL + l];
+    sb[3 * EL + l] = oinit ? cur : cur + gam * (sb[3 * EL + l] - (T)2 * cur + x[l]);
+    sa[3 * EL + l] = x[l];
+  }
+}
+
+// Finiteness of the spectral state after the last step.
+template <class T>
+__global__ void k_check_spec(const cx<T> *sa3, size_t n3, const cx<T> *sa2, size_t n2, Ctl *ctl) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  T acc = 0;
+  for (size_t q = idx; q < n3; q += stride) acc += sa3[q].x + sa3[q].y;
+  for (size_t q = idx; q < n2; q += stride) acc += sa2[q].x + sa2[q].y;
+  if (!isfinite(acc)) atomicMin(&ctl->bad_step, ctl->step);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions.
+// ---------------------------------------------------------------------------
+// reference grid (B, J, I, K) float64 -> internal [b][j][r0+k][I] of T,
+// optionally times rcos[j] (vector analysis pre-scale).  K may be 1 for a
+// surface field given as (B, J, I).
+template <class T>
+__global__ void k_pack_grid(const double *__restrict__ src, T *__restrict__ dst, int B, int J,
+                            int I, int K, int R, int r0, const double *__restrict__ rcos) {
+  const size_t total = (size_t)B * J * I * K;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    // q enumerates the destination order (b, j, k, i) for coalesced stores
+    const int i = (int)(q % I);
+    size_t rest = q / I;
+    const int k = (int)(rest % K);
+    rest /= K;
+    const int j = (int)(rest % J);
+    const int b = (int)(rest / J);
+    double v = src[(((size_t)b * J + j) * I + i) * K + k];
+    if (rcos) v *= rcos[j];
+    dst[(((size_t)b * J + j) * R + r0 + k) * I + i] = (T)v;
+  }
+}
+
+// internal [b][j][r0+k][I] -> reference (B, J, I, K) float64.
+// mode 0: copy; 1: ps' = ps_bar (exp(x) - 1) with ps_bar from psbar[b?][j][i].
+template <class T>
+__global__ void k_unpack_grid(const T *__restrict__ src, double *__restrict__ dst, int B, int J,
+                              int I, int K, int R, int r0, int mode, const T *psbar,
+                              int psbar_rows, int psbar_row, int nbasic) {
+  const size_t total = (size_t)B * J * I * K;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(q % K);
+    size_t rest = q / K;
+    const int i = (int)(rest % I);
+    rest /= I;
+    const int j = (int)(rest % J);
+    const int b = (int)(rest / J);
+    T v = src[(((size_t)b * J + j) * R + r0 + k) * I + i];
+    if (mode == 1) {
+      const int bb = nbasic > 1 ? b : 0;
+      const T pb = psbar[(((size_t)bb * J + j) * psbar_rows + psbar_row) * I + i];
+      v = pb * expm1(v);
+    }
+    dst[q] = (double)v;
+  }
+}
+
+// reference spectral (B, M, M, K) complex128 -> packed [s][f][ncols] with
+// column b*K + k (f fixed by the caller through the destination offset).
+template <class T>
+__global__ void k_pack_spec(const double *__restrict__ src, cx<T> *__restrict__ dst, int B, int M,
+                            int K, int nf, int f, int ncols) {
+  const int nspec = M * (M + 1) / 2;
+  const size_t total = (size_t)nspec * B * K;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int col = (int)(q % ((size_t)B * K));
+    const int s = (int)(q / ((size_t)B * K));
+    const int b = col / K, k = col - b * K;
+    int m = 0;
+    while (spec_off(m + 1, M) <= s) ++m;
+    const int n = m + (s - spec_off(m, M));
+    const size_t si = ((((size_t)b * M + m) * M + n) * K + k) * 2;
+    dst[((size_t)s * nf + f) * ncols + col] = mk((T)src[si], (T)src[si + 1]);
+  }
+}
+
+template <class T>
+__global__ void k_unpack_spec(const cx<T> *__restrict__ src, double *__restrict__ dst, int B,
+                              int M, int K, int nf, int f, int ncols) {
+  const size_t total = (size_t)B * M * M * K;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(q % K);
+    size_t rest = q / K;
+    const int n = (int)(rest % M);
+    rest /= M;
+    const int m = (int)(rest % M);
+    const int b = (int)(rest / M);
+    double re = 0, im = 0;
+    if (n >= m) {
+      const int s = spec_off(m, M) + n - m;
+      const cx<T> v = src[((size_t)s * nf + f) * ncols + (size_t)b * K + k];
+      re = (double)v.x;
+      im = (double)v.y;
+    }
+    dst[2 * q] = re;
+    dst[2 * q + 1] = im;
+  }
+}
+
+// ps' (B, J, I) float64 -> pi' = log1p(ps'/ps_bar) into an internal grid row.
+template <class T>
+__global__ void k_pack_pi(const double *__restrict__ ps, T *__restrict__ dst, int B, int J, int I,
+                          int R, int r0, const double *__restrict__ psbar, int nbasic) {
+  const size_t total = (size_t)B * J * I;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    const int j = (int)((q / I) % J);
+    const int b = (int)(q / ((size_t)I * J));
+    const int bb = nbasic > 1 ? b : 0;
+    const double pb = psbar[((size_t)bb * J + j) * I + i];
+    const double v = ps ? ps[q] : 0.0;
+    dst[(((size_t)b * J + j) * R + r0) * I + i] = (T)log1p(v / pb);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Basic-state terms (dynamics.py:54-93), FP64.  gsyn rows: vor (L), div (L);
+// ggrad rows: d/dx ln ps, d/dy ln ps.  Writes the BRow block.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_basic_terms(
+    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ t,
+    const double *__restrict__ q, const double *__restrict__ ps,  // reference layouts
+    const double *__restrict__ gsyn, const double *__restrict__ ggrad, const double *__restrict__ vc,
+    const double *__restrict__ mu, double *__restrict__ bsg, int J, int I, int L) {
+  const int i = threadIdx.x;
+  const int b = blockIdx.x / J, j = blockIdx.x - b * J;
+  const VOff vo{L};
+  const BRow br{L};
+  const size_t col = ((size_t)b * J + j) * I + i;  // (b, j, i) in 2-D reference layout
+  const double *vs = gsyn + ((size_t)(b * J + j) * 2 * L) * I + i;
+  const double *gg = ggrad + ((size_t)(b * J + j) * 2) * I + i;
+  double *o = bsg + ((size_t)(b * J + j) * br.rows()) * I + i;
+  const double pix = gg[0], piy = gg[(size_t)I];
+  const double f = 2.0 * OMEGA * mu[j];
+  double S = 0;
+  for (int k = L - 1; k >= 0; --k) {  // column sum from the top
+    const size_t g = col * L + k;
+    const double d = vs[(size_t)(L + k) * I];
+    const double padv = u[g] * pix + v[g] * piy;
+    S += vc[vo.dsig() + k] * (d + padv);
+  }
+  const double S0 = S;
+  // bottom-up: S_k = S_0 - sum_{l<k} dsig (D + A); sdot_k = sigm_k S_0 - S_k
+  double Sk = S0;
+  for (int k = 0; k < L; ++k) {
+    const size_t g = col * L + k;
+    const double d = vs[(size_t)(L + k) * I];
+    const double uu = u[g], vv = v[g], tt = t[g], qq = q[g];
+    const double padv = uu * pix + vv * piy;
+    const double sdk = (k == 0) ? 0.0 : vc[vo.sigm() + k] * S0 - Sk;
+    const double Sk1 = Sk - vc[vo.dsig() + k] * (d + padv);
+    o[(size_t)(br.u() + k) * I] = uu;
+    o[(size_t)(br.v() + k) * I] = vv;
+    o[(size_t)(br.t() + k) * I] = tt;
+    o[(size_t)(br.q() + k) * I] = qq;
+    o[(size_t)(br.tv() + k) * I] = tt * (1.0 + EPSVT * qq);
+    o[(size_t)(br.absvor() + k) * I] = vs[(size_t)k * I] + f;
+    o[(size_t)(br.div() + k) * I] = d;
+    o[(size_t)(br.x5() + k) * I] =
+        vc[vo.w3() + k] * (k + 1 == L ? 0.0 : Sk1) + vc[vo.alpha() + k] * d + vc[vo.wP() + k] * padv;
+    o[(size_t)(br.sdot() + k) * I] = sdk;
+    Sk = Sk1;
+  }
+  o[(size_t)(br.sdot() + L) * I] = 0.0;
+  o[(size_t)br.pix() * I] = pix;
+  o[(size_t)br.piy() * I] = piy;
+  o[(size_t)br.psbar() * I] = ps[col];
+}
+
+// Plain grid mean of basic Tv per level (vertical.py:206-216), fixed order.
+__global__ __launch_bounds__(256) void k_tbar(const double *__restrict__ bsg, double *__restrict__ tbar,
+                                              int J, int I, int L) {
+  __shared__ double red[256];
+  const int b = blockIdx.x / L, k = blockIdx.x - b * L;
+  const BRow br{L};
+  double acc = 0;
+  for (int p = threadIdx.x; p < J * I; p += 256) {
+    const int j = p / I, i = p - j * I;
+    acc += bsg[((size_t)(b * J + j) * br.rows() + br.tv() + k) * I + i];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tbar[(size_t)b * L + k] = red[0] / (double)(J * I);
+}
+
+// g[row r0] <- log(g[row r0]) for every (b, j, i)  (ln ps of dynamics.py:74)
+template <class T>
+__global__ void k_log_row(T *g, int B, int J, int I, int R, int r0) {
+  const size_t tot = (size_t)B * J * I;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    const size_t bj = q / I;
+    T *x = g + (bj * R + r0) * I + i;
+    *x = log(*x);
+  }
+}
+
+// zero K rows r0..r0+K-1 of an internal grid [b][j][R][I]
+template <class T>
+__global__ void k_zero_rows(T *g, int B, int J, int I, int R, int r0, int K) {
+  const size_t tot = (size_t)B * J * K * I;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    size_t rest = q / I;
+    const int k = (int)(rest % K);
+    const size_t bj = rest / K;
+    g[(bj * R + r0 + k) * I + i] = (T)0;
+  }
+}
+
+template <class S, class D>
+__global__ void k_convert(const S *__restrict__ src, D *__restrict__ dst, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = (D)src[q];
+}
+
+}  // namespace lbm

@@ -1,0 +1,244 @@
+"""``Config`` and ``LBM`` with the reference's interface (``pylbm_isca/model.py:34-290``).
+
+``LBM.run`` executes the whole time loop on the GPU: the start-up half
+steps (dt/4, dt/2, clock not advanced), the fused step kernels and the
+Robert-Asselin filter, optionally replayed as a hipGraph, with an on-device
+finiteness flag instead of a host sync per step.
+"""
+
+import time as _time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .backend import hip
+from .constants import S2D
+from .plan import Plan
+from .state import State
+
+DDRAG_DEFAULT = [0.5, 0.5, 0.5, 5, 15, 20, 20, 20, 20, 20,
+                 20, 20, 20, 20, 20, 20, 20, 20, 0.5, 0.5]
+
+
+@dataclass
+class Config:
+    """Namelist knobs and defaults of ``pylbm_isca/model.py:34-74``."""
+    delt: float = 1200.0
+    inistp: int = 2
+    ndays: float = 66.0
+    hdif_order: int = 4
+    hdif_tefold: float = 2.0
+    hdif_tunit: str = "HOUR"
+    sponge: bool = True
+    ray0: float = 30.0
+    ddragv: list = field(default_factory=lambda: list(DDRAG_DEFAULT))
+    ddragd: list = field(default_factory=lambda: list(DDRAG_DEFAULT))
+    ddragt: list = field(default_factory=lambda: list(DDRAG_DEFAULT))
+    ddragq: list = None
+    damp_tunit: str = "DAY"
+    vdifv: float = 1.0e3
+    vdifd: float = 1.0e3
+    vdift: float = 1.0e3
+    vdifq: float = None
+    robert: float = 0.04
+    out_interval_days: float = 1.0
+    time_average: bool = True
+    strictly_linear: bool = False
+    nmem: int = 1
+    device: str = "hip"
+    precision: str = "fp64"      # "fp64" or "fp32" arithmetic on the GPU
+
+    @property
+    def steps_per_day(self):
+        return int(round(86400.0 / self.delt))
+
+
+_UNIT = {"SEC": 1.0, "MIN": 60.0, "HOU": 3600.0, "DAY": 86400.0, "MON": 30 * 86400.0}
+
+
+def _seconds(value, unit):
+    return float(value) * _UNIT[unit.strip().upper()[:3]]
+
+
+def _drag(profile, kmax, unit):
+    """Per-level timescale -> s^-1, <= 0 disables (integrator.py:101-112)."""
+    out = np.zeros(kmax)
+    if profile is None:
+        return out
+    d = np.asarray(profile, dtype=np.float64)
+    if d.size != kmax:
+        raise ValueError(f"damping profile must have {kmax} entries")
+    sec = np.array([_seconds(x, unit) for x in d])
+    out[sec > 0] = 1.0 / sec[sec > 0]
+    return out
+
+
+def damping_coefficients(cfg, nmax, sig):
+    """(kv, kd, ktt, kq), each (nmax+1, nlev) in s^-1: implicit hyperdiffusion
+    [n(n+1)/(N(N+1))]^p / tau, tanh top sponge and Rayleigh/Newtonian drag
+    (integrator.py:41-98)."""
+    kmax = sig.size
+    n = np.arange(nmax + 1, dtype=np.float64)
+    p = cfg.hdif_order // 2
+    tau = _seconds(cfg.hdif_tefold, cfg.hdif_tunit)
+    difmax = (nmax * (nmax + 1.0) / nmax) ** p
+    dift = (n * (n + 1.0) / nmax) ** p
+    difv = dift - (2.0 / nmax) ** p
+    kt = dift / difmax / tau
+    kv = difv / difmax / tau
+    kt[0] = 0.0
+    kv[0] = 0.0
+    kv[1] = max(kv[1], 0.0)
+    sponge = np.zeros(kmax)
+    if cfg.sponge:
+        zsclh, rzsclh = 8.0e3, 7.0e3
+        z = -zsclh * np.log(sig)
+        zofs = -zsclh * np.log(float(sig[-1]))
+        sponge = np.maximum(1.0 + np.tanh((z - zofs) / rzsclh), 1e-20) / _seconds(cfg.ray0, cfg.hdif_tunit)
+    dv = _drag(cfg.ddragv, kmax, cfg.damp_tunit)
+    dd = _drag(cfg.ddragd, kmax, cfg.damp_tunit)
+    dt_ = _drag(cfg.ddragt, kmax, cfg.damp_tunit)
+    dq = _drag(cfg.ddragq, kmax, cfg.damp_tunit)
+    return (kv[:, None] + sponge + dv, kv[:, None] + sponge + dd,
+            kt[:, None] + dt_, kt[:, None] + dq)
+
+
+def _vdif_rate(v, kmax):
+    """1/(86400 v) per level; None or any v <= 0 disables (integrator.py:243-253)."""
+    if v is None:
+        return None
+    v = np.atleast_1d(np.asarray(v, dtype=np.float64))
+    if v.size == 1:
+        v = np.full(kmax, v[0])
+    if np.any(v <= 0):
+        return None
+    return 1.0 / (S2D * v)
+
+
+def _field(obj, name):
+    return getattr(obj, name)
+
+
+def _lead(x):
+    """Add the member axis to a single-member field (numpy or torch)."""
+    if x is None:
+        return None
+    if hasattr(x, "unsqueeze"):
+        return x.unsqueeze(0)
+    return np.asarray(x)[None]
+
+
+class LBM:
+    """Linear baroclinic model on the MI355X.
+
+    ``basic_state`` is any object with ``u, v, t, q`` (nlat, nlon, nlev) and
+    ``ps`` (nlat, nlon) in hPa (``dynamics.py:59-63``), or -- as an extension
+    of the reference, for ensembles whose members each have their own basic
+    state -- the same fields with a leading member axis when
+    ``per_member_basic=True``.
+    """
+
+    def __init__(self, basic_state, spectral, vertical, config=None, basic_spectral=None,
+                 per_member_basic=False):
+        self.cfg = config or Config()
+        c = self.cfg
+        self.sp, self.vg, self.bs = spectral, vertical, basic_state
+        self.xp = hip
+        self.batched = bool(getattr(spectral, "batched", False)) or c.nmem > 1
+        nmem = max(int(c.nmem), 1)
+        self.nmem = nmem
+        damping = damping_coefficients(c, spectral.nmax, vertical.sig)
+        kmax = vertical.kmax
+        vdif = tuple(_vdif_rate(v, kmax) for v in (c.vdifv, c.vdifd, c.vdift, c.vdifq))
+        nbasic = nmem if per_member_basic else 1
+        self.plan = Plan(spectral.nlon, spectral.nlat, spectral.nmax, spectral.a,
+                         vertical.sigm, nmem=nmem, nbasic=nbasic, precision=c.precision,
+                         damping=damping, vdif=vdif, delt=c.delt, robert=c.robert,
+                         inistp=c.inistp, strictly_linear=c.strictly_linear)
+        fields = [np.asarray(_field(basic_state, k), dtype=np.float64) for k in ("u", "v", "t", "q", "ps")]
+        if not per_member_basic:
+            fields = [f[None] for f in fields]
+        self.plan.set_basic_state(*fields)
+        self.ps_bar = fields[4] if per_member_basic else fields[4][0]
+        tb = self.plan.get("tbar", nbasic * kmax).reshape(nbasic, kmax)
+        vertical.set_reference_temperature(tb[0])
+        self.tbar = tb
+        self.plan.set_state()
+        self._ga = None
+        self.time = 0.0
+        self.istep = 0
+
+    @property
+    def device(self):
+        return "hip"
+
+    # -- state -----------------------------------------------------------
+    def set_initial_perturbation(self, u=None, v=None, t=None, q=None, ps=None):
+        """Sets both time levels (model.py:142-148).  Fields left out keep the
+        current values of the present level."""
+        given = dict(u=u, v=v, t=t, q=q, ps=ps)
+        if any(x is None for x in given.values()) and self.plan.steps_taken() > 0:
+            cur = dict(zip(("u", "v", "t", "q", "ps"), self.plan.get_state()))
+            if not self.batched:
+                cur = {k: x[0] for k, x in cur.items()}
+            given = {k: (cur[k] if x is None else x) for k, x in given.items()}
+        if not self.batched:
+            given = {k: _lead(x) for k, x in given.items()}
+        self.plan.set_state(**given)
+        self._ga = None
+
+    @property
+    def ga(self):
+        """Current time level as a grid :class:`State` (device tensors)."""
+        if self._ga is None:
+            u, v, t, q, ps = self.plan.get_state()
+            if not self.batched:
+                u, v, t, q, ps = (x[0] for x in (u, v, t, q, ps))
+            self._ga = State(self.ps_bar, u, v, t, q, ps)
+        return self._ga
+
+    def spectral_state(self, level=0):
+        """(vor, div, t, q, pi) spectral fields of the current (0) or
+        previous filtered (1) level; member axis always present."""
+        return self.plan.get_spectral(level)
+
+    # -- time loop ---------------------------------------------------------
+    def run(self, mjo_forcing=None, steady_forcing=None, writer=None, verbose=True,
+            progress_every=240, use_graph=True, lengths=None, check_every=0):
+        """Integrate ``cfg.ndays`` (model.py:168-290).
+
+        ``lengths`` (extension): per-member numbers of full steps for an
+        uneven ensemble; members stop when they reach theirs.
+        """
+        if writer is not None:
+            raise NotImplementedError("history output (writer=) is not built on the hip backend")
+        c = self.cfg
+        frc = None
+        if mjo_forcing is not None:
+            if not getattr(mjo_forcing, "constant", False) and type(mjo_forcing).__name__ not in (
+                    "EnsembleForcing", "ConstForcing"):
+                raise NotImplementedError(
+                    "time-varying forcing: only forcings whose step() is constant are built")
+            frc = mjo_forcing.step()
+        elif steady_forcing is not None:
+            for name in ("vor", "div", "ps"):
+                val = getattr(steady_forcing, name, None)
+                if val is not None and np.any(np.asarray(val) != 0):
+                    raise NotImplementedError(f"non-zero steady {name} forcing is not built")
+            frc = steady_forcing.t
+        self.plan.set_forcing(frc)
+        nsteps = int(round(c.ndays * c.steps_per_day))
+        if lengths is not None:
+            self.plan.set_lengths(lengths)
+            nsteps = 0
+        t0 = _time.time()
+        self.plan.run(nsteps, use_graph=use_graph, check_every=check_every)
+        adv = nsteps if lengths is None else int(max(lengths))
+        self.time += adv * c.delt
+        self.istep += adv
+        self._ga = None
+        if verbose:
+            import torch
+            torch.cuda.synchronize()
+            print(f"  done: {adv} steps, {_time.time() - t0:.1f}s", flush=True)
+        return self

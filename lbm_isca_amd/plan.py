@@ -1,0 +1,225 @@
+"""Owning wrapper of an ``lbm_plan`` (``include/lbm_hip.h``).
+
+All device buffers passed to the library are contiguous float64 torch
+tensors on the current HIP device, in the reference's layouts; the plan's
+internal precision (FP64 or FP32) only changes the arithmetic inside.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .constants import ER, NMAX, IMAX, JMAX, SIGM
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(x, dtype=None):
+    """float64 (or complex128) contiguous device tensor."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device="cuda")
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def zero_damping(nmax, nlev):
+    z = np.zeros((nmax + 1) * nlev)
+    return z, z, z, z
+
+
+class Plan:
+    """One model/transform plan: grid, tables, operators, state, work buffers."""
+
+    def __init__(self, nlon=IMAX, nlat=JMAX, nmax=NMAX, radius=ER, sigm=SIGM, nmem=1,
+                 nbasic=1, precision="fp64", damping=None, vdif=(None, None, None, None),
+                 delt=1200.0, robert=0.04, inistp=2, strictly_linear=False):
+        lib = _lib.load()
+        self.nlon, self.nlat, self.nmax = int(nlon), int(nlat), int(nmax)
+        self.sigm = np.ascontiguousarray(sigm, dtype=np.float64)
+        self.nlev = self.sigm.size - 1
+        self.nmem, self.nbasic = int(nmem), int(nbasic)
+        self.precision = precision
+        prec = {"fp64": _lib.LBM_FP64, "fp32": _lib.LBM_FP32}.get(str(precision).lower())
+        if prec is None:
+            raise ValueError(f"unknown precision {precision!r} (fp64 or fp32)")
+        if damping is None:
+            damping = zero_damping(self.nmax, self.nlev)
+        self._keep = [self.sigm]
+        d = _lib.LbmDesc()
+        d.nlon, d.nlat, d.nmax, d.nlev = self.nlon, self.nlat, self.nmax, self.nlev
+        d.nmem, d.nbasic, d.precision = self.nmem, self.nbasic, prec
+        d.strictly_linear = int(bool(strictly_linear))
+        d.inistp = int(inistp)
+        d.skip_low_n_vdiff = 1
+        d.radius, d.delt, d.robert = float(radius), float(delt), float(robert)
+        d.h_sigm = _lib.dptr(self.sigm)
+        arrs = []
+        for a in damping:
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != (self.nmax + 1) * self.nlev:
+                raise ValueError("damping arrays must be (nmax+1, nlev)")
+            arrs.append(a)
+        self._keep += arrs
+        d.h_kv, d.h_kd, d.h_ktt, d.h_kq = (_lib.dptr(a) for a in arrs)
+        vd = []
+        for v in vdif:
+            if v is None:
+                vd.append(None)
+            else:
+                v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.nlev,)))
+                self._keep.append(v)
+                vd.append(_lib.dptr(v))
+        d.h_vdif_vor, d.h_vdif_div, d.h_vdif_t, d.h_vdif_q = vd
+        h = ctypes.c_void_p()
+        _lib.check(lib.lbm_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lbm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- host setup arrays -------------------------------------------------
+    def get(self, name, count):
+        out = np.empty(count, dtype=np.float64)
+        n = _lib.check(self._lib.lbm_plan_get(self._h, name.encode(), _lib.dptr(out), count))
+        return out[:n]
+
+    # -- transforms --------------------------------------------------------
+    def _call(self, fn, *ptrs, B, nk):
+        _lib.check(fn(self._h, *ptrs, int(B), int(nk), _stream()))
+
+    def grid_to_wave(self, g):
+        torch = _torch()
+        g = _dev(g, torch.float64)
+        B, J, I, K = g.shape
+        w = torch.empty((B, self.nmax + 1, self.nmax + 1, K), dtype=torch.complex128, device="cuda")
+        self._call(self._lib.lbm_grid_to_wave, _lib.ptr(g), _lib.ptr(w), B=B, nk=K)
+        return w
+
+    def wave_to_grid(self, w):
+        torch = _torch()
+        w = _dev(w, torch.complex128)
+        B, M, _, K = w.shape
+        g = torch.empty((B, self.nlat, self.nlon, K), dtype=torch.float64, device="cuda")
+        self._call(self._lib.lbm_wave_to_grid, _lib.ptr(w), _lib.ptr(g), B=B, nk=K)
+        return g
+
+    def uv_to_vordiv(self, u, v):
+        torch = _torch()
+        u = _dev(u, torch.float64)
+        v = _dev(v, torch.float64)
+        B, J, I, K = u.shape
+        shp = (B, self.nmax + 1, self.nmax + 1, K)
+        vor = torch.empty(shp, dtype=torch.complex128, device="cuda")
+        div = torch.empty(shp, dtype=torch.complex128, device="cuda")
+        self._call(self._lib.lbm_uv_to_vordiv, _lib.ptr(u), _lib.ptr(v), _lib.ptr(vor),
+                   _lib.ptr(div), B=B, nk=K)
+        return vor, div
+
+    def psichi_to_uv(self, psi, chi):
+        torch = _torch()
+        psi = _dev(psi, torch.complex128)
+        chi = _dev(chi, torch.complex128)
+        B, M, _, K = psi.shape
+        shp = (B, self.nlat, self.nlon, K)
+        u = torch.empty(shp, dtype=torch.float64, device="cuda")
+        v = torch.empty(shp, dtype=torch.float64, device="cuda")
+        self._call(self._lib.lbm_psichi_to_uv, _lib.ptr(psi), _lib.ptr(chi), _lib.ptr(u),
+                   _lib.ptr(v), B=B, nk=K)
+        return u, v
+
+    def hgrad(self, w):
+        torch = _torch()
+        w = _dev(w, torch.complex128)
+        B, M, _, K = w.shape
+        shp = (B, self.nlat, self.nlon, K)
+        gx = torch.empty(shp, dtype=torch.float64, device="cuda")
+        gy = torch.empty(shp, dtype=torch.float64, device="cuda")
+        self._call(self._lib.lbm_hgrad, _lib.ptr(w), _lib.ptr(gx), _lib.ptr(gy), B=B, nk=K)
+        return gx, gy
+
+    # -- model -------------------------------------------------------------
+    def set_basic_state(self, u, v, t, q, ps):
+        torch = _torch()
+        f = [_dev(x, torch.float64) for x in (u, v, t, q, ps)]
+        _lib.check(self._lib.lbm_set_basic_state(self._h, *[_lib.ptr(x) for x in f], _stream()))
+
+    def set_forcing(self, frc):
+        torch = _torch()
+        if frc is None:
+            _lib.check(self._lib.lbm_set_forcing(self._h, None, 0, _stream()))
+            return
+        f = _dev(frc, torch.float64)
+        if f.dim() == 3:
+            f = f.unsqueeze(0)
+        _lib.check(self._lib.lbm_set_forcing(self._h, _lib.ptr(f), int(f.shape[0]), _stream()))
+
+    def set_state(self, u=None, v=None, t=None, q=None, ps=None):
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        args = []
+        for x, shp in ((u, (E, J, I, L)), (v, (E, J, I, L)), (t, (E, J, I, L)),
+                       (q, (E, J, I, L)), (ps, (E, J, I))):
+            if x is None:
+                args.append(None)
+                continue
+            x = _dev(x, torch.float64)
+            x = x.expand(shp).contiguous() if tuple(x.shape) != shp else x
+            args.append(x)
+        self._state_inputs = args
+        _lib.check(self._lib.lbm_set_state(self._h, *[_lib.ptr(x) for x in args], _stream()))
+
+    def set_lengths(self, nsteps):
+        if nsteps is None:
+            _lib.check(self._lib.lbm_set_lengths(self._h, None, None))
+            return
+        arr = (ctypes.c_int32 * self.nmem)(*[int(x) for x in nsteps])
+        _lib.check(self._lib.lbm_set_lengths(self._h, arr, None))
+
+    def run(self, nsteps, use_graph=True, check_every=0):
+        _lib.check(self._lib.lbm_run(self._h, int(nsteps), int(bool(use_graph)),
+                                     int(check_every), _stream()))
+
+    def get_state(self):
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        out = [torch.empty((E, J, I, L), dtype=torch.float64, device="cuda") for _ in range(4)]
+        ps = torch.empty((E, J, I), dtype=torch.float64, device="cuda")
+        _lib.check(self._lib.lbm_get_state(self._h, *[_lib.ptr(x) for x in out], _lib.ptr(ps),
+                                           _stream()))
+        return out + [ps]
+
+    def get_spectral(self, level=0):
+        torch = _torch()
+        E, M, L = self.nmem, self.nmax + 1, self.nlev
+        out = [torch.empty((E, M, M, L), dtype=torch.complex128, device="cuda") for _ in range(4)]
+        pi = torch.empty((E, M, M), dtype=torch.complex128, device="cuda")
+        _lib.check(self._lib.lbm_get_spectral(self._h, int(level), *[_lib.ptr(x) for x in out],
+                                              _lib.ptr(pi), _stream()))
+        return out + [pi]
+
+    def steps_taken(self):
+        n = ctypes.c_int64()
+        _lib.check(self._lib.lbm_steps_taken(self._h, ctypes.byref(n)))
+        return n.value
