@@ -78,8 +78,9 @@ int lbm_plan_destroy(lbm_plan *plan);
 
 /* Host copies of setup arrays: "mu", "gw", "coslat", "lat", "lon", "sig",
  * "dsig", "alpha", "cpg", "w3", "wP", "dsigm", "ww" (nlev*nlev), "dlog3",
- * "tbar" (nbasic*nlev, after lbm_set_basic_state), "minv" (nbasic*(nmax+1)*
- * nlev*nlev for dt=delt), "cond" (nbasic*(nmax+1)), "lop" (nlev*nlev).
+ * "sigm", "tbar" (nbasic*nlev, after lbm_set_basic_state), "minv"
+ * ((inistp+1)*nbasic*(nmax+1)*nlev*nlev: one block per start-up dt, then
+ * delt), "vinv" ((inistp+1)*4*nlev*nlev), "lop" (nlev*nlev).
  * n is the capacity of h_out in doubles; returns the count written. */
 int lbm_plan_get(const lbm_plan *plan, const char *name, double *h_out, int64_t n);
 
@@ -143,6 +144,17 @@ int lbm_get_spectral(lbm_plan *plan, int level, double *d_vor, double *d_div, do
 
 /* Number of steps taken since lbm_set_state (start-up included). */
 int lbm_steps_taken(const lbm_plan *plan, int64_t *out);
+
+/* Kernel timing with HIP events on the plan's stream (measurement only).
+ * lbm_profile(plan, 1) makes every following eager step record an event
+ * pair around each of its kernels and accumulate the elapsed times (one
+ * stream synchronisation per step); lbm_profile(plan, 0) stops.  Slots, in
+ * launch order: 0 synthesis (level fields), 1 synthesis (surface), 2
+ * inverse FFT, 3 grid tendencies, 4 forward FFT, 5 analysis (level), 6
+ * analysis (surface), 7 spectral step.  lbm_profile_read copies up to n
+ * accumulated milliseconds and launch counts and resets them. */
+int lbm_profile(lbm_plan *plan, int enable);
+int lbm_profile_read(lbm_plan *plan, double *h_ms, int64_t *h_count, int n);
 
 /* Copies the last error message into buf (always NUL terminated). */
 int lbm_last_error(char *buf, int64_t len);

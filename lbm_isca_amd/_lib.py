@@ -26,8 +26,10 @@ EXPORTS = (
     "lbm_grid_to_wave", "lbm_wave_to_grid", "lbm_uv_to_vordiv", "lbm_psichi_to_uv",
     "lbm_hgrad", "lbm_set_basic_state", "lbm_set_forcing", "lbm_set_state",
     "lbm_set_lengths", "lbm_run", "lbm_get_state", "lbm_get_spectral",
-    "lbm_steps_taken", "lbm_last_error",
+    "lbm_steps_taken", "lbm_profile", "lbm_profile_read", "lbm_last_error",
 )
+
+PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -78,6 +80,8 @@ def load():
         "lbm_get_state": ([P, P, P, P, P, P, P], ctypes.c_int),
         "lbm_get_spectral": ([P, ctypes.c_int, P, P, P, P, P, P], ctypes.c_int),
         "lbm_steps_taken": ([P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "lbm_profile": ([P, ctypes.c_int], ctypes.c_int),
+        "lbm_profile_read": ([P, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
         "lbm_last_error": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

@@ -49,38 +49,54 @@ int fail(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------
 // FP64 host setup
 // ---------------------------------------------------------------------------
-// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_J, sorted
-// north first (descending mu), weights halved to sum to one
+// Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_J in
+// extended precision, w = 2/(J P_{J-1}(x) P'_J(x)), symmetrised and
+// normalised (the construction numpy.polynomial.legendre.leggauss
+// publishes), sorted north first and halved to sum to one
 // (spectral.py:57-61).
 void gauss_legendre(int J, std::vector<double> &mu, std::vector<double> &gw) {
+  typedef long double R;
+  std::vector<R> x(J), w(J);
+  const R pi = 3.14159265358979323846264338327950288L;
+  for (int i = 0; i < J; ++i) {
+    R z = std::cos(pi * (i + 0.75L) / (J + 0.5L));
+    R p0 = 1, p1 = z, dp = 0;
+    for (int it = 0; it < 100; ++it) {
+      p0 = 1;
+      p1 = z;
+      for (int n = 2; n <= J; ++n) {
+        const R p2 = ((2 * n - 1) * z * p1 - (n - 1) * p0) / n;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = J * (z * p1 - p0) / (z * z - 1);
+      const R dz = p1 / dp;
+      z -= dz;
+      if (std::fabs((double)dz) < 1e-19 && it > 2) break;
+    }
+    p0 = 1;
+    p1 = z;
+    for (int n = 2; n <= J; ++n) {
+      const R p2 = ((2 * n - 1) * z * p1 - (n - 1) * p0) / n;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = J * (z * p1 - p0) / (z * z - 1);
+    x[i] = z;  // descending: i = 0 is the northernmost root
+    w[i] = 2 / (J * p0 * dp);
+  }
+  R sum = 0;
+  std::vector<R> ws(J), xs(J);
+  for (int i = 0; i < J; ++i) {
+    ws[i] = (w[i] + w[J - 1 - i]) / 2;
+    xs[i] = (x[i] - x[J - 1 - i]) / 2;
+    sum += ws[i];
+  }
   mu.assign(J, 0.0);
   gw.assign(J, 0.0);
   for (int i = 0; i < J; ++i) {
-    double x = std::cos(M_PI * (i + 0.75) / (J + 0.5));
-    double dp = 0.0;
-    for (int it = 0; it < 100; ++it) {
-      double p0 = 1.0, p1 = x;
-      for (int n = 2; n <= J; ++n) {
-        const double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
-        p0 = p1;
-        p1 = p2;
-      }
-      dp = J * (x * p1 - p0) / (x * x - 1.0);
-      const double dx = p1 / dp;
-      x -= dx;
-      if (std::fabs(dx) < 1e-16 && it > 2) break;
-    }
-    {
-      double p0 = 1.0, p1 = x;
-      for (int n = 2; n <= J; ++n) {
-        const double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
-        p0 = p1;
-        p1 = p2;
-      }
-      dp = J * (x * p1 - p0) / (x * x - 1.0);
-    }
-    mu[i] = x;  // descending already: i=0 is the northernmost root
-    gw[i] = 1.0 / ((1.0 - x * x) * dp * dp);  // = w/2
+    mu[i] = (double)xs[i];
+    gw[i] = (double)(ws[i] / sum);  // sum(gw) = 1
   }
 }
 
@@ -218,6 +234,12 @@ struct lbm_plan {
   hipStream_t own = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gsteps[2] = {0, 0};
+  // kernel timing (lbm_profile)
+  static constexpr int NPROF = 8;
+  bool prof = false;
+  hipEvent_t ev[NPROF + 1] = {};
+  double prof_ms[NPROF] = {};
+  long long prof_n[NPROF] = {};
 
   template <class T>
   DevTabs<T> &tabs();
@@ -823,11 +845,28 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st) {
   return LBM_OK;
 }
 
+// Event k marks the start of profiling slot k (slot order of lbm_profile).
+inline int mark(lbm_plan *p, int k, hipStream_t st) {
+  if (p->prof) HIPCHK(hipEventRecord(p->ev[k], st));
+  return LBM_OK;
+}
+
 template <class T>
 int one_step(lbm_plan *p, hipStream_t st) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
-  CHK(synthesize_state<T>(p, st));
+  {
+    const ColMap cm{E, L, 6, 3};
+    const int C = cm.ncols();
+    OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
+    CHK(mark(p, 0, st));
+    CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, C, sl, st)));
+    OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E}};
+    CHK(mark(p, 1, st));
+    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, C, s2, st)));
+    CHK(mark(p, 2, st));
+    CHK(launch_ifft<T>(p, p->fin, p->gin, cm, st));
+  }
   TendArgs<T> ta;
   ta.gin = (const T *)p->gin;
   ta.raw = p->have_raw ? (const T *)p->raw : nullptr;
@@ -849,23 +888,38 @@ int one_step(lbm_plan *p, hipStream_t st) {
   ta.nfrc = p->nfrc;
   ta.strictly_linear = p->strictly_linear;
   ta.robert = (T)p->robert;
+  CHK(mark(p, 3, st));
   hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
   HIPCHK(hipGetLastError());
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
+  CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st));
   InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
+  CHK(mark(p, 5, st));
   CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st)));
   InSlots s2{{9 * EL}};
+  CHK(mark(p, 6, st));
   CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st)));
+  CHK(mark(p, 7, st));
   if (L == 20)
     CHK((launch_step_kernel<T, 20>(p, st)));
   else if (L == 40)
     CHK((launch_step_kernel<T, 40>(p, st)));
   else
     return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
+  CHK(mark(p, 8, st));
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, p->ctl);
   HIPCHK(hipGetLastError());
+  if (p->prof) {
+    HIPCHK(hipEventSynchronize(p->ev[lbm_plan::NPROF]));
+    for (int k = 0; k < lbm_plan::NPROF; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
+      p->prof_ms[k] += ms;
+      p->prof_n[k] += 1;
+    }
+  }
   return LBM_OK;
 }
 
@@ -874,6 +928,7 @@ int step_dispatch(lbm_plan *p, hipStream_t st) {
 }
 
 int capture_graph(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
+  if (p->prof) return fail(LBM_ERR_STATE, "profiling runs eager launches only (use_graph=0)");
   if (p->gexec[slot] && p->gsteps[slot] == nsteps) return LBM_OK;
   if (p->gexec[slot]) {
     (void)hipGraphExecDestroy(p->gexec[slot]);
@@ -945,6 +1000,8 @@ int lbm_plan_destroy(lbm_plan *p) {
                   p->ctl, p->pend, p->d_nsteps, p->scratch};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
   if (p->own) (void)hipStreamDestroy(p->own);
   delete p;
   return LBM_OK;
@@ -1274,6 +1331,26 @@ int lbm_get_spectral(lbm_plan *p, int level, double *vor, double *div, double *t
   }
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+int lbm_profile(lbm_plan *p, int enable) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (enable && !p->ev[0])
+    for (auto &e : p->ev) HIPCHK(hipEventCreate(&e));
+  p->prof = enable != 0;
+  return LBM_OK;
+}
+
+int lbm_profile_read(lbm_plan *p, double *h_ms, int64_t *h_count, int n) {
+  if (!p || !h_ms || !h_count) return fail(LBM_ERR_VALUE, "null argument");
+  const int k = std::min(n, lbm_plan::NPROF);
+  for (int i = 0; i < k; ++i) {
+    h_ms[i] = p->prof_ms[i];
+    h_count[i] = p->prof_n[i];
+    p->prof_ms[i] = 0.0;
+    p->prof_n[i] = 0;
+  }
+  return k;
 }
 
 int lbm_steps_taken(const lbm_plan *p, int64_t *out) {

@@ -206,6 +206,7 @@ __global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *
   __syncthreads();
 #pragma unroll
   for (int n1 = 0; n1 < 8; ++n1) v[n1] = sh[rl][n1][tt];
+  dft8<-1>(v);
   __syncthreads();
   // Z[tt + 8 k1] = v[k1]; stage the 64 values of the row for the real split
   cx<T> *z = &sh[rl][0][0];

@@ -223,3 +223,16 @@ class Plan:
         n = ctypes.c_int64()
         _lib.check(self._lib.lbm_steps_taken(self._h, ctypes.byref(n)))
         return n.value
+
+    # -- measurement -------------------------------------------------------
+    def profile(self, enable):
+        _lib.check(self._lib.lbm_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        """{slot: (total ms, launches)} accumulated since the last read."""
+        n = len(_lib.PROFILE_SLOTS)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        k = _lib.check(self._lib.lbm_profile_read(
+            self._h, _lib.dptr(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(_lib.PROFILE_SLOTS[:k])}

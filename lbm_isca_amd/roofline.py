@@ -1,0 +1,43 @@
+"""Algorithmic HBM bytes per launch of each step kernel.
+
+Each figure counts the bytes a kernel must read and write at the internal
+layouts, each array once: inputs it consumes, outputs it produces, the
+Legendre tables once per launch and a shared basic state once per launch.
+``s`` is the element size (8 for FP64, 4 for FP32); complex values are 2 s.
+"""
+
+MI355X_HBM_GBS = 8000.0          # vendor peak, GB/s
+MI355X_FP64_TFLOPS = 78.6
+MI355X_FP32_TFLOPS = 157.3
+
+
+def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):
+    M = nmax + 1
+    nspec = M * (M + 1) // 2
+    Jh = J // 2
+    EL = E * L
+    c = 2 * s
+    tables = 2 * nspec * Jh * s
+    cin = 6 * EL + 3 * E           # Fourier/grid columns of the synthesised state
+    cout = 9 * EL + 2 * E          # columns of the grid tendencies
+    rb = 9 * L + 4                 # basic-state rows
+    return {
+        "syn_level": (nspec * 4 * EL + M * J * 6 * EL) * c + tables,
+        "syn_surf": (nspec * E + M * J * 3 * E) * c + tables,
+        "ifft": M * J * cin * c + J * cin * I * s,
+        "tend": (J * cin * I + nbasic * J * rb * I + nfrc * J * L * I
+                 + 4 * E * J * I + J * cout * I) * s,
+        "fft": J * cout * I * s + M * J * cout * c,
+        "ana_level": M * J * 9 * EL * c + nspec * 4 * EL * c + tables,
+        "ana_surf": M * J * 2 * E * c + nspec * 2 * E * c + tables,
+        "step": (5 * nspec * 4 * EL + nspec * 2 * E + nspec * E) * c + nbasic * M * L * L * s,
+    }
+
+
+def compulsory_bytes_per_member_step(L=20, J=64, I=128, s=8, per_member_basic=False, nmax=42):
+    """The path's compulsory traffic per member-step: read + write two grid
+    time levels and read the forcing, plus a per-member basic state."""
+    b = 4 * (4 * L + 1) * J * I * s + L * J * I * s
+    if per_member_basic:
+        b += (9 * L * J * I + L + 4 * J * I + (nmax + 1) * L * L) * s
+    return b
