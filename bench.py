@@ -107,8 +107,8 @@ def main():
     prof = plan.profile_read()
     plan.profile(False)
     s = 8 if args.precision == "fp64" else 4
-    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s)
-    avg = {k: ms / n for k, (ms, n) in prof.items() if n}
+    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s, nfrc=E)
+    avg = {k: ms / n for k, (ms, n) in prof.items() if n and ms > 0}
     dom = max(avg, key=lambda k: prof[k][0])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
     log("kernel ms/launch: " + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()))

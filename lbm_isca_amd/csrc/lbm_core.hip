@@ -227,7 +227,7 @@ struct lbm_plan {
   Ctl *ctl = nullptr;
   int *pend = nullptr, *d_nsteps = nullptr;
   std::vector<int> nsteps_h;
-  bool basic_set = false, have_raw = false;
+  bool basic_set = false, have_raw = false, raw_pending = false;
   long long steps_total = 0;
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -349,22 +349,22 @@ int set_twiddles() {
 // ---------------------------------------------------------------------------
 template <class T, class R>
 int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
-               hipStream_t st) {
+               hipStream_t st, int in_nf = R::NIN) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid((ncols + 7) / 8, p->M);
   hipLaunchKernelGGL((k_syn<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     t.Ps, t.Hs, t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M);
+                     t.Ps, t.Hs, t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M, in_nf);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 template <class T, class R>
 int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
-               hipStream_t st) {
+               hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid((ncols + 15) / 16, p->M);
   hipLaunchKernelGGL((k_ana<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     t.Pa, t.Ha, t.nn1a2, ncols, C, sl, p->J, p->M);
+                     t.Pa, t.Ha, t.nn1a2, ncols, C, sl, p->J, p->M, out_nf, out_f0);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -655,7 +655,7 @@ int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, co
     InSlots sl{{0, ncl}};
     CHK((launch_ana<double, AnaVorDiv>(p, F, S, ncl, C, sl, st)));
     InSlots s2{{2 * ncl}};
-    CHK((launch_ana<double, AnaScalar>(p, F, Sln, nb, C, s2, st)));
+    CHK((launch_ana<double, AnaScalar>(p, F, Sln, nb, C, s2, st, 1, 0)));
   }
   // grid vor, div (two fields as 2*ncl columns of one) and grad ln ps
   {
@@ -697,7 +697,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
   const int J = p->J, I = p->I, L = p->L, E = p->E, M = p->M;
   const int EL = E * L;
   const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
-  const size_t s3 = (size_t)p->nspec * 4 * EL, s2 = (size_t)p->nspec * E;
+  const size_t s3 = (size_t)p->nspec * 4 * EL, s2 = (size_t)p->nspec * 3 * E;
   HIPCHK(hipMemsetAsync(p->pend, 0, E * sizeof(int), st));
   if (ps) {
     hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, ps, (T *)p->psa, E, J, I,
@@ -713,6 +713,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
     HIPCHK(hipMemsetAsync(p->sb3, 0, s3 * sizeof(cx<T>), st));
     HIPCHK(hipMemsetAsync(p->sa2, 0, s2 * sizeof(cx<T>), st));
     p->have_raw = false;
+    p->raw_pending = false;
     return LBM_OK;
   }
   // raw grid (u, v, t, q) for the first tendency call
@@ -751,10 +752,14 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
     InSlots sl{{0, EL, 2 * EL, 3 * EL}};
     CHK((launch_ana<T, AnaState>(p, F, p->sa3, EL, C, sl, st)));
     InSlots s2{{4 * EL}};
-    CHK((launch_ana<T, AnaScalar>(p, F, p->sa2, E, C, s2, st)));
+    CHK((launch_ana<T, AnaScalar>(p, F, p->sa2, E, C, s2, st, 3, 0)));
   }
+  hipLaunchKernelGGL((k_vsum<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
+                     (const cx<T> *)p->sa3, (cx<T> *)p->sa2, p->tabs<T>().vc, E, L, p->nspec);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(p->sb3, p->sa3, s3 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
   p->have_raw = true;
+  p->raw_pending = true;
   return LBM_OK;
 }
 
@@ -762,11 +767,11 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
 template <class T>
 int synthesize_state(lbm_plan *p, hipStream_t st) {
   const int E = p->E, L = p->L, EL = E * L;
-  const ColMap cm{E, L, 6, 3};
+  const ColMap cm{E, L, 6, 6};
   const int C = cm.ncols();
   OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
   CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, C, sl, st)));
-  OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E}};
+  OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
   CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, C, s2, st)));
   CHK(launch_ifft<T>(p, p->fin, p->gin, cm, st));
   return LBM_OK;
@@ -790,7 +795,7 @@ int get_state_t(lbm_plan *p, double *u, double *v, double *t, double *q, double 
     return LBM_OK;
   }
   CHK(synthesize_state<T>(p, st));
-  const int rows = 6 * L + 3;
+  const int rows = 6 * L + 6;
   const int r0[4] = {0, L, 4 * L, 5 * L};
   for (int f = 0; f < 4; ++f)
     if (dst[f])
@@ -851,25 +856,26 @@ inline int mark(lbm_plan *p, int k, hipStream_t st) {
   return LBM_OK;
 }
 
+// One time step: synthesis, inverse FFT, grid tendencies, forward FFT,
+// analysis, spectral step.  raw_path: the first step after a raw initial
+// state (k_tend reads the raw grid).
 template <class T>
-int one_step(lbm_plan *p, hipStream_t st) {
+int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
+  const ColMap cmi{E, L, 6, 6};
+  const int Ci = cmi.ncols();
   {
-    const ColMap cm{E, L, 6, 3};
-    const int C = cm.ncols();
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
     CHK(mark(p, 0, st));
-    CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, C, sl, st)));
-    OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E}};
+    CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, Ci, sl, st)));
+    OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
     CHK(mark(p, 1, st));
-    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, C, s2, st)));
-    CHK(mark(p, 2, st));
-    CHK(launch_ifft<T>(p, p->fin, p->gin, cm, st));
+    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, Ci, s2, st)));
   }
   TendArgs<T> ta;
   ta.gin = (const T *)p->gin;
-  ta.raw = p->have_raw ? (const T *)p->raw : nullptr;
+  ta.raw = (raw_path && p->have_raw) ? (const T *)p->raw : nullptr;
   ta.bsg = (const T *)p->bsg;
   ta.frc = (const T *)p->frc;
   ta.gout = (T *)p->gout;
@@ -888,11 +894,13 @@ int one_step(lbm_plan *p, hipStream_t st) {
   ta.nfrc = p->nfrc;
   ta.strictly_linear = p->strictly_linear;
   ta.robert = (T)p->robert;
+  const ColMap cmo{E, L, 9, 2};
+  const int Co = cmo.ncols();
+  CHK(mark(p, 2, st));
+  CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st));
   CHK(mark(p, 3, st));
   hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
   HIPCHK(hipGetLastError());
-  const ColMap cmo{E, L, 9, 2};
-  const int Co = cmo.ncols();
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st));
   InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
@@ -900,7 +908,7 @@ int one_step(lbm_plan *p, hipStream_t st) {
   CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st)));
   InSlots s2{{9 * EL}};
   CHK(mark(p, 6, st));
-  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st)));
+  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0)));
   CHK(mark(p, 7, st));
   if (L == 20)
     CHK((launch_step_kernel<T, 20>(p, st)));
@@ -923,8 +931,8 @@ int one_step(lbm_plan *p, hipStream_t st) {
   return LBM_OK;
 }
 
-int step_dispatch(lbm_plan *p, hipStream_t st) {
-  return p->prec == LBM_FP64 ? one_step<double>(p, st) : one_step<float>(p, st);
+int step_dispatch(lbm_plan *p, hipStream_t st, bool raw_path = false) {
+  return p->prec == LBM_FP64 ? one_step<double>(p, st, raw_path) : one_step<float>(p, st, raw_path);
 }
 
 int capture_graph(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
@@ -964,7 +972,7 @@ int check_bad(lbm_plan *p, hipStream_t st, long long base_step) {
 
 template <class T>
 int final_check(lbm_plan *p, hipStream_t st) {
-  const size_t n3 = (size_t)p->nspec * 4 * p->E * p->L, n2 = (size_t)p->nspec * p->E;
+  const size_t n3 = (size_t)p->nspec * 4 * p->E * p->L, n2 = (size_t)p->nspec * 3 * p->E;
   hipLaunchKernelGGL((k_check_spec<T>), dim3(256), dim3(256), 0, st, (const cx<T> *)p->sa3, n3,
                      (const cx<T> *)p->sa2, n2, p->ctl);
   HIPCHK(hipGetLastError());
@@ -1081,7 +1089,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->tbar.assign((size_t)p->nb * L, 0.0);
   const size_t rs = p->rsize(), cs = 2 * rs;
   const size_t EL = (size_t)E * L;
-  const ColMap cin{E, L, 6, 3}, cout{E, L, 9, 2};
+  const ColMap cin{E, L, 6, 6}, cout{E, L, 9, 2};
   struct A {
     void **ptr;
     size_t bytes;
@@ -1091,7 +1099,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
       {(void **)&p->d_psbar, sizeof(double) * p->nb * J * I},
       {&p->sa3, cs * p->nspec * 4 * EL},
       {&p->sb3, cs * p->nspec * 4 * EL},
-      {&p->sa2, cs * p->nspec * E},
+      {&p->sa2, cs * p->nspec * 3 * E},
       {&p->tend3, cs * p->nspec * 4 * EL},
       {&p->tend2, cs * p->nspec * 2 * E},
       {&p->fin, cs * M * J * cin.ncols()},
@@ -1116,7 +1124,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
       hipMemset(p->pend, 0, sizeof(int) * E) != hipSuccess ||
       hipMemset(p->sa3, 0, cs * p->nspec * 4 * EL) != hipSuccess ||
       hipMemset(p->sb3, 0, cs * p->nspec * 4 * EL) != hipSuccess ||
-      hipMemset(p->sa2, 0, cs * p->nspec * E) != hipSuccess ||
+      hipMemset(p->sa2, 0, cs * p->nspec * 3 * E) != hipSuccess ||
       hipMemset(p->psa, 0, rs * (size_t)E * J * I) != hipSuccess ||
       hipMemset(p->psb, 0, rs * (size_t)E * J * I) != hipSuccess)
     return bail(fail(LBM_ERR_RUNTIME, "initialisation failed"));
@@ -1267,6 +1275,12 @@ int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *strea
   const int chunk = 8;
   int done = 0;
   int rc = LBM_OK;
+  if (p->raw_pending) {
+    // the first step after a raw initial state reads the raw grid (unfused)
+    CHK(step_dispatch(p, st, true));
+    p->raw_pending = false;
+    done = 1;
+  }
   while (done < total) {
     int todo = total - done;
     if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
@@ -1324,10 +1338,10 @@ int lbm_get_spectral(lbm_plan *p, int level, double *vor, double *div, double *t
   if (pi && level == 0) {
     if (p->prec == LBM_FP64)
       hipLaunchKernelGGL((k_unpack_spec<double>), dim3(nblk((size_t)E * M * M)), dim3(256), 0, st,
-                         (const cx<double> *)p->sa2, pi, E, M, 1, 1, 0, E);
+                         (const cx<double> *)p->sa2, pi, E, M, 1, 3, 0, E);
     else
       hipLaunchKernelGGL((k_unpack_spec<float>), dim3(nblk((size_t)E * M * M)), dim3(256), 0, st,
-                         (const cx<float> *)p->sa2, pi, E, M, 1, 1, 0, E);
+                         (const cx<float> *)p->sa2, pi, E, M, 1, 3, 0, E);
   }
   HIPCHK(hipGetLastError());
   return LBM_OK;

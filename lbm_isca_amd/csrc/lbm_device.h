@@ -88,7 +88,9 @@ __host__ __device__ inline void dft8(cx<T> *v) {
   v[7] = e3 - w3o3;
 }
 
-// Column-space map: Fourier/grid columns -> (member, grid row).
+// Column-space map: Fourier/grid columns -> (member, grid row).  Columns of
+// level fields are (field, level, member) with the member fastest; surface
+// fields follow as (field, member).
 struct ColMap {
   int E, L, nF3, nF2;  // members, levels, level fields, surface fields
   __host__ __device__ int ncols() const { return nF3 * E * L + nF2 * E; }
@@ -97,8 +99,9 @@ struct ColMap {
     const int EL = E * L;
     if (col < nF3 * EL) {
       const int f = col / EL, rem = col - f * EL;
-      e = rem / L;
-      r = f * L + (rem - e * L);
+      const int l = rem / E;
+      e = rem - l * E;
+      r = f * L + l;
     } else {
       const int c2 = col - nF3 * EL;
       const int g = c2 / E;
@@ -110,8 +113,10 @@ struct ColMap {
 
 // ---------------------------------------------------------------------------
 // Length-128 real FFTs, 8 threads per row (64 = 8 x 8 complex FFT of the
-// even/odd packed samples, then the real split).  32 rows per 256-thread
-// block.  Fourier columns are rows j*C + col.
+// even/odd packed samples, then the real split).  Each transform is split
+// into stages separated by a block barrier: the caller places the
+// __syncthreads() between stages, so several rows (and other work) share
+// the barriers.  scr is an 8x9 complex scratch per row.
 // ---------------------------------------------------------------------------
 constexpr int FFT_I = 128;
 constexpr int FFT_ROWS = 32;
@@ -132,100 +137,129 @@ __constant__ cx<float> c_tw128_f[FFT_I];
 __device__ inline const cx<double> *Tw<double>::get() { return c_tw128_d; }
 __device__ inline const cx<float> *Tw<float>::get() { return c_tw128_f; }
 
-// Fourier (m <= M-1) -> grid.  g = X0.re + 2 sum_k Re(X_k e^{i k lam}),
-// i.e. irfft(zero-filled * nlon) of spectral.py:166-172.
 template <class T>
-__global__ __launch_bounds__(256) void k_ifft128(const cx<T> *__restrict__ F, T *__restrict__ G,
-                                                 int C, ColMap cm, int J, int M) {
-  __shared__ cx<T> sh[FFT_ROWS][8][9];
+using Scr = cx<T> (*)[9];  // [8][9]
+
+// Inverse, stage a: X_m (m < M, from F with stride ms between m) -> scr.
+// g = X0.re + 2 sum_m Re(X_m e^{i m lam}) = irfft(zero-filled * nlon)
+// (spectral.py:166-172); the imaginary part of X0 is discarded.
+template <class T>
+__device__ inline void ifft_a(const cx<T> *__restrict__ F, size_t ms, int M, cx<T> (*scr)[9], int tt) {
   const cx<T> *tw = Tw<T>::get();
-  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
-  const long row = (long)blockIdx.x * FFT_ROWS + rl;
-  const bool live = row < (long)J * C;
-  int j = 0, col = 0;
-  if (live) {
-    j = (int)(row / C);
-    col = (int)(row - (long)j * C);
-  }
   cx<T> v[8];
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
     const int kk = tt + 8 * n2;
     cx<T> A = zero<T>(), B = zero<T>();
-    if (live && kk < M) {
-      A = F[((size_t)kk * J + j) * C + col];
+    if (kk < M) {
+      A = F[(size_t)kk * ms];
       if (kk == 0) A.y = 0;
     }
     const int kb = 64 - kk;
-    if (live && kb < M) B = conj(F[((size_t)kb * J + j) * C + col]);
+    if (kb < M) B = conj(F[(size_t)kb * ms]);
     const cx<T> w = conj(tw[kk]);  // exp(+2 pi i kk/128)
     v[n2] = (A + B) + cmul(mk((T)0, (T)1), cmul(w, A - B));
   }
   dft8<+1>(v);
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) sh[rl][tt][k2] = cmul(v[k2], conj(tw[(2 * tt * k2) & 127]));
-  __syncthreads();
+  for (int k2 = 0; k2 < 8; ++k2) scr[tt][k2] = cmul(v[k2], conj(tw[(2 * tt * k2) & 127]));
+}
+
+// Inverse, stage b: scr -> 128 real samples (as 64 complex pairs) in out.
+template <class T>
+__device__ inline void ifft_b(cx<T> (*scr)[9], cx<T> *out, int tt) {
+  cx<T> v[8];
 #pragma unroll
-  for (int n1 = 0; n1 < 8; ++n1) v[n1] = sh[rl][n1][tt];
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[n1][tt];
   dft8<+1>(v);
-  if (!live) return;
-  int e, r;
-  cm.map(col, e, r);
-  cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I);
 #pragma unroll
   for (int k1 = 0; k1 < 8; ++k1) out[tt + 8 * k1] = v[k1];
 }
 
-// grid -> Fourier (m <= M-1): rfft / nlon of spectral.py:159-164.
+// Forward, stage a: 128 real samples (64 complex pairs in) -> scr.
 template <class T>
-__global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *__restrict__ F,
-                                                int C, ColMap cm, int J, int M) {
-  __shared__ cx<T> sh[FFT_ROWS][8][9];
+__device__ inline void fft_a(const cx<T> *in, cx<T> (*scr)[9], int tt) {
   const cx<T> *tw = Tw<T>::get();
-  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
-  const long row = (long)blockIdx.x * FFT_ROWS + rl;
-  const bool live = row < (long)J * C;
-  int j = 0, col = 0;
   cx<T> v[8];
-  if (live) {
-    j = (int)(row / C);
-    col = (int)(row - (long)j * C);
-    int e, r;
-    cm.map(col, e, r);
-    const cx<T> *in =
-        reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I);
 #pragma unroll
-    for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[tt + 8 * n2];
-  } else {
-#pragma unroll
-    for (int n2 = 0; n2 < 8; ++n2) v[n2] = zero<T>();
-  }
+  for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[tt + 8 * n2];
   dft8<-1>(v);
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) sh[rl][tt][k2] = cmul(v[k2], tw[(2 * tt * k2) & 127]);
-  __syncthreads();
+  for (int k2 = 0; k2 < 8; ++k2) scr[tt][k2] = cmul(v[k2], tw[(2 * tt * k2) & 127]);
+}
+
+// Forward, stage b: scr -> Z[0..63] (complex FFT of the packed samples).
+template <class T>
+__device__ inline void fft_b(cx<T> (*scr)[9], cx<T> *Z, int tt) {
+  cx<T> v[8];
 #pragma unroll
-  for (int n1 = 0; n1 < 8; ++n1) v[n1] = sh[rl][n1][tt];
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[n1][tt];
   dft8<-1>(v);
-  __syncthreads();
-  // Z[tt + 8 k1] = v[k1]; stage the 64 values of the row for the real split
-  cx<T> *z = &sh[rl][0][0];
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) z[tt + 8 * k1] = v[k1];
-  __syncthreads();
-  if (!live) return;
+  for (int k1 = 0; k1 < 8; ++k1) Z[tt + 8 * k1] = v[k1];
+}
+
+// Forward, stage c: real split, X_m = rfft/nlon for m < M (spectral.py:159-164),
+// written to F with stride ms between m.
+template <class T>
+__device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, int M, int tt) {
+  const cx<T> *tw = Tw<T>::get();
   const T half = (T)0.5, inv_n = (T)1 / (T)FFT_I;
 #pragma unroll
   for (int k1 = 0; k1 < 8; ++k1) {
     const int k = tt + 8 * k1;
     if (k >= M) break;
-    const cx<T> zk = z[k], zc = conj(z[(64 - k) & 63]);
+    const cx<T> zk = Z[k], zc = conj(Z[(64 - k) & 63]);
     const cx<T> s = zk + zc, d = zk - zc;
     // X = (s - i w d)/2, w = exp(-2 pi i k/128)
     const cx<T> wd = cmul(tw[k], d);
-    cx<T> X = mk(half * (s.x + wd.y), half * (s.y - wd.x));
-    F[((size_t)k * J + j) * C + col] = inv_n * X;
+    F[(size_t)k * ms] = inv_n * mk(half * (s.x + wd.y), half * (s.y - wd.x));
   }
+}
+
+// Fourier (m <= M-1) -> grid, 32 rows per block; rows are j*C + col.
+template <class T>
+__global__ __launch_bounds__(256) void k_ifft128(const cx<T> *__restrict__ F, T *__restrict__ G,
+                                                 int C, ColMap cm, int J, int M) {
+  __shared__ cx<T> sh[FFT_ROWS][8][9];
+  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
+  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+  const bool live = row < (long)J * C;
+  int j = 0, col = 0;
+  if (live) {
+    j = (int)(row / C);
+    col = (int)(row - (long)j * C);
+    ifft_a<T>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt);
+  }
+  __syncthreads();
+  if (!live) return;
+  int e, r;
+  cm.map(col, e, r);
+  ifft_b<T>(sh[rl], reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I), tt);
+}
+
+// grid -> Fourier (m <= M-1), 32 rows per block.
+template <class T>
+__global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *__restrict__ F,
+                                                int C, ColMap cm, int J, int M) {
+  __shared__ cx<T> sh[FFT_ROWS][8][9];
+  __shared__ cx<T> zb[FFT_ROWS][64];
+  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
+  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+  const bool live = row < (long)J * C;
+  int j = 0, col = 0;
+  if (live) {
+    j = (int)(row / C);
+    col = (int)(row - (long)j * C);
+    int e, r;
+    cm.map(col, e, r);
+    fft_a<T>(reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I),
+             sh[rl], tt);
+  }
+  __syncthreads();
+  if (live) fft_b<T>(sh[rl], zb[rl], tt);
+  __syncthreads();
+  if (live) fft_c<T>(zb[rl], F + (size_t)j * C + col, (size_t)J * C, M, tt);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,16 +285,24 @@ struct SynLevel {  // (vor, div, t, q) -> (U, V, vor, div, t, q): state.py:94-99
     A[5] = x[3];
   }
 };
-struct SynSurf {  // pi -> (d/dx, d/dy, pi): hgrad of spectral.py:210-225 + wave_to_grid
-  static constexpr int NIN = 1, NOUT = 3;
+struct SynSurf {  // (pi, sum dsig vor, sum dsig div) -> (d/dx pi, d/dy pi, pi, sum u, sum v, sum D)
+  // hgrad and wave_to_grid of spectral.py:193-225; the column sums of
+  // u, v, D (psichi_to_uv and wave_to_grid of the vertically summed spectra)
+  static constexpr int NIN = 3, NOUT = 6;
   static constexpr bool hasA(int o) { return o != 1; }
-  static constexpr bool hasB(int o) { return o == 1; }
-  static constexpr bool scaled(int o) { return o < 2; }
+  static constexpr bool hasB(int o) { return o == 1 || o == 3 || o == 4; }
+  static constexpr bool scaled(int o) { return o < 2 || o == 3 || o == 4; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+  __device__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
     A[0] = im_mul(fm, x[0]);
     B[1] = x[0];
     A[2] = x[0];
+    const cx<T> psi = il * x[1], chi = il * x[2];
+    A[3] = im_mul(fm, chi);
+    B[3] = -psi;
+    A[4] = im_mul(fm, psi);
+    B[4] = chi;
+    A[5] = x[2];
   }
 };
 struct SynScalar {  // wave_to_grid (spectral.py:193-197)
@@ -308,7 +350,7 @@ __global__ __launch_bounds__(256) void k_syn(const cx<T> *__restrict__ in, cx<T>
                                              const T *__restrict__ Ps, const T *__restrict__ Hs,
                                              const T *__restrict__ inv_lap,
                                              const T *__restrict__ rcos, int ncols, int C,
-                                             OutSlots slots, int J, int M) {
+                                             OutSlots slots, int J, int M, int in_nf) {
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
   const int t = threadIdx.x, cl = t & 7;
   const int col = blockIdx.x * 8 + cl, m = blockIdx.y;
@@ -323,7 +365,7 @@ __global__ __launch_bounds__(256) void k_syn(const cx<T> *__restrict__ in, cx<T>
     for (int n = m; n < M; ++n, ++s) {
       cx<T> x[NIN], A[NOUT], B[NOUT];
 #pragma unroll
-      for (int f = 0; f < NIN; ++f) x[f] = in[((size_t)s * NIN + f) * ncols + col];
+      for (int f = 0; f < NIN; ++f) x[f] = in[((size_t)s * in_nf + f) * ncols + col];
       R::template terms<T>(x, inv_lap[n], fm, A, B);
       const T p = Ps[(size_t)s * Jh + jp], h = Hs[(size_t)s * Jh + jp];
       if ((n - m) & 1) {
@@ -416,7 +458,7 @@ template <class T, class R>
 __global__ __launch_bounds__(256) void k_ana(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                              const T *__restrict__ Pa, const T *__restrict__ Ha,
                                              const T *__restrict__ nn1a2, int ncols, int C,
-                                             InSlots slots, int J, int M) {
+                                             InSlots slots, int J, int M, int out_nf, int out_f0) {
   constexpr int NIN = R::NIN, NOUT = R::NOUT, KN = 3;
   const int t = threadIdx.x, cl = t & 15, ns = t >> 4;
   const int col = blockIdx.x * 16 + cl, m = blockIdx.y;
@@ -472,7 +514,7 @@ __global__ __launch_bounds__(256) void k_ana(const cx<T> *__restrict__ in, cx<T>
     R::template combine<T>(ap, ah, fm, nn1a2[nk[k]], o);
     const int s = s0 + nk[k];
 #pragma unroll
-    for (int q = 0; q < NOUT; ++q) out[((size_t)s * NOUT + q) * ncols + col] = o[q];
+    for (int q = 0; q < NOUT; ++q) out[((size_t)s * out_nf + out_f0 + q) * ncols + col] = o[q];
   }
 }
 

@@ -39,7 +39,9 @@ struct BRow {
   __host__ __device__ int pix() const { return 9 * L + 1; }
   __host__ __device__ int piy() const { return 9 * L + 2; }
   __host__ __device__ int psbar() const { return 9 * L + 3; }
-  __host__ __device__ int rows() const { return 9 * L + 4; }
+  __host__ __device__ int sbu() const { return 9 * L + 4; }  // sum_l dsig_l u_l
+  __host__ __device__ int sbv() const { return 9 * L + 5; }  // sum_l dsig_l v_l
+  __host__ __device__ int rows() const { return 9 * L + 6; }
 };
 
 // Loop control read by the step kernels (graph-replayable: no per-step
@@ -57,20 +59,148 @@ __global__ void k_tick(Ctl *ctl) {
 }
 
 // ---------------------------------------------------------------------------
-// Grid-point tendencies of dynamics.py:156-262 for one column per thread.
-// Inputs (synthesised state, rows of gin): U V vor div T Q (L each), then
-// d/dx pi, d/dy pi, pi.  Outputs (rows of gout): fu' fv' ke xu' xv' gt yu'
-// yv' g (L each; ' = times 1/(a cos lat), the analysis metric), then gt_pi
-// and pi of the previous (filtered) level.  Vertical sums are prefix/suffix
-// sums: S_k = sum_{l>=k} dsig (D+A), sdot_k = sigm_k S_0 - S_k,
-// (tmtx div)_k = kappa Tref_k (alpha_k D_k + w3_k sum_{l>k} dsig_l D_l),
-// phi_k = sum_{l<k} R dlog3_l x_l + R alpha_k x_k.
+// Grid-point tendencies of dynamics.py:156-262, one column per thread,
+// levels bottom-up.  Vertical operators are running sums:
+//   S_k = sum_{l>=k} dsig (D+A) = S_0 - P_k,  sdot_k = sigm_k S_0 - S_k,
+//   (tmtx D)_k = kappa Tref_k (alpha_k D_k + w3_k sum_{l>k} dsig_l D_l),
+//   phi_k = sum_{l<k} R dlog3_l x_l + R alpha_k x_k,
+// and second-centred vertical advection carries the flux of half level k.
+// Outputs per level: fu' fv' ke xu' xv' gt yu' yv' g (' = times 1/(a cos
+// lat), the metric of the vector analysis).
 // ---------------------------------------------------------------------------
 template <class T>
+struct ColConst {
+  T S0, D0, pix, piy, bpix, bpiy, rc;
+  bool full_ke;
+};
+
+template <class T>
+struct ColState {
+  T P = 0, PD = 0, phacc = 0;
+  T fl_up = 0, fl_bu = 0, fl_vp = 0, fl_bv = 0, fl_tp = 0, fl_bt = 0, fl_qp = 0, fl_bq = 0;
+  T sd_lo = 0, bsd_lo = 0;
+  T chk = 0;
+};
+
+template <class T>
+__device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig) {
+  return (flux_hi - flux_lo - f * (s_hi - s_lo)) * rdsig;
+}
+
+// One level k of the column.  bs: basic-state column (row stride I); fr:
+// forcing column (row stride I) or null.  (up..qp) at level k, (upn..qpn)
+// at k+1 (ignored at the top).
+template <class T>
+__device__ inline void column_level(int k, int L, const T *__restrict__ vc, const T *__restrict__ tref,
+                                    const T *__restrict__ bs, const T *__restrict__ fr, int I,
+                                    const ColConst<T> &cc, ColState<T> &cs, T up, T vp, T vor, T div,
+                                    T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out) {
+  const VOff vo{L};
+  const BRow br{L};
+  auto B = [&](int row) { return bs[(size_t)row * I]; };
+  const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY, h = (T)0.5;
+  const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
+  const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
+  const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
+  const T tr = tref[k];
+  const T bu = B(br.u() + k), bv = B(br.v() + k), bt = B(br.t() + k), bq = B(br.q() + k);
+  const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
+  const T bx5 = B(br.x5() + k);
+  const bool top = (k + 1 == L);
+  const T bun = top ? (T)0 : B(br.u() + k + 1), bvn = top ? (T)0 : B(br.v() + k + 1);
+  const T btn = top ? (T)0 : B(br.t() + k + 1), bqn = top ? (T)0 : B(br.q() + k + 1);
+  const T bsd_hi = B(br.sdot() + k + 1);
+  if (top) upn = vpn = tpn = qpn = 0;
+  const T padv = up * cc.bpix + bu * cc.pix + vp * cc.bpiy + bv * cc.piy;
+  cs.chk += up + vp + tp + qp;
+
+  const T Pk1 = cs.P + ds * (div + padv), PDk1 = cs.PD + ds * div;
+  const T Sk1 = cc.S0 - Pk1;
+  const T sd_hi = top ? (T)0 : Pk1 - ((T)1 - vc[vo.sigm() + k + 1]) * cc.S0;
+  // flux at half level k+1: s_{k+1} (f_{k+1} + f_k)/2 (0 at the top)
+  const T fh_up = top ? 0 : bsd_hi * h * (upn + up);
+  const T fh_bu = top ? 0 : sd_hi * h * (bun + bu);
+  const T fh_vp = top ? 0 : bsd_hi * h * (vpn + vp);
+  const T fh_bv = top ? 0 : sd_hi * h * (bvn + bv);
+  const T fh_tp = top ? 0 : bsd_hi * h * (tpn + tp);
+  const T fh_bt = top ? 0 : sd_hi * h * (btn + bt);
+  const T fh_qp = top ? 0 : bsd_hi * h * (qpn + qp);
+  const T fh_bq = top ? 0 : sd_hi * h * (bqn + bq);
+  const T va_up = vadv_term(fh_up, cs.fl_up, up, bsd_hi, cs.bsd_lo, rds);
+  const T va_bu = vadv_term(fh_bu, cs.fl_bu, bu, sd_hi, cs.sd_lo, rds);
+  const T va_vp = vadv_term(fh_vp, cs.fl_vp, vp, bsd_hi, cs.bsd_lo, rds);
+  const T va_bv = vadv_term(fh_bv, cs.fl_bv, bv, sd_hi, cs.sd_lo, rds);
+  const T va_tp = vadv_term(fh_tp, cs.fl_tp, tp, bsd_hi, cs.bsd_lo, rds);
+  const T va_bt = vadv_term(fh_bt, cs.fl_bt, bt, sd_hi, cs.sd_lo, rds);
+  const T va_qp = vadv_term(fh_qp, cs.fl_qp, qp, bsd_hi, cs.bsd_lo, rds);
+  const T va_bq = vadv_term(fh_bq, cs.fl_bq, bq, sd_hi, cs.sd_lo, rds);
+
+  // temperature (dynamics.py:183, 197-218)
+  const T tv = tp + eps * (tp * bq + bt * qp);
+  const T x5p = alpha * div + wP * padv + w3 * Sk1;
+  const T adiab = -kap * (tv * bx5 + btv * x5p);
+  const T tdev = bt - tr, tvdev = btv - tr;
+  const T xu = up * tdev + bu * tp, xv = vp * tdev + bv * tp;
+  const T hadv = tp * bdiv + tdev * div;
+  const T tm = kap * tr * (alpha * div + w3 * (cc.D0 - PDk1));
+  T gt = adiab + hadv + va_tp + va_bt + tm;
+  if (fr) gt += fr[(size_t)k * I];
+  // momentum (dynamics.py:224-244)
+  const T fu = vp * babsv + bv * vor - rcpg * (tv * cc.bpix + tvdev * cc.pix) + va_up + va_bu;
+  const T fv = -up * babsv - bu * vor - rcpg * (tv * cc.bpiy + tvdev * cc.piy) + va_vp + va_bv;
+  const T xvirt = tv - tp;
+  const T phiv = cs.phacc + rgas * alpha * xvirt;
+  cs.phacc += rgas * dlog3 * xvirt;
+  T ke = phiv + up * bu + vp * bv;
+  if (cc.full_ke) ke += h * (up * up + vp * vp);
+  // humidity (dynamics.py:256-259)
+  const T yu = up * bq + bu * qp, yv = vp * bq + bv * qp;
+  const T g = qp * bdiv + bq * div + va_qp + va_bq;
+
+  out[0] = fu * cc.rc;
+  out[1] = fv * cc.rc;
+  out[2] = ke;
+  out[3] = xu * cc.rc;
+  out[4] = xv * cc.rc;
+  out[5] = gt;
+  out[6] = yu * cc.rc;
+  out[7] = yv * cc.rc;
+  out[8] = g;
+
+  cs.P = Pk1;
+  cs.PD = PDk1;
+  cs.fl_up = fh_up; cs.fl_bu = fh_bu; cs.fl_vp = fh_vp; cs.fl_bv = fh_bv;
+  cs.fl_tp = fh_tp; cs.fl_bt = fh_bt; cs.fl_qp = fh_qp; cs.fl_bq = fh_bq;
+  cs.sd_lo = sd_hi;
+  cs.bsd_lo = bsd_hi;
+}
+
+// Surface part of a column: deferred ps' update of the previous step
+// (state.py:94-128 on ps': ps'_new = ps_bar (exp(pi) - 1); gb <- ga at
+// start-up or the Robert-Asselin filter; ga <- new), continuity
+// gt_pi = -S_0 + sum dsig D (dynamics.py:188) and pi of the previous level
+// ln(ps_bar + ps'_b) - ln ps_bar = log1p(ps'_b/ps_bar) (state.py:77-79).
+template <class T>
+__device__ inline void column_surface(T *psa_p, T *psb_p, int mode, T pig, T psbar, T robert,
+                                      T S0, T D0, T &gt_pi, T &pi_prev, T &chk) {
+  T psa = *psa_p, psb = *psb_p;
+  if (mode != 0) {
+    const T psn = psbar * expm1(pig);
+    psb = (mode == 1) ? psa : psa + robert * (psb - (T)2 * psa + psn);
+    psa = psn;
+    *psa_p = psa;
+    *psb_p = psb;
+  }
+  chk += psa;
+  gt_pi = D0 - S0;
+  pi_prev = log1p(psb / psbar);
+}
+
+template <class T>
 struct TendArgs {
-  const T *gin;     // [e][j][6L+3][I]
+  const T *gin;     // [e][j][6L+6][I] synthesised state on the grid
   const T *raw;     // [e][j][4L][I] raw initial (u,v,t,q) or null
-  const T *bsg;     // [eb][j][9L+4][I]
+  const T *bsg;     // [eb][j][9L+6][I]
   const T *frc;     // [ef][j][L][I] or null
   T *gout;          // [e][j][9L+2][I]
   T *psa, *psb;     // [e][j][I] ps' at t and at t-dt (filtered)
@@ -83,11 +213,12 @@ struct TendArgs {
   T robert;
 };
 
-template <class T>
-__device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig) {
-  return (flux_hi - flux_lo - f * (s_hi - s_lo)) * rdsig;
-}
-
+// Tendencies on grid rows, one thread per column, one pass over the levels
+// with S_0 and sum dsig D from the synthesised column sums.  The first
+// step after a raw (not band-limited) initial perturbation reads the raw
+// u, v, T, q and forms S_0 in a first pass over the column, as the
+// reference does on its grid.  gin rows: U V vor div T Q (L each), then
+// d/dx pi, d/dy pi, pi, sum u, sum v, sum D.
 template <class T>
 __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const int i = threadIdx.x;
@@ -96,151 +227,70 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};
   const BRow br{L};
-  const T *vc = a.vc;
-  const T *tref = a.tref + (size_t)eb * L;
-  const int Rin = 6 * L + 3, Rout = 9 * L + 2;
+  const int Rin = 6 * L + 6, Rout = 9 * L + 2;
   const bool use_raw = a.raw != nullptr && a.ctl->raw;
   const T *gi = a.gin + (size_t)(e * J + j) * Rin * I + i;
   const T *rw = use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I + i : nullptr;
   const T *bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
   const T *fr = a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
   T *go = a.gout + (size_t)(e * J + j) * Rout * I + i;
-  const T rc = a.rcos[j];
-
-  // perturbation inputs: raw grid on the first step after set_state
-  auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : gi[(size_t)k * I]; };
-  auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : gi[(size_t)(L + k) * I]; };
-  auto TP = [&](int k) { return use_raw ? rw[(size_t)(2 * L + k) * I] : gi[(size_t)(4 * L + k) * I]; };
-  auto QP = [&](int k) { return use_raw ? rw[(size_t)(3 * L + k) * I] : gi[(size_t)(5 * L + k) * I]; };
-  auto VOR = [&](int k) { return gi[(size_t)(2 * L + k) * I]; };
-  auto DIV = [&](int k) { return gi[(size_t)(3 * L + k) * I]; };
-  auto B = [&](int row) { return bs[(size_t)row * I]; };
-
-  const T pix = gi[(size_t)(6 * L) * I], piy = gi[(size_t)(6 * L + 1) * I];
-  const T pig = gi[(size_t)(6 * L + 2) * I];
-  const T bpix = B(br.pix()), bpiy = B(br.piy()), psbar = B(br.psbar());
-
-  // deferred ps' update of the previous step (state.py:94-128 on ps'):
-  // ps'_new = ps_bar (exp(pi) - 1); gb <- ga (start-up) or RA filter; ga <- new
-  const size_t pidx = (size_t)(e * J + j) * I + i;
-  T psa = a.psa[pidx], psb = a.psb[pidx];
-  const int mode = a.pend[e];
-  if (mode != 0) {
-    const T psn = psbar * expm1(pig);
-    psb = (mode == 1) ? psa : psa + a.robert * (psb - (T)2 * psa + psn);
-    psa = psn;
-    a.psa[pidx] = psa;
-    a.psb[pidx] = psb;
-  }
-  T chk = psa;
-
-  // pass 1: column totals S_0 and sum dsig*D
-  T S0 = 0, D0 = 0;
-  for (int k = 0; k < L; ++k) {
-    const T d = DIV(k);
-    const T padv = UP(k) * bpix + B(br.u() + k) * pix + VP(k) * bpiy + B(br.v() + k) * piy;
-    const T ds = vc[vo.dsig() + k];
-    S0 += ds * (d + padv);
-    D0 += ds * d;
-  }
-  // continuity: gt_pi = -S_0 + sum dsig D (dynamics.py:188)
-  go[(size_t)(9 * L) * I] = D0 - S0;
-  // previous-level pi' = ln(ps_bar + ps'_b) - ln ps_bar (state.py:77-79)
-  go[(size_t)(9 * L + 1) * I] = log1p(psb / psbar);
-
-  const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY;
-  const bool full_ke = !a.strictly_linear;
-  T P = 0, PD = 0, phacc = 0;
-  // vertical-advection fluxes at half level k for the 8 (field; sdot) pairs
-  T fl_up = 0, fl_bu = 0, fl_vp = 0, fl_bv = 0, fl_tp = 0, fl_bt = 0, fl_qp = 0, fl_bq = 0;
-  T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
-  T bu = B(br.u()), bv = B(br.v()), bt = B(br.t()), bq = B(br.q());
-  T sd_lo = 0;               // perturbation sdot at half level k
-  T bsd_lo = B(br.sdot());   // basic sdot at half level k
-  for (int k = 0; k < L; ++k) {
-    const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
-    const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
-    const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
-    const T tr = tref[k];
-    const T d = DIV(k), vo_ = VOR(k);
-    const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
-    const T bx5 = B(br.x5() + k);
-    const T padv = up * bpix + bu * pix + vp * bpiy + bv * piy;
-    chk += up + vp + tp + qp;
-
-    const T Pk1 = P + ds * (d + padv), PDk1 = PD + ds * d;
-    const T Sk1 = S0 - Pk1;
-    const bool top = (k + 1 == L);
-    const T sd_hi = top ? (T)0 : Pk1 - ((T)1 - vc[vo.sigm() + k + 1]) * S0;
-    const T bsd_hi = B(br.sdot() + k + 1);
-
-    T upn = 0, vpn = 0, tpn = 0, qpn = 0, bun = 0, bvn = 0, btn = 0, bqn = 0;
-    if (!top) {
-      upn = UP(k + 1); vpn = VP(k + 1); tpn = TP(k + 1); qpn = QP(k + 1);
-      bun = B(br.u() + k + 1); bvn = B(br.v() + k + 1);
-      btn = B(br.t() + k + 1); bqn = B(br.q() + k + 1);
+  auto G = [&](int row) { return gi[(size_t)row * I]; };
+  auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : G(k); };
+  auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : G(L + k); };
+  auto TP = [&](int k) { return use_raw ? rw[(size_t)(2 * L + k) * I] : G(4 * L + k); };
+  auto QP = [&](int k) { return use_raw ? rw[(size_t)(3 * L + k) * I] : G(5 * L + k); };
+  ColConst<T> cc;
+  cc.pix = G(6 * L);
+  cc.piy = G(6 * L + 1);
+  const T pig = G(6 * L + 2);
+  cc.bpix = bs[(size_t)br.pix() * I];
+  cc.bpiy = bs[(size_t)br.piy() * I];
+  const T psbar = bs[(size_t)br.psbar() * I];
+  cc.rc = a.rcos[j];
+  cc.full_ke = !a.strictly_linear;
+  if (use_raw) {
+    T S0 = 0, D0 = 0;
+    for (int k = 0; k < L; ++k) {
+      const T d = G(3 * L + k);
+      const T padv = UP(k) * cc.bpix + bs[(size_t)(br.u() + k) * I] * cc.pix + VP(k) * cc.bpiy +
+                     bs[(size_t)(br.v() + k) * I] * cc.piy;
+      const T ds = a.vc[vo.dsig() + k];
+      S0 += ds * (d + padv);
+      D0 += ds * d;
     }
-    const T h = (T)0.5;
-    // flux at half level k+1: s_{k+1} (f_{k+1} + f_k)/2 (0 at the top)
-    const T fh_up = top ? 0 : bsd_hi * h * (upn + up);
-    const T fh_bu = top ? 0 : sd_hi * h * (bun + bu);
-    const T fh_vp = top ? 0 : bsd_hi * h * (vpn + vp);
-    const T fh_bv = top ? 0 : sd_hi * h * (bvn + bv);
-    const T fh_tp = top ? 0 : bsd_hi * h * (tpn + tp);
-    const T fh_bt = top ? 0 : sd_hi * h * (btn + bt);
-    const T fh_qp = top ? 0 : bsd_hi * h * (qpn + qp);
-    const T fh_bq = top ? 0 : sd_hi * h * (bqn + bq);
-    const T va_up = vadv_term(fh_up, fl_up, up, bsd_hi, bsd_lo, rds);
-    const T va_bu = vadv_term(fh_bu, fl_bu, bu, sd_hi, sd_lo, rds);
-    const T va_vp = vadv_term(fh_vp, fl_vp, vp, bsd_hi, bsd_lo, rds);
-    const T va_bv = vadv_term(fh_bv, fl_bv, bv, sd_hi, sd_lo, rds);
-    const T va_tp = vadv_term(fh_tp, fl_tp, tp, bsd_hi, bsd_lo, rds);
-    const T va_bt = vadv_term(fh_bt, fl_bt, bt, sd_hi, sd_lo, rds);
-    const T va_qp = vadv_term(fh_qp, fl_qp, qp, bsd_hi, bsd_lo, rds);
-    const T va_bq = vadv_term(fh_bq, fl_bq, bq, sd_hi, sd_lo, rds);
-
-    // temperature (dynamics.py:183, 197-218)
-    const T tv = tp + eps * (tp * bq + bt * qp);
-    const T x5p = alpha * d + wP * padv + w3 * Sk1;
-    const T adiab = -kap * (tv * bx5 + btv * x5p);
-    const T tdev = bt - tr, tvdev = btv - tr;
-    const T xu = up * tdev + bu * tp, xv = vp * tdev + bv * tp;
-    const T hadv = tp * bdiv + tdev * d;
-    const T tm = kap * tr * (alpha * d + w3 * (D0 - PDk1));
-    T gt = adiab + hadv + va_tp + va_bt + tm;
-    if (fr) gt += fr[(size_t)k * I];
-    // momentum (dynamics.py:224-244)
-    const T fu = vp * babsv + bv * vo_ - rcpg * (tv * bpix + tvdev * pix) + va_up + va_bu;
-    const T fv = -up * babsv - bu * vo_ - rcpg * (tv * bpiy + tvdev * piy) + va_vp + va_bv;
-    const T xv_ = tv - tp;
-    const T phiv = phacc + rgas * alpha * xv_;
-    phacc += rgas * dlog3 * xv_;
-    T ke = phiv + up * bu + vp * bv;
-    if (full_ke) ke += h * (up * up + vp * vp);
-    // humidity (dynamics.py:256-259)
-    const T yu = up * bq + bu * qp, yv = vp * bq + bv * qp;
-    const T g = qp * bdiv + bq * d + va_qp + va_bq;
-
-    go[(size_t)(0 * L + k) * I] = fu * rc;
-    go[(size_t)(1 * L + k) * I] = fv * rc;
-    go[(size_t)(2 * L + k) * I] = ke;
-    go[(size_t)(3 * L + k) * I] = xu * rc;
-    go[(size_t)(4 * L + k) * I] = xv * rc;
-    go[(size_t)(5 * L + k) * I] = gt;
-    go[(size_t)(6 * L + k) * I] = yu * rc;
-    go[(size_t)(7 * L + k) * I] = yv * rc;
-    go[(size_t)(8 * L + k) * I] = g;
-
-    P = Pk1;
-    PD = PDk1;
-    fl_up = fh_up; fl_bu = fh_bu; fl_vp = fh_vp; fl_bv = fh_bv;
-    fl_tp = fh_tp; fl_bt = fh_bt; fl_qp = fh_qp; fl_bq = fh_bq;
-    sd_lo = sd_hi;
-    bsd_lo = bsd_hi;
-    up = upn; vp = vpn; tp = tpn; qp = qpn;
-    bu = bun; bv = bvn; bt = btn; bq = bqn;
+    cc.S0 = S0;
+    cc.D0 = D0;
+  } else {
+    const T su = G(6 * L + 3), sv = G(6 * L + 4), sd = G(6 * L + 5);
+    cc.D0 = sd;
+    cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * bs[(size_t)br.sbu() * I] +
+            cc.piy * bs[(size_t)br.sbv() * I];
   }
-  if (!isfinite(chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
+  ColState<T> cs;
+  cs.bsd_lo = bs[(size_t)br.sdot() * I];
+  const size_t pidx = (size_t)(e * J + j) * I + i;
+  T gpi, pib;
+  column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0, gpi,
+                    pib, cs.chk);
+  go[(size_t)(9 * L) * I] = gpi;
+  go[(size_t)(9 * L + 1) * I] = pib;
+  const T *tref = a.tref + (size_t)eb * L;
+  T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
+  for (int k = 0; k < L; ++k) {
+    const bool top = (k + 1 == L);
+    const T upn = top ? (T)0 : UP(k + 1), vpn = top ? (T)0 : VP(k + 1);
+    const T tpn = top ? (T)0 : TP(k + 1), qpn = top ? (T)0 : QP(k + 1);
+    T out[9];
+    column_level<T>(k, L, a.vc, tref, bs, fr, I, cc, cs, up, vp, G(2 * L + k), G(3 * L + k), tp, qp,
+                    upn, vpn, tpn, qpn, out);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) go[(size_t)(f * L + k) * I] = out[f];
+    up = upn;
+    vp = vpn;
+    tp = tpn;
+    qp = qpn;
+  }
+  if (!isfinite(cs.chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,8 +301,8 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <class T>
 struct StepArgs {
-  cx<T> *sa3, *sa2, *sb3;        // [s][4][E*L], [s][E], [s][4][E*L]
-  const cx<T> *tend3, *tend2;    // [s][4][E*L], [s][2][E] (wt_pi, pi_prev)
+  cx<T> *sa3, *sa2, *sb3;        // [s][4][L][E], [s][3][E] (pi, sum vor, sum div), [s][4][L][E]
+  const cx<T> *tend3, *tend2;    // [s][4][L][E], [s][2][E] (wt_pi, pi_prev)
   const T *kv, *kd, *ktt, *kq;   // [M][L]
   const T *minv;                 // [nset][nbasic][M][L][L]
   const T *vinv;                 // [nset][4][L][L]
@@ -276,6 +326,8 @@ __device__ inline void matvec(const T *__restrict__ A, const cx<T> *x, cx<T> *y)
   }
 }
 
+// One thread per (s, member), members fastest across lanes so every field
+// access of a wave is contiguous; all levels in registers.
 template <class T, int L>
 __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   const int idx = blockIdx.x * 64 + threadIdx.x;
@@ -298,18 +350,20 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   const T *ktt = a.ktt + (size_t)n * L, *kd = a.kd + (size_t)n * L;
   const T *kv = a.kv + (size_t)n * L, *kq = a.kq + (size_t)n * L;
   const size_t EL = (size_t)E * L;
-  const size_t b3 = (size_t)s * 4 * EL + (size_t)e * L;
+  const size_t b3 = (size_t)s * 4 * EL + e;
   cx<T> *sa = a.sa3 + b3, *sb = a.sb3 + b3;
   const cx<T> *tn = a.tend3 + b3;
+  auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
   const cx<T> wt_pi = a.tend2[(size_t)s * 2 * E + e];
   const cx<T> pi_prev = a.tend2[(size_t)s * 2 * E + E + e];
   const T lapn = a.lap[n];
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
+  cx<T> *s2 = a.sa2 + (size_t)s * 3 * E + e;
 
   cx<T> w[L], dbar[L], x[L];
   // temperature estimate and its geopotential (integrator.py:287-292)
 #pragma unroll
-  for (int l = 0; l < L; ++l) w[l] = ((T)1 + dt * ktt[l]) * sb[2 * EL + l] + dt * tn[2 * EL + l];
+  for (int l = 0; l < L; ++l) w[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
   {
     cx<T> acc = zero<T>();
 #pragma unroll
@@ -325,7 +379,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
     const T ft = (T)1 + dt2 * ktt[l];
     const T gg = rg * tref[l] * vc[vo.cpg() + l];
     const cx<T> phy = w[l] + (ft * gg) * wdpix;
-    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[EL + l] + dt * (ft * tn[EL + l] - lapn * phy);
+    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
   matvec<T, L>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, x, dbar);
@@ -337,21 +391,26 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
     cx<T> sdb = zero<T>();
 #pragma unroll
     for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
-    a.sa2[(size_t)s * E + e] = pi_prev + dt2 * (wt_pi - sdb);
+    s2[0] = pi_prev + dt2 * (wt_pi - sdb);
   }
   // divergence
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[EL + l];
+  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[at(1, l)];
   if ((a.vmask & 2) && vd_low) {
     matvec<T, L>(vinv + 1 * L * L, x, w);
 #pragma unroll
     for (int l = 0; l < L; ++l) x[l] = w[l];
   }
+  {
+    cx<T> sum = zero<T>();
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[EL + l];
-    sb[EL + l] = oinit ? cur : cur + gam * (sb[EL + l] - (T)2 * cur + x[l]);
-    sa[EL + l] = x[l];
+    for (int l = 0; l < L; ++l) {
+      const cx<T> cur = sa[at(1, l)];
+      sb[at(1, l)] = oinit ? cur : cur + gam * (sb[at(1, l)] - (T)2 * cur + x[l]);
+      sa[at(1, l)] = x[l];
+      sum = sum + vc[vo.dsig() + l] * x[l];
+    }
+    s2[2 * E] = sum;
   }
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
@@ -361,7 +420,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
       const T ft = (T)1 + dt2 * ktt[l];
       const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * dbar[l] + vc[vo.w3() + l] * suf);
       suf = suf + vc[vo.dsig() + l] * dbar[l];
-      x[l] = (T)(1) / ft * (sb[2 * EL + l] + dt2 * (tn[2 * EL + l] - tmd));
+      x[l] = (T)(1) / ft * (sb[at(2, l)] + dt2 * (tn[at(2, l)] - tmd));
     }
   }
   if (a.vmask & 4) {
@@ -371,28 +430,33 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   }
 #pragma unroll
   for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[2 * EL + l];
-    sb[2 * EL + l] = oinit ? cur : cur + gam * (sb[2 * EL + l] - (T)2 * cur + x[l]);
-    sa[2 * EL + l] = x[l];
+    const cx<T> cur = sa[at(2, l)];
+    sb[at(2, l)] = oinit ? cur : cur + gam * (sb[at(2, l)] - (T)2 * cur + x[l]);
+    sa[at(2, l)] = x[l];
   }
   // vorticity
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[l] + dt2 * tn[l]);
+  for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
   if ((a.vmask & 1) && vd_low) {
     matvec<T, L>(vinv, x, w);
 #pragma unroll
     for (int l = 0; l < L; ++l) x[l] = w[l];
   }
+  {
+    cx<T> sum = zero<T>();
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[l];
-    sb[l] = oinit ? cur : cur + gam * (sb[l] - (T)2 * cur + x[l]);
-    sa[l] = x[l];
+    for (int l = 0; l < L; ++l) {
+      const cx<T> cur = sa[at(0, l)];
+      sb[at(0, l)] = oinit ? cur : cur + gam * (sb[at(0, l)] - (T)2 * cur + x[l]);
+      sa[at(0, l)] = x[l];
+      sum = sum + vc[vo.dsig() + l] * x[l];
+    }
+    s2[E] = sum;
   }
   // humidity
 #pragma unroll
   for (int l = 0; l < L; ++l)
-    x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[3 * EL + l] + dt2 * tn[3 * EL + l]);
+    x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
   if (a.vmask & 8) {
     matvec<T, L>(vinv + 3 * L * L, x, w);
 #pragma unroll
@@ -400,9 +464,30 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   }
 #pragma unroll
   for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[3 * EL + l];
-    sb[3 * EL + l] = oinit ? cur : cur + gam * (sb[3 * EL + l] - (T)2 * cur + x[l]);
-    sa[3 * EL + l] = x[l];
+    const cx<T> cur = sa[at(3, l)];
+    sb[at(3, l)] = oinit ? cur : cur + gam * (sb[at(3, l)] - (T)2 * cur + x[l]);
+    sa[at(3, l)] = x[l];
+  }
+}
+
+// Column sums sum_l dsig_l x_l of the spectral vorticity and divergence of
+// the current level into sa2 fields 1 and 2 (after a state is loaded).
+template <class T>
+__global__ void k_vsum(const cx<T> *__restrict__ sa3, cx<T> *__restrict__ sa2, const T *__restrict__ vc,
+                       int E, int L, int nspec) {
+  const size_t tot = (size_t)nspec * E;
+  const VOff vo{L};
+  const size_t EL = (size_t)E * L;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = q / E, e = q - s * E;
+    cx<T> sv = zero<T>(), sd = zero<T>();
+    for (int l = 0; l < L; ++l) {
+      sv = sv + vc[vo.dsig() + l] * sa3[s * 4 * EL + (size_t)l * E + e];
+      sd = sd + vc[vo.dsig() + l] * sa3[s * 4 * EL + EL + (size_t)l * E + e];
+    }
+    sa2[s * 3 * E + E + e] = sv;
+    sa2[s * 3 * E + 2 * E + e] = sd;
   }
 }
 
@@ -478,7 +563,7 @@ __global__ void k_pack_spec(const double *__restrict__ src, cx<T> *__restrict__ 
        q += (size_t)gridDim.x * blockDim.x) {
     const int col = (int)(q % ((size_t)B * K));
     const int s = (int)(q / ((size_t)B * K));
-    const int b = col / K, k = col - b * K;
+    const int k = col / B, b = col - k * B;  // columns (level, member), member fastest
     int m = 0;
     while (spec_off(m + 1, M) <= s) ++m;
     const int n = m + (s - spec_off(m, M));
@@ -502,7 +587,7 @@ __global__ void k_unpack_spec(const cx<T> *__restrict__ src, double *__restrict_
     double re = 0, im = 0;
     if (n >= m) {
       const int s = spec_off(m, M) + n - m;
-      const cx<T> v = src[((size_t)s * nf + f) * ncols + (size_t)b * K + k];
+      const cx<T> v = src[((size_t)s * nf + f) * ncols + (size_t)k * B + b];
       re = (double)v.x;
       im = (double)v.y;
     }
@@ -556,7 +641,7 @@ __global__ __launch_bounds__(128) void k_basic_terms(
   }
   const double S0 = S;
   // bottom-up: S_k = S_0 - sum_{l<k} dsig (D + A); sdot_k = sigm_k S_0 - S_k
-  double Sk = S0;
+  double Sk = S0, su = 0, sv = 0;
   for (int k = 0; k < L; ++k) {
     const size_t g = col * L + k;
     const double d = vs[(size_t)(L + k) * I];
@@ -574,8 +659,12 @@ __global__ __launch_bounds__(128) void k_basic_terms(
     o[(size_t)(br.x5() + k) * I] =
         vc[vo.w3() + k] * (k + 1 == L ? 0.0 : Sk1) + vc[vo.alpha() + k] * d + vc[vo.wP() + k] * padv;
     o[(size_t)(br.sdot() + k) * I] = sdk;
+    su += vc[vo.dsig() + k] * uu;
+    sv += vc[vo.dsig() + k] * vv;
     Sk = Sk1;
   }
+  o[(size_t)br.sbu() * I] = su;
+  o[(size_t)br.sbv() * I] = sv;
   o[(size_t)(br.sdot() + L) * I] = 0.0;
   o[(size_t)br.pix() * I] = pix;
   o[(size_t)br.piy() * I] = piy;

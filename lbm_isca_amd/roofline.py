@@ -18,19 +18,19 @@ def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):
     EL = E * L
     c = 2 * s
     tables = 2 * nspec * Jh * s
-    cin = 6 * EL + 3 * E           # Fourier/grid columns of the synthesised state
+    cin = 6 * EL + 6 * E           # Fourier/grid columns of the synthesised state
     cout = 9 * EL + 2 * E          # columns of the grid tendencies
-    rb = 9 * L + 4                 # basic-state rows
+    rb = 9 * L + 6                 # basic-state rows
     return {
         "syn_level": (nspec * 4 * EL + M * J * 6 * EL) * c + tables,
-        "syn_surf": (nspec * E + M * J * 3 * E) * c + tables,
+        "syn_surf": (nspec * 3 * E + M * J * 6 * E) * c + tables,
         "ifft": M * J * cin * c + J * cin * I * s,
         "tend": (J * cin * I + nbasic * J * rb * I + nfrc * J * L * I
                  + 4 * E * J * I + J * cout * I) * s,
         "fft": J * cout * I * s + M * J * cout * c,
         "ana_level": M * J * 9 * EL * c + nspec * 4 * EL * c + tables,
         "ana_surf": M * J * 2 * E * c + nspec * 2 * E * c + tables,
-        "step": (5 * nspec * 4 * EL + nspec * 2 * E + nspec * E) * c + nbasic * M * L * L * s,
+        "step": (5 * nspec * 4 * EL + nspec * 2 * E + nspec * 3 * E) * c + nbasic * M * L * L * s,
     }
 
 
