@@ -15,6 +15,7 @@
 
 #include "../../include/lbm_hip.h"
 #include "lbm_model.h"
+#include "lbm_mfma.h"
 
 using namespace lbm;
 
@@ -351,8 +352,8 @@ template <class T, class R>
 int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
                hipStream_t st, int in_nf = R::NIN) {
   DevTabs<T> &t = p->tabs<T>();
-  dim3 grid((ncols + 7) / 8, p->M);
-  hipLaunchKernelGGL((k_syn<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
+  dim3 grid((ncols + 7) / 8, p->M, (p->Jh + 31) / 32);
+  hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      t.Ps, t.Hs, t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M, in_nf);
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -362,8 +363,9 @@ template <class T, class R>
 int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
                hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0) {
   DevTabs<T> &t = p->tabs<T>();
-  dim3 grid((ncols + 15) / 16, p->M);
-  hipLaunchKernelGGL((k_ana<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
+  // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
+  dim3 grid((ncols + 7) / 8, p->M, (p->M + 63) / 64);
+  hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      t.Pa, t.Ha, t.nn1a2, ncols, C, sl, p->J, p->M, out_nf, out_f0);
   HIPCHK(hipGetLastError());
   return LBM_OK;

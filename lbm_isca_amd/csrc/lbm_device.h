@@ -273,7 +273,7 @@ struct SynLevel {  // (vor, div, t, q) -> (U, V, vor, div, t, q): state.py:94-99
   static constexpr bool hasB(int o) { return o < 2; }
   static constexpr bool scaled(int o) { return o < 2; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
+  __device__ __forceinline__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
     const cx<T> psi = il * x[0], chi = il * x[1];
     A[0] = im_mul(fm, chi);  // U = P (im chi) - H psi   (spectral.py:271)
     B[0] = -psi;
@@ -293,7 +293,7 @@ struct SynSurf {  // (pi, sum dsig vor, sum dsig div) -> (d/dx pi, d/dy pi, pi, 
   static constexpr bool hasB(int o) { return o == 1 || o == 3 || o == 4; }
   static constexpr bool scaled(int o) { return o < 2 || o == 3 || o == 4; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
+  __device__ __forceinline__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
     A[0] = im_mul(fm, x[0]);
     B[1] = x[0];
     A[2] = x[0];
@@ -311,7 +311,7 @@ struct SynScalar {  // wave_to_grid (spectral.py:193-197)
   static constexpr bool hasB(int) { return false; }
   static constexpr bool scaled(int) { return false; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T, T, cx<T> *A, cx<T> *) { A[0] = x[0]; }
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T, cx<T> *A, cx<T> *) { A[0] = x[0]; }
 };
 struct SynPsiChi {  // psichi_to_uv (spectral.py:262-276)
   static constexpr int NIN = 2, NOUT = 2;
@@ -319,7 +319,7 @@ struct SynPsiChi {  // psichi_to_uv (spectral.py:262-276)
   static constexpr bool hasB(int) { return true; }
   static constexpr bool scaled(int) { return true; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
     A[0] = im_mul(fm, x[1]);
     B[0] = -x[0];
     A[1] = im_mul(fm, x[0]);
@@ -332,7 +332,7 @@ struct SynGrad {  // hgrad (spectral.py:210-225)
   static constexpr bool hasB(int o) { return o == 1; }
   static constexpr bool scaled(int) { return true; }
   template <class T>
-  __device__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
     A[0] = im_mul(fm, x[0]);
     B[1] = x[0];
   }
@@ -410,7 +410,7 @@ struct AnaTend {  // (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t, q): dy
   static constexpr bool needP(int f) { return f != 4 && f != 7; }
   static constexpr bool needH(int f) { return f == 0 || f == 1 || f == 4 || f == 7; }
   template <class T>
-  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T nn1a2, cx<T> *o) {
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T nn1a2, cx<T> *o) {
     o[0] = im_mul(fm, AP[1]) + AH[0];                                   // curl(fu, fv)
     o[1] = im_mul(fm, AP[0]) - AH[1] + nn1a2 * AP[2];                   // div - lap(ke)
     o[2] = AP[5] - im_mul(fm, AP[3]) + AH[4];                           // -div(xu,xv) + gt
@@ -422,7 +422,7 @@ struct AnaState {  // (u', v', t, q) -> (vor, div, t, q): state.py:86-92
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int f) { return f < 2; }
   template <class T>
-  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
     o[0] = im_mul(fm, AP[1]) + AH[0];
     o[1] = im_mul(fm, AP[0]) - AH[1];
     o[2] = AP[2];
@@ -434,7 +434,7 @@ struct AnaVorDiv {  // uv_to_vordiv (spectral.py:227-244)
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int) { return true; }
   template <class T>
-  __device__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
     o[0] = im_mul(fm, AP[1]) + AH[0];
     o[1] = im_mul(fm, AP[0]) - AH[1];
   }
@@ -444,7 +444,7 @@ struct AnaScalar {  // grid_to_wave (spectral.py:186-191)
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int) { return false; }
   template <class T>
-  __device__ static void combine(const cx<T> *AP, const cx<T> *, T, T, cx<T> *o) { o[0] = AP[0]; }
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *, T, T, cx<T> *o) { o[0] = AP[0]; }
 };
 
 struct InSlots {

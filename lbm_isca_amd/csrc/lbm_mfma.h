@@ -1,0 +1,290 @@
+// Legendre analysis and synthesis on the CDNA4 matrix cores.
+//
+// For one zonal wavenumber m both transforms are small dense contractions
+// over latitude (analysis) or degree n (synthesis), folded north/south with
+// P(-mu) = (-1)^(n+m) P(mu), H(-mu) = -(-1)^(n+m) H(mu).  They are computed
+// as 16x16 output tiles with 16x16x4 MFMA steps, real tables times the real
+// and imaginary parts of the complex data as separate columns: a tile covers
+// 8 complex columns, real column j holding part j>>3 (0 re, 1 im) of column
+// j&7.
+//
+// Operand layout of v_mfma_{f64,f32}_16x16x4 (64 lanes):
+//   A[i][k]: lane i + 16k;  B[k][j]: lane j + 16k;
+//   D[i][j] value v of lane l: j = l%16, i = l/16 + 4v (f64) or
+//   4(l/16) + v (f32).
+#pragma once
+#include "lbm_device.h"
+
+namespace lbm {
+
+template <class T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// select between two loaded values (keeps aggregates out of private memory)
+template <class T>
+__device__ __forceinline__ T pick(T re, T im, int part) {
+  return part == 0 ? re : im;
+}
+
+// complex value of real column j of a tile from the lane's own part and the
+// partner lane j^8 (same k-group)
+template <class T>
+__device__ inline cx<T> pair_of(T mine, int part) {
+  const T other = __shfl_xor(mine, 8);
+  return part == 0 ? mk(mine, other) : mk(other, mine);
+}
+
+// ---------------------------------------------------------------------------
+// Analysis.  Block = 4 waves for (parity p = w&1, n-tile t = w>>1) of one m
+// and 8 columns; n = m + p + 2(16t + i); blockIdx.z selects n-tiles 2z,
+// 2z+1 (more than one only when N+1-m > 64).  K = latitude pairs, staged 32
+// at a time per field, double-buffered in LDS.
+// ---------------------------------------------------------------------------
+template <class T, class R>
+__global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
+                                                  cx<T> *__restrict__ out,
+                                                  const T *__restrict__ Pa,
+                                                  const T *__restrict__ Ha,
+                                                  const T *__restrict__ nn1a2, int ncols, int C,
+                                                  InSlots slots, int J, int M, int out_nf,
+                                                  int out_f0) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NIN = R::NIN, NOUT = R::NOUT;
+  __shared__ T sym[2][32][16], anti[2][32][16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int p = w & 1, tile = (w >> 1) + 2 * blockIdx.z;
+  const int m = blockIdx.y, col0 = blockIdx.x * 8;
+  const int Jh = J / 2;
+  const int N = M - 1;
+  const int s0 = spec_off(m, M) - m;
+  const int nch = (Jh + 31) / 32;
+
+  // loader mapping: thread -> (latitude pair lj of the chunk, column lc)
+  const int lj = t >> 3, lc = t & 7;
+  const bool lcol_ok = (col0 + lc) < ncols;
+  int off[NIN];
+#pragma unroll
+  for (int f = 0; f < NIN; ++f) off[f] = slots.off[f];
+  auto stage = [&](int jc, int f, int buf) __attribute__((always_inline)) {
+    const int jp = 32 * jc + lj;
+    cx<T> fn = zero<T>(), fs = zero<T>();
+    if (lcol_ok && jp < Jh) {
+      fn = in[((size_t)m * J + jp) * C + off[f] + col0 + lc];
+      fs = in[((size_t)m * J + (J - 1 - jp)) * C + off[f] + col0 + lc];
+    }
+    sym[buf][lj][lc] = fn.x + fs.x;
+    sym[buf][lj][8 + lc] = fn.y + fs.y;
+    anti[buf][lj][lc] = fn.x - fs.x;
+    anti[buf][lj][8 + lc] = fn.y - fs.y;
+  };
+
+  // A operand lane (i = l%16 -> n row, k = l/16 -> latitude pair); B lane (j = l%16)
+  const int i_a = lane & 15, k_a = lane >> 4;
+  const int n_a = m + p + 2 * (16 * tile + i_a);
+  const bool row_ok = n_a <= N;
+  const bool wave_live = (m + p + 32 * tile) <= N;
+  const int j_b = lane & 15;
+  const size_t trow = (size_t)(s0 + (row_ok ? n_a : m)) * Jh;
+
+  v4 AP[NIN], AH[NIN];
+#pragma unroll
+  for (int f = 0; f < NIN; ++f) AP[f] = AH[f] = (v4){0, 0, 0, 0};
+
+  stage(0, 0, 0);
+  __syncthreads();
+  int it = 0;
+  for (int jc = 0; jc < nch; ++jc) {
+#pragma unroll
+    for (int f = 0; f < NIN; ++f, ++it) {
+      const int buf = it & 1;
+      if (f + 1 < NIN)
+        stage(jc, f + 1, buf ^ 1);
+      else if (jc + 1 < nch)
+        stage(jc + 1, 0, buf ^ 1);
+      if (wave_live) {
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 4) {
+          const int jp = 32 * jc + ks + k_a;
+          const bool ok = row_ok && jp < Jh;
+          const T xs = sym[buf][ks + k_a][j_b], xa = anti[buf][ks + k_a][j_b];
+          if (R::needP(f)) AP[f] = Mfma<T>::mma(ok ? Pa[trow + jp] : (T)0, p == 0 ? xs : xa, AP[f]);
+          if (R::needH(f)) AH[f] = Mfma<T>::mma(ok ? Ha[trow + jp] : (T)0, p == 0 ? xa : xs, AH[f]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (!wave_live) return;
+  // combine and store: lane holds rows i = row(lane, v), column j = lane%16
+  const int c = j_b & 7, part = j_b >> 3;
+  const int col = col0 + c;
+  const T fm = (T)m;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int i = Mfma<T>::row(lane, v);
+    const int n = m + p + 2 * (16 * tile + i);
+    cx<T> ap[NIN], ah[NIN], o[NOUT];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      ap[f] = pair_of<T>(AP[f][v], part);
+      ah[f] = pair_of<T>(AH[f][v], part);
+    }
+    if (n > N || col >= ncols) continue;
+    R::template combine<T>(ap, ah, fm, nn1a2[n], o);
+    const int s = s0 + n;
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      T *dst = reinterpret_cast<T *>(out + ((size_t)s * out_nf + out_f0 + q) * ncols + col);
+      dst[part] = pick<T>(o[q].x, o[q].y, part);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis.  Block = 4 waves: latitude tile rt = w&1 (pairs 32z + 16rt ..
+// +15) and output group g = w>>1 (outputs o with o%2 == g, so each wave
+// holds half of the accumulators), for one m and 8 columns.  K = degree n:
+// 32 consecutive rows n = m + 32c .. are staged per chunk c, each parity p
+// contributing 16 rows n = m + p + 2kk; the recipe terms A_o(n), B_o(n) are
+// formed per lane.
+// ---------------------------------------------------------------------------
+template <class T, class R, int G>
+__device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_chunk, int r0, int nn, int m,
+                                 int s0, int Jh, int jp_a, bool jp_ok, int lane,
+                                 const T *__restrict__ Ps, const T *__restrict__ Hs,
+                                 const T *__restrict__ inv_lap,
+                                 typename Mfma<T>::v4 (&EA)[R::NOUT], typename Mfma<T>::v4 (&OA)[R::NOUT],
+                                 typename Mfma<T>::v4 (&EB)[R::NOUT], typename Mfma<T>::v4 (&OB)[R::NOUT]) {
+  constexpr int NIN = R::NIN, NOUT = R::NOUT;
+  const int k_a = lane >> 4, j_b = lane & 15;
+  const int c_b = j_b & 7, part_b = j_b >> 3;
+  const T fm = (T)m;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      const int ra = p + 2 * (ks + k_a);
+      const bool a_ok = jp_ok && (r0 + ra) < nn;
+      const size_t ti = (size_t)(s0 + (a_ok ? m + r0 + ra : m)) * Jh + (jp_ok ? jp_a : 0);
+      const T pa = a_ok ? Ps[ti] : (T)0;
+      const T ha = a_ok ? Hs[ti] : (T)0;
+      const int rb = p + 2 * (ks + (lane >> 4));
+      cx<T> x[NIN], A[NOUT], B[NOUT];
+#pragma unroll
+      for (int o = 0; o < NOUT; ++o) A[o] = B[o] = zero<T>();
+      if (r0 + rb < nn) {
+#pragma unroll
+        for (int f = 0; f < NIN; ++f) x[f] = xin[f][rb][c_b];
+        R::template terms<T>(x, inv_lap[m + r0 + rb], fm, A, B);
+      }
+#pragma unroll
+      for (int o = G; o < NOUT; o += 2) {
+        if (R::hasA(o)) {
+          const T b = pick<T>(A[o].x, A[o].y, part_b);
+          if (p == 0)
+            EA[o] = Mfma<T>::mma(pa, b, EA[o]);
+          else
+            OA[o] = Mfma<T>::mma(pa, b, OA[o]);
+        }
+        if (R::hasB(o)) {
+          const T b = pick<T>(B[o].x, B[o].y, part_b);
+          if (p == 0)
+            EB[o] = Mfma<T>::mma(ha, b, EB[o]);
+          else
+            OB[o] = Mfma<T>::mma(ha, b, OB[o]);
+        }
+      }
+    }
+  }
+  (void)first_chunk;
+}
+
+template <class T, class R, int G>
+__device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
+                                const T *__restrict__ Ps, const T *__restrict__ Hs,
+                                const T *__restrict__ inv_lap, const T *__restrict__ rcos, int ncols,
+                                int C, const OutSlots &slots, int J, int M, int in_nf,
+                                cx<T> (*xin)[32][8]) {
+  typedef typename Mfma<T>::v4 v4;
+  constexpr int NIN = R::NIN, NOUT = R::NOUT;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int rt = w & 1;
+  const int m = blockIdx.y, col0 = blockIdx.x * 8;
+  const int Jh = J / 2;
+  const int nn = M - m;
+  const int s0 = spec_off(m, M) - m;
+  const int jp_a = 32 * blockIdx.z + 16 * rt + (lane & 15);
+  const bool jp_ok = jp_a < Jh;
+  v4 EA[NOUT], OA[NOUT], EB[NOUT], OB[NOUT];
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) EA[o] = OA[o] = EB[o] = OB[o] = (v4){0, 0, 0, 0};
+  for (int r0 = 0; r0 < nn; r0 += 32) {
+    __syncthreads();
+    for (int q = t; q < 32 * 8; q += 256) {
+      const int r = q >> 3, c = q & 7;
+      const bool ok = (col0 + c) < ncols && (r0 + r) < nn;
+#pragma unroll
+      for (int f = 0; f < NIN; ++f)
+        xin[f][r][c] = ok ? in[((size_t)(s0 + m + r0 + r) * in_nf + f) * ncols + col0 + c] : zero<T>();
+    }
+    __syncthreads();
+    syn_group<T, R, G>(xin, r0 == 0, r0, nn, m, s0, Jh, jp_a, jp_ok, lane, Ps, Hs, inv_lap, EA, OA,
+                       EB, OB);
+  }
+  const int j_b = lane & 15;
+  const int c_b = j_b & 7, part_b = j_b >> 3;
+  const int col = col0 + c_b;
+  if (col >= ncols) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int jp = 32 * blockIdx.z + 16 * rt + Mfma<T>::row(lane, v);
+    if (jp >= Jh) continue;
+    const T sc = rcos[jp];
+#pragma unroll
+    for (int o = G; o < NOUT; o += 2) {
+      T vn = EA[o][v] + OA[o][v] + EB[o][v] + OB[o][v];
+      T vs = EA[o][v] - OA[o][v] + OB[o][v] - EB[o][v];
+      if (R::scaled(o)) {
+        vn *= sc;
+        vs *= sc;
+      }
+      reinterpret_cast<T *>(out + ((size_t)m * J + jp) * C + slots.off[o] + col)[part_b] = vn;
+      reinterpret_cast<T *>(out + ((size_t)m * J + (J - 1 - jp)) * C + slots.off[o] + col)[part_b] = vs;
+    }
+  }
+}
+
+template <class T, class R>
+__global__ __launch_bounds__(256) void k_syn_mfma(const cx<T> *__restrict__ in,
+                                                  cx<T> *__restrict__ out,
+                                                  const T *__restrict__ Ps,
+                                                  const T *__restrict__ Hs,
+                                                  const T *__restrict__ inv_lap,
+                                                  const T *__restrict__ rcos, int ncols, int C,
+                                                  OutSlots slots, int J, int M, int in_nf) {
+  __shared__ cx<T> xin[R::NIN][32][8];
+  if ((threadIdx.x >> 7) == 0)
+    syn_body<T, R, 0>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+  else
+    syn_body<T, R, 1>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+}
+
+}  // namespace lbm
