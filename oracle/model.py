@@ -40,6 +40,11 @@ class OracleConfig:
     vdifq: float = None
     robert: float = 0.04
     strictly_linear: bool = False
+    # linear mode: strictly linear tendencies and the linearised change of
+    # variable ps' = ps_bar * pi' (the tangent-linear model about a zero
+    # perturbation); not a reference option, used to pin the adjoint's
+    # forward model
+    linear: bool = False
 
     @property
     def steps_per_day(self):
@@ -47,7 +52,12 @@ class OracleConfig:
 
 
 class GridState:
-    """(u, v, t, q) (E, J, I, L) and ps' (E, J, I) perturbations (``state.py:26-71``)."""
+    """(u, v, t, q) (E, J, I, L) and ps' (E, J, I) perturbations (``state.py:26-71``).
+
+    ``linear=True`` replaces the exact pi' <-> ps' pair by its linearisation.
+    """
+
+    linear = False
 
     def __init__(self, ps_bar, nmem, nlev):
         self.ps_bar = np.asarray(ps_bar, dtype=np.float64)
@@ -63,10 +73,15 @@ class GridState:
 
     def lnps_pert(self):
         """pi' = ln(ps_bar + ps') - ln(ps_bar) (``state.py:77-79``)."""
+        if self.linear:
+            return self.ps / self.ps_bar
         return np.log(self.ps_bar + self.ps) - self.lnps_bar
 
     def set_from_lnps(self, pi):
         """ps' = ps_bar (exp(pi') - 1) (``state.py:81-83``)."""
+        if self.linear:
+            self.ps = self.ps_bar * pi
+            return
         self.ps = self.ps_bar * (np.exp(pi) - 1.0)
 
     def to_spectral(self, sht):
@@ -114,6 +129,8 @@ class OracleModel:
         self.nmem = nmem
         mk = lambda: GridState(bs.ps, nmem, self.vg.kmax)  # noqa: E731
         self.ga, self.gb, self.gc = mk(), mk(), mk()
+        for g in (self.ga, self.gb, self.gc):
+            g.linear = c.linear
         self.time = 0.0
         self.last_diag = None
 
@@ -130,7 +147,7 @@ class OracleModel:
 
     def step(self, dt, oinit, frc_t=None):
         tend, diag = tendencies(self.sht, self.vg, self.bst, self.ga, frc_t=frc_t,
-                                strictly_linear=self.cfg.strictly_linear)
+                                strictly_linear=self.cfg.strictly_linear or self.cfg.linear)
         self.last_diag = diag
         prev = self.gb.to_spectral(self.sht)
         new = self.stepper.step(dt, prev, tend)
