@@ -65,6 +65,13 @@ typedef struct {
   const double *h_kv, *h_kd, *h_ktt, *h_kq; /* each (nmax+1) x nlev          */
   /* vertical diffusion nu per level in s^-1, NULL = off (integrator.py:243) */
   const double *h_vdif_vor, *h_vdif_div, *h_vdif_t, *h_vdif_q;
+  /* 1: the tangent-linear model about a zero perturbation -- strictly linear
+   * tendencies and the linearised pair ps' = ps_bar pi', so the state is
+   * the spectral (vor, div, t, q, pi) pair of time levels and the model is
+   * exactly linear; required by lbm_adjoint_run.  Not a reference option
+   * (the reference's strictly_linear keeps ps' = ps_bar (exp(pi') - 1),
+   * state.py:77-83). */
+  int32_t tangent_linear;
 } lbm_desc;
 
 typedef struct lbm_plan lbm_plan;
@@ -121,6 +128,16 @@ int lbm_set_forcing(lbm_plan *plan, const double *d_frc_t, int nfrc, void *strea
 int lbm_set_state(lbm_plan *plan, const double *d_u, const double *d_v, const double *d_t,
                   const double *d_q, const double *d_ps, void *stream);
 
+/* Spectral initial perturbation for BOTH time levels: d_vor, d_div, d_t,
+ * d_q (nmem,M,M,L) and d_pi = ln ps perturbation (nmem,M,M), complex128;
+ * NULL = zero.  The spectral form of set_initial_perturbation
+ * (model.py:142-148) followed by State.to_spectral (state.py:86-92): the
+ * state is taken as band-limited, so no raw grid step follows.  Resets the
+ * clock and the response sum. */
+int lbm_set_spectral_state(lbm_plan *plan, const double *d_vor, const double *d_div,
+                           const double *d_t, const double *d_q, const double *d_pi,
+                           void *stream);
+
 /* Per-member number of full steps (excluding start-up), NULL = all equal to
  * nsteps.  Members with fewer steps stop advancing when they reach theirs. */
 int lbm_set_lengths(lbm_plan *plan, const int32_t *h_nsteps, void *stream);
@@ -155,6 +172,30 @@ int lbm_steps_taken(const lbm_plan *plan, int64_t *out);
  * accumulated milliseconds and launch counts and resets them. */
 int lbm_profile(lbm_plan *plan, int enable);
 int lbm_profile_read(lbm_plan *plan, double *h_ms, int64_t *h_count, int n);
+
+/* Linear response (no counterpart in the reference, which has no adjoint):
+ *   J_e = sum_k h_wstep[k] sum_l h_wlev[l] sum_{j,i} h_wgrid[j,i] psi_l(j,i)
+ * over the states after step k = 0..nstep-1 of the next lbm_run (start-up
+ * steps count), for each member e while it is still advancing; psi_l is the
+ * streamfunction (inverse Laplacian of the vorticity) of level l on the
+ * grid.  h_wgrid (nlat,nlon), h_wlev (nlev), h_wstep (nstep) are host
+ * arrays; h_wgrid = NULL removes the response.  lbm_run accumulates J in
+ * FP64; lbm_get_response copies the nmem values to h_j. */
+int lbm_set_response(lbm_plan *plan, const double *h_wgrid, const double *h_wlev,
+                     const double *h_wstep, int nstep);
+int lbm_get_response(lbm_plan *plan, double *h_j);
+
+/* Adjoint of the tangent-linear run (no counterpart in the reference): one
+ * backward sweep over the schedule lbm_run(plan, nsteps, ...) follows
+ * (start-up half steps, per-member lengths) gives, for every member e, the
+ * gradient of J_e of lbm_set_response with respect to that member's
+ * spectral initial state of lbm_set_spectral_state (d_gvor, d_gdiv, d_gt,
+ * d_gq (nmem,M,M,L), d_gpi (nmem,M,M); complex128 dJ/dRe + i dJ/dIm) and
+ * to its temperature forcing (d_gfrc (nmem,J,I,L), K/s).  Any output may be
+ * NULL.  Needs tangent_linear = 1 and a response; the forward state is left
+ * untouched (the model is linear, so no trajectory is stored). */
+int lbm_adjoint_run(lbm_plan *plan, int nsteps, double *d_gvor, double *d_gdiv, double *d_gt,
+                    double *d_gq, double *d_gpi, double *d_gfrc, int use_graph, void *stream);
 
 /* Copies the last error message into buf (always NUL terminated). */
 int lbm_last_error(char *buf, int64_t len);

@@ -27,6 +27,7 @@ EXPORTS = (
     "lbm_hgrad", "lbm_set_basic_state", "lbm_set_forcing", "lbm_set_state",
     "lbm_set_lengths", "lbm_run", "lbm_get_state", "lbm_get_spectral",
     "lbm_steps_taken", "lbm_profile", "lbm_profile_read", "lbm_last_error",
+    "lbm_set_spectral_state", "lbm_set_response", "lbm_get_response", "lbm_adjoint_run",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -46,6 +47,7 @@ class LbmDesc(ctypes.Structure):
         ("h_sigm", _dp),
         ("h_kv", _dp), ("h_kd", _dp), ("h_ktt", _dp), ("h_kq", _dp),
         ("h_vdif_vor", _dp), ("h_vdif_div", _dp), ("h_vdif_t", _dp), ("h_vdif_q", _dp),
+        ("tangent_linear", ctypes.c_int32),
     ]
 
 
@@ -83,6 +85,10 @@ def load():
         "lbm_profile": ([P, ctypes.c_int], ctypes.c_int),
         "lbm_profile_read": ([P, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int], ctypes.c_int),
         "lbm_last_error": ([ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
+        "lbm_set_spectral_state": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_set_response": ([P, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
+        "lbm_get_response": ([P, _dp], ctypes.c_int),
+        "lbm_adjoint_run": ([P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,575 @@
+// Adjoint of the linear time step: strictly linear tendencies and the
+// linearised pi' <-> ps' pair, so the state is purely spectral and the
+// Robert-Asselin filter acts on all five fields.  Every stage of the forward
+// step is real-linear; each kernel here is its exact transpose for the
+// Euclidean inner product of the stored reals (re/im of spectral and Fourier
+// coefficients, grid values):
+//   spectral step      -> k_step_adj  (M^T, V^T, tmtx^T, WW^T, RA^T)
+//   Legendre analysis  -> k_syn_mfma with the analysis tables (gw P, gw H)
+//   forward FFT (/N)   -> inverse FFT of X_m (1 or 1/2)/N
+//   grid tendencies    -> k_tend_adj  (levels top-down, suffix sums)
+//   inverse FFT        -> forward FFT times N (1 or 2)
+//   Legendre synthesis -> k_ana_mfma with the synthesis tables (P, H)
+// (i m)^T = -(i m); the FFT factors are folded into the Legendre recipes and
+// the 1/(a cos lat) factors of the vector fields into the grid kernels.
+#pragma once
+#include "lbm_mfma.h"
+#include "lbm_model.h"
+
+namespace lbm {
+
+// transpose of rfft/nlon seen through the real inverse transform
+template <class T>
+__device__ __forceinline__ T fft_adj_scale(T fm) {
+  return (fm == (T)0 ? (T)1 : (T)0.5) / (T)FFT_I;
+}
+// transpose of the real inverse transform seen through rfft/nlon
+template <class T>
+__device__ __forceinline__ T ifft_adj_scale(T fm) {
+  return (fm == (T)0 ? (T)1 : (T)2) * (T)FFT_I;
+}
+
+// ---------------------------------------------------------------------------
+// Synthesis-kernel recipes (run with the analysis tables gw P, gw H and the
+// per-degree table n(n+1)/a^2).
+// ---------------------------------------------------------------------------
+// transpose of AnaTend: (vor, div, t, q)* -> Fourier (fu' fv' ke xu' xv' gt
+// yu' yv' g)*
+struct AdjTend {
+  static constexpr int NIN = 4, NOUT = 9;
+  static constexpr bool hasA(int o) { return o != 4 && o != 7; }
+  static constexpr bool hasB(int o) { return o == 0 || o == 1 || o == 4 || o == 7; }
+  static constexpr bool scaled(int) { return false; }
+  template <class T>
+  __device__ __forceinline__ static void terms(const cx<T> *x, T nn1a2, T fm, cx<T> *A, cx<T> *B) {
+    const T sc = fft_adj_scale<T>(fm);
+    const cx<T> zv = sc * x[0], zd = sc * x[1], zt = sc * x[2], zq = sc * x[3];
+    A[0] = im_mul(-fm, zd);  // fu': div = i m AP[fu'] + ...
+    B[0] = zv;               //      vor = AH[fu'] + ...
+    A[1] = im_mul(-fm, zv);  // fv': vor = i m AP[fv'] + ...
+    B[1] = -zd;              //      div = -AH[fv'] + ...
+    A[2] = nn1a2 * zd;       // ke
+    A[3] = im_mul(fm, zt);   // xu': t = -i m AP[xu'] + ...
+    B[4] = zt;               // xv'
+    A[5] = zt;               // gt
+    A[6] = im_mul(fm, zq);   // yu'
+    B[7] = zq;               // yv'
+    A[8] = zq;               // g
+  }
+};
+// transpose of the surface analysis of gt_pi
+struct AdjScalarFft {
+  static constexpr int NIN = 1, NOUT = 1;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int) { return false; }
+  static constexpr bool scaled(int) { return false; }
+  template <class T>
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *) {
+    A[0] = fft_adj_scale<T>(fm) * x[0];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Analysis-kernel recipes (run with the synthesis tables P, H and the
+// per-degree table -a^2/(n(n+1)), 0 at n = 0).
+// ---------------------------------------------------------------------------
+// transpose of SynLevel: Fourier (U V vor div T Q)* -> (vor, div, t, q)*,
+// added to the state adjoint
+struct AdjSynLevel {
+  static constexpr int NIN = 6, NOUT = 4;
+  static constexpr bool ACC = true;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int f) { return f < 2; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T il,
+                                                 cx<T> *o) {
+    const T sc = ifft_adj_scale<T>(fm);
+    // U = P (i m il div) - H (il vor),  V = P (i m il vor) + H (il div)
+    o[0] = sc * ((-il) * AH[0] + il * im_mul(-fm, AP[1]) + AP[2]);
+    o[1] = sc * (il * im_mul(-fm, AP[0]) + il * AH[1] + AP[3]);
+    o[2] = sc * AP[4];
+    o[3] = sc * AP[5];
+  }
+};
+// transpose of SynSurf: (d/dx pi, d/dy pi, pi, sum u, sum v, sum D)* ->
+// (pi, sum vor, sum div)*
+struct AdjSynSurf {
+  static constexpr int NIN = 6, NOUT = 3;
+  static constexpr bool ACC = false;
+  static constexpr bool needP(int f) { return f != 1; }
+  static constexpr bool needH(int f) { return f == 1 || f == 3 || f == 4; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T il,
+                                                 cx<T> *o) {
+    const T sc = ifft_adj_scale<T>(fm);
+    o[0] = sc * (im_mul(-fm, AP[0]) + AH[1] + AP[2]);
+    o[1] = sc * ((-il) * AH[3] + il * im_mul(-fm, AP[4]));
+    o[2] = sc * (il * im_mul(-fm, AP[3]) + il * AH[4] + AP[5]);
+  }
+};
+// transpose of vorticity -> streamfunction on the grid (inverse Laplacian,
+// then wave_to_grid): grid weights -> the linear functional of vorticity
+struct AdjPsi {
+  static constexpr int NIN = 1, NOUT = 1;
+  static constexpr bool ACC = false;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int) { return false; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *, T fm, T il,
+                                                 cx<T> *o) {
+    o[0] = (ifft_adj_scale<T>(fm) * il) * AP[0];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Transposed grid tendencies of the linear model.  gout* rows: fu' fv' ke
+// xu' xv' gt yu' yv' g (L each), gt_pi, (pi_prev, not read).  gin* rows: U V
+// vor div T Q (L each), d/dx pi, d/dy pi, pi, sum u, sum v, sum D, with the
+// 1/(a cos lat) factor of the scaled synthesis outputs applied.  The forcing
+// adjoint accumulates gt* ([e][j][L][I]).
+// ---------------------------------------------------------------------------
+template <class T>
+struct TendAdjArgs {
+  const T *gout_a;  // [e][j][9L+2][I]
+  T *gin_a;         // [e][j][6L+6][I]
+  T *frc_a;         // [e][j][L][I], accumulated
+  const T *bsg;     // [eb][j][9L+6][I]
+  const T *vc, *tref, *rcos;
+  int E, J, I, L, nbasic;
+};
+
+template <class T>
+__global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
+  const int i = threadIdx.x;
+  const int e = blockIdx.x / a.J, j = blockIdx.x - e * a.J;
+  const int L = a.L, I = a.I, J = a.J;
+  const int eb = a.nbasic > 1 ? e : 0;
+  const VOff vo{L};
+  const BRow br{L};
+  const T *vc = a.vc;
+  const T *tref = a.tref + (size_t)eb * L;
+  const T *bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
+  const T *go = a.gout_a + (size_t)(e * J + j) * (9 * L + 2) * I + i;
+  T *gi = a.gin_a + (size_t)(e * J + j) * (6 * L + 6) * I + i;
+  T *fg = a.frc_a + (size_t)(e * J + j) * L * I + i;
+  auto B = [&](int row) { return bs[(size_t)row * I]; };
+  auto O = [&](int f, int k) { return go[(size_t)(f * L + k) * I]; };
+  const T rc = a.rcos[j];
+  const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY, h = (T)0.5;
+  const T bpix = B(br.pix()), bpiy = B(br.piy());
+
+  // adjoints of the four tendencies carrying vertical advection (u, v, T, q)
+  auto W = [&](int k, T *w) {
+    w[0] = O(0, k) * rc;
+    w[1] = O(1, k) * rc;
+    w[2] = O(5, k);
+    w[3] = O(8, k);
+  };
+  T Pst = 0, PDst = 0, SKE = 0, S0st = 0, D0st = 0, pixst = 0, piyst = 0;
+  T wk[4], wn[4] = {0, 0, 0, 0}, wp[4] = {0, 0, 0, 0};
+  W(L - 1, wk);
+  for (int k = L - 1; k >= 0; --k) {
+    if (k > 0) W(k - 1, wp);
+    const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
+    const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
+    const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
+    const T tr = tref[k];
+    const T bu = B(br.u() + k), bv = B(br.v() + k), bt = B(br.t() + k), bq = B(br.q() + k);
+    const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
+    const T bx5 = B(br.x5() + k);
+    const T bsd_lo = B(br.sdot() + k), bsd_hi = B(br.sdot() + k + 1);
+    const bool top = (k + 1 == L);
+    const T fu_a = wk[0], fv_a = wk[1], gt_a = wk[2], g_a = wk[3];
+    const T ke_a = O(2, k), xu_a = O(3, k) * rc, xv_a = O(4, k) * rc;
+    const T yu_a = O(6, k) * rc, yv_a = O(7, k) * rc;
+    T up_a = 0, vp_a = 0, vor_a = 0, d_a = 0, tp_a = 0, qp_a = 0, tv_a = 0, padv_a = 0;
+    const T tdev = bt - tr, tvdev = btv - tr;
+    // temperature: adiabatic term -kappa (tv x5 + tv_bar x5'), energy
+    // conversion kappa Tr (alpha D + w3 (D_0 - PD_{k+1})), advection
+    const T x5p_a = -kap * btv * gt_a;
+    tv_a += -kap * bx5 * gt_a;
+    d_a += kap * tr * alpha * gt_a;
+    D0st += kap * tr * w3 * gt_a;
+    const T PD_a = PDst - kap * tr * w3 * gt_a;  // adjoint of PD_{k+1}
+    tp_a += bdiv * gt_a;
+    d_a += tdev * gt_a;
+    d_a += alpha * x5p_a;
+    padv_a += wP * x5p_a;
+    const T S_a = w3 * x5p_a;  // adjoint of S_{k+1} = S_0 - P_{k+1}
+    up_a += tdev * xu_a;
+    tp_a += bu * xu_a;
+    vp_a += tdev * xv_a;
+    tp_a += bv * xv_a;
+    // momentum
+    vp_a += babsv * fu_a;
+    vor_a += bv * fu_a;
+    tv_a += -rcpg * bpix * fu_a;
+    pixst += -rcpg * tvdev * fu_a;
+    up_a += -babsv * fv_a;
+    vor_a += -bu * fv_a;
+    tv_a += -rcpg * bpiy * fv_a;
+    piyst += -rcpg * tvdev * fv_a;
+    // kinetic energy u_bar u' + v_bar v' and the virtual geopotential
+    up_a += bu * ke_a;
+    vp_a += bv * ke_a;
+    const T x_a = rgas * alpha * ke_a + rgas * dlog3 * SKE;
+    SKE += ke_a;
+    tv_a += x_a;
+    tp_a -= x_a;
+    // humidity
+    up_a += bq * yu_a;
+    qp_a += bu * yu_a;
+    vp_a += bq * yv_a;
+    qp_a += bv * yv_a;
+    qp_a += bdiv * g_a;
+    d_a += bq * g_a;
+    // virtual temperature tv' = t' + eps (t' q_bar + t_bar q')
+    tp_a += ((T)1 + eps * bq) * tv_a;
+    qp_a += eps * bt * tv_a;
+    // vertical advection of the perturbation by the basic sigma-dot: level
+    // k reads f_{k-1}, f_k, f_{k+1}; the bottom and top fluxes are zero
+    {
+      T *fa[4] = {&up_a, &vp_a, &tp_a, &qp_a};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        T acc = -wk[q] * (bsd_hi - bsd_lo) * rds;
+        if (!top) acc += h * bsd_hi * (wk[q] * rds - wn[q] / vc[vo.dsig() + k + 1]);
+        if (k > 0) acc += h * bsd_lo * (wp[q] / vc[vo.dsig() + k - 1] - wk[q] * rds);
+        *fa[q] += acc;
+      }
+    }
+    // perturbation sigma-dot at half level k+1 advects the basic fields of
+    // levels k and k+1
+    T sd_a = 0;
+    if (!top) {
+      const T bun = B(br.u() + k + 1), bvn = B(br.v() + k + 1);
+      const T btn = B(br.t() + k + 1), bqn = B(br.q() + k + 1);
+      const T r1 = (T)1 / vc[vo.dsig() + k + 1];
+      sd_a = h * ((bun - bu) * (wk[0] * rds + wn[0] * r1) + (bvn - bv) * (wk[1] * rds + wn[1] * r1) +
+                  (btn - bt) * (wk[2] * rds + wn[2] * r1) + (bqn - bq) * (wk[3] * rds + wn[3] * r1));
+      S0st += -((T)1 - vc[vo.sigm() + k + 1]) * sd_a;
+    }
+    // P_{k+1} = P_k + dsig (D + A'), PD_{k+1} = PD_k + dsig D
+    const T P_a = Pst + sd_a - S_a;
+    S0st += S_a;
+    d_a += ds * (P_a + PD_a);
+    padv_a += ds * P_a;
+    Pst = P_a;
+    PDst = PD_a;
+    // A' = u' d/dx ln ps_bar + u_bar d/dx pi' + v' d/dy ln ps_bar + v_bar d/dy pi'
+    up_a += bpix * padv_a;
+    pixst += bu * padv_a;
+    vp_a += bpiy * padv_a;
+    piyst += bv * padv_a;
+
+    gi[(size_t)(0 * L + k) * I] = up_a * rc;
+    gi[(size_t)(1 * L + k) * I] = vp_a * rc;
+    gi[(size_t)(2 * L + k) * I] = vor_a;
+    gi[(size_t)(3 * L + k) * I] = d_a;
+    gi[(size_t)(4 * L + k) * I] = tp_a;
+    gi[(size_t)(5 * L + k) * I] = qp_a;
+    fg[(size_t)k * I] += gt_a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      wn[q] = wk[q];
+      wk[q] = wp[q];
+    }
+  }
+  // gt_pi = D_0 - S_0;  S_0 = sum D + bpix sum u + bpiy sum v + pix sbu + piy sbv
+  const T gpi_a = go[(size_t)(9 * L) * I];
+  D0st += gpi_a;
+  S0st -= gpi_a;
+  pixst += B(br.sbu()) * S0st;
+  piyst += B(br.sbv()) * S0st;
+  gi[(size_t)(6 * L) * I] = pixst * rc;
+  gi[(size_t)(6 * L + 1) * I] = piyst * rc;
+  gi[(size_t)(6 * L + 2) * I] = 0;
+  gi[(size_t)(6 * L + 3) * I] = bpix * S0st * rc;
+  gi[(size_t)(6 * L + 4) * I] = bpiy * S0st * rc;
+  gi[(size_t)(6 * L + 5) * I] = S0st + D0st;
+}
+
+// ---------------------------------------------------------------------------
+// Transposed spectral step.  On entry aa3/aa2 and ab3/ab2 hold the adjoints
+// of the state after the step (Sa_new, Sb_new); on exit those of the state
+// before it.  Also folded in: the synthesis adjoint of the column sums and
+// pi from the step after this one (s3a: pi*, (sum dsig vor)*, (sum dsig
+// div)*), and the response weight of the state after this step.  The
+// tendency adjoints go to at3 ([s][4][L][E]) and at2 ([s][2][E], field 0).
+// ---------------------------------------------------------------------------
+template <class T>
+struct StepAdjArgs {
+  cx<T> *aa3, *ab3;  // [s][4][L][E]
+  cx<T> *aa2, *ab2;  // [s][E] pi
+  const cx<T> *s3a;  // [s][3][E]
+  cx<T> *at3, *at2;
+  const T *kv, *kd, *ktt, *kq, *minv, *vinv, *vc, *tref, *lap;
+  const short *n_of_s;
+  const cx<T> *obs;  // [nspec] response functional of the vorticity, or null
+  const T *wlev;     // [L] response weight per level
+  const T *wstep;    // [nwstep] response weight per step
+  int nwstep;
+  const Ctl *ctl;
+  const int *nsteps;
+  int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  T delt, robert;
+};
+
+template <class T, int L>
+__device__ inline void matvec_t(const T *__restrict__ A, const cx<T> *x, cx<T> *y) {
+#pragma unroll
+  for (int c = 0; c < L; ++c) {
+    cx<T> acc = zero<T>();
+#pragma unroll
+    for (int r = 0; r < L; ++r) fma_acc(acc, A[r * L + c], x[r]);
+    y[c] = acc;
+  }
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  const int E = a.E;
+  const int s = idx / E, e = idx - s * E;
+  if (s >= a.nspec) return;
+  const int st = a.ctl->step;
+  const size_t EL = (size_t)E * L;
+  const size_t b3 = (size_t)s * 4 * EL + e;
+  cx<T> *A3 = a.aa3 + b3, *B3 = a.ab3 + b3, *T3 = a.at3 + b3;
+  auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
+  const VOff vo{L};
+  const T *vc = a.vc;
+  const size_t i2 = (size_t)s * E + e;
+  const cx<T> spi = a.s3a[(size_t)s * 3 * E + e];
+  const cx<T> svor = a.s3a[(size_t)s * 3 * E + E + e];
+  const cx<T> sdiv = a.s3a[(size_t)s * 3 * E + 2 * E + e];
+  cx<T> *at2 = a.at2 + (size_t)s * 2 * E + e;
+  const bool active = st < a.inistp + a.nsteps[e];
+  if (!active) {
+    // a member past its length: the step is the identity
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      A3[at(0, l)] = A3[at(0, l)] + vc[vo.dsig() + l] * svor;
+      A3[at(1, l)] = A3[at(1, l)] + vc[vo.dsig() + l] * sdiv;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) T3[at(f, l)] = zero<T>();
+    }
+    a.aa2[i2] = a.aa2[i2] + spi;
+    *at2 = zero<T>();
+    return;
+  }
+  const bool oinit = st < a.inistp;
+  const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
+  const int set = oinit ? st : a.inistp;
+  const T dt2 = (T)2 * dt;
+  const int n = a.n_of_s[s];
+  const int eb = a.nbasic > 1 ? e : 0;
+  const T *tref = a.tref + (size_t)eb * L;
+  const T *ktt = a.ktt + (size_t)n * L, *kd = a.kd + (size_t)n * L;
+  const T *kv = a.kv + (size_t)n * L, *kq = a.kq + (size_t)n * L;
+  const T lapn = a.lap[n];
+  const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
+  const T *vinv = a.vinv + (size_t)set * 4 * L * L;
+  const bool vd_low = !(a.skip_low_n && n <= 1);
+  const bool resp = a.obs != nullptr;
+  const T wr = (resp && st < a.nwstep) ? a.wstep[st] : (T)0;
+  const cx<T> ob = resp ? a.obs[s] : zero<T>();
+  // Robert-Asselin with the start-up assignment:
+  //   Sb_new = oinit ? Sa : (1 - 2g) Sa + g Sb + g S,  Sa_new = S
+  const T ca = oinit ? (T)1 : (T)1 - (T)2 * gam;
+  const T cb = oinit ? (T)0 : gam;
+  const T cn = oinit ? (T)0 : gam;
+  cx<T> x[L], y[L], dbar[L];
+
+  // vorticity: vor = Vv (vor_b + 2dt wt_vor) / (1 + 2dt kv)
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+    x[l] = A3[at(0, l)] + vc[vo.dsig() + l] * svor + (resp ? wr * a.wlev[l] : (T)0) * ob +
+           cn * B3[at(0, l)];
+  if ((a.vmask & 1) && vd_low) {
+    matvec_t<T, L>(vinv, x, y);
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = x[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> yl = ((T)1 / ((T)1 + dt2 * kv[l])) * y[l];
+    const cx<T> bb = B3[at(0, l)];
+    B3[at(0, l)] = cb * bb + yl;
+    A3[at(0, l)] = ca * bb;
+    T3[at(0, l)] = dt2 * yl;
+  }
+  // humidity
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = A3[at(3, l)] + cn * B3[at(3, l)];
+  if (a.vmask & 8) {
+    matvec_t<T, L>(vinv + 3 * L * L, x, y);
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = x[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> yl = ((T)1 / ((T)1 + dt2 * kq[l])) * y[l];
+    const cx<T> bb = B3[at(3, l)];
+    B3[at(3, l)] = cb * bb + yl;
+    A3[at(3, l)] = ca * bb;
+    T3[at(3, l)] = dt2 * yl;
+  }
+  // pi: pi = pi_b + 2dt (wt_pi - sum dsig dbar)
+  const cx<T> bpi = a.ab2[i2];
+  const cx<T> pin_a = a.aa2[i2] + spi + cn * bpi;
+  a.aa2[i2] = ca * bpi;
+  cx<T> pib_a = cb * bpi + pin_a;
+  // divergence: D = Vd (2 dbar - D_b)
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = A3[at(1, l)] + vc[vo.dsig() + l] * sdiv + cn * B3[at(1, l)];
+  if ((a.vmask & 2) && vd_low) {
+    matvec_t<T, L>(vinv + 1 * L * L, x, y);
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = x[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const cx<T> bb = B3[at(1, l)];
+    A3[at(1, l)] = ca * bb;
+    B3[at(1, l)] = cb * bb - y[l];
+    dbar[l] = (T)2 * y[l];
+  }
+  // temperature: T = Vt (T_b + 2dt (wt_t - tmtx dbar)) / ft
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = A3[at(2, l)] + cn * B3[at(2, l)];
+  if (a.vmask & 4) {
+    matvec_t<T, L>(vinv + 2 * L * L, x, y);
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l) y[l] = x[l];
+  }
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const T ft = (T)1 + dt2 * ktt[l];
+    const cx<T> yl = ((T)1 / ft) * y[l];
+    const cx<T> bb = B3[at(2, l)];
+    A3[at(2, l)] = ca * bb;
+    B3[at(2, l)] = cb * bb + yl;
+    y[l] = dt2 * yl;  // wt_t* so far, and -(tmtx dbar)*
+    T3[at(2, l)] = y[l];
+  }
+  // (tmtx dbar)_k = kappa Tr_k (alpha_k dbar_k + w3_k sum_{l>k} dsig_l dbar_l)
+  {
+    cx<T> pre = zero<T>();
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const cx<T> tt = (kap * tref[l] * vc[vo.alpha() + l]) * y[l] + vc[vo.dsig() + l] * pre;
+      pre = pre + (kap * tref[l] * vc[vo.w3() + l]) * y[l];
+      dbar[l] = dbar[l] - tt;
+    }
+  }
+  const cx<T> wtp = dt2 * pin_a;
+#pragma unroll
+  for (int l = 0; l < L; ++l) dbar[l] = dbar[l] - vc[vo.dsig() + l] * wtp;
+  // dbar = M(n)^-1 rhs,  rhs = (1 + dt kd) ft D_b + dt (ft wt_d - lap phy),
+  // phy_l = phi_l + ft_l R Tr_l cpg_l (pi_b + dt wt_pi)
+  matvec_t<T, L>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, dbar, x);
+  cx<T> wdpix_a = zero<T>();
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const T ft = (T)1 + dt2 * ktt[l];
+    B3[at(1, l)] = B3[at(1, l)] + (((T)1 + dt * kd[l]) * ft) * x[l];
+    T3[at(1, l)] = (dt * ft) * x[l];
+    y[l] = (-dt * lapn) * x[l];
+    wdpix_a = wdpix_a + (ft * (rg * tref[l] * vc[vo.cpg() + l])) * y[l];
+  }
+  pib_a = pib_a + wdpix_a;
+  a.ab2[i2] = pib_a;
+  *at2 = wtp + dt * wdpix_a;
+  // phi_l = sum_{k<l} R dlog3_k w_k + R alpha_l w_l,  w = (1 + dt ktt) T_b + dt wt_t
+  {
+    cx<T> suf = zero<T>();
+#pragma unroll
+    for (int l = L - 1; l >= 0; --l) {
+      const cx<T> w_a = (rg * vc[vo.alpha() + l]) * y[l] + (rg * vc[vo.dlog3() + l]) * suf;
+      suf = suf + y[l];
+      B3[at(2, l)] = B3[at(2, l)] + ((T)1 + dt * ktt[l]) * w_a;
+      T3[at(2, l)] = T3[at(2, l)] + dt * w_a;
+    }
+  }
+}
+
+// Adds the column-sum and pi adjoints s3a into the state adjoint:
+// vor*_l += dsig_l (sum vor)*, div*_l += dsig_l (sum div)*, pi* += pi*.
+template <class T>
+__global__ void k_vsum_adj(const cx<T> *__restrict__ s3a, cx<T> *__restrict__ aa3,
+                           cx<T> *__restrict__ aa2, const T *__restrict__ vc, int E, int L,
+                           int nspec) {
+  const size_t tot = (size_t)nspec * E;
+  const VOff vo{L};
+  const size_t EL = (size_t)E * L;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = q / E, e = q - s * E;
+    const cx<T> pa = s3a[s * 3 * E + e], zv = s3a[s * 3 * E + E + e];
+    const cx<T> zd = s3a[s * 3 * E + 2 * E + e];
+    aa2[s * E + e] = aa2[s * E + e] + pa;
+    for (int l = 0; l < L; ++l) {
+      const T ds = vc[vo.dsig() + l];
+      cx<T> *pv = aa3 + s * 4 * EL + (size_t)l * E + e;
+      pv[0] = pv[0] + ds * zv;
+      pv[EL] = pv[EL] + ds * zd;
+    }
+  }
+}
+
+// Response J_e += w_step sum_l w_l <obs, vor_l> over (re, im) of every
+// coefficient of the state after the step; one block per member, fixed
+// summation order, accumulated in FP64.
+template <class T>
+__global__ __launch_bounds__(256) void k_response(const cx<T> *__restrict__ sa3,
+                                                  const cx<T> *__restrict__ obs,
+                                                  const T *__restrict__ wlev,
+                                                  const T *__restrict__ wstep, int nwstep, int E,
+                                                  int L, int nspec, const Ctl *ctl,
+                                                  const int *nsteps, int inistp,
+                                                  double *__restrict__ J) {
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  const int st = ctl->step;
+  const double w = (st >= 0 && st < nwstep) ? (double)wstep[st] : 0.0;
+  const bool active = st < inistp + nsteps[e];
+  double acc = 0;
+  if (w != 0.0 && active) {
+    const size_t EL = (size_t)E * L;
+    for (int l = 0; l < L; ++l) {
+      const double wl = (double)wlev[l];
+      if (wl == 0.0) continue;
+      double al = 0;
+      for (int s = threadIdx.x; s < nspec; s += 256) {
+        const cx<T> v = sa3[(size_t)s * 4 * EL + (size_t)l * E + e];
+        al += (double)obs[s].x * (double)v.x + (double)obs[s].y * (double)v.y;
+      }
+      acc += wl * al;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J[e] += w * red[0];
+}
+
+// out = a + b over n complex values (the initial Sa and Sb are one state)
+template <class T>
+__global__ void k_add_cx(const cx<T> *__restrict__ a, const cx<T> *__restrict__ b,
+                         cx<T> *__restrict__ out, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    out[q] = a[q] + b[q];
+}
+
+__global__ void k_tick_back(Ctl *ctl) { ctl->step -= 1; }
+
+}  // namespace lbm

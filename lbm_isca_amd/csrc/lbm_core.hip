@@ -14,8 +14,7 @@
 #include <vector>
 
 #include "../../include/lbm_hip.h"
-#include "lbm_model.h"
-#include "lbm_mfma.h"
+#include "lbm_adjoint.h"
 
 using namespace lbm;
 
@@ -206,7 +205,7 @@ inline unsigned nblk(size_t n, unsigned bs = 256) {
 struct lbm_plan {
   // sizes
   int J = 0, I = 0, N = 0, M = 0, L = 0, E = 0, nb = 0, prec = 0, nspec = 0, Jh = 0, nset = 0;
-  int inistp = 2, strictly_linear = 0, skip_low_n = 1, vmask = 0;
+  int inistp = 2, strictly_linear = 0, skip_low_n = 1, vmask = 0, linear = 0;
   double a = 0, delt = 0, robert = 0;
   // host FP64 setup
   std::vector<double> mu, gw, coslat, lat, lon;
@@ -225,6 +224,16 @@ struct lbm_plan {
   void *fin = nullptr, *gin = nullptr, *gout = nullptr, *fout = nullptr;
   void *raw = nullptr, *psa = nullptr, *psb = nullptr, *bsg = nullptr, *frc = nullptr;
   int nfrc = 0;
+  void *sb2 = nullptr;  // [s][3][E] pi at t-dt in tangent-linear mode (field 0)
+  // adjoint sweep: state adjoints of both time levels, synthesis adjoint of
+  // the column sums, forcing adjoint
+  void *aa3 = nullptr, *ab3 = nullptr, *aa2 = nullptr, *ab2 = nullptr, *s3a = nullptr;
+  void *frc_a = nullptr;
+  // linear response: vorticity functional, level and step weights, FP64 sums
+  void *obs = nullptr, *wlev = nullptr, *wstep = nullptr;
+  double *jacc = nullptr;
+  int nwstep = 0, wstep_cap = 0;
+  bool resp_set = false;
   Ctl *ctl = nullptr;
   int *pend = nullptr, *d_nsteps = nullptr;
   std::vector<int> nsteps_h;
@@ -233,8 +242,9 @@ struct lbm_plan {
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   hipStream_t own = nullptr;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  int gsteps[2] = {0, 0};
+  // graphs: 0 forward chunk, 1 forward single step, 2 adjoint chunk, 3 adjoint step
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+  int gsteps[4] = {0, 0, 0, 0};
   // kernel timing (lbm_profile)
   static constexpr int NPROF = 8;
   bool prof = false;
@@ -269,7 +279,7 @@ int scratch_get(lbm_plan *p, size_t bytes, void **out, hipStream_t st) {
 }
 
 void destroy_graphs(lbm_plan *p) {
-  for (int g = 0; g < 2; ++g)
+  for (int g = 0; g < 4; ++g)
     if (p->gexec[g]) {
       (void)hipGraphExecDestroy(p->gexec[g]);
       p->gexec[g] = nullptr;
@@ -348,25 +358,33 @@ int set_twiddles() {
 // ---------------------------------------------------------------------------
 // Transform building blocks on internal buffers.
 // ---------------------------------------------------------------------------
+// Synthesis with the synthesis tables P, H and -a^2/(n(n+1)) by default; the
+// transposed analysis passes the analysis tables instead.
 template <class T, class R>
 int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
-               hipStream_t st, int in_nf = R::NIN) {
+               hipStream_t st, int in_nf = R::NIN, const T *P = nullptr, const T *H = nullptr,
+               const T *per_n = nullptr) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid((ncols + 7) / 8, p->M, (p->Jh + 31) / 32);
   hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     t.Ps, t.Hs, t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M, in_nf);
+                     P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
+                     p->J, p->M, in_nf);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
+// Analysis with the analysis tables gw P, gw H and n(n+1)/a^2 by default;
+// the transposed synthesis passes the synthesis tables instead.
 template <class T, class R>
 int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
-               hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0) {
+               hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0, const T *P = nullptr,
+               const T *H = nullptr, const T *per_n = nullptr) {
   DevTabs<T> &t = p->tabs<T>();
   // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
   dim3 grid((ncols + 7) / 8, p->M, (p->M + 63) / 64);
   hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     t.Pa, t.Ha, t.nn1a2, ncols, C, sl, p->J, p->M, out_nf, out_f0);
+                     P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
+                     out_nf, out_f0);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -714,6 +732,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
     HIPCHK(hipMemsetAsync(p->sa3, 0, s3 * sizeof(cx<T>), st));
     HIPCHK(hipMemsetAsync(p->sb3, 0, s3 * sizeof(cx<T>), st));
     HIPCHK(hipMemsetAsync(p->sa2, 0, s2 * sizeof(cx<T>), st));
+    if (p->sb2) HIPCHK(hipMemsetAsync(p->sb2, 0, s2 * sizeof(cx<T>), st));
     p->have_raw = false;
     p->raw_pending = false;
     return LBM_OK;
@@ -747,7 +766,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
                          f * L, L);
   }
   hipLaunchKernelGGL((k_pack_pi<T>), dim3(nblk(n2)), dim3(256), 0, st, ps, g, E, J, I, cm.rows(),
-                     4 * L, (const double *)p->d_psbar, p->nb);
+                     4 * L, (const double *)p->d_psbar, p->nb, p->linear);
   HIPCHK(hipGetLastError());
   CHK(launch_fft<T>(p, g, F, cm, st));
   {
@@ -760,6 +779,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
                      (const cx<T> *)p->sa3, (cx<T> *)p->sa2, p->tabs<T>().vc, E, L, p->nspec);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(p->sb3, p->sa3, s3 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
+  if (p->sb2) HIPCHK(hipMemcpyAsync(p->sb2, p->sa2, s2 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
   p->have_raw = true;
   p->raw_pending = true;
   return LBM_OK;
@@ -806,7 +826,8 @@ int get_state_t(lbm_plan *p, double *u, double *v, double *t, double *q, double 
   if (ps) {
     const BRow br{L};
     hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->gin, ps, E,
-                       J, I, 1, rows, 6 * L + 2, 1, (const T *)p->bsg, br.rows(), br.psbar(), p->nb);
+                       J, I, 1, rows, 6 * L + 2, p->linear ? 2 : 1, (const T *)p->bsg, br.rows(),
+                       br.psbar(), p->nb);
   }
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -822,6 +843,7 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st) {
   a.sa3 = (cx<T> *)p->sa3;
   a.sa2 = (cx<T> *)p->sa2;
   a.sb3 = (cx<T> *)p->sb3;
+  a.sb2 = (cx<T> *)p->sb2;
   a.tend3 = (const cx<T> *)p->tend3;
   a.tend2 = (const cx<T> *)p->tend2;
   a.kv = t.kv;
@@ -894,7 +916,8 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   ta.L = L;
   ta.nbasic = p->nb;
   ta.nfrc = p->nfrc;
-  ta.strictly_linear = p->strictly_linear;
+  ta.strictly_linear = p->strictly_linear || p->linear;
+  ta.linear = p->linear;
   ta.robert = (T)p->robert;
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
@@ -919,6 +942,11 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   else
     return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
   CHK(mark(p, 8, st));
+  if (p->resp_set)
+    hipLaunchKernelGGL((k_response<T>), dim3(E), dim3(256), 0, st, (const cx<T> *)p->sa3,
+                       (const cx<T> *)p->obs, (const T *)p->wlev, (const T *)p->wstep, p->nwstep, E,
+                       L, p->nspec, (const Ctl *)p->ctl, (const int *)p->d_nsteps, p->inistp,
+                       p->jacc);
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, p->ctl);
   HIPCHK(hipGetLastError());
   if (p->prof) {
@@ -981,6 +1009,276 @@ int final_check(lbm_plan *p, hipStream_t st) {
   return LBM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Spectral initial state, linear response and the adjoint sweep.
+// ---------------------------------------------------------------------------
+template <class T>
+int set_spectral_t(lbm_plan *p, const double *vor, const double *div, const double *t,
+                   const double *q, const double *pi, hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, E = p->E, M = p->M;
+  const int EL = E * L;
+  const size_t s3 = (size_t)p->nspec * 4 * EL, s2 = (size_t)p->nspec * 3 * E;
+  HIPCHK(hipMemsetAsync(p->pend, 0, E * sizeof(int), st));
+  const double *src[4] = {vor, div, t, q};
+  HIPCHK(hipMemsetAsync(p->sa3, 0, s3 * sizeof(cx<T>), st));
+  HIPCHK(hipMemsetAsync(p->sa2, 0, s2 * sizeof(cx<T>), st));
+  const size_t n3 = (size_t)p->nspec * EL;
+  for (int f = 0; f < 4; ++f)
+    if (src[f])
+      hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(n3)), dim3(256), 0, st, src[f], (cx<T> *)p->sa3,
+                         E, M, L, 4, f, EL);
+  if (pi)
+    hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st, pi,
+                       (cx<T> *)p->sa2, E, M, 1, 3, 0, E);
+  hipLaunchKernelGGL((k_vsum<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
+                     (const cx<T> *)p->sa3, (cx<T> *)p->sa2, p->tabs<T>().vc, E, L, p->nspec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(p->sb3, p->sa3, s3 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
+  if (p->sb2) HIPCHK(hipMemcpyAsync(p->sb2, p->sa2, s2 * sizeof(cx<T>), hipMemcpyDeviceToDevice, st));
+  const size_t n2 = (size_t)E * J * I;
+  if (p->linear || !pi) {
+    HIPCHK(hipMemsetAsync(p->psa, 0, n2 * sizeof(T), st));
+    HIPCHK(hipMemsetAsync(p->psb, 0, n2 * sizeof(T), st));
+  } else {
+    // ps' of both levels from the synthesised pi (state.py:94-99)
+    const ColMap cm{E, 1, 0, 1};
+    const size_t gb = (n2 * sizeof(T) + 255) / 256 * 256;
+    void *base;
+    CHK(scratch_get(p, gb + (size_t)M * J * E * sizeof(cx<T>), &base, st));
+    T *g = (T *)base;
+    cx<T> *F = (cx<T> *)((char *)base + gb);
+    OutSlots sl{{0}};
+    CHK((launch_syn<T, SynScalar>(p, p->sa2, F, E, E, sl, st, 3)));
+    CHK(launch_ifft<T>(p, F, g, cm, st));
+    const BRow br{L};
+    hipLaunchKernelGGL((k_ps_from_pi<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)g, 1, 0,
+                       (const T *)p->bsg, br.rows(), br.psbar(), (T *)p->psa, (T *)p->psb, E, J, I,
+                       p->nb);
+    HIPCHK(hipGetLastError());
+  }
+  p->have_raw = false;
+  p->raw_pending = false;
+  return LBM_OK;
+}
+
+template <class T>
+int set_response_t(lbm_plan *p, const double *wgrid, const double *wlev, const double *wstep,
+                   int nst, hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, M = p->M;
+  if (!p->obs) HIPCHK(hipMalloc(&p->obs, sizeof(cx<T>) * p->nspec));
+  if (!p->wlev) HIPCHK(hipMalloc(&p->wlev, sizeof(T) * L));
+  if (!p->jacc) HIPCHK(hipMalloc(&p->jacc, sizeof(double) * p->E));
+  if (nst > p->wstep_cap) {
+    if (p->wstep) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(p->wstep));
+      p->wstep = nullptr;
+    }
+    HIPCHK(hipMalloc(&p->wstep, sizeof(T) * std::max(nst, 1)));
+    p->wstep_cap = nst;
+  }
+  std::vector<T> wl(wlev, wlev + L), ws(wstep, wstep + nst);
+  HIPCHK(hipMemcpy(p->wlev, wl.data(), sizeof(T) * L, hipMemcpyHostToDevice));
+  if (nst > 0) HIPCHK(hipMemcpy(p->wstep, ws.data(), sizeof(T) * nst, hipMemcpyHostToDevice));
+  p->nwstep = nst;
+  // obs = (wave_to_grid o inverse Laplacian)^T of the grid weights
+  const size_t ng = (size_t)J * I;
+  const size_t al = 256;
+  auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+  void *base;
+  CHK(scratch_get(p, up(ng * sizeof(double)) + up(ng * sizeof(T)) + (size_t)M * J * sizeof(cx<T>),
+                  &base, st));
+  double *w64 = (double *)base;
+  T *g = (T *)((char *)base + up(ng * sizeof(double)));
+  cx<T> *F = (cx<T> *)((char *)base + up(ng * sizeof(double)) + up(ng * sizeof(T)));
+  HIPCHK(hipMemcpyAsync(w64, wgrid, ng * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(ng)), dim3(256), 0, st, (const double *)w64, g, 1,
+                     J, I, 1, 1, 0, (const double *)nullptr);
+  HIPCHK(hipGetLastError());
+  const ColMap cm{1, 1, 1, 0};
+  CHK(launch_fft<T>(p, g, F, cm, st));
+  DevTabs<T> &t = p->tabs<T>();
+  InSlots sl{{0}};
+  CHK((launch_ana<T, AdjPsi>(p, F, p->obs, 1, 1, sl, st, 1, 0, t.Ps, t.Hs, t.inv_lap)));
+  HIPCHK(hipMemsetAsync(p->jacc, 0, sizeof(double) * p->E, st));
+  HIPCHK(hipStreamSynchronize(st));
+  p->resp_set = true;
+  return LBM_OK;
+}
+
+// One adjoint step: transposed spectral step, transposed analysis (synthesis
+// kernels with the analysis tables), inverse FFT, transposed tendencies,
+// forward FFT, transposed synthesis (analysis kernels with the synthesis
+// tables), then the step counter back by one.
+template <class T, int L>
+int launch_step_adj(lbm_plan *p, hipStream_t st) {
+  DevTabs<T> &t = p->tabs<T>();
+  StepAdjArgs<T> a;
+  a.aa3 = (cx<T> *)p->aa3;
+  a.ab3 = (cx<T> *)p->ab3;
+  a.aa2 = (cx<T> *)p->aa2;
+  a.ab2 = (cx<T> *)p->ab2;
+  a.s3a = (const cx<T> *)p->s3a;
+  a.at3 = (cx<T> *)p->tend3;
+  a.at2 = (cx<T> *)p->tend2;
+  a.kv = t.kv;
+  a.kd = t.kd;
+  a.ktt = t.ktt;
+  a.kq = t.kq;
+  a.minv = t.minv;
+  a.vinv = t.vinv;
+  a.vc = t.vc;
+  a.tref = t.tref;
+  a.lap = t.lap;
+  a.n_of_s = p->d_n_of_s;
+  a.obs = p->resp_set ? (const cx<T> *)p->obs : nullptr;
+  a.wlev = (const T *)p->wlev;
+  a.wstep = (const T *)p->wstep;
+  a.nwstep = p->nwstep;
+  a.ctl = p->ctl;
+  a.nsteps = p->d_nsteps;
+  a.E = p->E;
+  a.M = p->M;
+  a.nspec = p->nspec;
+  a.nbasic = p->nb;
+  a.inistp = p->inistp;
+  a.vmask = p->vmask;
+  a.skip_low_n = p->skip_low_n;
+  a.delt = (T)p->delt;
+  a.robert = (T)p->robert;
+  const size_t nthr = (size_t)p->nspec * p->E;
+  hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int adjoint_step(lbm_plan *p, hipStream_t st) {
+  const int E = p->E, L = p->L, EL = E * L;
+  DevTabs<T> &t = p->tabs<T>();
+  CHK(mark(p, 7, st));
+  if (L == 20)
+    CHK((launch_step_adj<T, 20>(p, st)));
+  else
+    CHK((launch_step_adj<T, 40>(p, st)));
+  const ColMap cmo{E, L, 9, 2}, cmi{E, L, 6, 6};
+  const int Co = cmo.ncols(), Ci = cmi.ncols();
+  CHK(mark(p, 5, st));
+  {
+    OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
+    CHK((launch_syn<T, AdjTend>(p, p->tend3, p->fout, EL, Co, sl, st, 4, t.Pa, t.Ha, t.nn1a2)));
+  }
+  CHK(mark(p, 6, st));
+  {
+    OutSlots sl{{9 * EL}};
+    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, E, Co, sl, st, 2, t.Pa, t.Ha, t.nn1a2)));
+  }
+  CHK(mark(p, 4, st));
+  CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st));
+  CHK(mark(p, 3, st));
+  TendAdjArgs<T> ta;
+  ta.gout_a = (const T *)p->gout;
+  ta.gin_a = (T *)p->gin;
+  ta.frc_a = (T *)p->frc_a;
+  ta.bsg = (const T *)p->bsg;
+  ta.vc = t.vc;
+  ta.tref = t.tref;
+  ta.rcos = t.rcos;
+  ta.E = E;
+  ta.J = p->J;
+  ta.I = p->I;
+  ta.L = L;
+  ta.nbasic = p->nb;
+  hipLaunchKernelGGL((k_tend_adj<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
+  HIPCHK(hipGetLastError());
+  CHK(mark(p, 2, st));
+  CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st));
+  CHK(mark(p, 0, st));
+  {
+    InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
+    CHK((launch_ana<T, AdjSynLevel>(p, p->fin, p->aa3, EL, Ci, sl, st, 4, 0, t.Ps, t.Hs, t.inv_lap)));
+  }
+  CHK(mark(p, 1, st));
+  {
+    InSlots sl{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
+    CHK((launch_ana<T, AdjSynSurf>(p, p->fin, p->s3a, E, Ci, sl, st, 3, 0, t.Ps, t.Hs, t.inv_lap)));
+  }
+  CHK(mark(p, 8, st));
+  hipLaunchKernelGGL(k_tick_back, dim3(1), dim3(1), 0, st, p->ctl);
+  HIPCHK(hipGetLastError());
+  if (p->prof) {
+    // slots keep the forward meaning of the transposed kernel
+    const int order[8] = {7, 5, 6, 4, 3, 2, 0, 1};
+    const int next[8] = {5, 6, 4, 3, 2, 0, 1, 8};
+    HIPCHK(hipEventSynchronize(p->ev[8]));
+    for (int k = 0; k < 8; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, p->ev[order[k]], p->ev[next[k]]));
+      p->prof_ms[order[k]] += ms;
+      p->prof_n[order[k]] += 1;
+    }
+  }
+  return LBM_OK;
+}
+
+int adjoint_dispatch(lbm_plan *p, hipStream_t st) {
+  return p->prec == LBM_FP64 ? adjoint_step<double>(p, st) : adjoint_step<float>(p, st);
+}
+
+template <class T>
+int adjoint_finish(lbm_plan *p, double *gvor, double *gdiv, double *gt, double *gq, double *gpi,
+                   double *gfrc, hipStream_t st) {
+  const int E = p->E, L = p->L, M = p->M, J = p->J, I = p->I;
+  const size_t EL = (size_t)E * L;
+  hipLaunchKernelGGL((k_vsum_adj<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
+                     (const cx<T> *)p->s3a, (cx<T> *)p->aa3, (cx<T> *)p->aa2, p->tabs<T>().vc, E, L,
+                     p->nspec);
+  // the initial state fills both time levels: gradient = Sa* + Sb*
+  const size_t n3 = (size_t)p->nspec * 4 * EL, n2 = (size_t)p->nspec * E;
+  hipLaunchKernelGGL((k_add_cx<T>), dim3(nblk(n3)), dim3(256), 0, st, (const cx<T> *)p->aa3,
+                     (const cx<T> *)p->ab3, (cx<T> *)p->aa3, n3);
+  hipLaunchKernelGGL((k_add_cx<T>), dim3(nblk(n2)), dim3(256), 0, st, (const cx<T> *)p->aa2,
+                     (const cx<T> *)p->ab2, (cx<T> *)p->aa2, n2);
+  double *dst[4] = {gvor, gdiv, gt, gq};
+  const size_t tot = (size_t)E * M * M * L;
+  for (int f = 0; f < 4; ++f)
+    if (dst[f])
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk(tot)), dim3(256), 0, st,
+                         (const cx<T> *)p->aa3, dst[f], E, M, L, 4, f, (int)EL);
+  if (gpi)
+    hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)E * M * M)), dim3(256), 0, st,
+                       (const cx<T> *)p->aa2, gpi, E, M, 1, 1, 0, E);
+  if (gfrc)
+    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk((size_t)E * J * I * L)), dim3(256), 0, st,
+                       (const T *)p->frc_a, gfrc, E, J, I, L, L, 0, 0, (const T *)nullptr, 0, 0, 1);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int capture_adjoint(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
+  if (p->prof) return fail(LBM_ERR_STATE, "profiling runs eager launches only (use_graph=0)");
+  if (p->gexec[slot] && p->gsteps[slot] == nsteps) return LBM_OK;
+  if (p->gexec[slot]) {
+    (void)hipGraphExecDestroy(p->gexec[slot]);
+    p->gexec[slot] = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = LBM_OK;
+  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = adjoint_dispatch(p, st);
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (rc != LBM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e2 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph capture failed: %s", hipGetErrorString(e2));
+  hipError_t e3 = hipGraphInstantiate(&p->gexec[slot], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e3 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph instantiate failed: %s", hipGetErrorString(e3));
+  p->gsteps[slot] = nsteps;
+  return LBM_OK;
+}
+
 template <class T>
 size_t alloc_bytes(const lbm_plan *p) {
   (void)p;
@@ -1007,7 +1305,8 @@ int lbm_plan_destroy(lbm_plan *p) {
   free_tabs(p->d32);
   void *bufs[] = {p->d_n_of_s, p->d_mu, p->d_psbar, p->sa3, p->sa2, p->sb3, p->tend3, p->tend2,
                   p->fin, p->gin, p->gout, p->fout, p->raw, p->psa, p->psb, p->bsg, p->frc,
-                  p->ctl, p->pend, p->d_nsteps, p->scratch};
+                  p->ctl, p->pend, p->d_nsteps, p->scratch, p->sb2, p->aa3, p->ab3, p->aa2,
+                  p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
@@ -1046,6 +1345,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->inistp = d->inistp;
   p->nset = p->inistp + 1;
   p->strictly_linear = d->strictly_linear;
+  p->linear = d->tangent_linear ? 1 : 0;
   p->skip_low_n = d->skip_low_n_vdiff;
   p->a = d->radius;
   p->delt = d->delt;
@@ -1118,6 +1418,11 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   };
   for (auto &al : allocs)
     if ((rc = dalloc(al.ptr, al.bytes)) != LBM_OK) return bail(rc);
+  if (p->linear) {
+    if ((rc = dalloc(&p->sb2, cs * p->nspec * 3 * E)) != LBM_OK) return bail(rc);
+    if (hipMemset(p->sb2, 0, cs * p->nspec * 3 * E) != hipSuccess)
+      return bail(fail(LBM_ERR_RUNTIME, "initialisation failed"));
+  }
   if (hipMemcpy(p->d_n_of_s, p->n_of_s.data(), sizeof(short) * p->nspec, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(p->d_mu, p->mu.data(), sizeof(double) * J, hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(LBM_ERR_RUNTIME, "table upload failed"));
@@ -1372,6 +1677,102 @@ int lbm_profile_read(lbm_plan *p, double *h_ms, int64_t *h_count, int n) {
 int lbm_steps_taken(const lbm_plan *p, int64_t *out) {
   if (!p || !out) return fail(LBM_ERR_VALUE, "null argument");
   *out = p->steps_total;
+  return LBM_OK;
+}
+
+int lbm_set_spectral_state(lbm_plan *p, const double *vor, const double *div, const double *t,
+                           const double *q, const double *pi, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before the perturbation");
+  hipStream_t st = p->strm(stream);
+  p->steps_total = 0;
+  int rc = p->prec == LBM_FP64 ? set_spectral_t<double>(p, vor, div, t, q, pi, st)
+                               : set_spectral_t<float>(p, vor, div, t, q, pi, st);
+  if (rc != LBM_OK) return rc;
+  if (p->jacc) HIPCHK(hipMemsetAsync(p->jacc, 0, sizeof(double) * p->E, st));
+  Ctl c0{0, 0, INT_MAX, 0};
+  HIPCHK(hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_set_response(lbm_plan *p, const double *wgrid, const double *wlev, const double *wstep,
+                     int nstep) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  destroy_graphs(p);
+  if (!wgrid) {
+    p->resp_set = false;
+    return LBM_OK;
+  }
+  if (!wlev || (nstep > 0 && !wstep) || nstep < 0)
+    return fail(LBM_ERR_VALUE, "response needs level weights and nstep >= 0 step weights");
+  hipStream_t st = p->own;
+  return p->prec == LBM_FP64 ? set_response_t<double>(p, wgrid, wlev, wstep, nstep, st)
+                             : set_response_t<float>(p, wgrid, wlev, wstep, nstep, st);
+}
+
+int lbm_get_response(lbm_plan *p, double *h_j) {
+  if (!p || !h_j) return fail(LBM_ERR_VALUE, "null argument");
+  if (!p->resp_set) return fail(LBM_ERR_STATE, "no response set");
+  HIPCHK(hipStreamSynchronize(p->own));
+  HIPCHK(hipMemcpy(h_j, p->jacc, sizeof(double) * p->E, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double *gt, double *gq,
+                    double *gpi, double *gfrc, int use_graph, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_adjoint_run before lbm_set_basic_state");
+  if (!p->linear) return fail(LBM_ERR_STATE, "the adjoint needs a plan with tangent_linear = 1");
+  if (!p->resp_set) return fail(LBM_ERR_STATE, "the adjoint needs a response (lbm_set_response)");
+  hipStream_t st = use_graph ? p->own : p->strm(stream);
+  if (use_graph && stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  const int E = p->E, L = p->L, J = p->J, I = p->I;
+  const size_t EL = (size_t)E * L, cs = 2 * p->rsize();
+  std::vector<int> len(E);
+  for (int e = 0; e < E; ++e) len[e] = nsteps > 0 ? nsteps : p->nsteps_h[e];
+  const int maxlen = *std::max_element(len.begin(), len.end());
+  const int total = p->inistp + maxlen;
+  struct A {
+    void **ptr;
+    size_t bytes;
+  } bufs[] = {{&p->aa3, cs * p->nspec * 4 * EL}, {&p->ab3, cs * p->nspec * 4 * EL},
+              {&p->aa2, cs * p->nspec * E},      {&p->ab2, cs * p->nspec * E},
+              {&p->s3a, cs * p->nspec * 3 * E},  {&p->frc_a, p->rsize() * E * J * I * L}};
+  for (auto &b : bufs) {
+    if (!*b.ptr) HIPCHK(hipMalloc(b.ptr, b.bytes));
+    HIPCHK(hipMemsetAsync(*b.ptr, 0, b.bytes, st));
+  }
+  HIPCHK(hipMemcpyAsync(p->d_nsteps, len.data(), sizeof(int) * E, hipMemcpyHostToDevice, st));
+  Ctl saved;
+  HIPCHK(hipMemcpyAsync(&saved, p->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  Ctl c0{total - 1, 0, INT_MAX, 0};
+  HIPCHK(hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  const int chunk = 8;
+  int done = 0;
+  while (done < total) {
+    const int todo = total - done;
+    if (use_graph) {
+      if (todo >= chunk) {
+        CHK(capture_adjoint(p, 2, chunk, st));
+        HIPCHK(hipGraphLaunch(p->gexec[2], st));
+        done += chunk;
+      } else {
+        CHK(capture_adjoint(p, 3, 1, st));
+        HIPCHK(hipGraphLaunch(p->gexec[3], st));
+        done += 1;
+      }
+    } else {
+      CHK(adjoint_dispatch(p, st));
+      done += 1;
+    }
+  }
+  CHK(p->prec == LBM_FP64 ? adjoint_finish<double>(p, gvor, gdiv, gt, gq, gpi, gfrc, st)
+                          : adjoint_finish<float>(p, gvor, gdiv, gt, gq, gpi, gfrc, st));
+  // the forward loop control is left as it was
+  HIPCHK(hipMemcpyAsync(p->ctl, &saved, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   return LBM_OK;
 }
 

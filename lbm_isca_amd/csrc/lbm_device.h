@@ -339,64 +339,8 @@ struct SynGrad {  // hgrad (spectral.py:210-225)
 };
 
 struct OutSlots {
-  int off[8];  // column offset in the Fourier row of each output field
+  int off[12];  // column offset in the Fourier row of each output field
 };
-
-// Synthesis, one block per (m, 8 columns); threads (col-lane, lat-pair).
-// P(-mu) = (-1)^(n+m) P(mu), H(-mu) = -(-1)^(n+m) H(mu): the north and
-// south rows come from the even/odd partial sums.
-template <class T, class R>
-__global__ __launch_bounds__(256) void k_syn(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
-                                             const T *__restrict__ Ps, const T *__restrict__ Hs,
-                                             const T *__restrict__ inv_lap,
-                                             const T *__restrict__ rcos, int ncols, int C,
-                                             OutSlots slots, int J, int M, int in_nf) {
-  constexpr int NIN = R::NIN, NOUT = R::NOUT;
-  const int t = threadIdx.x, cl = t & 7;
-  const int col = blockIdx.x * 8 + cl, m = blockIdx.y;
-  const int Jh = J / 2;
-  if (col >= ncols) return;
-  const T fm = (T)m;
-  for (int jp = t >> 3; jp < Jh; jp += 32) {
-    cx<T> EP[NOUT], OP[NOUT], EH[NOUT], OH[NOUT];
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) EP[o] = OP[o] = EH[o] = OH[o] = zero<T>();
-    int s = spec_off(m, M);
-    for (int n = m; n < M; ++n, ++s) {
-      cx<T> x[NIN], A[NOUT], B[NOUT];
-#pragma unroll
-      for (int f = 0; f < NIN; ++f) x[f] = in[((size_t)s * in_nf + f) * ncols + col];
-      R::template terms<T>(x, inv_lap[n], fm, A, B);
-      const T p = Ps[(size_t)s * Jh + jp], h = Hs[(size_t)s * Jh + jp];
-      if ((n - m) & 1) {
-#pragma unroll
-        for (int o = 0; o < NOUT; ++o) {
-          if (R::hasA(o)) fma_acc(OP[o], p, A[o]);
-          if (R::hasB(o)) fma_acc(OH[o], h, B[o]);
-        }
-      } else {
-#pragma unroll
-        for (int o = 0; o < NOUT; ++o) {
-          if (R::hasA(o)) fma_acc(EP[o], p, A[o]);
-          if (R::hasB(o)) fma_acc(EH[o], h, B[o]);
-        }
-      }
-    }
-    const int jn = jp, js = J - 1 - jp;
-    const T sc = rcos[jp];
-#pragma unroll
-    for (int o = 0; o < NOUT; ++o) {
-      cx<T> vn = (EP[o] + OP[o]) + (EH[o] + OH[o]);
-      cx<T> vs = (EP[o] - OP[o]) + (OH[o] - EH[o]);
-      if (R::scaled(o)) {
-        vn = sc * vn;
-        vs = sc * vs;
-      }
-      out[((size_t)m * J + jn) * C + slots.off[o] + col] = vn;
-      out[((size_t)m * J + js) * C + slots.off[o] + col] = vs;
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Legendre analysis recipes.  For each input field f the kernel forms
@@ -407,6 +351,7 @@ __global__ __launch_bounds__(256) void k_syn(const cx<T> *__restrict__ in, cx<T>
 // ---------------------------------------------------------------------------
 struct AnaTend {  // (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t, q): dynamics.py:219,246-262
   static constexpr int NIN = 9, NOUT = 4;
+  static constexpr bool ACC = false;
   static constexpr bool needP(int f) { return f != 4 && f != 7; }
   static constexpr bool needH(int f) { return f == 0 || f == 1 || f == 4 || f == 7; }
   template <class T>
@@ -419,6 +364,7 @@ struct AnaTend {  // (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t, q): dy
 };
 struct AnaState {  // (u', v', t, q) -> (vor, div, t, q): state.py:86-92
   static constexpr int NIN = 4, NOUT = 4;
+  static constexpr bool ACC = false;
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int f) { return f < 2; }
   template <class T>
@@ -431,6 +377,7 @@ struct AnaState {  // (u', v', t, q) -> (vor, div, t, q): state.py:86-92
 };
 struct AnaVorDiv {  // uv_to_vordiv (spectral.py:227-244)
   static constexpr int NIN = 2, NOUT = 2;
+  static constexpr bool ACC = false;
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int) { return true; }
   template <class T>
@@ -441,6 +388,7 @@ struct AnaVorDiv {  // uv_to_vordiv (spectral.py:227-244)
 };
 struct AnaScalar {  // grid_to_wave (spectral.py:186-191)
   static constexpr int NIN = 1, NOUT = 1;
+  static constexpr bool ACC = false;
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int) { return false; }
   template <class T>
@@ -450,72 +398,5 @@ struct AnaScalar {  // grid_to_wave (spectral.py:186-191)
 struct InSlots {
   int off[12];  // column offset in the Fourier row of each input field
 };
-
-// Analysis, one block per (m, 16 columns); threads (col-lane, n-slot).  A
-// thread owns n = m + ns + 16k (k = 0..2), all of one parity, so it folds
-// the north/south rows once: symmetric sums for even n-m with P (odd with H).
-template <class T, class R>
-__global__ __launch_bounds__(256) void k_ana(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
-                                             const T *__restrict__ Pa, const T *__restrict__ Ha,
-                                             const T *__restrict__ nn1a2, int ncols, int C,
-                                             InSlots slots, int J, int M, int out_nf, int out_f0) {
-  constexpr int NIN = R::NIN, NOUT = R::NOUT, KN = 3;
-  const int t = threadIdx.x, cl = t & 15, ns = t >> 4;
-  const int col = blockIdx.x * 16 + cl, m = blockIdx.y;
-  const int Jh = J / 2;
-  if (col >= ncols) return;
-  const bool odd = (ns & 1) != 0;
-  const int s0 = spec_off(m, M) - m;  // s = s0 + n
-  cx<T> AP[NIN][KN], AH[NIN][KN];
-#pragma unroll
-  for (int f = 0; f < NIN; ++f)
-#pragma unroll
-    for (int k = 0; k < KN; ++k) AP[f][k] = AH[f][k] = zero<T>();
-  int nk[KN];
-  bool ok[KN];
-#pragma unroll
-  for (int k = 0; k < KN; ++k) {
-    nk[k] = m + ns + 16 * k;
-    ok[k] = nk[k] < M;
-  }
-  if (!ok[0]) return;
-  for (int jp = 0; jp < Jh; ++jp) {
-    const cx<T> *rn = in + ((size_t)m * J + jp) * C + col;
-    const cx<T> *rs = in + ((size_t)m * J + (J - 1 - jp)) * C + col;
-    T p[KN], h[KN];
-#pragma unroll
-    for (int k = 0; k < KN; ++k) {
-      const size_t ti = (size_t)(s0 + (ok[k] ? nk[k] : m)) * Jh + jp;
-      p[k] = ok[k] ? Pa[ti] : (T)0;
-      h[k] = ok[k] ? Ha[ti] : (T)0;
-    }
-#pragma unroll
-    for (int f = 0; f < NIN; ++f) {
-      const cx<T> fn = rn[slots.off[f]], fs = rs[slots.off[f]];
-      const cx<T> sym = fn + fs, anti = fn - fs;
-      const cx<T> fp = odd ? anti : sym, fh = odd ? sym : anti;
-#pragma unroll
-      for (int k = 0; k < KN; ++k) {
-        if (R::needP(f)) fma_acc(AP[f][k], p[k], fp);
-        if (R::needH(f)) fma_acc(AH[f][k], h[k], fh);
-      }
-    }
-  }
-  const T fm = (T)m;
-#pragma unroll
-  for (int k = 0; k < KN; ++k) {
-    if (!ok[k]) continue;
-    cx<T> ap[NIN], ah[NIN], o[NOUT];
-#pragma unroll
-    for (int f = 0; f < NIN; ++f) {
-      ap[f] = AP[f][k];
-      ah[f] = AH[f][k];
-    }
-    R::template combine<T>(ap, ah, fm, nn1a2[nk[k]], o);
-    const int s = s0 + nk[k];
-#pragma unroll
-    for (int q = 0; q < NOUT; ++q) out[((size_t)s * out_nf + out_f0 + q) * ncols + col] = o[q];
-  }
-}
 
 }  // namespace lbm

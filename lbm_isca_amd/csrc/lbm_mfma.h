@@ -153,7 +153,10 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
 #pragma unroll
     for (int q = 0; q < NOUT; ++q) {
       T *dst = reinterpret_cast<T *>(out + ((size_t)s * out_nf + out_f0 + q) * ncols + col);
-      dst[part] = pick<T>(o[q].x, o[q].y, part);
+      if constexpr (R::ACC)
+        dst[part] += pick<T>(o[q].x, o[q].y, part);
+      else
+        dst[part] = pick<T>(o[q].x, o[q].y, part);
     }
   }
 }

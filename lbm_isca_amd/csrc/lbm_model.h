@@ -210,6 +210,7 @@ struct TendArgs {
   const T *rcos;    // [J] 1/(a cos lat)
   Ctl *ctl;
   int E, J, I, L, nbasic, nfrc, strictly_linear;
+  int linear;       // tangent-linear mode: no ps' grid state (pi' is spectral)
   T robert;
 };
 
@@ -270,8 +271,13 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   cs.bsd_lo = bs[(size_t)br.sdot() * I];
   const size_t pidx = (size_t)(e * J + j) * I + i;
   T gpi, pib;
-  column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0, gpi,
-                    pib, cs.chk);
+  if (a.linear) {
+    gpi = cc.D0 - cc.S0;
+    pib = 0;
+  } else {
+    column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0,
+                      gpi, pib, cs.chk);
+  }
   go[(size_t)(9 * L) * I] = gpi;
   go[(size_t)(9 * L + 1) * I] = pib;
   const T *tref = a.tref + (size_t)eb * L;
@@ -302,6 +308,7 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
 template <class T>
 struct StepArgs {
   cx<T> *sa3, *sa2, *sb3;        // [s][4][L][E], [s][3][E] (pi, sum vor, sum div), [s][4][L][E]
+  cx<T> *sb2;                    // [s][3][E] field 0: pi at t-dt (tangent-linear mode) or null
   const cx<T> *tend3, *tend2;    // [s][4][L][E], [s][2][E] (wt_pi, pi_prev)
   const T *kv, *kd, *ktt, *kq;   // [M][L]
   const T *minv;                 // [nset][nbasic][M][L][L]
@@ -355,7 +362,9 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   const cx<T> *tn = a.tend3 + b3;
   auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
   const cx<T> wt_pi = a.tend2[(size_t)s * 2 * E + e];
-  const cx<T> pi_prev = a.tend2[(size_t)s * 2 * E + E + e];
+  // pi at t-dt: the analysed log1p(ps'_b/ps_bar), or the spectral level
+  // itself in tangent-linear mode
+  const cx<T> pi_prev = a.sb2 ? a.sb2[(size_t)s * 3 * E + e] : a.tend2[(size_t)s * 2 * E + E + e];
   const T lapn = a.lap[n];
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
   cx<T> *s2 = a.sa2 + (size_t)s * 3 * E + e;
@@ -391,7 +400,12 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
     cx<T> sdb = zero<T>();
 #pragma unroll
     for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
-    s2[0] = pi_prev + dt2 * (wt_pi - sdb);
+    const cx<T> pin = pi_prev + dt2 * (wt_pi - sdb);
+    if (a.sb2) {
+      const cx<T> cur = s2[0];
+      a.sb2[(size_t)s * 3 * E + e] = oinit ? cur : cur + gam * (pi_prev - (T)2 * cur + pin);
+    }
+    s2[0] = pin;
   }
   // divergence
 #pragma unroll
@@ -528,7 +542,8 @@ __global__ void k_pack_grid(const double *__restrict__ src, T *__restrict__ dst,
 }
 
 // internal [b][j][r0+k][I] -> reference (B, J, I, K) float64.
-// mode 0: copy; 1: ps' = ps_bar (exp(x) - 1) with ps_bar from psbar[b?][j][i].
+// mode 0: copy; 1: ps' = ps_bar (exp(x) - 1) with ps_bar from psbar[b?][j][i];
+// 2: ps' = ps_bar x (the linearised pair).
 template <class T>
 __global__ void k_unpack_grid(const T *__restrict__ src, double *__restrict__ dst, int B, int J,
                               int I, int K, int R, int r0, int mode, const T *psbar,
@@ -543,10 +558,10 @@ __global__ void k_unpack_grid(const T *__restrict__ src, double *__restrict__ ds
     const int j = (int)(rest % J);
     const int b = (int)(rest / J);
     T v = src[(((size_t)b * J + j) * R + r0 + k) * I + i];
-    if (mode == 1) {
+    if (mode != 0) {
       const int bb = nbasic > 1 ? b : 0;
       const T pb = psbar[(((size_t)bb * J + j) * psbar_rows + psbar_row) * I + i];
-      v = pb * expm1(v);
+      v = mode == 1 ? pb * expm1(v) : pb * v;
     }
     dst[q] = (double)v;
   }
@@ -596,10 +611,11 @@ __global__ void k_unpack_spec(const cx<T> *__restrict__ src, double *__restrict_
   }
 }
 
-// ps' (B, J, I) float64 -> pi' = log1p(ps'/ps_bar) into an internal grid row.
+// ps' (B, J, I) float64 -> pi' = log1p(ps'/ps_bar) (linear: ps'/ps_bar) into
+// an internal grid row.
 template <class T>
 __global__ void k_pack_pi(const double *__restrict__ ps, T *__restrict__ dst, int B, int J, int I,
-                          int R, int r0, const double *__restrict__ psbar, int nbasic) {
+                          int R, int r0, const double *__restrict__ psbar, int nbasic, int linear) {
   const size_t total = (size_t)B * J * I;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
        q += (size_t)gridDim.x * blockDim.x) {
@@ -609,7 +625,26 @@ __global__ void k_pack_pi(const double *__restrict__ ps, T *__restrict__ dst, in
     const int bb = nbasic > 1 ? b : 0;
     const double pb = psbar[((size_t)bb * J + j) * I + i];
     const double v = ps ? ps[q] : 0.0;
-    dst[(((size_t)b * J + j) * R + r0) * I + i] = (T)log1p(v / pb);
+    dst[(((size_t)b * J + j) * R + r0) * I + i] = (T)(linear ? v / pb : log1p(v / pb));
+  }
+}
+
+// psa = psb = ps_bar (exp(pi) - 1) from a synthesised pi row (state.py:81-83)
+template <class T>
+__global__ void k_ps_from_pi(const T *__restrict__ g, int R, int r0, const T *__restrict__ bsg,
+                             int brows, int brow, T *__restrict__ psa, T *__restrict__ psb, int B,
+                             int J, int I, int nbasic) {
+  const size_t tot = (size_t)B * J * I;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    const size_t bj = q / I;
+    const int j = (int)(bj % J), b = (int)(bj / J);
+    const int bb = nbasic > 1 ? b : 0;
+    const T pb = bsg[(((size_t)bb * J + j) * brows + brow) * I + i];
+    const T v = pb * expm1(g[(bj * R + r0) * I + i]);
+    psa[q] = v;
+    psb[q] = v;
   }
 }
 

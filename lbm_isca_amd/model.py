@@ -44,6 +44,9 @@ class Config:
     out_interval_days: float = 1.0
     time_average: bool = True
     strictly_linear: bool = False
+    # tangent-linear model about a zero perturbation (strictly linear, with
+    # ps' = ps_bar pi'): exactly linear, the model the adjoint transposes
+    tangent_linear: bool = False
     nmem: int = 1
     device: str = "hip"
     precision: str = "fp64"      # "fp64" or "fp32" arithmetic on the GPU
@@ -154,7 +157,8 @@ class LBM:
         self.plan = Plan(spectral.nlon, spectral.nlat, spectral.nmax, spectral.a,
                          vertical.sigm, nmem=nmem, nbasic=nbasic, precision=c.precision,
                          damping=damping, vdif=vdif, delt=c.delt, robert=c.robert,
-                         inistp=c.inistp, strictly_linear=c.strictly_linear)
+                         inistp=c.inistp, strictly_linear=c.strictly_linear,
+                         tangent_linear=c.tangent_linear)
         fields = [np.asarray(_field(basic_state, k), dtype=np.float64) for k in ("u", "v", "t", "q", "ps")]
         if not per_member_basic:
             fields = [f[None] for f in fields]

@@ -45,7 +45,8 @@ class Plan:
 
     def __init__(self, nlon=IMAX, nlat=JMAX, nmax=NMAX, radius=ER, sigm=SIGM, nmem=1,
                  nbasic=1, precision="fp64", damping=None, vdif=(None, None, None, None),
-                 delt=1200.0, robert=0.04, inistp=2, strictly_linear=False):
+                 delt=1200.0, robert=0.04, inistp=2, strictly_linear=False,
+                 tangent_linear=False):
         lib = _lib.load()
         self.nlon, self.nlat, self.nmax = int(nlon), int(nlat), int(nmax)
         self.sigm = np.ascontiguousarray(sigm, dtype=np.float64)
@@ -62,6 +63,8 @@ class Plan:
         d.nlon, d.nlat, d.nmax, d.nlev = self.nlon, self.nlat, self.nmax, self.nlev
         d.nmem, d.nbasic, d.precision = self.nmem, self.nbasic, prec
         d.strictly_linear = int(bool(strictly_linear))
+        d.tangent_linear = int(bool(tangent_linear))
+        self.tangent_linear = bool(tangent_linear)
         d.inistp = int(inistp)
         d.skip_low_n_vdiff = 1
         d.radius, d.delt, d.robert = float(radius), float(delt), float(robert)
@@ -189,6 +192,63 @@ class Plan:
             args.append(x)
         self._state_inputs = args
         _lib.check(self._lib.lbm_set_state(self._h, *[_lib.ptr(x) for x in args], _stream()))
+
+    def set_spectral_state(self, vor=None, div=None, t=None, q=None, pi=None):
+        """Spectral initial perturbation (E, M, M, L) / (E, M, M) complex128 for
+        both time levels; None means zero."""
+        torch = _torch()
+        E, M, L = self.nmem, self.nmax + 1, self.nlev
+        args = []
+        for x, shp in ((vor, (E, M, M, L)), (div, (E, M, M, L)), (t, (E, M, M, L)),
+                       (q, (E, M, M, L)), (pi, (E, M, M))):
+            if x is None:
+                args.append(None)
+                continue
+            x = _dev(x, torch.complex128)
+            x = x.expand(shp).contiguous() if tuple(x.shape) != shp else x
+            args.append(x)
+        self._state_inputs = args
+        _lib.check(self._lib.lbm_set_spectral_state(self._h, *[_lib.ptr(x) for x in args],
+                                                    _stream()))
+
+    def set_response(self, wgrid, wlev=None, wstep=None):
+        """J_e = sum_k wstep[k] sum_l wlev[l] sum_ji wgrid[j,i] psi_l(j,i) over the
+        states after steps k of the next run; wgrid None removes it."""
+        torch = _torch()
+        torch.cuda.current_stream().synchronize()
+        if wgrid is None:
+            _lib.check(self._lib.lbm_set_response(self._h, None, None, None, 0))
+            return
+        wg = np.ascontiguousarray(wgrid, dtype=np.float64)
+        if wg.shape != (self.nlat, self.nlon):
+            raise ValueError(f"wgrid must be ({self.nlat}, {self.nlon})")
+        wl = np.ascontiguousarray(np.broadcast_to(
+            np.asarray(1.0 if wlev is None else wlev, np.float64), (self.nlev,)))
+        ws = np.ascontiguousarray(np.asarray(wstep, dtype=np.float64).reshape(-1))
+        self._resp = (wg, wl, ws)
+        _lib.check(self._lib.lbm_set_response(self._h, _lib.dptr(wg), _lib.dptr(wl),
+                                              _lib.dptr(ws), int(ws.size)))
+
+    def get_response(self):
+        torch = _torch()
+        torch.cuda.current_stream().synchronize()
+        out = np.zeros(self.nmem)
+        _lib.check(self._lib.lbm_get_response(self._h, _lib.dptr(out)))
+        return out
+
+    def adjoint_run(self, nsteps, use_graph=True):
+        """Gradients of J_e with respect to member e's spectral initial state
+        (vor, div, t, q, pi; complex128 dJ/dRe + i dJ/dIm) and its temperature
+        forcing (E, J, I, L)."""
+        torch = _torch()
+        E, M, L, J, I = self.nmem, self.nmax + 1, self.nlev, self.nlat, self.nlon
+        g3 = [torch.empty((E, M, M, L), dtype=torch.complex128, device="cuda") for _ in range(4)]
+        gpi = torch.empty((E, M, M), dtype=torch.complex128, device="cuda")
+        gf = torch.empty((E, J, I, L), dtype=torch.float64, device="cuda")
+        _lib.check(self._lib.lbm_adjoint_run(self._h, int(nsteps), *[_lib.ptr(x) for x in g3],
+                                             _lib.ptr(gpi), _lib.ptr(gf), int(bool(use_graph)),
+                                             _stream()))
+        return g3 + [gpi], gf
 
     def set_lengths(self, nsteps):
         if nsteps is None:

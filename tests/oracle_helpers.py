@@ -35,3 +35,22 @@ def rel_err(a, b):
     b = np.asarray(b)
     den = np.sqrt((np.abs(b) ** 2).sum())
     return float(np.sqrt((np.abs(a - b) ** 2).sum()) / (den if den > 0 else 1.0))
+
+
+def oracle_response(bs, nmem, spec, frc, nsteps, wgrid, wlev, wstep, sigm=None):
+    """J_e = sum_k wstep[k] sum_l wlev[l] sum_ji wgrid psi_l of the oracle's
+    linear-mode run from a spectral state (both time levels), with psi the
+    inverse Laplacian of the new vorticity synthesised on the grid."""
+    m = OracleModel(bs, nmem, OracleConfig(linear=True), sigm=sigm)
+    m.ga.load_spectral(m.sht, spec)
+    m.gb.load_spectral(m.sht, spec)
+    c = m.cfg
+    J = np.zeros(nmem)
+    for k in range(c.inistp + nsteps):
+        oinit = k < c.inistp
+        dt = c.delt / (2.0 ** (c.inistp - k)) if oinit else c.delt
+        new = m.step(dt, oinit, frc)
+        if k < len(wstep) and wstep[k] != 0.0:
+            psi = m.sht.wave_to_grid(m.sht.inverse_laplacian(new["vor"]))
+            J += wstep[k] * np.einsum("ejil,ji,l->e", psi, wgrid, wlev)
+    return J
