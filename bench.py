@@ -15,6 +15,18 @@ counted as days), the dominant kernel's achieved HBM bandwidth against the
 8 TB/s peak (HIP events on the plan's stream, eager replay of the same
 steps) and the numpy FP64 oracle timed on the host cores on a bounded
 single-member sample of the same workload.
+
+Two more parts of the metric ride along in the same line:
+
+* ``fp32_vs_fp64``: the uneven ensemble -- 256 members, each with its own
+  jet basic state and heating and a length of U(5, 60) days -- run in FP32
+  and in FP64 (one hipGraph replay each); member-days/s of both, the
+  speed-up, and the relative L2 error of the FP32 final state against the
+  FP64 one per field (the FP64 path matches the FP64 oracle to 1e-9).
+* ``adjoint``: 32 of those members in the tangent-linear model with the
+  response J = area mean of the sigma = 0.2 streamfunction over 30-60N,
+  150-210E averaged over days 20-30; wall time of the adjoint sweep (both
+  gradients, all members) over the forward run, FP64 and FP32.
 """
 
 import argparse
@@ -31,6 +43,31 @@ METRIC = "T42L20 member-days/s, FP32 vs FP64 at <=1e-4 rel err; adjoint/forward 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+# kernel of each profiling slot (FP64 names; FP32 replaces double by float)
+SLOT_KERNEL = {"syn_level": "k_syn_mfma<double, lbm::SynLevel>",
+               "syn_surf": "k_syn_mfma<double, lbm::SynSurf>",
+               "ifft": "k_ifft128<double>", "tend": "k_tend<double>", "fft": "k_fft128<double>",
+               "ana_level": "k_ana_mfma<double, lbm::AnaTend>",
+               "ana_surf": "k_ana_mfma<double, lbm::AnaScalar>", "step": "k_step<double, 20>"}
+
+
+def measured_traffic(slot, E, precision):
+    """HBM bytes per launch of a slot's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_T42L20_E{E}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        table = json.load(fh)["bytes_per_launch"]
+    name = SLOT_KERNEL[slot]
+    if precision == "fp32":
+        name = name.replace("double", "float")
+    for k, v in table.items():
+        if name in k:
+            return int(v)
+    return None
 
 
 def build_model(members, precision):
@@ -65,6 +102,118 @@ def cpu_baseline(bs, frc, sigm, seconds=15.0):
                       f"({dt:.1f} s) after one start-up step, BLAS threads = host default"}
 
 
+def uneven_ensemble(E, seed=1, max_days=None):
+    """Per-member jet basic states, heatings and lengths of the uneven
+    ensemble (E, J, I, L) on 20 equal sigma levels; ``max_days`` caps the
+    lengths (profiling runs only)."""
+    import numpy as np
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases
+    sigm = cases.EQUAL_SIGM_L20
+    sp = L.BatchedSpectral()
+    vg = L.VerticalGrid(sigm)
+    bs, _ = cases.jet_states(sp, vg.sig, E, seed=seed)
+    frc = cases.random_heatings(sp.lat, sp.lon, vg.sig, E, seed=0)
+    lengths = cases.member_lengths(E, seed=seed)
+    if max_days is not None:
+        lengths = np.minimum(lengths, int(round(72 * max_days)))
+    return bs, frc, lengths, sigm
+
+
+def measure_fp32_vs_fp64(E=256, max_days=None):
+    """FP32 and FP64 runs of the uneven ensemble: throughput and the FP32
+    error against FP64."""
+    import numpy as np
+    import torch
+    import lbm_isca_amd as L
+    bs, frc, lengths, sigm = uneven_ensemble(E, max_days=max_days)
+    out, states = {}, {}
+    for prec in ("fp64", "fp32"):
+        cfg = L.Config(precision=prec)
+        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
+        m.plan.set_forcing(frc)
+        m.plan.set_lengths(lengths)
+        m.plan.run(8, use_graph=True)               # capture + warm caches
+        m.plan.set_state()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.plan.run(0, use_graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[prec] = {"member_days_per_s": round(float(lengths.sum()) / 72.0 / dt, 2),
+                     "seconds": round(dt, 3)}
+        states[prec] = [x.cpu().numpy() for x in m.plan.get_state()]
+        log(f"uneven ensemble {prec}: {out[prec]}")
+        del m
+        torch.cuda.empty_cache()
+    err = {}
+    for f, a, b in zip(("u", "v", "t", "q", "ps"), states["fp32"], states["fp64"]):
+        den = np.sqrt((b ** 2).sum())
+        err[f] = float(np.sqrt(((a - b) ** 2).sum()) / den) if den > 0 else 0.0
+    return {"workload": f"T42L20 {E}-member uneven ensemble, per-member jet basic states, "
+                        "heatings of the main workload, lengths U(5,60) days, one graph replay",
+            "members": E, "member_days": round(float(lengths.sum()) / 72.0, 2),
+            "fp64": out["fp64"], "fp32": out["fp32"],
+            "speedup_fp32_over_fp64": round(out["fp32"]["member_days_per_s"]
+                                            / out["fp64"]["member_days_per_s"], 3),
+            "fp32_rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
+            "max_rel_l2": float(f"{max(err.values()):.3e}")}
+
+
+def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0)):
+    """Adjoint sweep time over forward run time for the streamfunction
+    response of the uneven ensemble's first E members."""
+    import numpy as np
+    import torch
+    import lbm_isca_amd as L
+    from lbm_isca_amd import adjoint
+    bs, frc, lengths, sigm = uneven_ensemble(E, max_days=max_days)
+    nmax_steps = int(lengths.max())
+    res = {}
+    for prec in ("fp64", "fp32"):
+        cfg = L.Config(precision=prec, tangent_linear=True)
+        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
+        plan = m.plan
+        lat = np.degrees(plan.get("lat", plan.nlat))
+        lon = np.degrees(plan.get("lon", plan.nlon))
+        adjoint.StreamfunctionResponse(plan, lat, lon, plan.get("gw", plan.nlat),
+                                       plan.get("sig", plan.nlev), 0.2, (30.0, 60.0),
+                                       (150.0, 210.0), window[0], window[1], nmax_steps, 72,
+                                       cfg.inistp)
+        plan.set_forcing(frc)
+        plan.set_lengths(lengths)
+        plan.set_spectral_state()
+        plan.run(0, use_graph=True)
+        plan.adjoint_run(0)
+        torch.cuda.synchronize()
+        plan.set_spectral_state()
+        t0 = time.perf_counter()
+        plan.run(0, use_graph=True)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        J = plan.get_response()
+        t0 = time.perf_counter()
+        gx, gf = plan.adjoint_run(0)
+        torch.cuda.synchronize()
+        ta = time.perf_counter() - t0
+        # from a zero initial state J_e = <dJ_e/df_e, f_e> exactly (linearity)
+        pred = (gf.cpu().numpy() * frc).reshape(E, -1).sum(1)
+        sel = np.abs(J) > 0
+        lin = float(np.max(np.abs(pred[sel] - J[sel]) / np.abs(J[sel]))) if sel.any() else 0.0
+        res[prec] = {"forward_s": round(tf, 4), "adjoint_s": round(ta, 4),
+                     "adjoint_over_forward": round(ta / tf, 3),
+                     "members_with_window": int(sel.sum()),
+                     "max_rel_err_J_vs_grad_dot_forcing": float(f"{lin:.2e}")}
+        log(f"adjoint {prec}: {res[prec]}")
+        del m, plan
+        torch.cuda.empty_cache()
+    return {"workload": f"T42L20 {E} members of the uneven ensemble, tangent-linear model, "
+                        "J = area mean of psi(sigma=0.2) over 30-60N 150-210E, days "
+                        f"{window[0]:g}-{window[1]:g}; gradients w.r.t. forcing and spectral "
+                        "initial state",
+            "members": E, **res}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +224,16 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP32-vs-FP64 uneven ensemble and the adjoint timing")
+    ap.add_argument("--only", choices=("uneven", "adjoint"), default=None,
+                    help="run only one of the extra measurements (profiling)")
+    ap.add_argument("--uneven-members", type=int, default=256)
+    ap.add_argument("--adjoint-members", type=int, default=32)
+    ap.add_argument("--max-days", type=float, default=None,
+                    help="cap the uneven lengths (profiling runs only)")
+    ap.add_argument("--window", type=float, nargs=2, default=(20.0, 30.0),
+                    help="response averaging window in days")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); members are "
@@ -85,6 +244,13 @@ def main():
     from lbm_isca_amd import roofline
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a HIP device")
+    if args.only is not None:
+        if args.only == "uneven":
+            res = measure_fp32_vs_fp64(args.uneven_members, args.max_days)
+        else:
+            res = measure_adjoint(args.adjoint_members, args.max_days, tuple(args.window))
+        print(json.dumps({args.only: res}), flush=True)
+        return
     E = args.members
     m, bs, frc, vg = build_model(E, args.precision)
     plan = m.plan
@@ -114,9 +280,27 @@ def main():
     log("kernel ms/launch: " + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()))
     rl = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
           "peak": roofline.MI355X_HBM_GBS, "unit": "GB/s",
-          "frac": round(achieved / roofline.MI355X_HBM_GBS, 4), "traffic": None,
+          "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
+          "traffic": measured_traffic(dom, E, args.precision),
           "algorithmic_bytes": kb[dom], "avg_ms": round(avg[dom], 5),
           "kernels_ms": {k: round(v, 5) for k, v in avg.items()}}
+
+    extras = {}
+    if not args.no_extras:
+        del m, plan
+        torch.cuda.empty_cache()
+        for key, fn in (("fp32_vs_fp64",
+                         lambda: measure_fp32_vs_fp64(args.uneven_members, args.max_days)),
+                        ("adjoint", lambda: measure_adjoint(args.adjoint_members, args.max_days,
+                                                            tuple(args.window)))):
+            log(f"{key} ...")
+            try:
+                extras[key] = fn()
+            except (FloatingPointError, RuntimeError, ValueError) as exc:
+                # the main line stands; the failed part says why
+                extras[key] = {"error": f"{type(exc).__name__}: {exc}"}
+                log(f"{key} failed: {exc}")
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -133,7 +317,7 @@ def main():
                                f"{args.precision.upper()}, 20 equal sigma levels",
                    "members": E, "resolution": "T42L20", "dt_s": 1200.0,
                    "graph": True, "startup_steps_in_timed_region": 2},
-        "roofline": rl, "cpu_baseline": cpu,
+        "roofline": rl, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(out), flush=True)
 
