@@ -22,6 +22,47 @@ import numpy as np
 from .constants import GRAV, RDGAS
 
 EQUAL_SIGM_L20 = np.linspace(1.0, 0.0, 21)
+EQUAL_SIGM_L40 = np.linspace(1.0, 0.0, 41)
+
+# Per-level drag (days) of 40 equally spaced levels: each entry of the
+# reference's 20-level profile (model.py:30-31) covers two levels, so the
+# strong surface drag, the free-atmosphere 20 days and the weak top drag sit
+# at the same sigma ranges.
+DDRAG_L40 = list(np.repeat([0.5, 0.5, 0.5, 5, 15, 20, 20, 20, 20, 20,
+                            20, 20, 20, 20, 20, 20, 20, 20, 0.5, 0.5], 2).astype(float))
+
+
+def t127l40_config(**kw):
+    """Config keywords of the T127 L40 workload: the 40-level drag for
+    vorticity, divergence and temperature, defaults otherwise."""
+    out = dict(ddragv=list(DDRAG_L40), ddragd=list(DDRAG_L40), ddragt=list(DDRAG_L40))
+    out.update(kw)
+    return out
+
+
+def vorticity_perturbation(sp, nlev, nmem, seed=4, rms=1.0e-6, slope=-1.5):
+    """Seeded spectral vorticity (E, M, M, L) with amplitude spectrum
+    proportional to n^slope (energy ~ n^(2 slope)), random phases, real
+    m = 0, zero at n = 0, scaled to the given grid rms (s^-1) per member."""
+    rng = np.random.default_rng(seed)
+    M = sp.nmax + 1
+    mm, nn = np.meshgrid(np.arange(M), np.arange(M), indexing="ij")
+    sel = (nn >= mm) & (nn >= 1)
+    amp = np.where(sel, np.maximum(nn, 1).astype(float) ** slope, 0.0)
+    out = np.zeros((nmem, M, M, nlev), dtype=np.complex128)
+    for e in range(nmem):
+        ph = rng.uniform(0.0, 2.0 * np.pi, (M, M, nlev))
+        x = amp[:, :, None] * np.exp(1j * ph)
+        x[0] = x[0].real
+        out[e] = np.where(sel[:, :, None], x, 0.0)
+    g = np.asarray(_host(sp.wave_to_grid(out)))
+    r = np.sqrt((g ** 2).mean(axis=(1, 2, 3)))
+    return out * (rms / r)[:, None, None, None]
+
+
+def _host(x):
+    from .backend import asnumpy
+    return asnumpy(x)
 
 
 def solid_body_state(lat, lon, sig, u0=20.0, ps=1000.0):

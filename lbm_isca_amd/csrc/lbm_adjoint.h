@@ -6,28 +6,18 @@
 // coefficients, grid values):
 //   spectral step      -> k_step_adj  (M^T, V^T, tmtx^T, WW^T, RA^T)
 //   Legendre analysis  -> k_syn_mfma with the analysis tables (gw P, gw H)
-//   forward FFT (/N)   -> inverse FFT of X_m (1 or 1/2)/N
+//   forward FFT (/N)   -> inverse FFT of X_m (1 or 1/2)/N       (mode 1)
 //   grid tendencies    -> k_tend_adj  (levels top-down, suffix sums)
-//   inverse FFT        -> forward FFT times N (1 or 2)
+//   inverse FFT        -> forward FFT times N (1 or 2), no 1/N   (mode 2)
 //   Legendre synthesis -> k_ana_mfma with the synthesis tables (P, H)
-// (i m)^T = -(i m); the FFT factors are folded into the Legendre recipes and
-// the 1/(a cos lat) factors of the vector fields into the grid kernels.
+// (i m)^T = -(i m); the FFT factors are applied by the FFT kernels (their
+// mode argument) and the 1/(a cos lat) factors of the vector fields by the
+// grid kernels.
 #pragma once
 #include "lbm_mfma.h"
 #include "lbm_model.h"
 
 namespace lbm {
-
-// transpose of rfft/nlon seen through the real inverse transform
-template <class T>
-__device__ __forceinline__ T fft_adj_scale(T fm) {
-  return (fm == (T)0 ? (T)1 : (T)0.5) / (T)FFT_I;
-}
-// transpose of the real inverse transform seen through rfft/nlon
-template <class T>
-__device__ __forceinline__ T ifft_adj_scale(T fm) {
-  return (fm == (T)0 ? (T)1 : (T)2) * (T)FFT_I;
-}
 
 // ---------------------------------------------------------------------------
 // Synthesis-kernel recipes (run with the analysis tables gw P, gw H and the
@@ -42,8 +32,7 @@ struct AdjTend {
   static constexpr bool scaled(int) { return false; }
   template <class T>
   __device__ __forceinline__ static void terms(const cx<T> *x, T nn1a2, T fm, cx<T> *A, cx<T> *B) {
-    const T sc = fft_adj_scale<T>(fm);
-    const cx<T> zv = sc * x[0], zd = sc * x[1], zt = sc * x[2], zq = sc * x[3];
+    const cx<T> zv = x[0], zd = x[1], zt = x[2], zq = x[3];
     A[0] = im_mul(-fm, zd);  // fu': div = i m AP[fu'] + ...
     B[0] = zv;               //      vor = AH[fu'] + ...
     A[1] = im_mul(-fm, zv);  // fv': vor = i m AP[fv'] + ...
@@ -64,8 +53,8 @@ struct AdjScalarFft {
   static constexpr bool hasB(int) { return false; }
   static constexpr bool scaled(int) { return false; }
   template <class T>
-  __device__ __forceinline__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *) {
-    A[0] = fft_adj_scale<T>(fm) * x[0];
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T, cx<T> *A, cx<T> *) {
+    A[0] = x[0];
   }
 };
 
@@ -83,12 +72,11 @@ struct AdjSynLevel {
   template <class T>
   __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T il,
                                                  cx<T> *o) {
-    const T sc = ifft_adj_scale<T>(fm);
     // U = P (i m il div) - H (il vor),  V = P (i m il vor) + H (il div)
-    o[0] = sc * ((-il) * AH[0] + il * im_mul(-fm, AP[1]) + AP[2]);
-    o[1] = sc * (il * im_mul(-fm, AP[0]) + il * AH[1] + AP[3]);
-    o[2] = sc * AP[4];
-    o[3] = sc * AP[5];
+    o[0] = (-il) * AH[0] + il * im_mul(-fm, AP[1]) + AP[2];
+    o[1] = il * im_mul(-fm, AP[0]) + il * AH[1] + AP[3];
+    o[2] = AP[4];
+    o[3] = AP[5];
   }
 };
 // transpose of SynSurf: (d/dx pi, d/dy pi, pi, sum u, sum v, sum D)* ->
@@ -101,10 +89,9 @@ struct AdjSynSurf {
   template <class T>
   __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T il,
                                                  cx<T> *o) {
-    const T sc = ifft_adj_scale<T>(fm);
-    o[0] = sc * (im_mul(-fm, AP[0]) + AH[1] + AP[2]);
-    o[1] = sc * ((-il) * AH[3] + il * im_mul(-fm, AP[4]));
-    o[2] = sc * (il * im_mul(-fm, AP[3]) + il * AH[4] + AP[5]);
+    o[0] = im_mul(-fm, AP[0]) + AH[1] + AP[2];
+    o[1] = (-il) * AH[3] + il * im_mul(-fm, AP[4]);
+    o[2] = il * im_mul(-fm, AP[3]) + il * AH[4] + AP[5];
   }
 };
 // transpose of vorticity -> streamfunction on the grid (inverse Laplacian,
@@ -115,9 +102,9 @@ struct AdjPsi {
   static constexpr bool needP(int) { return true; }
   static constexpr bool needH(int) { return false; }
   template <class T>
-  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *, T fm, T il,
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *, T, T il,
                                                  cx<T> *o) {
-    o[0] = (ifft_adj_scale<T>(fm) * il) * AP[0];
+    o[0] = il * AP[0];
   }
 };
 
@@ -140,8 +127,10 @@ struct TendAdjArgs {
 
 template <class T>
 __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
-  const int i = threadIdx.x;
-  const int e = blockIdx.x / a.J, j = blockIdx.x - e * a.J;
+  const int nch = a.I / 128;
+  const int ej = blockIdx.x / nch;
+  const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
+  const int e = ej / a.J, j = ej - e * a.J;
   const int L = a.L, I = a.I, J = a.J;
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};

@@ -341,17 +341,23 @@ int build_tabs(lbm_plan *p) {
   return LBM_OK;
 }
 
+// exp(-2 pi i k / n), k < n, for the 128- and 384-point transforms
 template <class T>
 int set_twiddles() {
-  cx<T> tw[FFT_I];
-  for (int k = 0; k < FFT_I; ++k) {
-    const double ang = -2.0 * M_PI * k / FFT_I;
-    tw[k] = mk((T)std::cos(ang), (T)std::sin(ang));
+  for (int n : {128, 384}) {
+    std::vector<cx<T>> tw(n);
+    for (int k = 0; k < n; ++k) {
+      const long double ang = -2.0L * (long double)M_PI * k / n;
+      tw[k] = mk((T)std::cos(ang), (T)std::sin(ang));
+    }
+    const size_t b = sizeof(cx<T>) * n;
+    if (sizeof(T) == 8)
+      HIPCHK(n == 128 ? hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_d), tw.data(), b)
+                      : hipMemcpyToSymbol(HIP_SYMBOL(c_tw384_d), tw.data(), b));
+    else
+      HIPCHK(n == 128 ? hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_f), tw.data(), b)
+                      : hipMemcpyToSymbol(HIP_SYMBOL(c_tw384_f), tw.data(), b));
   }
-  if (sizeof(T) == 8)
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_d), tw, sizeof(tw)));
-  else
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tw128_f), tw, sizeof(tw)));
   return LBM_OK;
 }
 
@@ -389,24 +395,40 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
   return LBM_OK;
 }
 
-template <class T>
-int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st) {
+// FFTs over the rows j*C + col of a column map; mode: fourier_scale (0 for
+// the model, 1/2 for the transposed transforms of the adjoint)
+template <class T, int NI>
+int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode) {
+  using S = FftShape<NI>;
   const int C = cm.ncols();
   const long rows = (long)p->J * C;
-  hipLaunchKernelGGL((k_ifft128<T>), dim3((unsigned)((rows + FFT_ROWS - 1) / FFT_ROWS)), dim3(256),
-                     0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M);
+  hipLaunchKernelGGL((k_ifft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
+                     dim3(S::THREADS), 0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M, mode);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T, int NI>
+int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode) {
+  using S = FftShape<NI>;
+  const int C = cm.ncols();
+  const long rows = (long)p->J * C;
+  hipLaunchKernelGGL((k_fft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
+                     dim3(S::THREADS), 0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M, mode);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 template <class T>
-int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st) {
-  const int C = cm.ncols();
-  const long rows = (long)p->J * C;
-  hipLaunchKernelGGL((k_fft128<T>), dim3((unsigned)((rows + FFT_ROWS - 1) / FFT_ROWS)), dim3(256),
-                     0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M);
-  HIPCHK(hipGetLastError());
-  return LBM_OK;
+int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode = 0) {
+  return p->I == 128 ? launch_ifft_n<T, 128>(p, F, G, cm, st, mode)
+                     : launch_ifft_n<T, 384>(p, F, G, cm, st, mode);
+}
+
+template <class T>
+int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode = 0) {
+  return p->I == 128 ? launch_fft_n<T, 128>(p, G, F, cm, st, mode)
+                     : launch_fft_n<T, 384>(p, G, F, cm, st, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -686,7 +708,7 @@ int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, co
     CHK((launch_syn<double, SynGrad>(p, Sln, F, nb, 2 * nb, s2, st)));
     CHK(launch_ifft<double>(p, F, gg, ColMap{nb, 1, 0, 2}, st));
   }
-  hipLaunchKernelGGL(k_basic_terms, dim3(nb * J), dim3(I), 0, st, u, v, t, q, ps, gs, gg,
+  hipLaunchKernelGGL(k_basic_terms, dim3(nb * J * (I / 128)), dim3(128), 0, st, u, v, t, q, ps, gs, gg,
                      p->d64.vc, p->d_mu, bsg64, J, I, L);
   hipLaunchKernelGGL(k_tbar, dim3(nb * L), dim3(256), 0, st, bsg64, tb, J, I, L);
   HIPCHK(hipGetLastError());
@@ -924,7 +946,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   CHK(mark(p, 2, st));
   CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st));
   CHK(mark(p, 3, st));
-  hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
+  hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st));
@@ -1096,7 +1118,7 @@ int set_response_t(lbm_plan *p, const double *wgrid, const double *wlev, const d
                      J, I, 1, 1, 0, (const double *)nullptr);
   HIPCHK(hipGetLastError());
   const ColMap cm{1, 1, 1, 0};
-  CHK(launch_fft<T>(p, g, F, cm, st));
+  CHK(launch_fft<T>(p, g, F, cm, st, 2));
   DevTabs<T> &t = p->tabs<T>();
   InSlots sl{{0}};
   CHK((launch_ana<T, AdjPsi>(p, F, p->obs, 1, 1, sl, st, 1, 0, t.Ps, t.Hs, t.inv_lap)));
@@ -1174,7 +1196,7 @@ int adjoint_step(lbm_plan *p, hipStream_t st) {
     CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, E, Co, sl, st, 2, t.Pa, t.Ha, t.nn1a2)));
   }
   CHK(mark(p, 4, st));
-  CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st));
+  CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st, 1));
   CHK(mark(p, 3, st));
   TendAdjArgs<T> ta;
   ta.gout_a = (const T *)p->gout;
@@ -1189,10 +1211,10 @@ int adjoint_step(lbm_plan *p, hipStream_t st) {
   ta.I = p->I;
   ta.L = L;
   ta.nbasic = p->nb;
-  hipLaunchKernelGGL((k_tend_adj<T>), dim3(E * p->J), dim3(p->I), 0, st, ta);
+  hipLaunchKernelGGL((k_tend_adj<T>), dim3(E * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
-  CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st));
+  CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2));
   CHK(mark(p, 0, st));
   {
     InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
@@ -1319,11 +1341,13 @@ int lbm_plan_destroy(lbm_plan *p) {
 int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   if (!d || !out) return fail(LBM_ERR_VALUE, "null descriptor or output");
   *out = nullptr;
-  if (d->nlon != FFT_I) return fail(LBM_ERR_VALUE, "nlon=%d: only 128 longitudes are built", d->nlon);
+  if (d->nlon != 128 && d->nlon != 384)
+    return fail(LBM_ERR_VALUE, "nlon=%d: 128 or 384 longitudes are built", d->nlon);
   if (d->nlat <= 0 || d->nlat % 2 || d->nlat > 64 * 32)
     return fail(LBM_ERR_VALUE, "nlat must be even and positive");
-  if (d->nmax < 1 || d->nmax + 1 > 64 || d->nmax >= d->nlat)
-    return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=128", d->nmax);
+  if (d->nmax < 1 || d->nmax + 1 > d->nlon / 2 || d->nmax >= d->nlat)
+    return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=%d, nlat=%d", d->nmax, d->nlon,
+                d->nlat);
   if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
   if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
   if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");

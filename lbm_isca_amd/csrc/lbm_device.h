@@ -1,5 +1,5 @@
 // Device code of the gfx950 time-stepping core: complex helpers, the
-// length-128 real FFT pair, the per-m Legendre synthesis/analysis with
+// real FFT pair (128 or 384 longitudes), the per-m Legendre synthesis/analysis with
 // north/south folding, the fused grid-point tendency kernel and the fused
 // spectral semi-implicit leapfrog + vertical diffusion + Robert-Asselin
 // kernel.  Everything is templated on the arithmetic type T (double/float).
@@ -112,140 +112,248 @@ struct ColMap {
 };
 
 // ---------------------------------------------------------------------------
-// Length-128 real FFTs, 8 threads per row (64 = 8 x 8 complex FFT of the
-// even/odd packed samples, then the real split).  Each transform is split
-// into stages separated by a block barrier: the caller places the
-// __syncthreads() between stages, so several rows (and other work) share
-// the barriers.  scr is an 8x9 complex scratch per row.
+// Real FFTs of length NI = 128 R (R = 1: 128 longitudes, R = 3: 384).  The
+// NI real samples of a row are NC = NI/2 complex pairs z_n = x_2n + i x_2n+1;
+// their complex DFT of length NC = 64 R is split as
+//   n = n_b + 64 n_a,  k = r + R q   (n_b, q < 64; n_a, r < R)
+// into R 64-point DFTs (over q for the inverse, over n_b for the forward),
+// twiddles exp(-+2 pi i n_b r / NC) and a radix-R DFT across r / n_a.  Each
+// 64-point DFT is 8 x 8 (radix 8, LDS transpose) on 8 threads, so a row
+// uses 8R threads; then the real split to/from X_m = rfft/nlon.  Stages are
+// separated by block barriers placed by the kernels.
 // ---------------------------------------------------------------------------
-constexpr int FFT_I = 128;
-constexpr int FFT_ROWS = 32;
-
-// Twiddles exp(-2 pi i k / 128), k = 0..127, filled by the host.
-template <class T>
-struct Tw;
-template <>
-struct Tw<double> {
-  static __device__ inline const cx<double> *get();
+template <int NI>
+struct FftShape {
+  static constexpr int R = NI / 128;
+  static constexpr int NC = NI / 2;
+  static constexpr int TPR = 8 * R;             // threads per row
+  static constexpr int ROWS = 256 / TPR;        // rows per block
+  static constexpr int THREADS = ROWS * TPR;
+  static constexpr int LDS = 8 * 9 * R;         // complex scratch per row (>= NC)
 };
+
+// Twiddles exp(-2 pi i k / NI), k = 0..NI-1, filled by the host.
+__constant__ cx<double> c_tw128_d[128];
+__constant__ cx<float> c_tw128_f[128];
+__constant__ cx<double> c_tw384_d[384];
+__constant__ cx<float> c_tw384_f[384];
+template <class T, int NI>
+__device__ inline const cx<T> *twiddles();
 template <>
-struct Tw<float> {
-  static __device__ inline const cx<float> *get();
-};
-__constant__ cx<double> c_tw128_d[FFT_I];
-__constant__ cx<float> c_tw128_f[FFT_I];
-__device__ inline const cx<double> *Tw<double>::get() { return c_tw128_d; }
-__device__ inline const cx<float> *Tw<float>::get() { return c_tw128_f; }
+__device__ inline const cx<double> *twiddles<double, 128>() { return c_tw128_d; }
+template <>
+__device__ inline const cx<float> *twiddles<float, 128>() { return c_tw128_f; }
+template <>
+__device__ inline const cx<double> *twiddles<double, 384>() { return c_tw384_d; }
+template <>
+__device__ inline const cx<float> *twiddles<float, 384>() { return c_tw384_f; }
 
-template <class T>
-using Scr = cx<T> (*)[9];  // [8][9]
+// exp(SIGN 2 pi i a / b) from the table, b dividing NI
+template <class T, int NI, int SIGN>
+__device__ __forceinline__ cx<T> twf(const cx<T> *tw, int a, int b) {
+  const cx<T> w = tw[(a * (NI / b)) % NI];
+  return SIGN < 0 ? w : conj(w);
+}
 
-// Inverse, stage a: X_m (m < M, from F with stride ms between m) -> scr.
+// radix-R DFT of v[0..R-1] in place (R = 1 or 3), sign SIGN
+template <int R, int SIGN, class T>
+__device__ __forceinline__ void dftR(cx<T> *v) {
+  if constexpr (R == 3) {
+    // exp(SIGN 2 pi i / 3) = -1/2 + SIGN i sqrt(3)/2
+    const T c = (T)-0.5, sn = (T)SIGN * (T)0.86602540378443864676;
+    const cx<T> s = v[1] + v[2], d = v[1] - v[2];
+    const cx<T> m = v[0] + c * s;
+    const cx<T> jd = mk(-sn * d.y, sn * d.x);  // i sn d
+    v[0] = v[0] + s;
+    v[1] = m + jd;
+    v[2] = m - jd;
+  }
+}
+
+// Scale of the Fourier coefficient m applied by the transforms: mode 0
+// none (the forward keeps its 1/nlon); 1 the transpose of rfft/nlon seen
+// through the real inverse ((1 or 1/2)/nlon, on the inverse's input); 2 the
+// transpose of the real inverse (1 or 2, on the forward's unnormalised
+// output in place of 1/nlon).
+template <class T, int NI>
+__device__ __forceinline__ T fourier_scale(int mode, int m) {
+  if (mode == 1) return (m == 0 ? (T)1 : (T)0.5) / (T)NI;
+  if (mode == 2) return m == 0 ? (T)1 : (T)2;
+  return (T)1;
+}
+
+// Inverse, stage a (thread (r, t) of the row): X_m (m < M, stride ms) ->
+// Z_k for k = r + R(t + 8 n2) -> first radix-8 pass -> scr.
 // g = X0.re + 2 sum_m Re(X_m e^{i m lam}) = irfft(zero-filled * nlon)
 // (spectral.py:166-172); the imaginary part of X0 is discarded.
-template <class T>
-__device__ inline void ifft_a(const cx<T> *__restrict__ F, size_t ms, int M, cx<T> (*scr)[9], int tt) {
-  const cx<T> *tw = Tw<T>::get();
+template <class T, int NI>
+__device__ inline void ifft_a(const cx<T> *__restrict__ F, size_t ms, int M, cx<T> *scr, int tt,
+                              int mode) {
+  using S = FftShape<NI>;
+  const cx<T> *tw = twiddles<T, NI>();
+  const int r = tt >> 3, t = tt & 7;
   cx<T> v[8];
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
-    const int kk = tt + 8 * n2;
+    const int k = r + S::R * (t + 8 * n2);
     cx<T> A = zero<T>(), B = zero<T>();
-    if (kk < M) {
-      A = F[(size_t)kk * ms];
-      if (kk == 0) A.y = 0;
+    if (k < M) {
+      A = fourier_scale<T, NI>(mode, k) * F[(size_t)k * ms];
+      if (k == 0) A.y = 0;
     }
-    const int kb = 64 - kk;
-    if (kb < M) B = conj(F[(size_t)kb * ms]);
-    const cx<T> w = conj(tw[kk]);  // exp(+2 pi i kk/128)
+    const int kb = S::NC - k;
+    if (kb < M) B = conj(fourier_scale<T, NI>(mode, kb) * F[(size_t)kb * ms]);
+    const cx<T> w = conj(tw[k]);  // exp(+2 pi i k / NI)
     v[n2] = (A + B) + cmul(mk((T)0, (T)1), cmul(w, A - B));
   }
   dft8<+1>(v);
+  cx<T> *sc = scr + r * 72 + t * 9;
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) scr[tt][k2] = cmul(v[k2], conj(tw[(2 * tt * k2) & 127]));
+  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = cmul(v[k2], twf<T, NI, +1>(tw, t * k2, 64));
 }
 
-// Inverse, stage b: scr -> 128 real samples (as 64 complex pairs) in out.
-template <class T>
-__device__ inline void ifft_b(cx<T> (*scr)[9], cx<T> *out, int tt) {
-  cx<T> v[8];
+// Inverse, stage b: second radix-8 pass of sub-transform r, then the
+// twiddle exp(+2 pi i n_b r / NC); returns Y_r[n_b = t + 8 k1] in v.
+template <class T, int NI>
+__device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt) {
+  const cx<T> *tw = twiddles<T, NI>();
+  const int r = tt >> 3, t = tt & 7;
 #pragma unroll
-  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[n1][tt];
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[r * 72 + n1 * 9 + t];
   dft8<+1>(v);
+  if (FftShape<NI>::R > 1) {
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) out[tt + 8 * k1] = v[k1];
-}
-
-// Forward, stage a: 128 real samples (64 complex pairs in) -> scr.
-template <class T>
-__device__ inline void fft_a(const cx<T> *in, cx<T> (*scr)[9], int tt) {
-  const cx<T> *tw = Tw<T>::get();
-  cx<T> v[8];
-#pragma unroll
-  for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[tt + 8 * n2];
-  dft8<-1>(v);
-#pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) scr[tt][k2] = cmul(v[k2], tw[(2 * tt * k2) & 127]);
-}
-
-// Forward, stage b: scr -> Z[0..63] (complex FFT of the packed samples).
-template <class T>
-__device__ inline void fft_b(cx<T> (*scr)[9], cx<T> *Z, int tt) {
-  cx<T> v[8];
-#pragma unroll
-  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[n1][tt];
-  dft8<-1>(v);
-#pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) Z[tt + 8 * k1] = v[k1];
-}
-
-// Forward, stage c: real split, X_m = rfft/nlon for m < M (spectral.py:159-164),
-// written to F with stride ms between m.
-template <class T>
-__device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, int M, int tt) {
-  const cx<T> *tw = Tw<T>::get();
-  const T half = (T)0.5, inv_n = (T)1 / (T)FFT_I;
-#pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) {
-    const int k = tt + 8 * k1;
-    if (k >= M) break;
-    const cx<T> zk = Z[k], zc = conj(Z[(64 - k) & 63]);
-    const cx<T> s = zk + zc, d = zk - zc;
-    // X = (s - i w d)/2, w = exp(-2 pi i k/128)
-    const cx<T> wd = cmul(tw[k], d);
-    F[(size_t)k * ms] = inv_n * mk(half * (s.x + wd.y), half * (s.y - wd.x));
+    for (int k1 = 0; k1 < 8; ++k1) v[k1] = cmul(v[k1], twf<T, NI, +1>(tw, (t + 8 * k1) * r, FftShape<NI>::NC));
   }
 }
 
-// Fourier (m <= M-1) -> grid, 32 rows per block; rows are j*C + col.
-template <class T>
-__global__ __launch_bounds__(256) void k_ifft128(const cx<T> *__restrict__ F, T *__restrict__ G,
-                                                 int C, ColMap cm, int J, int M) {
-  __shared__ cx<T> sh[FFT_ROWS][8][9];
-  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
-  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+// Inverse, stage c (R > 1): radix-R across r from ex[r][n_b] for output
+// block n_a = r of this thread: z[n_b + 64 n_a], n_b = t + 8 k1.
+template <class T, int NI>
+__device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
+  constexpr int R = FftShape<NI>::R;
+  const int na = tt >> 3, t = tt & 7;
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) {
+    const int nb = t + 8 * k1;
+    cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = ex[r * 64 + nb];
+    dftR<R, +1>(v);
+    cx<T> o = v[0];
+#pragma unroll
+    for (int q = 1; q < R; ++q)
+      if (q == na) o = v[q];
+    out[nb + 64 * na] = o;
+  }
+}
+
+// Forward, stage a (thread (r, t)): radix-R across n_a of z[n_b + 64 n_a]
+// for n_b = t + 8 n2, keep output r, twiddle exp(-2 pi i n_b r / NC), first
+// radix-8 pass -> scr.
+template <class T, int NI>
+__device__ inline void fft_a(const cx<T> *in, cx<T> *scr, int tt) {
+  constexpr int R = FftShape<NI>::R;
+  const cx<T> *tw = twiddles<T, NI>();
+  const int r = tt >> 3, t = tt & 7;
+  cx<T> v[8];
+#pragma unroll
+  for (int n2 = 0; n2 < 8; ++n2) {
+    const int nb = t + 8 * n2;
+    if constexpr (R == 1) {
+      v[n2] = in[nb];
+    } else {
+      cx<T> u[R];
+#pragma unroll
+      for (int na = 0; na < R; ++na) u[na] = in[nb + 64 * na];
+      dftR<R, -1>(u);
+      cx<T> ur = u[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q)
+        if (q == r) ur = u[q];
+      v[n2] = cmul(ur, twf<T, NI, -1>(tw, nb * r, FftShape<NI>::NC));
+    }
+  }
+  dft8<-1>(v);
+  cx<T> *sc = scr + r * 72 + t * 9;
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = cmul(v[k2], twf<T, NI, -1>(tw, t * k2, 64));
+}
+
+// Forward, stage b: second radix-8 pass; v[k1] = Z[r + R (t + 8 k1)].
+template <class T, int NI>
+__device__ inline void fft_b(const cx<T> *scr, cx<T> *v, int tt) {
+  const int r = tt >> 3, t = tt & 7;
+#pragma unroll
+  for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[r * 72 + n1 * 9 + t];
+  dft8<-1>(v);
+}
+
+// Forward, stage c: real split, X_m = rfft/nlon for m < M (spectral.py:159-164)
+// (times the mode's scale), written to F with stride ms between m.
+template <class T, int NI>
+__device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, int M, int tt,
+                             int mode) {
+  using S = FftShape<NI>;
+  const cx<T> *tw = twiddles<T, NI>();
+  const T half = (T)0.5;
+  for (int k = tt; k < M; k += S::TPR) {
+    const cx<T> zk = Z[k], zc = conj(Z[(S::NC - k) % S::NC]);
+    const cx<T> s = zk + zc, d = zk - zc;
+    // X = (s - i w d)/2, w = exp(-2 pi i k / NI)
+    const cx<T> wd = cmul(tw[k], d);
+    const T sc = mode == 2 ? fourier_scale<T, NI>(2, k) : (T)1 / (T)NI;
+    F[(size_t)k * ms] = sc * mk(half * (s.x + wd.y), half * (s.y - wd.x));
+  }
+}
+
+// Fourier (m <= M-1) -> grid; rows are j*C + col.  mode: fourier_scale.
+template <class T, int NI>
+__global__ __launch_bounds__(FftShape<NI>::THREADS) void k_ifft(const cx<T> *__restrict__ F,
+                                                                T *__restrict__ G, int C, ColMap cm,
+                                                                int J, int M, int mode) {
+  using S = FftShape<NI>;
+  __shared__ cx<T> sh[S::ROWS][S::LDS];
+  const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
+  const long row = (long)blockIdx.x * S::ROWS + rl;
   const bool live = row < (long)J * C;
   int j = 0, col = 0;
   if (live) {
     j = (int)(row / C);
     col = (int)(row - (long)j * C);
-    ifft_a<T>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt);
+    ifft_a<T, NI>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt, mode);
   }
   __syncthreads();
-  if (!live) return;
-  int e, r;
-  cm.map(col, e, r);
-  ifft_b<T>(sh[rl], reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I), tt);
+  cx<T> v[8];
+  if (live) ifft_b<T, NI>(sh[rl], v, tt);
+  int e = 0, r = 0;
+  if (live) cm.map(col, e, r);
+  cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+  if constexpr (S::R == 1) {
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) out[tt + 8 * k1] = v[k1];
+    }
+  } else {
+    __syncthreads();  // scr is reused as the exchange buffer
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) sh[rl][(tt >> 3) * 64 + (tt & 7) + 8 * k1] = v[k1];
+    }
+    __syncthreads();
+    if (live) ifft_c<T, NI>(sh[rl], out, tt);
+  }
 }
 
-// grid -> Fourier (m <= M-1), 32 rows per block.
-template <class T>
-__global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *__restrict__ F,
-                                                int C, ColMap cm, int J, int M) {
-  __shared__ cx<T> sh[FFT_ROWS][8][9];
-  __shared__ cx<T> zb[FFT_ROWS][64];
-  const int t = threadIdx.x, rl = t >> 3, tt = t & 7;
-  const long row = (long)blockIdx.x * FFT_ROWS + rl;
+// grid -> Fourier (m <= M-1).  mode: fourier_scale.
+template <class T, int NI>
+__global__ __launch_bounds__(FftShape<NI>::THREADS) void k_fft(const T *__restrict__ G,
+                                                               cx<T> *__restrict__ F, int C,
+                                                               ColMap cm, int J, int M, int mode) {
+  using S = FftShape<NI>;
+  __shared__ cx<T> sh[S::ROWS][S::LDS];
+  const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
+  const long row = (long)blockIdx.x * S::ROWS + rl;
   const bool live = row < (long)J * C;
   int j = 0, col = 0;
   if (live) {
@@ -253,13 +361,20 @@ __global__ __launch_bounds__(256) void k_fft128(const T *__restrict__ G, cx<T> *
     col = (int)(row - (long)j * C);
     int e, r;
     cm.map(col, e, r);
-    fft_a<T>(reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * FFT_I),
-             sh[rl], tt);
+    fft_a<T, NI>(reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI),
+                 sh[rl], tt);
   }
   __syncthreads();
-  if (live) fft_b<T>(sh[rl], zb[rl], tt);
+  cx<T> v[8];
+  if (live) fft_b<T, NI>(sh[rl], v, tt);
+  __syncthreads();  // scr is reused for Z
+  if (live) {
+    const int rr = tt >> 3, tq = tt & 7;
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) sh[rl][rr + S::R * (tq + 8 * k1)] = v[k1];
+  }
   __syncthreads();
-  if (live) fft_c<T>(zb[rl], F + (size_t)j * C + col, (size_t)J * C, M, tt);
+  if (live) fft_c<T, NI>(sh[rl], F + (size_t)j * C + col, (size_t)J * C, M, tt, mode);
 }
 
 // ---------------------------------------------------------------------------

@@ -222,8 +222,11 @@ struct TendArgs {
 // d/dx pi, d/dy pi, pi, sum u, sum v, sum D.
 template <class T>
 __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
-  const int i = threadIdx.x;
-  const int e = blockIdx.x / a.J, j = blockIdx.x - e * a.J;
+  // blocks of 128 longitudes: blockIdx = (e * J + j) * (I / 128) + chunk
+  const int nch = a.I / 128;
+  const int ej = blockIdx.x / nch;
+  const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
+  const int e = ej / a.J, j = ej - e * a.J;
   const int L = a.L, I = a.I, J = a.J;
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};
@@ -657,8 +660,10 @@ __global__ __launch_bounds__(128) void k_basic_terms(
     const double *__restrict__ q, const double *__restrict__ ps,  // reference layouts
     const double *__restrict__ gsyn, const double *__restrict__ ggrad, const double *__restrict__ vc,
     const double *__restrict__ mu, double *__restrict__ bsg, int J, int I, int L) {
-  const int i = threadIdx.x;
-  const int b = blockIdx.x / J, j = blockIdx.x - b * J;
+  const int nch = I / 128;
+  const int bj = blockIdx.x / nch;
+  const int i = (blockIdx.x - bj * nch) * 128 + threadIdx.x;
+  const int b = bj / J, j = bj - b * J;
   const VOff vo{L};
   const BRow br{L};
   const size_t col = ((size_t)b * J + j) * I + i;  // (b, j, i) in 2-D reference layout

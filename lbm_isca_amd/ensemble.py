@@ -53,10 +53,11 @@ def build_heating_sweep(base_heat, nmem, mode="longitude", xp=None):
 
 
 def make_ensemble_model(basic_state, nmem, config=None, device="hip", sigm=None,
-                        per_member_basic=False):
+                        per_member_basic=False, spectral=None):
     """E-member :class:`LBM` on one device; returns ``(model, spectral, vgrid)``
-    (``ensemble.py:152-165``)."""
-    sp = BatchedSpectral(device=device)
+    (``ensemble.py:152-165``).  ``spectral`` (a BatchedSpectral) selects
+    another resolution than T42 (e.g. ``BatchedSpectral(384, 192, 127)``)."""
+    sp = spectral if spectral is not None else BatchedSpectral(device=device)
     vg = VerticalGrid(device=device) if sigm is None else VerticalGrid(sigm, device=device)
     cfg = config or Config()
     cfg.nmem = nmem

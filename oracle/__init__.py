@@ -27,7 +27,10 @@ Pinning (what checks the restatement itself, see ``tests/test_oracle_*.py``):
   step output with active damping and vertical diffusion.
 * ``model``    -- the reference-held Gill golden (gill_isca.json) and
   notebook maxima, reference-generated multi-step samples, ensemble
-  fold semantics (identical members == single run).
+  fold semantics (identical members == single run); linear mode by
+  superposition and first-order agreement with the strictly linear model.
+* ``gauss_grid(refine=True)`` -- extended-precision weights (not the
+  reference's ``leggauss``), checked against 40-digit Newton (mpmath).
 """
 
 from .sht import SHT, gauss_grid, legendre_tables  # noqa: F401

@@ -115,10 +115,11 @@ class GridState:
 class OracleModel:
     """The LBM time loop (``model.py:92-290``), E members sharing one basic state."""
 
-    def __init__(self, bs, nmem=1, cfg=None, sigm=None, nlon=128, nlat=64, nmax=42):
+    def __init__(self, bs, nmem=1, cfg=None, sigm=None, nlon=128, nlat=64, nmax=42,
+                 refine_gauss=False):
         self.cfg = cfg or OracleConfig()
         c = self.cfg
-        self.sht = SHT(nlon, nlat, nmax)
+        self.sht = SHT(nlon, nlat, nmax, refine_gauss=refine_gauss)
         self.vg = VerticalOps() if sigm is None else VerticalOps(sigm)
         self.bst = BasicTerms(self.sht, self.vg, bs)
         self.dmp = Damping(nmax, self.vg, order=c.hdif_order, tefold=c.hdif_tefold,

@@ -22,12 +22,33 @@ the arithmetic per column is identical.
 import numpy as np
 
 
-def gauss_grid(nlat):
+def gauss_grid(nlat, refine=False):
     """North-first Gauss-Legendre nodes and weights summing to one.
 
-    Reference: ``pylbm_isca/spectral.py:57-68``.
+    Reference: ``pylbm_isca/spectral.py:57-68`` (numpy ``leggauss``).
+    ``refine=True`` (not the reference's) polishes the nodes by Newton steps
+    on P_J and recomputes w = 2/((1-x^2) P_J'(x)^2) in extended precision:
+    ``leggauss`` weights carry relative errors growing with J (6e-11 at J =
+    192), which the device's weights do not.
     """
     mu, w = np.polynomial.legendre.leggauss(nlat)
+    if refine:
+        x = mu.astype(np.longdouble)
+        for _ in range(3):
+            p0, p1 = np.ones_like(x), x.copy()
+            for k in range(2, nlat + 1):
+                p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+            dp = nlat * (x * p1 - p0) / (x * x - 1)
+            x = x - p1 / dp
+        p0, p1 = np.ones_like(x), x.copy()
+        for k in range(2, nlat + 1):
+            p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+        dp = nlat * (x * p1 - p0) / (x * x - 1)
+        wl = 2 / ((1 - x * x) * dp * dp)
+        x = (x - x[::-1]) / 2
+        wl = (wl + wl[::-1]) / 2
+        wl = wl * (2 / wl.sum())
+        mu, w = x.astype(np.float64), wl.astype(np.float64)
     order = np.argsort(-mu)
     return mu[order], w[order] / 2.0
 
@@ -72,10 +93,10 @@ class SHT:
     Reference: ``pylbm_isca/spectral.py:43-289`` (class ``Spectral``).
     """
 
-    def __init__(self, nlon=128, nlat=64, nmax=42, radius=6.371e6):
+    def __init__(self, nlon=128, nlat=64, nmax=42, radius=6.371e6, refine_gauss=False):
         self.nlon, self.nlat, self.nmax = int(nlon), int(nlat), int(nmax)
         self.a = float(radius)
-        self.mu, self.gw = gauss_grid(self.nlat)
+        self.mu, self.gw = gauss_grid(self.nlat, refine_gauss)
         self.lat = np.arcsin(self.mu)
         self.coslat = np.sqrt(1.0 - self.mu ** 2)
         self.lon = np.arange(self.nlon) * (2.0 * np.pi / self.nlon)

@@ -82,3 +82,34 @@ def test_response_weights():
     assert ws.size == 2162 and abs(ws.sum() - 1.0) < 1e-12
     k = np.nonzero(ws)[0]
     assert k[0] == 2 + 1440 - 1 and k[-1] == 2 + 2160 - 1 and k.size == 721
+
+
+def test_refined_gauss_weights_against_mpmath():
+    """gauss_grid(refine=True) to ~1e-15 at J = 192, where leggauss's
+    weights are off by ~6e-11 (what separates the oracle's reference weights
+    from the device's)."""
+    import mpmath
+    from oracle import gauss_grid
+    J = 192
+    mpmath.mp.dps = 40
+    mu_r, gw_r = gauss_grid(J, refine=True)
+    mu_n, gw_n = gauss_grid(J)
+    ref_w = []
+    for x0 in mu_r[: J // 2]:
+        z = mpmath.mpf(x0)
+        for _ in range(4):
+            p0, p1 = mpmath.mpf(1), z
+            for k in range(2, J + 1):
+                p0, p1 = p1, ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+            dp = J * (z * p1 - p0) / (z * z - 1)
+            z = z - p1 / dp
+        p0, p1 = mpmath.mpf(1), z
+        for k in range(2, J + 1):
+            p0, p1 = p1, ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+        dp = J * (z * p1 - p0) / (z * z - 1)
+        ref_w.append(float(1 / ((1 - z * z) * dp * dp)))   # halved weight
+    ref_w = np.array(ref_w)
+    assert np.max(np.abs(gw_r[: J // 2] / ref_w - 1)) < 5e-15
+    assert np.max(np.abs(gw_n[: J // 2] / ref_w - 1)) > 1e-12
+    assert abs(gw_r.sum() - 1.0) < 1e-15
+    np.testing.assert_allclose(mu_r, mu_n, atol=3e-16)
