@@ -43,7 +43,12 @@ typedef enum {
   LBM_ERR_STATE = -4      /* call order (e.g. run before basic state): RuntimeError */
 } lbm_status;
 
-typedef enum { LBM_FP64 = 0, LBM_FP32 = 1 } lbm_precision;
+/* FP64 or FP32 arithmetic throughout; BF16X3: FP32 with the Legendre
+ * contractions as split-BF16x3 on the matrix cores (each FP32 operand split
+ * into a BF16 high and low part, three BF16 products in FP32 accumulation).
+ * Setup (Gaussian nodes, Legendre recurrences, semi-implicit inverses) is
+ * FP64 in every mode. */
+typedef enum { LBM_FP64 = 0, LBM_FP32 = 1, LBM_BF16X3 = 2 } lbm_precision;
 
 /* Model description.  Mirrors Spectral(nlon, nlat, nmax, radius)
  * (spectral.py:49-50), VerticalGrid(sigm) (vertical.py:59) and Config

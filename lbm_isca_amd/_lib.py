@@ -19,6 +19,7 @@ LBM_ERR_STATE = -4
 
 LBM_FP64 = 0
 LBM_FP32 = 1
+LBM_BF16X3 = 2
 
 # Every symbol include/lbm_hip.h declares.
 EXPORTS = (

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -206,6 +207,7 @@ struct lbm_plan {
   // sizes
   int J = 0, I = 0, N = 0, M = 0, L = 0, E = 0, nb = 0, prec = 0, nspec = 0, Jh = 0, nset = 0;
   int inistp = 2, strictly_linear = 0, skip_low_n = 1, vmask = 0, linear = 0;
+  bool split = false;  // LBM_BF16X3: split-BF16x3 Legendre contractions
   double a = 0, delt = 0, robert = 0;
   // host FP64 setup
   std::vector<double> mu, gw, coslat, lat, lon;
@@ -372,6 +374,15 @@ int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlot
                const T *per_n = nullptr) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid((ncols + 7) / 8, p->M, (p->Jh + 31) / 32);
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->split) {
+      hipLaunchKernelGGL((k_syn_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
+                         (cx<T> *)out, P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap,
+                         t.rcos, ncols, C, sl, p->J, p->M, in_nf);
+      HIPCHK(hipGetLastError());
+      return LBM_OK;
+    }
+  }
   hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
                      p->J, p->M, in_nf);
@@ -388,6 +399,15 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
   DevTabs<T> &t = p->tabs<T>();
   // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
   dim3 grid((ncols + 7) / 8, p->M, (p->M + 63) / 64);
+  if constexpr (std::is_same<T, float>::value) {
+    if (p->split) {
+      hipLaunchKernelGGL((k_ana_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
+                         (cx<T> *)out, P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols,
+                         C, sl, p->J, p->M, out_nf, out_f0);
+      HIPCHK(hipGetLastError());
+      return LBM_OK;
+    }
+  }
   hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
                      out_nf, out_f0);
@@ -725,7 +745,7 @@ int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, co
   HIPCHK(hipFree(w));
   CHK(build_matrices(p));
   CHK(upload_operators<double>(p));
-  if (p->prec == LBM_FP32) CHK(upload_operators<float>(p));
+  if (p->prec != LBM_FP64) CHK(upload_operators<float>(p));
   p->basic_set = true;
   return LBM_OK;
 }
@@ -1351,7 +1371,8 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
   if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
   if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");
-  if (d->precision != LBM_FP64 && d->precision != LBM_FP32) return fail(LBM_ERR_VALUE, "bad precision");
+  if (d->precision != LBM_FP64 && d->precision != LBM_FP32 && d->precision != LBM_BF16X3)
+    return fail(LBM_ERR_VALUE, "bad precision");
   if (!d->h_sigm || !d->h_kv || !d->h_kd || !d->h_ktt || !d->h_kq)
     return fail(LBM_ERR_VALUE, "sigm and damping arrays are required");
   if (d->inistp < 0 || d->inistp > 8) return fail(LBM_ERR_VALUE, "inistp out of range");
@@ -1364,6 +1385,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->E = d->nmem;
   p->nb = d->nbasic;
   p->prec = d->precision;
+  p->split = d->precision == LBM_BF16X3;
   p->nspec = p->M * (p->M + 1) / 2;
   p->Jh = p->J / 2;
   p->inistp = d->inistp;
@@ -1408,7 +1430,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   };
   if (hipStreamCreate(&p->own) != hipSuccess) return bail(fail(LBM_ERR_RUNTIME, "stream create failed"));
   if ((rc = build_tabs<double>(p)) != LBM_OK) return bail(rc);
-  if (p->prec == LBM_FP32 && (rc = build_tabs<float>(p)) != LBM_OK) return bail(rc);
+  if (p->prec != LBM_FP64 && (rc = build_tabs<float>(p)) != LBM_OK) return bail(rc);
   if ((rc = set_twiddles<double>()) != LBM_OK) return bail(rc);
   if ((rc = set_twiddles<float>()) != LBM_OK) return bail(rc);
   // identity-free defaults for the reference temperature until a basic state

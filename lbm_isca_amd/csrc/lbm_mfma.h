@@ -12,6 +12,11 @@
 //   A[i][k]: lane i + 16k;  B[k][j]: lane j + 16k;
 //   D[i][j] value v of lane l: j = l%16, i = l/16 + 4v (f64) or
 //   4(l/16) + v (f32).
+// Split-BF16x3 (FP32 data, SPLIT kernels): v_mfma_f32_16x16x16bf16_1k with
+//   A[i][k]: lane i + 16(k/4), element k%4;  B[k][j]: lane j + 16(k/4),
+//   element k%4;  D as f32.  Each operand x = hi + lo (hi = bf16(x), lo =
+//   bf16(x - hi)) and a b ~ hi_a hi_b + hi_a lo_b + lo_a hi_b in FP32
+//   accumulation (the lo_a lo_b term, ~2^-16 relative, is dropped).
 #pragma once
 #include "lbm_device.h"
 
@@ -38,6 +43,33 @@ struct Mfma<float> {
   static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+struct Bf16x3 {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef short s4 __attribute__((ext_vector_type(4)));
+  // round to nearest even (finite inputs)
+  static __device__ __forceinline__ short rne(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+  }
+  static __device__ __forceinline__ float up(short h) {
+    return __uint_as_float(((unsigned)(unsigned short)h) << 16);
+  }
+  static __device__ __forceinline__ void split(const float *x, s4 &hi, s4 &lo) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const short h = rne(x[q]);
+      hi[q] = h;
+      lo[q] = rne(x[q] - up(h));
+    }
+  }
+  static __device__ __forceinline__ v4 mma3(s4 ah, s4 al, s4 bh, s4 bl, v4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+  }
+};
+
 // select between two loaded values (keeps aggregates out of private memory)
 template <class T>
 __device__ __forceinline__ T pick(T re, T im, int part) {
@@ -58,7 +90,7 @@ __device__ inline cx<T> pair_of(T mine, int part) {
 // 2z+1 (more than one only when N+1-m > 64).  K = latitude pairs, staged 32
 // at a time per field, double-buffered in LDS.
 // ---------------------------------------------------------------------------
-template <class T, class R>
+template <class T, class R, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Pa,
@@ -120,13 +152,42 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
       else if (jc + 1 < nch)
         stage(jc + 1, 0, buf ^ 1);
       if (wave_live) {
+        if constexpr (SPLIT) {
 #pragma unroll
-        for (int ks = 0; ks < 32; ks += 4) {
-          const int jp = 32 * jc + ks + k_a;
-          const bool ok = row_ok && jp < Jh;
-          const T xs = sym[buf][ks + k_a][j_b], xa = anti[buf][ks + k_a][j_b];
-          if (R::needP(f)) AP[f] = Mfma<T>::mma(ok ? Pa[trow + jp] : (T)0, p == 0 ? xs : xa, AP[f]);
-          if (R::needH(f)) AH[f] = Mfma<T>::mma(ok ? Ha[trow + jp] : (T)0, p == 0 ? xa : xs, AH[f]);
+          for (int ks = 0; ks < 32; ks += 16) {
+            float pa[4], ha[4], xp[4], xh[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int kk = ks + 4 * k_a + q;
+              const int jp = 32 * jc + kk;
+              const bool ok = row_ok && jp < Jh;
+              pa[q] = (ok && R::needP(f)) ? (float)Pa[trow + jp] : 0.f;
+              ha[q] = (ok && R::needH(f)) ? (float)Ha[trow + jp] : 0.f;
+              const float xs = sym[buf][kk][j_b], xa = anti[buf][kk][j_b];
+              xp[q] = p == 0 ? xs : xa;
+              xh[q] = p == 0 ? xa : xs;
+            }
+            Bf16x3::s4 ah, al, bh, bl;
+            if (R::needP(f)) {
+              Bf16x3::split(pa, ah, al);
+              Bf16x3::split(xp, bh, bl);
+              AP[f] = Bf16x3::mma3(ah, al, bh, bl, AP[f]);
+            }
+            if (R::needH(f)) {
+              Bf16x3::split(ha, ah, al);
+              Bf16x3::split(xh, bh, bl);
+              AH[f] = Bf16x3::mma3(ah, al, bh, bl, AH[f]);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < 32; ks += 4) {
+            const int jp = 32 * jc + ks + k_a;
+            const bool ok = row_ok && jp < Jh;
+            const T xs = sym[buf][ks + k_a][j_b], xa = anti[buf][ks + k_a][j_b];
+            if (R::needP(f)) AP[f] = Mfma<T>::mma(ok ? Pa[trow + jp] : (T)0, p == 0 ? xs : xa, AP[f]);
+            if (R::needH(f)) AH[f] = Mfma<T>::mma(ok ? Ha[trow + jp] : (T)0, p == 0 ? xa : xs, AH[f]);
+          }
         }
       }
       __syncthreads();
@@ -169,7 +230,7 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
 // contributing 16 rows n = m + p + 2kk; the recipe terms A_o(n), B_o(n) are
 // formed per lane.
 // ---------------------------------------------------------------------------
-template <class T, class R, int G>
+template <class T, class R, int G, bool SPLIT>
 __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_chunk, int r0, int nn, int m,
                                  int s0, int Jh, int jp_a, bool jp_ok, int lane,
                                  const T *__restrict__ Ps, const T *__restrict__ Hs,
@@ -180,6 +241,57 @@ __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_
   const int k_a = lane >> 4, j_b = lane & 15;
   const int c_b = j_b & 7, part_b = j_b >> 3;
   const T fm = (T)m;
+  if constexpr (SPLIT) {
+    // one K = 16 step per parity: rows p + 2 (4 (lane/16) + q), q < 4
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float pa[4], ha[4];
+      float bA[NOUT][4], bB[NOUT][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ra = p + 2 * (4 * k_a + q);
+        const bool a_ok = jp_ok && (r0 + ra) < nn;
+        const size_t ti = (size_t)(s0 + (a_ok ? m + r0 + ra : m)) * Jh + (jp_ok ? jp_a : 0);
+        pa[q] = a_ok ? (float)Ps[ti] : 0.f;
+        ha[q] = a_ok ? (float)Hs[ti] : 0.f;
+        cx<T> x[NIN], A[NOUT], B[NOUT];
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o) A[o] = B[o] = zero<T>();
+        if (r0 + ra < nn) {
+#pragma unroll
+          for (int f = 0; f < NIN; ++f) x[f] = xin[f][ra][c_b];
+          R::template terms<T>(x, inv_lap[m + r0 + ra], fm, A, B);
+        }
+#pragma unroll
+        for (int o = G; o < NOUT; o += 2) {
+          bA[o][q] = (float)pick<T>(A[o].x, A[o].y, part_b);
+          bB[o][q] = (float)pick<T>(B[o].x, B[o].y, part_b);
+        }
+      }
+      Bf16x3::s4 ph, pl, hh, hl, bh, bl;
+      Bf16x3::split(pa, ph, pl);
+      Bf16x3::split(ha, hh, hl);
+#pragma unroll
+      for (int o = G; o < NOUT; o += 2) {
+        if (R::hasA(o)) {
+          Bf16x3::split(bA[o], bh, bl);
+          if (p == 0)
+            EA[o] = Bf16x3::mma3(ph, pl, bh, bl, EA[o]);
+          else
+            OA[o] = Bf16x3::mma3(ph, pl, bh, bl, OA[o]);
+        }
+        if (R::hasB(o)) {
+          Bf16x3::split(bB[o], bh, bl);
+          if (p == 0)
+            EB[o] = Bf16x3::mma3(hh, hl, bh, bl, EB[o]);
+          else
+            OB[o] = Bf16x3::mma3(hh, hl, bh, bl, OB[o]);
+        }
+      }
+    }
+    (void)first_chunk;
+    return;
+  }
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
 #pragma unroll
@@ -220,7 +332,7 @@ __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_
   (void)first_chunk;
 }
 
-template <class T, class R, int G>
+template <class T, class R, int G, bool SPLIT>
 __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                 const T *__restrict__ Ps, const T *__restrict__ Hs,
                                 const T *__restrict__ inv_lap, const T *__restrict__ rcos, int ncols,
@@ -249,8 +361,8 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
         xin[f][r][c] = ok ? in[((size_t)(s0 + m + r0 + r) * in_nf + f) * ncols + col0 + c] : zero<T>();
     }
     __syncthreads();
-    syn_group<T, R, G>(xin, r0 == 0, r0, nn, m, s0, Jh, jp_a, jp_ok, lane, Ps, Hs, inv_lap, EA, OA,
-                       EB, OB);
+    syn_group<T, R, G, SPLIT>(xin, r0 == 0, r0, nn, m, s0, Jh, jp_a, jp_ok, lane, Ps, Hs, inv_lap,
+                              EA, OA, EB, OB);
   }
   const int j_b = lane & 15;
   const int c_b = j_b & 7, part_b = j_b >> 3;
@@ -275,7 +387,7 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
   }
 }
 
-template <class T, class R>
+template <class T, class R, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_syn_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Ps,
@@ -285,9 +397,9 @@ __global__ __launch_bounds__(256) void k_syn_mfma(const cx<T> *__restrict__ in,
                                                   OutSlots slots, int J, int M, int in_nf) {
   __shared__ cx<T> xin[R::NIN][32][8];
   if ((threadIdx.x >> 7) == 0)
-    syn_body<T, R, 0>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+    syn_body<T, R, 0, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
   else
-    syn_body<T, R, 1>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+    syn_body<T, R, 1, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
 }
 
 }  // namespace lbm

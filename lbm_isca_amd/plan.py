@@ -53,9 +53,10 @@ class Plan:
         self.nlev = self.sigm.size - 1
         self.nmem, self.nbasic = int(nmem), int(nbasic)
         self.precision = precision
-        prec = {"fp64": _lib.LBM_FP64, "fp32": _lib.LBM_FP32}.get(str(precision).lower())
+        prec = {"fp64": _lib.LBM_FP64, "fp32": _lib.LBM_FP32,
+                "bf16x3": _lib.LBM_BF16X3}.get(str(precision).lower())
         if prec is None:
-            raise ValueError(f"unknown precision {precision!r} (fp64 or fp32)")
+            raise ValueError(f"unknown precision {precision!r} (fp64, fp32 or bf16x3)")
         if damping is None:
             damping = zero_damping(self.nmax, self.nlev)
         self._keep = [self.sigm]
