@@ -27,6 +27,12 @@ Two more parts of the metric ride along in the same line:
   response J = area mean of the sigma = 0.2 streamfunction over 30-60N,
   150-210E averaged over days 20-30; wall time of the adjoint sweep (both
   gradients, all members) over the forward run, FP64 and FP32.
+* ``t127l40``: 32 balanced-jet members at T127 L40 (384 x 192 grid, 40
+  equal sigma levels) from a seeded n^-1.5 vorticity perturbation of rms
+  1e-6 1/s, 20 days, with the Legendre contractions in FP64, FP32 and
+  split-BF16x3: member-days/s, the relative L2 error of each low-precision
+  final state against FP64, and the Legendre kernels' matrix-core
+  utilisation (useful flops / time / the precision's dense peak).
 """
 
 import argparse
@@ -104,8 +110,10 @@ def cpu_baseline(bs, frc, sigm, seconds=15.0):
 
 def uneven_ensemble(E, seed=1, max_days=None):
     """Per-member jet basic states, heatings and lengths of the uneven
-    ensemble (E, J, I, L) on 20 equal sigma levels; ``max_days`` caps the
-    lengths (profiling runs only)."""
+    ensemble (E, J, I, L) on 20 equal sigma levels, members ordered by
+    decreasing length (the plan skips finished members in blocks, so the
+    finished ones should be neighbours); ``max_days`` caps the lengths
+    (profiling runs only)."""
     import numpy as np
     import lbm_isca_amd as L
     from lbm_isca_amd import cases
@@ -117,7 +125,10 @@ def uneven_ensemble(E, seed=1, max_days=None):
     lengths = cases.member_lengths(E, seed=seed)
     if max_days is not None:
         lengths = np.minimum(lengths, int(round(72 * max_days)))
-    return bs, frc, lengths, sigm
+    order = np.argsort(-lengths, kind="stable")
+    for k in ("u", "v", "t", "q", "ps"):
+        setattr(bs, k, np.ascontiguousarray(getattr(bs, k)[order]))
+    return bs, np.ascontiguousarray(frc[order]), lengths[order], sigm
 
 
 def measure_fp32_vs_fp64(E=256, max_days=None):
@@ -214,6 +225,79 @@ def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0)):
             "members": E, **res}
 
 
+def legendre_flops(E, L, nspec, Jh):
+    """Useful flops of the level Legendre kernels of one step: each real
+    contraction is 2 nspec Jh (2 E L); synthesis has 8 of them (P for the six
+    synthesised fields, H for U and V), analysis 11 (P for seven grid
+    tendencies, H for four)."""
+    one = 2.0 * nspec * Jh * 2 * E * L
+    return {"syn_level": 8 * one, "ana_level": 11 * one}
+
+
+def measure_t127(E=32, days=20.0):
+    """T127 L40 in the three precisions of the Legendre contractions."""
+    import numpy as np
+    import torch
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases, roofline
+    sigm = cases.EQUAL_SIGM_L40
+    sp = L.BatchedSpectral(384, 192, 127)
+    vg = L.VerticalGrid(sigm)
+    t0 = time.perf_counter()
+    bs, _ = cases.jet_states(sp, vg.sig, E, seed=1)
+    vor = cases.vorticity_perturbation(sp, vg.kmax, E, seed=4)
+    log(f"t127l40 setup {time.perf_counter() - t0:.1f} s")
+    nsteps = int(round(72 * days))
+    M, Jh = 128, 96
+    nspec = M * (M + 1) // 2
+    flops = legendre_flops(E, vg.kmax, nspec, Jh)
+    peaks = {"fp64": roofline.MI355X_FP64_TFLOPS, "fp32": roofline.MI355X_FP32_TFLOPS,
+             "bf16x3": roofline.MI355X_BF16_TFLOPS}
+    res, states = {}, {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        cfg = L.Config(**cases.t127l40_config(precision=prec))
+        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True, spectral=sp)
+        plan = m.plan
+        plan.set_spectral_state(vor=vor)
+        plan.run(8, use_graph=True)              # capture + warm caches
+        plan.set_spectral_state(vor=vor)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(nsteps, use_graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        states[prec] = [x.cpu().numpy() for x in plan.get_state()]
+        plan.profile(True)
+        plan.run(3, use_graph=False)
+        prof = plan.profile_read()
+        plan.profile(False)
+        avg = {k: ms / n for k, (ms, n) in prof.items() if n}
+        mfma = {}
+        for k in ("syn_level", "ana_level"):
+            tf = flops[k] / (avg[k] * 1e-3) / 1e12
+            executed = 3.0 if prec == "bf16x3" else 1.0
+            mfma[k] = {"ms": round(avg[k], 4), "useful_tflops": round(tf, 2),
+                       "frac_of_peak": round(executed * tf / peaks[prec], 4)}
+        res[prec] = {"member_days_per_s": round(E * nsteps / 72.0 / dt, 3),
+                     "ms_per_step": round(dt * 1e3 / (nsteps + cfg.inistp), 3),
+                     "legendre": mfma, "peak_tflops": peaks[prec]}
+        log(f"t127l40 {prec}: {res[prec]}")
+        del m, plan
+        torch.cuda.empty_cache()
+    err = {}
+    for prec in ("fp32", "bf16x3"):
+        e = {}
+        for f, a, b in zip(("u", "v", "t", "q", "ps"), states[prec], states["fp64"]):
+            den = np.sqrt((b ** 2).sum())
+            e[f] = float(f"{np.sqrt(((a - b) ** 2).sum()) / den:.3e}") if den > 0 else 0.0
+        err[prec] = e
+    return {"workload": f"T127L40 {E} balanced-jet members (own basic states), n^-1.5 vorticity "
+                        f"perturbation rms 1e-6 1/s, {days:g} days, one graph replay per precision",
+            "members": E, "days": days, **res, "rel_l2_vs_fp64": err,
+            "note": "useful Legendre flops (executed on the matrix cores: x3 for bf16x3) over the "
+                    "dense peak of the precision"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +310,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP32-vs-FP64 uneven ensemble and the adjoint timing")
-    ap.add_argument("--only", choices=("uneven", "adjoint"), default=None,
+    ap.add_argument("--only", choices=("uneven", "adjoint", "t127"), default=None,
                     help="run only one of the extra measurements (profiling)")
     ap.add_argument("--uneven-members", type=int, default=256)
     ap.add_argument("--adjoint-members", type=int, default=32)
@@ -234,6 +318,8 @@ def main():
                     help="cap the uneven lengths (profiling runs only)")
     ap.add_argument("--window", type=float, nargs=2, default=(20.0, 30.0),
                     help="response averaging window in days")
+    ap.add_argument("--t127-members", type=int, default=32)
+    ap.add_argument("--t127-days", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); members are "
@@ -247,6 +333,8 @@ def main():
     if args.only is not None:
         if args.only == "uneven":
             res = measure_fp32_vs_fp64(args.uneven_members, args.max_days)
+        elif args.only == "t127":
+            res = measure_t127(args.t127_members, args.t127_days)
         else:
             res = measure_adjoint(args.adjoint_members, args.max_days, tuple(args.window))
         print(json.dumps({args.only: res}), flush=True)
@@ -292,7 +380,8 @@ def main():
         for key, fn in (("fp32_vs_fp64",
                          lambda: measure_fp32_vs_fp64(args.uneven_members, args.max_days)),
                         ("adjoint", lambda: measure_adjoint(args.adjoint_members, args.max_days,
-                                                            tuple(args.window)))):
+                                                            tuple(args.window))),
+                        ("t127l40", lambda: measure_t127(args.t127_members, args.t127_days))):
             log(f"{key} ...")
             try:
                 extras[key] = fn()

@@ -112,6 +112,26 @@ int lbm_psichi_to_uv(lbm_plan *plan, const double *d_psi, const double *d_chi, d
 int lbm_hgrad(lbm_plan *plan, const double *d_w, double *d_gx, double *d_gy, int B, int nk,
               void *stream);
 
+/* Transposes of the five transforms above (no counterpart in the reference,
+ * which has no adjoint): for the Euclidean inner products of grid values
+ * and of the re/im parts of the n >= m coefficients, <T x, y> = <x, T^T y>.
+ * Same layouts, inputs and outputs exchanged:
+ *   lbm_grid_to_wave_adj : d_w (B,M,M,nk)            -> d_g (grid)
+ *   lbm_wave_to_grid_adj : d_g (B,J,I,nk)            -> d_w (spectral)
+ *   lbm_uv_to_vordiv_adj : d_vor, d_div (spectral)   -> d_u, d_v (grid)
+ *   lbm_psichi_to_uv_adj : d_u, d_v (grid)           -> d_psi, d_chi
+ *   lbm_hgrad_adj        : d_gx, d_gy (grid)         -> d_w (spectral) */
+int lbm_grid_to_wave_adj(lbm_plan *plan, const double *d_w, double *d_g, int B, int nk,
+                         void *stream);
+int lbm_wave_to_grid_adj(lbm_plan *plan, const double *d_g, double *d_w, int B, int nk,
+                         void *stream);
+int lbm_uv_to_vordiv_adj(lbm_plan *plan, const double *d_vor, const double *d_div, double *d_u,
+                         double *d_v, int B, int nk, void *stream);
+int lbm_psichi_to_uv_adj(lbm_plan *plan, const double *d_u, const double *d_v, double *d_psi,
+                         double *d_chi, int B, int nk, void *stream);
+int lbm_hgrad_adj(lbm_plan *plan, const double *d_gx, const double *d_gy, double *d_w, int B,
+                  int nk, void *stream);
+
 /* Basic state(s): nbasic states, u,v,t,q (nbasic,J,I,L), ps (nbasic,J,I) in
  * hPa.  Computes the BasicStateTerms of dynamics.py:54-93 on the device,
  * the reference temperature, tmtx/gg (vertical.py:139-193) and the FP64

@@ -29,6 +29,8 @@ EXPORTS = (
     "lbm_set_lengths", "lbm_run", "lbm_get_state", "lbm_get_spectral",
     "lbm_steps_taken", "lbm_profile", "lbm_profile_read", "lbm_last_error",
     "lbm_set_spectral_state", "lbm_set_response", "lbm_get_response", "lbm_adjoint_run",
+    "lbm_grid_to_wave_adj", "lbm_wave_to_grid_adj", "lbm_uv_to_vordiv_adj",
+    "lbm_psichi_to_uv_adj", "lbm_hgrad_adj",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -90,6 +92,11 @@ def load():
         "lbm_set_response": ([P, _dp, _dp, _dp, ctypes.c_int], ctypes.c_int),
         "lbm_get_response": ([P, _dp], ctypes.c_int),
         "lbm_adjoint_run": ([P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int, P], ctypes.c_int),
+        "lbm_grid_to_wave_adj": ([P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_wave_to_grid_adj": ([P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_uv_to_vordiv_adj": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_psichi_to_uv_adj": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_hgrad_adj": ([P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -108,6 +108,51 @@ struct AdjPsi {
   }
 };
 
+// Transposes of the API transforms (spectral.py:210-276 forward).
+// uv_to_vordiv^T (synthesis kernel, analysis tables): (vor*, div*) ->
+// Fourier (U*, V*), times 1/(a cos lat) for the forward's pre-scale
+struct AdjVorDiv {
+  static constexpr int NIN = 2, NOUT = 2;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int) { return true; }
+  static constexpr bool scaled(int) { return true; }
+  template <class T>
+  __device__ __forceinline__ static void terms(const cx<T> *x, T, T fm, cx<T> *A, cx<T> *B) {
+    A[0] = im_mul(-fm, x[1]);  // vor = i m AP[V] + AH[U], div = i m AP[U] - AH[V]
+    B[0] = x[0];
+    A[1] = im_mul(-fm, x[0]);
+    B[1] = -x[1];
+  }
+};
+// psichi_to_uv^T (analysis kernel, synthesis tables): (U*, V*) pre-scaled
+// by 1/(a cos lat) -> (psi*, chi*)
+struct AdjPsiChi {
+  static constexpr int NIN = 2, NOUT = 2;
+  static constexpr bool ACC = false;
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int) { return true; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T,
+                                                 cx<T> *o) {
+    // U = P (i m chi) - H psi,  V = P (i m psi) + H chi
+    o[0] = im_mul(-fm, AP[1]) - AH[0];
+    o[1] = im_mul(-fm, AP[0]) + AH[1];
+  }
+};
+// hgrad^T (analysis kernel, synthesis tables): (gx*, gy*) pre-scaled by
+// 1/(a cos lat) -> w*
+struct AdjGrad {
+  static constexpr int NIN = 2, NOUT = 1;
+  static constexpr bool ACC = false;
+  static constexpr bool needP(int f) { return f == 0; }
+  static constexpr bool needH(int f) { return f == 1; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T,
+                                                 cx<T> *o) {
+    o[0] = im_mul(-fm, AP[0]) + AH[1];  // gx = P (i m w), gy = H w
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Transposed grid tendencies of the linear model.  gout* rows: fu' fv' ke
 // xu' xv' gt yu' yv' g (L each), gt_pi, (pi_prev, not read).  gin* rows: U V
@@ -123,6 +168,7 @@ struct TendAdjArgs {
   const T *bsg;     // [eb][j][9L+6][I]
   const T *vc, *tref, *rcos;
   int E, J, I, L, nbasic;
+  ActiveMask am;
 };
 
 template <class T>
@@ -131,6 +177,7 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
   const int ej = blockIdx.x / nch;
   const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
   const int e = ej / a.J, j = ej - e * a.J;
+  if (!a.am(e)) return;  // a finished member contributes nothing
   const int L = a.L, I = a.I, J = a.J;
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};
@@ -301,6 +348,7 @@ struct StepAdjArgs {
   const Ctl *ctl;
   const int *nsteps;
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  int ecnt;  // members covered by the launch (<= E)
   T delt, robert;
 };
 
@@ -319,7 +367,7 @@ template <class T, int L>
 __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
-  const int s = idx / E, e = idx - s * E;
+  const int s = idx / a.ecnt, e = idx - s * a.ecnt;
   if (s >= a.nspec) return;
   const int st = a.ctl->step;
   const size_t EL = (size_t)E * L;

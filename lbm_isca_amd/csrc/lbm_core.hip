@@ -7,11 +7,14 @@
 #include <algorithm>
 #include <type_traits>
 #include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/lbm_hip.h"
@@ -244,9 +247,9 @@ struct lbm_plan {
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   hipStream_t own = nullptr;
-  // graphs: 0 forward chunk, 1 forward single step, 2 adjoint chunk, 3 adjoint step
-  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
-  int gsteps[4] = {0, 0, 0, 0};
+  // graphs keyed by (kind, member count of the launches): kind 0 forward
+  // chunk, 1 forward single step, 2 adjoint chunk, 3 adjoint single step
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
   // kernel timing (lbm_profile)
   static constexpr int NPROF = 8;
   bool prof = false;
@@ -281,12 +284,27 @@ int scratch_get(lbm_plan *p, size_t bytes, void **out, hipStream_t st) {
 }
 
 void destroy_graphs(lbm_plan *p) {
-  for (int g = 0; g < 4; ++g)
-    if (p->gexec[g]) {
-      (void)hipGraphExecDestroy(p->gexec[g]);
-      p->gexec[g] = nullptr;
-      p->gsteps[g] = 0;
-    }
+  for (auto &kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+  p->graphs.clear();
+}
+
+// Members that can still be live from step st of a run on (advancing, or,
+// forward, with the pending ps' update of their last step), as a count of
+// leading members rounded up to 8; E when E is not a multiple of 8.
+int live_members(const lbm_plan *p, const std::vector<int> &len, int st, bool pending) {
+  if (p->E % 8) return p->E;
+  int last = -1;
+  for (int e = 0; e < p->E; ++e)
+    if (st < p->inistp + len[e] + (pending ? 1 : 0)) last = e;
+  return std::max(8, (last + 1 + 7) / 8 * 8);
+}
+
+// launch extents of the column-tiled kernels for a mask
+unsigned tiles_x(int ncols, const ActiveMask &am) {
+  return am.reduced() ? (unsigned)((ncols / am.E) * (am.ecnt / 8)) : (unsigned)((ncols + 7) / 8);
+}
+long fft_rows(long J, int C, const ActiveMask &am) {
+  return am.reduced() ? J * (C / am.E) * am.ecnt : J * C;
 }
 
 // Tables of one precision from the FP64 host setup.
@@ -371,21 +389,21 @@ int set_twiddles() {
 template <class T, class R>
 int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
                hipStream_t st, int in_nf = R::NIN, const T *P = nullptr, const T *H = nullptr,
-               const T *per_n = nullptr) {
+               const T *per_n = nullptr, ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
   DevTabs<T> &t = p->tabs<T>();
-  dim3 grid((ncols + 7) / 8, p->M, (p->Jh + 31) / 32);
+  dim3 grid(tiles_x(ncols, am), p->M, (p->Jh + 31) / 32);
   if constexpr (std::is_same<T, float>::value) {
     if (p->split) {
       hipLaunchKernelGGL((k_syn_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
                          (cx<T> *)out, P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap,
-                         t.rcos, ncols, C, sl, p->J, p->M, in_nf);
+                         t.rcos, ncols, C, sl, p->J, p->M, in_nf, am);
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
   hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
-                     p->J, p->M, in_nf);
+                     p->J, p->M, in_nf, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -395,22 +413,23 @@ int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlot
 template <class T, class R>
 int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
                hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0, const T *P = nullptr,
-               const T *H = nullptr, const T *per_n = nullptr) {
+               const T *H = nullptr, const T *per_n = nullptr,
+               ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
   DevTabs<T> &t = p->tabs<T>();
   // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
-  dim3 grid((ncols + 7) / 8, p->M, (p->M + 63) / 64);
+  dim3 grid(tiles_x(ncols, am), p->M, (p->M + 63) / 64);
   if constexpr (std::is_same<T, float>::value) {
     if (p->split) {
       hipLaunchKernelGGL((k_ana_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
                          (cx<T> *)out, P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols,
-                         C, sl, p->J, p->M, out_nf, out_f0);
+                         C, sl, p->J, p->M, out_nf, out_f0, am);
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
   hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
                      P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
-                     out_nf, out_f0);
+                     out_nf, out_f0, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -418,43 +437,62 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
 // FFTs over the rows j*C + col of a column map; mode: fourier_scale (0 for
 // the model, 1/2 for the transposed transforms of the adjoint)
 template <class T, int NI>
-int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode) {
+int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
+                  ActiveMask am) {
   using S = FftShape<NI>;
   const int C = cm.ncols();
-  const long rows = (long)p->J * C;
+  const long rows = fft_rows(p->J, C, am);
   hipLaunchKernelGGL((k_ifft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
-                     dim3(S::THREADS), 0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M, mode);
+                     dim3(S::THREADS), 0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M, mode, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 template <class T, int NI>
-int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode) {
+int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode,
+                 ActiveMask am) {
   using S = FftShape<NI>;
   const int C = cm.ncols();
-  const long rows = (long)p->J * C;
+  const long rows = fft_rows(p->J, C, am);
   hipLaunchKernelGGL((k_fft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
-                     dim3(S::THREADS), 0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M, mode);
+                     dim3(S::THREADS), 0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M, mode, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 template <class T>
-int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode = 0) {
-  return p->I == 128 ? launch_ifft_n<T, 128>(p, F, G, cm, st, mode)
-                     : launch_ifft_n<T, 384>(p, F, G, cm, st, mode);
+int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode = 0,
+                ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  return p->I == 128 ? launch_ifft_n<T, 128>(p, F, G, cm, st, mode, am)
+                     : launch_ifft_n<T, 384>(p, F, G, cm, st, mode, am);
 }
 
 template <class T>
-int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode = 0) {
-  return p->I == 128 ? launch_fft_n<T, 128>(p, G, F, cm, st, mode)
-                     : launch_fft_n<T, 384>(p, G, F, cm, st, mode);
+int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode = 0,
+               ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  return p->I == 128 ? launch_fft_n<T, 128>(p, G, F, cm, st, mode, am)
+                     : launch_fft_n<T, 384>(p, G, F, cm, st, mode, am);
+}
+
+// members still advancing at the device step counter of the time loop
+// (Ctl::step is the first field of the control block); the adjoint has no
+// deferred ps' updates
+static_assert(offsetof(Ctl, step) == 0, "the mask reads the step counter at the block's start");
+ActiveMask loop_mask(const lbm_plan *p, bool with_pending, int ecnt) {
+  return ActiveMask{reinterpret_cast<const int *>(p->ctl), p->d_nsteps,
+                    with_pending ? p->pend : nullptr, p->inistp, p->E,
+                    ecnt < p->E ? ecnt : 0};
 }
 
 // ---------------------------------------------------------------------------
-// Spectral API: (B, nk) transforms on caller FP64 buffers.
+// Spectral API: (B, nk) transforms on caller FP64 buffers, and their
+// transposes (the adjoint twins, for the Euclidean inner products of the
+// grid values and of the re/im parts of the n >= m coefficients).
 // ---------------------------------------------------------------------------
-enum SpecOp { OP_G2W, OP_W2G, OP_UV2VD, OP_PC2UV, OP_HGRAD };
+enum SpecOp {
+  OP_G2W, OP_W2G, OP_UV2VD, OP_PC2UV, OP_HGRAD,
+  OP_G2W_ADJ, OP_W2G_ADJ, OP_UV2VD_ADJ, OP_PC2UV_ADJ, OP_HGRAD_ADJ
+};
 
 template <class T>
 int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, double *out0,
@@ -462,10 +500,19 @@ int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, do
   if (B <= 0 || nk <= 0) return fail(LBM_ERR_VALUE, "batch and column counts must be positive");
   const int J = p->J, I = p->I, M = p->M;
   const int ncols = B * nk;
-  const int nf = (op == OP_G2W || op == OP_W2G) ? 1 : 2;
-  const size_t gbytes = (size_t)B * J * nf * nk * I * sizeof(T);
-  const size_t fbytes = (size_t)M * J * nf * ncols * sizeof(cx<T>);
-  const size_t sbytes = (size_t)p->nspec * nf * ncols * sizeof(cx<T>);
+  // grid -> spectral (analysis kernels) or spectral -> grid (synthesis kernels)
+  const bool to_spec = op == OP_G2W || op == OP_UV2VD || op == OP_W2G_ADJ || op == OP_PC2UV_ADJ ||
+                       op == OP_HGRAD_ADJ;
+  const bool adj = op >= OP_G2W_ADJ;
+  int ngrid, nspecf;  // grid and spectral fields of the call
+  switch (op) {
+    case OP_G2W: case OP_W2G: case OP_G2W_ADJ: case OP_W2G_ADJ: ngrid = 1; nspecf = 1; break;
+    case OP_HGRAD: case OP_HGRAD_ADJ: ngrid = 2; nspecf = 1; break;
+    default: ngrid = 2; nspecf = 2; break;
+  }
+  const size_t gbytes = (size_t)B * J * ngrid * nk * I * sizeof(T);
+  const size_t fbytes = (size_t)M * J * ngrid * ncols * sizeof(cx<T>);
+  const size_t sbytes = (size_t)p->nspec * nspecf * ncols * sizeof(cx<T>);
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
   void *base;
@@ -474,58 +521,68 @@ int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, do
   T *g = (T *)cb;
   cx<T> *f = (cx<T> *)(cb + up(gbytes));
   cx<T> *s = (cx<T> *)(cb + up(gbytes) + up(fbytes));
-  ColMap cm{B, nk, nf, 0};
+  ColMap cm{B, nk, ngrid, 0};
   const size_t gtot = (size_t)B * J * I * nk;
   const double *rc64 = p->d64.rcos;
   const size_t stot = (size_t)p->nspec * ncols;
-  if (op == OP_G2W || op == OP_UV2VD) {
-    hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, in0, g, B, J, I, nk,
-                       nf * nk, 0, op == OP_UV2VD ? rc64 : (const double *)nullptr);
-    if (op == OP_UV2VD)
-      hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, in1, g, B, J, I, nk,
-                         nf * nk, nk, rc64);
+  const size_t spec_out = (size_t)B * M * M * nk;
+  DevTabs<T> &t = p->tabs<T>();
+  if (to_spec) {
+    // vector grid inputs carry 1/(a cos lat): the forward's pre-scale, or
+    // the transpose of a scaled synthesis output
+    const bool scale = op != OP_G2W && op != OP_W2G_ADJ;
+    const double *gin[2] = {in0, in1};
+    for (int k = 0; k < ngrid; ++k)
+      hipLaunchKernelGGL((k_pack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, gin[k], g, B, J, I,
+                         nk, ngrid * nk, k * nk, scale ? rc64 : (const double *)nullptr);
     HIPCHK(hipGetLastError());
-    CHK(launch_fft<T>(p, g, f, cm, st));
-    if (op == OP_G2W) {
-      InSlots sl{{0}};
-      CHK((launch_ana<T, AnaScalar>(p, f, s, ncols, ncols, sl, st)));
-      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
-                         out0, B, M, nk, 1, 0, ncols);
-    } else {
-      InSlots sl{{0, ncols}};
-      CHK((launch_ana<T, AnaVorDiv>(p, f, s, ncols, 2 * ncols, sl, st)));
-      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
-                         out0, B, M, nk, 2, 0, ncols);
-      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk((size_t)B * M * M * nk)), dim3(256), 0, st, s,
-                         out1, B, M, nk, 2, 1, ncols);
+    CHK(launch_fft<T>(p, g, f, cm, st, adj ? 2 : 0));
+    InSlots sl{{0, ncols}};
+    const int C = ngrid * ncols;
+    switch (op) {
+      case OP_G2W: CHK((launch_ana<T, AnaScalar>(p, f, s, ncols, C, sl, st))); break;
+      case OP_UV2VD: CHK((launch_ana<T, AnaVorDiv>(p, f, s, ncols, C, sl, st))); break;
+      case OP_W2G_ADJ:
+        CHK((launch_ana<T, AnaScalar>(p, f, s, ncols, C, sl, st, 1, 0, t.Ps, t.Hs, t.inv_lap)));
+        break;
+      case OP_PC2UV_ADJ:
+        CHK((launch_ana<T, AdjPsiChi>(p, f, s, ncols, C, sl, st, 2, 0, t.Ps, t.Hs, t.inv_lap)));
+        break;
+      default:
+        CHK((launch_ana<T, AdjGrad>(p, f, s, ncols, C, sl, st, 1, 0, t.Ps, t.Hs, t.inv_lap)));
+        break;
     }
+    double *sout[2] = {out0, out1};
+    for (int k = 0; k < nspecf; ++k)
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk(spec_out)), dim3(256), 0, st, s, sout[k], B,
+                         M, nk, nspecf, k, ncols);
     HIPCHK(hipGetLastError());
     return LBM_OK;
   }
   // synthesis family
-  const int nin = (op == OP_PC2UV) ? 2 : 1;
-  hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(stot)), dim3(256), 0, st, in0, s, B, M, nk, nin, 0,
-                     ncols);
-  if (op == OP_PC2UV)
-    hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(stot)), dim3(256), 0, st, in1, s, B, M, nk, nin,
-                       1, ncols);
+  const double *sin_[2] = {in0, in1};
+  for (int k = 0; k < nspecf; ++k)
+    hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(stot)), dim3(256), 0, st, sin_[k], s, B, M, nk,
+                       nspecf, k, ncols);
   HIPCHK(hipGetLastError());
-  if (op == OP_W2G) {
-    OutSlots sl{{0}};
-    CHK((launch_syn<T, SynScalar>(p, s, f, ncols, ncols, sl, st)));
-  } else if (op == OP_PC2UV) {
-    OutSlots sl{{0, ncols}};
-    CHK((launch_syn<T, SynPsiChi>(p, s, f, ncols, 2 * ncols, sl, st)));
-  } else {
-    OutSlots sl{{0, ncols}};
-    CHK((launch_syn<T, SynGrad>(p, s, f, ncols, 2 * ncols, sl, st)));
+  OutSlots sl{{0, ncols}};
+  const int C = ngrid * ncols;
+  switch (op) {
+    case OP_W2G: CHK((launch_syn<T, SynScalar>(p, s, f, ncols, C, sl, st))); break;
+    case OP_PC2UV: CHK((launch_syn<T, SynPsiChi>(p, s, f, ncols, C, sl, st))); break;
+    case OP_HGRAD: CHK((launch_syn<T, SynGrad>(p, s, f, ncols, C, sl, st))); break;
+    case OP_G2W_ADJ:
+      CHK((launch_syn<T, SynScalar>(p, s, f, ncols, C, sl, st, 1, t.Pa, t.Ha, t.nn1a2)));
+      break;
+    default:
+      CHK((launch_syn<T, AdjVorDiv>(p, s, f, ncols, C, sl, st, 2, t.Pa, t.Ha, t.nn1a2)));
+      break;
   }
-  CHK(launch_ifft<T>(p, f, g, cm, st));
-  hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, g, out0, B, J, I, nk,
-                     nf * nk, 0, 0, (const T *)nullptr, 0, 0, 1);
-  if (nf == 2)
-    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, g, out1, B, J, I, nk,
-                       nf * nk, nk, 0, (const T *)nullptr, 0, 0, 1);
+  CHK(launch_ifft<T>(p, f, g, cm, st, adj ? 1 : 0));
+  double *gout[2] = {out0, out1};
+  for (int k = 0; k < ngrid; ++k)
+    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(gtot)), dim3(256), 0, st, g, gout[k], B, J, I,
+                       nk, ngrid * nk, k * nk, 0, (const T *)nullptr, 0, 0, 1);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -879,7 +936,7 @@ int get_state_t(lbm_plan *p, double *u, double *v, double *t, double *q, double 
 // One time step: synthesis, grid tendencies, analysis, spectral step.
 // ---------------------------------------------------------------------------
 template <class T, int L>
-int launch_step_kernel(lbm_plan *p, hipStream_t st) {
+int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
   StepArgs<T> a;
   a.sa3 = (cx<T> *)p->sa3;
@@ -910,7 +967,8 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st) {
   a.skip_low_n = p->skip_low_n;
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
-  const size_t nthr = (size_t)p->nspec * p->E;
+  a.ecnt = ecnt;
+  const size_t nthr = (size_t)p->nspec * ecnt;
   hipLaunchKernelGGL((k_step<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -926,18 +984,21 @@ inline int mark(lbm_plan *p, int k, hipStream_t st) {
 // analysis, spectral step.  raw_path: the first step after a raw initial
 // state (k_tend reads the raw grid).
 template <class T>
-int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
+int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
   const ColMap cmi{E, L, 6, 6};
   const int Ci = cmi.ncols();
+  const ActiveMask am = loop_mask(p, true, ecnt);
   {
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
     CHK(mark(p, 0, st));
-    CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, Ci, sl, st)));
+    CHK((launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, Ci, sl, st, SynLevel::NIN, nullptr, nullptr,
+                                 nullptr, am)));
     OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
     CHK(mark(p, 1, st));
-    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, Ci, s2, st)));
+    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, Ci, s2, st, SynSurf::NIN, nullptr, nullptr,
+                                nullptr, am)));
   }
   TendArgs<T> ta;
   ta.gin = (const T *)p->gin;
@@ -961,31 +1022,34 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   ta.strictly_linear = p->strictly_linear || p->linear;
   ta.linear = p->linear;
   ta.robert = (T)p->robert;
+  ta.am = am;
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
   CHK(mark(p, 2, st));
-  CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st));
+  CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
   CHK(mark(p, 3, st));
-  hipLaunchKernelGGL((k_tend<T>), dim3(E * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  hipLaunchKernelGGL((k_tend<T>), dim3(ecnt * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
-  CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st));
+  CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
   InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
   CHK(mark(p, 5, st));
-  CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st)));
+  CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st, AnaTend::NOUT, 0, nullptr,
+                              nullptr, nullptr, am)));
   InSlots s2{{9 * EL}};
   CHK(mark(p, 6, st));
-  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0)));
+  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,
+                                nullptr, am)));
   CHK(mark(p, 7, st));
   if (L == 20)
-    CHK((launch_step_kernel<T, 20>(p, st)));
+    CHK((launch_step_kernel<T, 20>(p, st, ecnt)));
   else if (L == 40)
-    CHK((launch_step_kernel<T, 40>(p, st)));
+    CHK((launch_step_kernel<T, 40>(p, st, ecnt)));
   else
     return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
   CHK(mark(p, 8, st));
   if (p->resp_set)
-    hipLaunchKernelGGL((k_response<T>), dim3(E), dim3(256), 0, st, (const cx<T> *)p->sa3,
+    hipLaunchKernelGGL((k_response<T>), dim3(ecnt), dim3(256), 0, st, (const cx<T> *)p->sa3,
                        (const cx<T> *)p->obs, (const T *)p->wlev, (const T *)p->wstep, p->nwstep, E,
                        L, p->nspec, (const Ctl *)p->ctl, (const int *)p->d_nsteps, p->inistp,
                        p->jacc);
@@ -1003,31 +1067,38 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path) {
   return LBM_OK;
 }
 
-int step_dispatch(lbm_plan *p, hipStream_t st, bool raw_path = false) {
-  return p->prec == LBM_FP64 ? one_step<double>(p, st, raw_path) : one_step<float>(p, st, raw_path);
+int step_dispatch(lbm_plan *p, hipStream_t st, int ecnt, bool raw_path = false) {
+  return p->prec == LBM_FP64 ? one_step<double>(p, st, raw_path, ecnt)
+                             : one_step<float>(p, st, raw_path, ecnt);
 }
 
-int capture_graph(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
+// the graph of nsteps steps (kind 0: the 8-step chunk, 1: one step) over
+// the first ecnt members, captured on first use
+int capture_graph(lbm_plan *p, int kind, int nsteps, int ecnt, hipStream_t st,
+                  hipGraphExec_t *out) {
   if (p->prof) return fail(LBM_ERR_STATE, "profiling runs eager launches only (use_graph=0)");
-  if (p->gexec[slot] && p->gsteps[slot] == nsteps) return LBM_OK;
-  if (p->gexec[slot]) {
-    (void)hipGraphExecDestroy(p->gexec[slot]);
-    p->gexec[slot] = nullptr;
+  const auto key = std::make_pair(kind, ecnt);
+  auto it = p->graphs.find(key);
+  if (it != p->graphs.end()) {
+    *out = it->second;
+    return LBM_OK;
   }
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = LBM_OK;
-  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = step_dispatch(p, st);
+  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = step_dispatch(p, st, ecnt);
   hipError_t e2 = hipStreamEndCapture(st, &g);
   if (rc != LBM_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e2 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph capture failed: %s", hipGetErrorString(e2));
-  hipError_t e3 = hipGraphInstantiate(&p->gexec[slot], g, nullptr, nullptr, 0);
+  hipGraphExec_t ge = nullptr;
+  hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e3 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph instantiate failed: %s", hipGetErrorString(e3));
-  p->gsteps[slot] = nsteps;
+  p->graphs[key] = ge;
+  *out = ge;
   return LBM_OK;
 }
 
@@ -1153,7 +1224,7 @@ int set_response_t(lbm_plan *p, const double *wgrid, const double *wlev, const d
 // forward FFT, transposed synthesis (analysis kernels with the synthesis
 // tables), then the step counter back by one.
 template <class T, int L>
-int launch_step_adj(lbm_plan *p, hipStream_t st) {
+int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
   StepAdjArgs<T> a;
   a.aa3 = (cx<T> *)p->aa3;
@@ -1188,35 +1259,38 @@ int launch_step_adj(lbm_plan *p, hipStream_t st) {
   a.skip_low_n = p->skip_low_n;
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
-  const size_t nthr = (size_t)p->nspec * p->E;
+  a.ecnt = ecnt;
+  const size_t nthr = (size_t)p->nspec * ecnt;
   hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 template <class T>
-int adjoint_step(lbm_plan *p, hipStream_t st) {
+int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
+  const ActiveMask am = loop_mask(p, false, ecnt);
   CHK(mark(p, 7, st));
   if (L == 20)
-    CHK((launch_step_adj<T, 20>(p, st)));
+    CHK((launch_step_adj<T, 20>(p, st, ecnt)));
   else
-    CHK((launch_step_adj<T, 40>(p, st)));
+    CHK((launch_step_adj<T, 40>(p, st, ecnt)));
   const ColMap cmo{E, L, 9, 2}, cmi{E, L, 6, 6};
   const int Co = cmo.ncols(), Ci = cmi.ncols();
   CHK(mark(p, 5, st));
   {
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
-    CHK((launch_syn<T, AdjTend>(p, p->tend3, p->fout, EL, Co, sl, st, 4, t.Pa, t.Ha, t.nn1a2)));
+    CHK((launch_syn<T, AdjTend>(p, p->tend3, p->fout, EL, Co, sl, st, 4, t.Pa, t.Ha, t.nn1a2, am)));
   }
   CHK(mark(p, 6, st));
   {
     OutSlots sl{{9 * EL}};
-    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, E, Co, sl, st, 2, t.Pa, t.Ha, t.nn1a2)));
+    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, E, Co, sl, st, 2, t.Pa, t.Ha, t.nn1a2,
+                                     am)));
   }
   CHK(mark(p, 4, st));
-  CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st, 1));
+  CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st, 1, am));
   CHK(mark(p, 3, st));
   TendAdjArgs<T> ta;
   ta.gout_a = (const T *)p->gout;
@@ -1231,19 +1305,22 @@ int adjoint_step(lbm_plan *p, hipStream_t st) {
   ta.I = p->I;
   ta.L = L;
   ta.nbasic = p->nb;
-  hipLaunchKernelGGL((k_tend_adj<T>), dim3(E * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  ta.am = am;
+  hipLaunchKernelGGL((k_tend_adj<T>), dim3(ecnt * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
-  CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2));
+  CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2, am));
   CHK(mark(p, 0, st));
   {
     InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
-    CHK((launch_ana<T, AdjSynLevel>(p, p->fin, p->aa3, EL, Ci, sl, st, 4, 0, t.Ps, t.Hs, t.inv_lap)));
+    CHK((launch_ana<T, AdjSynLevel>(p, p->fin, p->aa3, EL, Ci, sl, st, 4, 0, t.Ps, t.Hs, t.inv_lap,
+                                    am)));
   }
   CHK(mark(p, 1, st));
   {
     InSlots sl{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
-    CHK((launch_ana<T, AdjSynSurf>(p, p->fin, p->s3a, E, Ci, sl, st, 3, 0, t.Ps, t.Hs, t.inv_lap)));
+    CHK((launch_ana<T, AdjSynSurf>(p, p->fin, p->s3a, E, Ci, sl, st, 3, 0, t.Ps, t.Hs, t.inv_lap,
+                                   am)));
   }
   CHK(mark(p, 8, st));
   hipLaunchKernelGGL(k_tick_back, dim3(1), dim3(1), 0, st, p->ctl);
@@ -1263,8 +1340,8 @@ int adjoint_step(lbm_plan *p, hipStream_t st) {
   return LBM_OK;
 }
 
-int adjoint_dispatch(lbm_plan *p, hipStream_t st) {
-  return p->prec == LBM_FP64 ? adjoint_step<double>(p, st) : adjoint_step<float>(p, st);
+int adjoint_dispatch(lbm_plan *p, hipStream_t st, int ecnt) {
+  return p->prec == LBM_FP64 ? adjoint_step<double>(p, st, ecnt) : adjoint_step<float>(p, st, ecnt);
 }
 
 template <class T>
@@ -1297,27 +1374,31 @@ int adjoint_finish(lbm_plan *p, double *gvor, double *gdiv, double *gt, double *
   return LBM_OK;
 }
 
-int capture_adjoint(lbm_plan *p, int slot, int nsteps, hipStream_t st) {
+int capture_adjoint(lbm_plan *p, int kind, int nsteps, int ecnt, hipStream_t st,
+                    hipGraphExec_t *out) {
   if (p->prof) return fail(LBM_ERR_STATE, "profiling runs eager launches only (use_graph=0)");
-  if (p->gexec[slot] && p->gsteps[slot] == nsteps) return LBM_OK;
-  if (p->gexec[slot]) {
-    (void)hipGraphExecDestroy(p->gexec[slot]);
-    p->gexec[slot] = nullptr;
+  const auto key = std::make_pair(kind, ecnt);
+  auto it = p->graphs.find(key);
+  if (it != p->graphs.end()) {
+    *out = it->second;
+    return LBM_OK;
   }
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = LBM_OK;
-  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = adjoint_dispatch(p, st);
+  for (int s = 0; s < nsteps && rc == LBM_OK; ++s) rc = adjoint_dispatch(p, st, ecnt);
   hipError_t e2 = hipStreamEndCapture(st, &g);
   if (rc != LBM_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e2 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph capture failed: %s", hipGetErrorString(e2));
-  hipError_t e3 = hipGraphInstantiate(&p->gexec[slot], g, nullptr, nullptr, 0);
+  hipGraphExec_t ge = nullptr;
+  hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e3 != hipSuccess) return fail(LBM_ERR_RUNTIME, "graph instantiate failed: %s", hipGetErrorString(e3));
-  p->gsteps[slot] = nsteps;
+  p->graphs[key] = ge;
+  *out = ge;
   return LBM_OK;
 }
 
@@ -1533,6 +1614,24 @@ int lbm_psichi_to_uv(lbm_plan *p, const double *psi, const double *chi, double *
 int lbm_hgrad(lbm_plan *p, const double *w, double *gx, double *gy, int B, int nk, void *s) {
   return spectral_dispatch(p, OP_HGRAD, w, nullptr, gx, gy, B, nk, s);
 }
+int lbm_grid_to_wave_adj(lbm_plan *p, const double *w, double *g, int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_G2W_ADJ, w, nullptr, g, nullptr, B, nk, s);
+}
+int lbm_wave_to_grid_adj(lbm_plan *p, const double *g, double *w, int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_W2G_ADJ, g, nullptr, w, nullptr, B, nk, s);
+}
+int lbm_uv_to_vordiv_adj(lbm_plan *p, const double *vor, const double *div, double *u, double *v,
+                         int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_UV2VD_ADJ, vor, div, u, v, B, nk, s);
+}
+int lbm_psichi_to_uv_adj(lbm_plan *p, const double *u, const double *v, double *psi, double *chi,
+                         int B, int nk, void *s) {
+  return spectral_dispatch(p, OP_PC2UV_ADJ, u, v, psi, chi, B, nk, s);
+}
+int lbm_hgrad_adj(lbm_plan *p, const double *gx, const double *gy, double *w, int B, int nk,
+                  void *s) {
+  return spectral_dispatch(p, OP_HGRAD_ADJ, gx, gy, w, nullptr, B, nk, s);
+}
 
 int lbm_set_basic_state(lbm_plan *p, const double *u, const double *v, const double *t,
                         const double *q, const double *ps, void *stream) {
@@ -1630,25 +1729,28 @@ int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *strea
   int rc = LBM_OK;
   if (p->raw_pending) {
     // the first step after a raw initial state reads the raw grid (unfused)
-    CHK(step_dispatch(p, st, true));
+    CHK(step_dispatch(p, st, live_members(p, len, 0, true), true));
     p->raw_pending = false;
     done = 1;
   }
   while (done < total) {
     int todo = total - done;
     if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
+    // the members still live at the first step of this stretch cover it
+    const int ecnt = live_members(p, len, done, true);
     if (use_graph) {
+      hipGraphExec_t ge = nullptr;
       if (todo >= chunk) {
-        CHK(capture_graph(p, 0, chunk, st));
-        HIPCHK(hipGraphLaunch(p->gexec[0], st));
+        CHK(capture_graph(p, 0, chunk, ecnt, st, &ge));
         todo = chunk;
       } else {
-        CHK(capture_graph(p, 1, 1, st));
-        HIPCHK(hipGraphLaunch(p->gexec[1], st));
+        CHK(capture_graph(p, 1, 1, ecnt, st, &ge));
         todo = 1;
       }
+      HIPCHK(hipGraphLaunch(ge, st));
     } else {
-      for (int s = 0; s < todo; ++s) CHK(step_dispatch(p, st));
+      todo = std::min(todo, chunk);
+      for (int s = 0; s < todo; ++s) CHK(step_dispatch(p, st, ecnt));
     }
     done += todo;
     if (check_every > 0 && done % check_every == 0 && (rc = check_bad(p, st, done)) != LBM_OK) break;
@@ -1799,20 +1901,18 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
   int done = 0;
   while (done < total) {
     const int todo = total - done;
+    const int n = (use_graph && todo < chunk) ? 1 : std::min(todo, chunk);
+    // adjoint steps total-1-done down to total-done-n: the earliest has the
+    // most live members
+    const int ecnt = live_members(p, len, total - done - n, false);
     if (use_graph) {
-      if (todo >= chunk) {
-        CHK(capture_adjoint(p, 2, chunk, st));
-        HIPCHK(hipGraphLaunch(p->gexec[2], st));
-        done += chunk;
-      } else {
-        CHK(capture_adjoint(p, 3, 1, st));
-        HIPCHK(hipGraphLaunch(p->gexec[3], st));
-        done += 1;
-      }
+      hipGraphExec_t ge = nullptr;
+      CHK(capture_adjoint(p, n == chunk ? 2 : 3, n, ecnt, st, &ge));
+      HIPCHK(hipGraphLaunch(ge, st));
     } else {
-      CHK(adjoint_dispatch(p, st));
-      done += 1;
+      for (int s = 0; s < n; ++s) CHK(adjoint_dispatch(p, st, ecnt));
     }
+    done += n;
   }
   CHK(p->prec == LBM_FP64 ? adjoint_finish<double>(p, gvor, gdiv, gt, gq, gpi, gfrc, st)
                           : adjoint_finish<float>(p, gvor, gdiv, gt, gq, gpi, gfrc, st));

@@ -111,6 +111,64 @@ struct ColMap {
   }
 };
 
+// Members still advancing at the current step of a run: member e is live
+// while *step < inistp + nsteps[e], or while a deferred surface-pressure
+// update of its last step is pending (pend[e] != 0, applied by the next
+// tendency call from that step's synthesised pi); all members when step is
+// null.  The time-loop kernels skip the work of finished members (whole
+// blocks when all their members have finished); read on the device so a
+// captured graph sees the step of each replay.
+//
+// ecnt (a multiple of 8 below E, or 0 for all) limits a launch to the first
+// ecnt members: the host knows the lengths and orders nothing, so the
+// members that can still be live must be a prefix for this to pay; columns
+// (field-level, member) then map to blocks as (field-level, 8-member tile).
+struct ActiveMask {
+  const int *step;    // the loop counter, or null
+  const int *nsteps;  // [E] full steps per member
+  const int *pend;    // [E] pending ps' updates, or null
+  int inistp, E;
+  int ecnt;           // members covered by the launch (0: all)
+  __host__ __device__ __forceinline__ bool reduced() const { return ecnt > 0 && ecnt < E; }
+  __device__ __forceinline__ bool operator()(int e) const {
+    return step == nullptr || *step < inistp + nsteps[e] || (pend != nullptr && pend[e] != 0);
+  }
+  // first column of 8-column tile bx; columns are (field-level) * E + member
+  __device__ __forceinline__ int tile_col0(int bx) const {
+    if (!reduced()) return bx * 8;
+    const int tpl = ecnt >> 3;
+    const int fl = bx / tpl;
+    return fl * E + (bx - fl * tpl) * 8;
+  }
+  // row of a transform over J latitudes and C columns -> (j, col); false
+  // past the last row of the launch
+  __device__ __forceinline__ bool row_col(long row, int J, int C, int &j, int &col) const {
+    if (!reduced()) {
+      if (row >= (long)J * C) return false;
+      j = (int)(row / C);
+      col = (int)(row - (long)j * C);
+      return true;
+    }
+    const int nfl = C / E;
+    if (row >= (long)J * nfl * ecnt) return false;
+    const long q = row / ecnt;
+    const int e = (int)(row - q * ecnt);
+    const int fl = (int)(q % nfl);
+    j = (int)(q / nfl);
+    col = fl * E + e;
+    return true;
+  }
+  // any member among columns col0 .. col0+7 (< ncols), member = col % E
+  __device__ __forceinline__ bool any8(int col0, int ncols) const {
+    if (step == nullptr) return true;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (col0 + c < ncols && (*this)((col0 + c) % E)) any = true;
+    return any;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Real FFTs of length NI = 128 R (R = 1: 128 longitudes, R = 3: 384).  The
 // NI real samples of a row are NC = NI/2 complex pairs z_n = x_2n + i x_2n+1;
@@ -311,23 +369,22 @@ __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, i
 template <class T, int NI>
 __global__ __launch_bounds__(FftShape<NI>::THREADS) void k_ifft(const cx<T> *__restrict__ F,
                                                                 T *__restrict__ G, int C, ColMap cm,
-                                                                int J, int M, int mode) {
+                                                                int J, int M, int mode,
+                                                                ActiveMask am) {
   using S = FftShape<NI>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
-  const bool live = row < (long)J * C;
-  int j = 0, col = 0;
+  int j = 0, col = 0, e = 0, r = 0;
+  bool live = am.row_col(row, J, C, j, col);
   if (live) {
-    j = (int)(row / C);
-    col = (int)(row - (long)j * C);
-    ifft_a<T, NI>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt, mode);
+    cm.map(col, e, r);
+    live = am(e);
   }
+  if (live) ifft_a<T, NI>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt, mode);
   __syncthreads();
   cx<T> v[8];
   if (live) ifft_b<T, NI>(sh[rl], v, tt);
-  int e = 0, r = 0;
-  if (live) cm.map(col, e, r);
   cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
   if constexpr (S::R == 1) {
     if (live) {
@@ -349,18 +406,19 @@ __global__ __launch_bounds__(FftShape<NI>::THREADS) void k_ifft(const cx<T> *__r
 template <class T, int NI>
 __global__ __launch_bounds__(FftShape<NI>::THREADS) void k_fft(const T *__restrict__ G,
                                                                cx<T> *__restrict__ F, int C,
-                                                               ColMap cm, int J, int M, int mode) {
+                                                               ColMap cm, int J, int M, int mode,
+                                                               ActiveMask am) {
   using S = FftShape<NI>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
-  const bool live = row < (long)J * C;
-  int j = 0, col = 0;
+  int j = 0, col = 0, e = 0, r = 0;
+  bool live = am.row_col(row, J, C, j, col);
   if (live) {
-    j = (int)(row / C);
-    col = (int)(row - (long)j * C);
-    int e, r;
     cm.map(col, e, r);
+    live = am(e);
+  }
+  if (live) {
     fft_a<T, NI>(reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI),
                  sh[rl], tt);
   }

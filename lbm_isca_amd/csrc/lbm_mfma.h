@@ -97,17 +97,22 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   const T *__restrict__ Ha,
                                                   const T *__restrict__ nn1a2, int ncols, int C,
                                                   InSlots slots, int J, int M, int out_nf,
-                                                  int out_f0) {
+                                                  int out_f0, ActiveMask am) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
   __shared__ T sym[2][32][16], anti[2][32][16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int p = w & 1, tile = (w >> 1) + 2 * blockIdx.z;
-  const int m = blockIdx.y, col0 = blockIdx.x * 8;
+  const int m = blockIdx.y, col0 = am.tile_col0(blockIdx.x);
   const int Jh = J / 2;
   const int N = M - 1;
   const int s0 = spec_off(m, M) - m;
   const int nch = (Jh + 31) / 32;
+  // the lowest degree of this block's tiles is m + 64 blockIdx.z: a block
+  // beyond N has no live wave and exits before any barrier
+  if (m + 64 * (int)blockIdx.z > N) return;
+  // a tile of finished members only: nothing to do (uniform per block)
+  if (!am.any8(col0, ncols)) return;
 
   // loader mapping: thread -> (latitude pair lj of the chunk, column lc)
   const int lj = t >> 3, lc = t & 7;
@@ -208,7 +213,7 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
       ap[f] = pair_of<T>(AP[f][v], part);
       ah[f] = pair_of<T>(AH[f][v], part);
     }
-    if (n > N || col >= ncols) continue;
+    if (n > N || col >= ncols || !am(col % am.E)) continue;
     R::template combine<T>(ap, ah, fm, nn1a2[n], o);
     const int s = s0 + n;
 #pragma unroll
@@ -337,12 +342,12 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
                                 const T *__restrict__ Ps, const T *__restrict__ Hs,
                                 const T *__restrict__ inv_lap, const T *__restrict__ rcos, int ncols,
                                 int C, const OutSlots &slots, int J, int M, int in_nf,
-                                cx<T> (*xin)[32][8]) {
+                                cx<T> (*xin)[32][8], int col0) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int rt = w & 1;
-  const int m = blockIdx.y, col0 = blockIdx.x * 8;
+  const int m = blockIdx.y;
   const int Jh = J / 2;
   const int nn = M - m;
   const int s0 = spec_off(m, M) - m;
@@ -394,12 +399,17 @@ __global__ __launch_bounds__(256) void k_syn_mfma(const cx<T> *__restrict__ in,
                                                   const T *__restrict__ Hs,
                                                   const T *__restrict__ inv_lap,
                                                   const T *__restrict__ rcos, int ncols, int C,
-                                                  OutSlots slots, int J, int M, int in_nf) {
+                                                  OutSlots slots, int J, int M, int in_nf,
+                                                  ActiveMask am) {
   __shared__ cx<T> xin[R::NIN][32][8];
+  const int col0 = am.tile_col0(blockIdx.x);
+  if (!am.any8(col0, ncols)) return;
   if ((threadIdx.x >> 7) == 0)
-    syn_body<T, R, 0, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+    syn_body<T, R, 0, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
+                             col0);
   else
-    syn_body<T, R, 1, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin);
+    syn_body<T, R, 1, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
+                             col0);
 }
 
 }  // namespace lbm

@@ -212,6 +212,7 @@ struct TendArgs {
   int E, J, I, L, nbasic, nfrc, strictly_linear;
   int linear;       // tangent-linear mode: no ps' grid state (pi' is spectral)
   T robert;
+  ActiveMask am;
 };
 
 // Tendencies on grid rows, one thread per column, one pass over the levels
@@ -227,6 +228,7 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const int ej = blockIdx.x / nch;
   const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
   const int e = ej / a.J, j = ej - e * a.J;
+  if (!a.am(e)) return;  // a finished member: its tendencies are not used
   const int L = a.L, I = a.I, J = a.J;
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};
@@ -322,6 +324,7 @@ struct StepArgs {
   int *pend;                     // [E]
   const int *nsteps;             // [E] full steps per member
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  int ecnt;                      // members covered by the launch (<= E)
   T delt, robert;
 };
 
@@ -342,7 +345,7 @@ template <class T, int L>
 __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
-  const int s = idx / E, e = idx - s * E;
+  const int s = idx / a.ecnt, e = idx - s * a.ecnt;
   if (s >= a.nspec) return;
   const int st = a.ctl->step;
   const bool active = st < a.inistp + a.nsteps[e];

@@ -163,6 +163,62 @@ class Plan:
         self._call(self._lib.lbm_hgrad, _lib.ptr(w), _lib.ptr(gx), _lib.ptr(gy), B=B, nk=K)
         return gx, gy
 
+    # -- transposed transforms (adjoint twins) ------------------------------
+    def _grid_out(self, B, K, n):
+        torch = _torch()
+        return [torch.empty((B, self.nlat, self.nlon, K), dtype=torch.float64, device="cuda")
+                for _ in range(n)]
+
+    def _spec_out(self, B, K, n):
+        torch = _torch()
+        M = self.nmax + 1
+        return [torch.empty((B, M, M, K), dtype=torch.complex128, device="cuda") for _ in range(n)]
+
+    def grid_to_wave_adj(self, w):
+        """Transpose of grid_to_wave: spectral (B, M, M, K) -> grid."""
+        w = _dev(w, _torch().complex128)
+        B, _, _, K = w.shape
+        (g,) = self._grid_out(B, K, 1)
+        self._call(self._lib.lbm_grid_to_wave_adj, _lib.ptr(w), _lib.ptr(g), B=B, nk=K)
+        return g
+
+    def wave_to_grid_adj(self, g):
+        """Transpose of wave_to_grid: grid (B, J, I, K) -> spectral."""
+        g = _dev(g, _torch().float64)
+        B, _, _, K = g.shape
+        (w,) = self._spec_out(B, K, 1)
+        self._call(self._lib.lbm_wave_to_grid_adj, _lib.ptr(g), _lib.ptr(w), B=B, nk=K)
+        return w
+
+    def uv_to_vordiv_adj(self, vor, div):
+        """Transpose of uv_to_vordiv: spectral (vor, div) -> grid (u, v)."""
+        torch = _torch()
+        vor, div = _dev(vor, torch.complex128), _dev(div, torch.complex128)
+        B, _, _, K = vor.shape
+        u, v = self._grid_out(B, K, 2)
+        self._call(self._lib.lbm_uv_to_vordiv_adj, _lib.ptr(vor), _lib.ptr(div), _lib.ptr(u),
+                   _lib.ptr(v), B=B, nk=K)
+        return u, v
+
+    def psichi_to_uv_adj(self, u, v):
+        """Transpose of psichi_to_uv: grid (u, v) -> spectral (psi, chi)."""
+        torch = _torch()
+        u, v = _dev(u, torch.float64), _dev(v, torch.float64)
+        B, _, _, K = u.shape
+        psi, chi = self._spec_out(B, K, 2)
+        self._call(self._lib.lbm_psichi_to_uv_adj, _lib.ptr(u), _lib.ptr(v), _lib.ptr(psi),
+                   _lib.ptr(chi), B=B, nk=K)
+        return psi, chi
+
+    def hgrad_adj(self, gx, gy):
+        """Transpose of hgrad: grid (gx, gy) -> spectral."""
+        torch = _torch()
+        gx, gy = _dev(gx, torch.float64), _dev(gy, torch.float64)
+        B, _, _, K = gx.shape
+        (w,) = self._spec_out(B, K, 1)
+        self._call(self._lib.lbm_hgrad_adj, _lib.ptr(gx), _lib.ptr(gy), _lib.ptr(w), B=B, nk=K)
+        return w
+
     # -- model -------------------------------------------------------------
     def set_basic_state(self, u, v, t, q, ps):
         torch = _torch()

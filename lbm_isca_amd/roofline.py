@@ -9,6 +9,7 @@ Legendre tables once per launch and a shared basic state once per launch.
 MI355X_HBM_GBS = 8000.0          # vendor peak, GB/s
 MI355X_FP64_TFLOPS = 78.6
 MI355X_FP32_TFLOPS = 157.3
+MI355X_BF16_TFLOPS = 2500.0      # dense matrix cores
 
 
 def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):

@@ -292,3 +292,70 @@ def test_adjoint_needs_tangent_linear_and_response(env):
     m2 = tl_model(bs)
     with pytest.raises(RuntimeError, match="response"):
         m2.plan.adjoint_run(4)
+
+
+@pytest.mark.parametrize("nlon,nlat,nmax", [(128, 64, 42), (384, 192, 127)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_transform_adjoint_twins(nlon, nlat, nmax, precision):
+    """<T x, y> = <x, T^T y> for the five API transforms and their twins
+    (the transposed Legendre pair with the other table set, the FFT pair
+    with its transposed scalings); y spectral has zeros for n < m."""
+    from lbm_isca_amd.plan import Plan
+    p = Plan(nlon, nlat, nmax, precision=precision)
+    rng = np.random.default_rng(7)
+    M, B, K = nmax + 1, 2, 3
+    mask = (np.arange(M)[None, :] >= np.arange(M)[:, None])[None, :, :, None]
+
+    def spec():
+        return (rng.standard_normal((B, M, M, K)) + 1j * rng.standard_normal((B, M, M, K))) * mask
+
+    def grid():
+        return rng.standard_normal((B, nlat, nlon, K))
+
+    def ip(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        return float(np.sum(a.real * b.real + a.imag * b.imag))
+
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    cases_ = []
+    g, w = grid(), spec()
+    cases_.append(("g2w", ip(np_(p.grid_to_wave(g)), w), ip(g, np_(p.grid_to_wave_adj(w)))))
+    cases_.append(("w2g", ip(np_(p.wave_to_grid(w)), g), ip(w, np_(p.wave_to_grid_adj(g)))))
+    u, v, vo, dv = grid(), grid(), spec(), spec()
+    a, b = p.uv_to_vordiv(u, v)
+    ua, va = p.uv_to_vordiv_adj(vo, dv)
+    cases_.append(("uv2vd", ip(np_(a), vo) + ip(np_(b), dv), ip(u, np_(ua)) + ip(v, np_(va))))
+    ps_, ch = spec(), spec()
+    a, b = p.psichi_to_uv(ps_, ch)
+    pa, ca = p.psichi_to_uv_adj(u, v)
+    cases_.append(("pc2uv", ip(np_(a), u) + ip(np_(b), v), ip(ps_, np_(pa)) + ip(ch, np_(ca))))
+    a, b = p.hgrad(w)
+    wa = p.hgrad_adj(u, v)
+    cases_.append(("hgrad", ip(np_(a), u) + ip(np_(b), v), ip(w, np_(wa))))
+    for name, lhs, rhs in cases_:
+        print(f"{precision} {nlon}: {name} {lhs:.12e} {rhs:.12e}")
+        assert abs(lhs - rhs) <= tol * max(abs(lhs), abs(rhs)), (name, lhs, rhs)
+
+
+def test_adjoint_live_member_prefix_launches_are_exact(env):
+    """The adjoint of sixteen members ordered by decreasing length (shrunken
+    launches) equals that of the reversed order, bitwise."""
+    sht, vg, bs = env
+    E = 16
+    lengths = np.array([8, 8, 7, 7, 6, 6, 5, 5, 4, 4, 3, 3, 2, 2, 1, 1])
+    spec = spectral_state(sht, E, 91)
+    frc = heatings(sht, vg, E, 9)
+    res = []
+    for order in (np.arange(E), np.arange(E)[::-1]):
+        m = tl_model(bs, nmem=E)
+        plan = m.plan
+        response(plan, 8, steps=(1, 3, 5, 7, 9))
+        plan.set_lengths(lengths[order])
+        J = forward_J(plan, 0, {f: spec[f][order] for f in FIELDS}, frc[order])
+        gx, gf = plan.adjoint_run(0)
+        inv = np.argsort(order)
+        res.append((J[inv], [np_(g)[inv] for g in gx], np_(gf)[inv]))
+    assert np.array_equal(res[0][0], res[1][0])
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(res[0][2], res[1][2])

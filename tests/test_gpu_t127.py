@@ -28,7 +28,7 @@ def sht():
 def test_t127_transforms_match_oracle(sht):
     import lbm_isca_amd as L
     sp = L.Spectral(NLON, NLAT, NMAX)
-    np.testing.assert_allclose(sp.gw, sht.gw, rtol=5e-15)
+    np.testing.assert_allclose(sp.gw, sht.gw, rtol=3e-14)  # leggauss: 6e-11
     np.testing.assert_allclose(sp.mu, sht.mu, rtol=0, atol=3e-16)
     rng = np.random.default_rng(0)
     g = rng.standard_normal((NLAT, NLON, 3))
