@@ -299,6 +299,13 @@ int live_members(const lbm_plan *p, const std::vector<int> &len, int st, bool pe
   return std::max(8, (last + 1 + 7) / 8 * 8);
 }
 
+// every member advances through steps lo..hi (no mask needed there)
+bool all_live(const lbm_plan *p, const std::vector<int> &len, int hi) {
+  for (int e = 0; e < p->E; ++e)
+    if (hi >= p->inistp + len[e]) return false;
+  return true;
+}
+
 // launch extents of the column-tiled kernels for a mask
 unsigned tiles_x(int ncols, const ActiveMask &am) {
   return am.reduced() ? (unsigned)((ncols / am.E) * (am.ecnt / 8)) : (unsigned)((ncols + 7) / 8);
@@ -478,7 +485,9 @@ int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, i
 // (Ctl::step is the first field of the control block); the adjoint has no
 // deferred ps' updates
 static_assert(offsetof(Ctl, step) == 0, "the mask reads the step counter at the block's start");
+// ecnt <= 0: every member is live through the launches (no mask at all)
 ActiveMask loop_mask(const lbm_plan *p, bool with_pending, int ecnt) {
+  if (ecnt <= 0) return ActiveMask{nullptr, nullptr, nullptr, 0, p->E, 0};
   return ActiveMask{reinterpret_cast<const int *>(p->ctl), p->d_nsteps,
                     with_pending ? p->pend : nullptr, p->inistp, p->E,
                     ecnt < p->E ? ecnt : 0};
@@ -983,6 +992,8 @@ inline int mark(lbm_plan *p, int k, hipStream_t st) {
 // One time step: synthesis, inverse FFT, grid tendencies, forward FFT,
 // analysis, spectral step.  raw_path: the first step after a raw initial
 // state (k_tend reads the raw grid).
+// ecnt: leading members covered by the launches, or 0 when every member is
+// live through the step (no activity mask)
 template <class T>
 int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   const int E = p->E, L = p->L, EL = E * L;
@@ -990,6 +1001,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   const ColMap cmi{E, L, 6, 6};
   const int Ci = cmi.ncols();
   const ActiveMask am = loop_mask(p, true, ecnt);
+  const int ne = ecnt > 0 ? ecnt : E;  // members launched
   {
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
     CHK(mark(p, 0, st));
@@ -1028,7 +1040,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   CHK(mark(p, 2, st));
   CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
   CHK(mark(p, 3, st));
-  hipLaunchKernelGGL((k_tend<T>), dim3(ecnt * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
@@ -1042,14 +1054,14 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
                                 nullptr, am)));
   CHK(mark(p, 7, st));
   if (L == 20)
-    CHK((launch_step_kernel<T, 20>(p, st, ecnt)));
+    CHK((launch_step_kernel<T, 20>(p, st, ne)));
   else if (L == 40)
-    CHK((launch_step_kernel<T, 40>(p, st, ecnt)));
+    CHK((launch_step_kernel<T, 40>(p, st, ne)));
   else
     return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
   CHK(mark(p, 8, st));
   if (p->resp_set)
-    hipLaunchKernelGGL((k_response<T>), dim3(ecnt), dim3(256), 0, st, (const cx<T> *)p->sa3,
+    hipLaunchKernelGGL((k_response<T>), dim3(ne), dim3(256), 0, st, (const cx<T> *)p->sa3,
                        (const cx<T> *)p->obs, (const T *)p->wlev, (const T *)p->wstep, p->nwstep, E,
                        L, p->nspec, (const Ctl *)p->ctl, (const int *)p->d_nsteps, p->inistp,
                        p->jacc);
@@ -1271,11 +1283,12 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
   const ActiveMask am = loop_mask(p, false, ecnt);
+  const int ne = ecnt > 0 ? ecnt : E;
   CHK(mark(p, 7, st));
   if (L == 20)
-    CHK((launch_step_adj<T, 20>(p, st, ecnt)));
+    CHK((launch_step_adj<T, 20>(p, st, ne)));
   else
-    CHK((launch_step_adj<T, 40>(p, st, ecnt)));
+    CHK((launch_step_adj<T, 40>(p, st, ne)));
   const ColMap cmo{E, L, 9, 2}, cmi{E, L, 6, 6};
   const int Co = cmo.ncols(), Ci = cmi.ncols();
   CHK(mark(p, 5, st));
@@ -1306,7 +1319,7 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   ta.L = L;
   ta.nbasic = p->nb;
   ta.am = am;
-  hipLaunchKernelGGL((k_tend_adj<T>), dim3(ecnt * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  hipLaunchKernelGGL((k_tend_adj<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
   CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2, am));
@@ -1729,29 +1742,25 @@ int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *strea
   int rc = LBM_OK;
   if (p->raw_pending) {
     // the first step after a raw initial state reads the raw grid (unfused)
-    CHK(step_dispatch(p, st, live_members(p, len, 0, true), true));
+    CHK(step_dispatch(p, st, all_live(p, len, 0) ? 0 : live_members(p, len, 0, true), true));
     p->raw_pending = false;
     done = 1;
   }
   while (done < total) {
     int todo = total - done;
     if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
-    // the members still live at the first step of this stretch cover it
-    const int ecnt = live_members(p, len, done, true);
+    const int n = (use_graph && todo < chunk) ? 1 : std::min(todo, chunk);
+    // the members still live at the first step of this stretch cover it;
+    // no mask while every member advances through it
+    const int ecnt = all_live(p, len, done + n - 1) ? 0 : live_members(p, len, done, true);
     if (use_graph) {
       hipGraphExec_t ge = nullptr;
-      if (todo >= chunk) {
-        CHK(capture_graph(p, 0, chunk, ecnt, st, &ge));
-        todo = chunk;
-      } else {
-        CHK(capture_graph(p, 1, 1, ecnt, st, &ge));
-        todo = 1;
-      }
+      CHK(capture_graph(p, n == chunk ? 0 : 1, n, ecnt, st, &ge));
       HIPCHK(hipGraphLaunch(ge, st));
     } else {
-      todo = std::min(todo, chunk);
-      for (int s = 0; s < todo; ++s) CHK(step_dispatch(p, st, ecnt));
+      for (int s = 0; s < n; ++s) CHK(step_dispatch(p, st, ecnt));
     }
+    todo = n;
     done += todo;
     if (check_every > 0 && done % check_every == 0 && (rc = check_bad(p, st, done)) != LBM_OK) break;
   }
@@ -1904,7 +1913,8 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
     const int n = (use_graph && todo < chunk) ? 1 : std::min(todo, chunk);
     // adjoint steps total-1-done down to total-done-n: the earliest has the
     // most live members
-    const int ecnt = live_members(p, len, total - done - n, false);
+    const int ecnt =
+        all_live(p, len, total - done - 1) ? 0 : live_members(p, len, total - done - n, false);
     if (use_graph) {
       hipGraphExec_t ge = nullptr;
       CHK(capture_adjoint(p, n == chunk ? 2 : 3, n, ecnt, st, &ge));

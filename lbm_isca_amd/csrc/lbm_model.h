@@ -235,11 +235,12 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const BRow br{L};
   const int Rin = 6 * L + 6, Rout = 9 * L + 2;
   const bool use_raw = a.raw != nullptr && a.ctl->raw;
-  const T *gi = a.gin + (size_t)(e * J + j) * Rin * I + i;
-  const T *rw = use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I + i : nullptr;
-  const T *bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
-  const T *fr = a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
-  T *go = a.gout + (size_t)(e * J + j) * Rout * I + i;
+  const T *__restrict__ gi = a.gin + (size_t)(e * J + j) * Rin * I + i;
+  const T *__restrict__ rw = use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I + i : nullptr;
+  const T *__restrict__ bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
+  const T *__restrict__ fr =
+      a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
+  T *__restrict__ go = a.gout + (size_t)(e * J + j) * Rout * I + i;
   auto G = [&](int row) { return gi[(size_t)row * I]; };
   auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : G(k); };
   auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : G(L + k); };
@@ -342,7 +343,8 @@ __device__ inline void matvec(const T *__restrict__ A, const cx<T> *x, cx<T> *y)
 // One thread per (s, member), members fastest across lanes so every field
 // access of a wave is contiguous; all levels in registers.
 template <class T, int L>
-__global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L <= 20 ? 2 : 1)))
+void k_step(StepArgs<T> a) {
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
   const int s = idx / a.ecnt, e = idx - s * a.ecnt;
@@ -375,16 +377,18 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
   cx<T> *s2 = a.sa2 + (size_t)s * 3 * E + e;
 
-  cx<T> w[L], dbar[L], x[L];
+  // two level arrays (x, dbar): every mat-vec row goes straight to the
+  // Robert-Asselin update and to memory, which keeps two waves per SIMD
+  cx<T> x[L], dbar[L];
   // temperature estimate and its geopotential (integrator.py:287-292)
 #pragma unroll
-  for (int l = 0; l < L; ++l) w[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
+  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
   {
     cx<T> acc = zero<T>();
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      const cx<T> wl = w[l];
-      w[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
+      const cx<T> wl = x[l];
+      x[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
       acc = acc + (rg * vc[vo.dlog3() + l]) * wl;
     }
   }
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
   for (int l = 0; l < L; ++l) {
     const T ft = (T)1 + dt2 * ktt[l];
     const T gg = rg * tref[l] * vc[vo.cpg() + l];
-    const cx<T> phy = w[l] + (ft * gg) * wdpix;
+    const cx<T> phy = x[l] + (ft * gg) * wdpix;
     x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
@@ -413,25 +417,28 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
     }
     s2[0] = pin;
   }
+  // row r of V (or the identity) applied to x, then the filter and the store
+  auto update = [&](int f, const T *V, bool apply, cx<T> *sum) __attribute__((always_inline)) {
+    cx<T> acc = zero<T>();
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      cx<T> y = x[r];
+      if (apply) {
+        y = zero<T>();
+#pragma unroll
+        for (int c = 0; c < L; ++c) fma_acc(y, V[r * L + c], x[c]);
+      }
+      const cx<T> cur = sa[at(f, r)];
+      sb[at(f, r)] = oinit ? cur : cur + gam * (sb[at(f, r)] - (T)2 * cur + y);
+      sa[at(f, r)] = y;
+      if (sum) acc = acc + vc[vo.dsig() + r] * y;
+    }
+    if (sum) *sum = acc;
+  };
   // divergence
 #pragma unroll
   for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[at(1, l)];
-  if ((a.vmask & 2) && vd_low) {
-    matvec<T, L>(vinv + 1 * L * L, x, w);
-#pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = w[l];
-  }
-  {
-    cx<T> sum = zero<T>();
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const cx<T> cur = sa[at(1, l)];
-      sb[at(1, l)] = oinit ? cur : cur + gam * (sb[at(1, l)] - (T)2 * cur + x[l]);
-      sa[at(1, l)] = x[l];
-      sum = sum + vc[vo.dsig() + l] * x[l];
-    }
-    s2[2 * E] = sum;
-  }
+  update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E);
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
     cx<T> suf = zero<T>();
@@ -443,51 +450,16 @@ __global__ __launch_bounds__(64) void k_step(StepArgs<T> a) {
       x[l] = (T)(1) / ft * (sb[at(2, l)] + dt2 * (tn[at(2, l)] - tmd));
     }
   }
-  if (a.vmask & 4) {
-    matvec<T, L>(vinv + 2 * L * L, x, w);
-#pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = w[l];
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[at(2, l)];
-    sb[at(2, l)] = oinit ? cur : cur + gam * (sb[at(2, l)] - (T)2 * cur + x[l]);
-    sa[at(2, l)] = x[l];
-  }
+  update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr);
   // vorticity
 #pragma unroll
   for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
-  if ((a.vmask & 1) && vd_low) {
-    matvec<T, L>(vinv, x, w);
-#pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = w[l];
-  }
-  {
-    cx<T> sum = zero<T>();
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const cx<T> cur = sa[at(0, l)];
-      sb[at(0, l)] = oinit ? cur : cur + gam * (sb[at(0, l)] - (T)2 * cur + x[l]);
-      sa[at(0, l)] = x[l];
-      sum = sum + vc[vo.dsig() + l] * x[l];
-    }
-    s2[E] = sum;
-  }
+  update(0, vinv, (a.vmask & 1) && vd_low, s2 + E);
   // humidity
 #pragma unroll
   for (int l = 0; l < L; ++l)
     x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
-  if (a.vmask & 8) {
-    matvec<T, L>(vinv + 3 * L * L, x, w);
-#pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = w[l];
-  }
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const cx<T> cur = sa[at(3, l)];
-    sb[at(3, l)] = oinit ? cur : cur + gam * (sb[at(3, l)] - (T)2 * cur + x[l]);
-    sa[at(3, l)] = x[l];
-  }
+  update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr);
 }
 
 // Column sums sum_l dsig_l x_l of the spectral vorticity and divergence of
