@@ -247,7 +247,9 @@ def measure_t127(E=32, days=20.0):
     bs, _ = cases.jet_states(sp, vg.sig, E, seed=1)
     vor = cases.vorticity_perturbation(sp, vg.kmax, E, seed=4)
     log(f"t127l40 setup {time.perf_counter() - t0:.1f} s")
-    nsteps = int(round(72 * days))
+    delt = cases.t127l40_config()["delt"]
+    spd = int(round(86400.0 / delt))
+    nsteps = int(round(spd * days))
     M, Jh = 128, 96
     nspec = M * (M + 1) // 2
     flops = legendre_flops(E, vg.kmax, nspec, Jh)
@@ -278,7 +280,7 @@ def measure_t127(E=32, days=20.0):
             executed = 3.0 if prec == "bf16x3" else 1.0
             mfma[k] = {"ms": round(avg[k], 4), "useful_tflops": round(tf, 2),
                        "frac_of_peak": round(executed * tf / peaks[prec], 4)}
-        res[prec] = {"member_days_per_s": round(E * nsteps / 72.0 / dt, 3),
+        res[prec] = {"member_days_per_s": round(E * nsteps / spd / dt, 3),
                      "ms_per_step": round(dt * 1e3 / (nsteps + cfg.inistp), 3),
                      "legendre": mfma, "peak_tflops": peaks[prec]}
         log(f"t127l40 {prec}: {res[prec]}")
@@ -292,8 +294,9 @@ def measure_t127(E=32, days=20.0):
             e[f] = float(f"{np.sqrt(((a - b) ** 2).sum()) / den:.3e}") if den > 0 else 0.0
         err[prec] = e
     return {"workload": f"T127L40 {E} balanced-jet members (own basic states), n^-1.5 vorticity "
-                        f"perturbation rms 1e-6 1/s, {days:g} days, one graph replay per precision",
-            "members": E, "days": days, **res, "rel_l2_vs_fp64": err,
+                        f"perturbation rms 1e-6 1/s, {days:g} days at dt = {delt:g} s, one graph "
+                        "replay per precision",
+            "members": E, "days": days, "dt_s": delt, **res, "rel_l2_vs_fp64": err,
             "note": "useful Legendre flops (executed on the matrix cores: x3 for bf16x3) over the "
                     "dense peak of the precision"}
 

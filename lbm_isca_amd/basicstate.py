@@ -252,7 +252,7 @@ def jet_profile(plev, p_jet=200.0, taper=1.5):
 
 
 def balanced_jets(lat, lon, nmem, seed=1, plev=PLEV17, width_deg=12.0, wave_width_deg=20.0,
-                  p_jet=200.0, taper=1.5, f_min_lat_deg=15.0):
+                  p_jet=200.0, taper=1.5, eq_taper_deg=20.0):
     """Per-member jets u = U exp(-((lat - lat_j)/12 deg)^2) with U ~ U(20, 50)
     m/s and lat_j ~ U(25N, 45N), plus stationary waves m = 1, 2, 3 of
     amplitude U(0, 100) m and random phase in the geopotential, on the
@@ -264,8 +264,14 @@ def balanced_jets(lat, lon, nmem, seed=1, plev=PLEV17, width_deg=12.0, wave_widt
     hPa).  The jet's geopotential solves geostrophic
     balance exactly, z = z_std - (a/g) s(p) int f u dlat, and the
     temperature is hydrostatic, T = (g/R) dz/dln(1000/p), so T is in thermal
-    wind balance with u.  The waves carry geostrophic winds (with |f| held at
-    or above its value at 15 deg) and their hydrostatic temperature.  q = 0.
+    wind balance with u.  The waves' envelope exp(-((lat - lat_j)/20 deg)^2)
+    is tapered by mu = 1 - exp(-(lat/20 deg)^2) toward the equator; they carry
+    the geostrophic winds of the true Coriolis parameter and their
+    hydrostatic temperature.  Since mu ~ lat^2, dz/dlat / f and
+    dz/dlon / f stay finite at the equator and the winds are smooth there
+    (geostrophic winds of an untapered envelope, with |f| held at a floor,
+    jump at the equator: a vortex sheet whose shear instability T127
+    resolves).  q = 0.
     Returns dict of (nmem, nlat, nlon, nlev) arrays z, u, v, t, q and the
     drawn parameters.
     """
@@ -281,10 +287,17 @@ def balanced_jets(lat, lon, nmem, seed=1, plev=PLEV17, width_deg=12.0, wave_widt
     t_std, z_std = standard_profile(plev)
     wj = np.radians(width_deg)
     ww = np.radians(wave_width_deg)
-    f = 2.0 * OMEGA * np.sin(lat)
-    fmin = 2.0 * OMEGA * np.sin(np.radians(f_min_lat_deg))
-    fc = np.where(lat >= 0, 1.0, -1.0) * np.maximum(np.abs(f), fmin)
     coslat = np.cos(lat)
+    # equatorial taper mu and the ratios mu / sin(lat), mu' / sin(lat),
+    # written so that lat = 0 needs no division by zero
+    l0 = np.radians(eq_taper_deg)
+    y = (lat / l0) ** 2
+    lat_over_sin = 1.0 / np.sinc(lat / np.pi)
+    frac = np.where(y > 1e-12, -np.expm1(-y) / np.maximum(y, 1e-300), 1.0 - 0.5 * y)
+    mu = -np.expm1(-y)
+    mu_sin = (lat / l0 ** 2) * frac * lat_over_sin
+    dmu_sin = (2.0 / l0 ** 2) * np.exp(-y) * lat_over_sin
+    gf = GRAV / (2.0 * OMEGA * ER)            # g / (2 Omega a): u = -gf dz/dlat / sin(lat)
     # fine quadrature for F(lat) = int_{-pi/2}^{lat} f u_jet dlat
     fine = np.linspace(-np.pi / 2, np.pi / 2, 4001)
     out = {k: np.zeros((nmem, J, I, K)) for k in ("z", "u", "v", "t", "q")}
@@ -301,19 +314,19 @@ def balanced_jets(lat, lon, nmem, seed=1, plev=PLEV17, width_deg=12.0, wave_widt
         ubar = ujet[:, None] * s[None, :]
         env = np.exp(-((lat - latj[e]) / ww) ** 2)
         denv = -2.0 * (lat - latj[e]) / ww ** 2 * env
+        # z_w = sum_m A_m cos(arg_m) env mu; the winds use dz_w/dlat / sin(lat)
+        # = sum A cos (env' mu/sin + env mu'/sin) and dz_w/dlon / sin(lat)
         zw = np.zeros((J, I))
-        dzw_dlat = np.zeros((J, I))
-        dzw_dlon = np.zeros((J, I))
+        dlat_sin = np.zeros((J, I))
+        dlon_sin = np.zeros((J, I))
         for m in range(3):
             arg = (m + 1) * lon[None, :] + phase[e, m]
-            zw += amp[e, m] * env[:, None] * np.cos(arg)
-            dzw_dlat += amp[e, m] * denv[:, None] * np.cos(arg)
-            dzw_dlon += -amp[e, m] * (m + 1) * env[:, None] * np.sin(arg)
+            zw += amp[e, m] * (env * mu)[:, None] * np.cos(arg)
+            dlat_sin += amp[e, m] * (denv * mu_sin + env * dmu_sin)[:, None] * np.cos(arg)
+            dlon_sin += -amp[e, m] * (m + 1) * (env * mu_sin)[:, None] * np.sin(arg)
         out["z"][e] = zbar[:, None, :] + zw[:, :, None] * s[None, None, :]
-        out["u"][e] = ubar[:, None, :] - (GRAV / (fc * ER))[:, None, None] * dzw_dlat[:, :, None] \
-            * s[None, None, :]
-        out["v"][e] = (GRAV / (fc * ER * coslat))[:, None, None] * dzw_dlon[:, :, None] \
-            * s[None, None, :]
+        out["u"][e] = ubar[:, None, :] - gf * dlat_sin[:, :, None] * s[None, None, :]
+        out["v"][e] = (gf / coslat)[:, None, None] * dlon_sin[:, :, None] * s[None, None, :]
         out["t"][e] = tbar[:, None, :] + (GRAV / RDGAS) * zw[:, :, None] * ds[None, None, :]
     params = types.SimpleNamespace(U=U, lat_jet=latj, amp=amp, phase=phase)
     return out, params

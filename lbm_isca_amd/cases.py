@@ -34,8 +34,12 @@ DDRAG_L40 = list(np.repeat([0.5, 0.5, 0.5, 5, 15, 20, 20, 20, 20, 20,
 
 def t127l40_config(**kw):
     """Config keywords of the T127 L40 workload: the 40-level drag for
-    vorticity, divergence and temperature, defaults otherwise."""
-    out = dict(ddragv=list(DDRAG_L40), ddragd=list(DDRAG_L40), ddragt=list(DDRAG_L40))
+    vorticity, divergence and temperature, and dt = 450 s.  The leapfrog
+    advection needs u N dt / (a cos lat) < 1: at N = 127 the 50 m/s jets
+    give about 1.7 at 1200 s (the T42 step; runs blow up by day 3) and 0.85
+    at 600 s (blow-up by day 10); 450 s gives 0.64."""
+    out = dict(ddragv=list(DDRAG_L40), ddragd=list(DDRAG_L40), ddragt=list(DDRAG_L40),
+               delt=450.0)
     out.update(kw)
     return out
 

@@ -131,3 +131,36 @@ def test_balanced_jets_are_statically_stable():
     assert np.diff(theta, axis=-1).min() > 0.5
     s, ds = B.jet_profile(PLEV17)
     assert s[0] == 0.0 and abs(s[-1] - 1.0) < 1e-12 and np.all(np.diff(s) >= 0)
+
+
+def test_balanced_jet_waves_are_geostrophic_and_smooth_at_the_equator():
+    """The stationary waves' winds satisfy f u_w = -(g/a) dz_w/dlat and
+    f v_w = (g/(a cos)) dz_w/dlon with the true Coriolis parameter at every
+    latitude, the equator included, and are continuous across it (no vortex
+    sheet): on a 0.25 deg grid no wave wind changes by more than 0.5 m/s
+    between neighbouring latitudes."""
+    lat = np.radians(np.linspace(60, -60, 481))         # includes lat = 0
+    lon = np.linspace(0, 2 * np.pi, 96, endpoint=False)
+    f, par = B.balanced_jets(lat, lon, 8, seed=1)
+    assert np.all(np.isfinite(f["u"])) and np.all(np.isfinite(f["v"]))
+    fc = 2 * OMEGA * np.sin(lat)
+    for e in range(8):
+        k = -1                                              # strongest waves: s(p) = 1
+        zw = f["z"][e][..., k] - f["z"][e][..., k].mean(axis=1, keepdims=True)
+        uw = f["u"][e][..., k] - f["u"][e][..., k].mean(axis=1, keepdims=True)
+        vw = f["v"][e][..., k]
+        dz_lat = np.gradient(zw, lat, axis=0, edge_order=2)
+        kk = np.arange(lon.size // 2 + 1)
+        dz_lon = np.fft.irfft(1j * kk * np.fft.rfft(zw, axis=1), n=lon.size, axis=1)
+        scale = np.abs(uw).max() * np.abs(fc).max()
+        inner = slice(2, -2)
+        np.testing.assert_allclose((fc[:, None] * uw)[inner], (-(GRAV / ER) * dz_lat)[inner],
+                                   atol=2e-3 * scale)
+        np.testing.assert_allclose(fc[:, None] * vw,
+                                   (GRAV / ER) * dz_lon / np.cos(lat)[:, None],
+                                   atol=2e-3 * np.abs(vw).max() * np.abs(fc).max())
+        # an untapered envelope with a floor on |f| jumps by up to ~25 m/s here
+        assert np.abs(np.diff(uw, axis=0)).max() < 0.5
+        assert np.abs(np.diff(vw, axis=0)).max() < 0.5
+    # the taper leaves the waves near full strength at the jets
+    assert np.all(par.lat_jet >= np.radians(25.0))
