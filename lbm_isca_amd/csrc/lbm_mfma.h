@@ -90,6 +90,20 @@ __device__ inline cx<T> pair_of(T mine, int part) {
 // 2z+1 (more than one only when N+1-m > 64).  K = latitude pairs, staged 32
 // at a time per field, double-buffered in LDS.
 // ---------------------------------------------------------------------------
+// whether any input field of an analysis recipe contracts with P (H)
+template <class R>
+constexpr bool any_p() {
+  for (int f = 0; f < R::NIN; ++f)
+    if (R::needP(f)) return true;
+  return false;
+}
+template <class R>
+constexpr bool any_h() {
+  for (int f = 0; f < R::NIN; ++f)
+    if (R::needH(f)) return true;
+  return false;
+}
+
 template <class T, class R, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
@@ -144,11 +158,31 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
   v4 AP[NIN], AH[NIN];
 #pragma unroll
   for (int f = 0; f < NIN; ++f) AP[f] = AH[f] = (v4){0, 0, 0, 0};
+  constexpr bool anyP = any_p<R>(), anyH = any_h<R>();
 
   stage(0, 0, 0);
   __syncthreads();
   int it = 0;
   for (int jc = 0; jc < nch; ++jc) {
+    // Legendre operands of this chunk, shared by every field: lane row n_a,
+    // latitude pair 32 jc + 4 q + k_a (FP64/FP32 K = 4 steps) or
+    // 32 jc + 16 (q / 4) + 4 k_a + q % 4 (bf16 K = 16 steps)
+    T pr[8], hr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int jp = 32 * jc + (SPLIT ? 16 * (q >> 2) + 4 * k_a + (q & 3) : 4 * q + k_a);
+      const bool ok = wave_live && row_ok && jp < Jh;
+      pr[q] = (anyP && ok) ? Pa[trow + jp] : (T)0;
+      hr[q] = (anyH && ok) ? Ha[trow + jp] : (T)0;
+    }
+    Bf16x3::s4 prh[2], prl[2], hrh[2], hrl[2];
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (anyP) Bf16x3::split(reinterpret_cast<const float *>(pr) + 4 * h, prh[h], prl[h]);
+        if (anyH) Bf16x3::split(reinterpret_cast<const float *>(hr) + 4 * h, hrh[h], hrl[h]);
+      }
+    }
 #pragma unroll
     for (int f = 0; f < NIN; ++f, ++it) {
       const int buf = it & 1;
@@ -157,41 +191,36 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
       else if (jc + 1 < nch)
         stage(jc + 1, 0, buf ^ 1);
       if (wave_live) {
+        // the P rows of parity p contract with the symmetric sums when
+        // p = 0, the H rows with the other one
+        const T(*bP)[16] = p == 0 ? sym[buf] : anti[buf];
+        const T(*bH)[16] = p == 0 ? anti[buf] : sym[buf];
         if constexpr (SPLIT) {
 #pragma unroll
-          for (int ks = 0; ks < 32; ks += 16) {
-            float pa[4], ha[4], xp[4], xh[4];
+          for (int h = 0; h < 2; ++h) {
+            float xp[4], xh[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              const int kk = ks + 4 * k_a + q;
-              const int jp = 32 * jc + kk;
-              const bool ok = row_ok && jp < Jh;
-              pa[q] = (ok && R::needP(f)) ? (float)Pa[trow + jp] : 0.f;
-              ha[q] = (ok && R::needH(f)) ? (float)Ha[trow + jp] : 0.f;
-              const float xs = sym[buf][kk][j_b], xa = anti[buf][kk][j_b];
-              xp[q] = p == 0 ? xs : xa;
-              xh[q] = p == 0 ? xa : xs;
+              const int kk = 16 * h + 4 * k_a + q;
+              if (R::needP(f)) xp[q] = bP[kk][j_b];
+              if (R::needH(f)) xh[q] = bH[kk][j_b];
             }
-            Bf16x3::s4 ah, al, bh, bl;
+            Bf16x3::s4 bh, bl;
             if (R::needP(f)) {
-              Bf16x3::split(pa, ah, al);
               Bf16x3::split(xp, bh, bl);
-              AP[f] = Bf16x3::mma3(ah, al, bh, bl, AP[f]);
+              AP[f] = Bf16x3::mma3(prh[h], prl[h], bh, bl, AP[f]);
             }
             if (R::needH(f)) {
-              Bf16x3::split(ha, ah, al);
               Bf16x3::split(xh, bh, bl);
-              AH[f] = Bf16x3::mma3(ah, al, bh, bl, AH[f]);
+              AH[f] = Bf16x3::mma3(hrh[h], hrl[h], bh, bl, AH[f]);
             }
           }
         } else {
 #pragma unroll
-          for (int ks = 0; ks < 32; ks += 4) {
-            const int jp = 32 * jc + ks + k_a;
-            const bool ok = row_ok && jp < Jh;
-            const T xs = sym[buf][ks + k_a][j_b], xa = anti[buf][ks + k_a][j_b];
-            if (R::needP(f)) AP[f] = Mfma<T>::mma(ok ? Pa[trow + jp] : (T)0, p == 0 ? xs : xa, AP[f]);
-            if (R::needH(f)) AH[f] = Mfma<T>::mma(ok ? Ha[trow + jp] : (T)0, p == 0 ? xa : xs, AH[f]);
+          for (int q = 0; q < 8; ++q) {
+            const int kk = 4 * q + k_a;
+            if (R::needP(f)) AP[f] = Mfma<T>::mma(pr[q], bP[kk][j_b], AP[f]);
+            if (R::needH(f)) AH[f] = Mfma<T>::mma(hr[q], bH[kk][j_b], AH[f]);
           }
         }
       }
