@@ -446,7 +446,7 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
 template <class T, int NI>
 int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
                   ActiveMask am) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   const int C = cm.ncols();
   const long rows = fft_rows(p->J, C, am);
   hipLaunchKernelGGL((k_ifft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
@@ -458,7 +458,7 @@ int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st
 template <class T, int NI>
 int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode,
                  ActiveMask am) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   const int C = cm.ncols();
   const long rows = fft_rows(p->J, C, am);
   hipLaunchKernelGGL((k_fft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),

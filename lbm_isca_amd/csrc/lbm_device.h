@@ -180,12 +180,16 @@ struct ActiveMask {
 // uses 8R threads; then the real split to/from X_m = rfft/nlon.  Stages are
 // separated by block barriers placed by the kernels.
 // ---------------------------------------------------------------------------
-template <int NI>
+template <class T, int NI>
 struct FftShape {
   static constexpr int R = NI / 128;
   static constexpr int NC = NI / 2;
   static constexpr int TPR = 8 * R;             // threads per row
-  static constexpr int ROWS = 256 / TPR;        // rows per block
+  // rows per block: consecutive rows are consecutive Fourier columns, so a
+  // block's accesses to one coefficient m form ROWS x sizeof(cx<T>) bytes;
+  // at 384 points the LDS per row (3.4 KB FP64) bounds the blocks per CU
+  // (measured best: 20 rows FP64, 32 rows FP32)
+  static constexpr int ROWS = R == 1 ? 32 : (sizeof(T) == 8 ? 20 : 32);
   static constexpr int THREADS = ROWS * TPR;
   static constexpr int LDS = 8 * 9 * R;         // complex scratch per row (>= NC)
 };
@@ -247,7 +251,7 @@ __device__ __forceinline__ T fourier_scale(int mode, int m) {
 template <class T, int NI>
 __device__ inline void ifft_a(const cx<T> *__restrict__ F, size_t ms, int M, cx<T> *scr, int tt,
                               int mode) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
   cx<T> v[8];
@@ -279,9 +283,9 @@ __device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt) {
 #pragma unroll
   for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[r * 72 + n1 * 9 + t];
   dft8<+1>(v);
-  if (FftShape<NI>::R > 1) {
+  if (FftShape<T, NI>::R > 1) {
 #pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) v[k1] = cmul(v[k1], twf<T, NI, +1>(tw, (t + 8 * k1) * r, FftShape<NI>::NC));
+    for (int k1 = 0; k1 < 8; ++k1) v[k1] = cmul(v[k1], twf<T, NI, +1>(tw, (t + 8 * k1) * r, FftShape<T, NI>::NC));
   }
 }
 
@@ -289,7 +293,7 @@ __device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt) {
 // block n_a = r of this thread: z[n_b + 64 n_a], n_b = t + 8 k1.
 template <class T, int NI>
 __device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
-  constexpr int R = FftShape<NI>::R;
+  constexpr int R = FftShape<T, NI>::R;
   const int na = tt >> 3, t = tt & 7;
 #pragma unroll
   for (int k1 = 0; k1 < 8; ++k1) {
@@ -311,7 +315,7 @@ __device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
 // radix-8 pass -> scr.
 template <class T, int NI>
 __device__ inline void fft_a(const cx<T> *in, cx<T> *scr, int tt) {
-  constexpr int R = FftShape<NI>::R;
+  constexpr int R = FftShape<T, NI>::R;
   const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
   cx<T> v[8];
@@ -329,7 +333,7 @@ __device__ inline void fft_a(const cx<T> *in, cx<T> *scr, int tt) {
 #pragma unroll
       for (int q = 1; q < R; ++q)
         if (q == r) ur = u[q];
-      v[n2] = cmul(ur, twf<T, NI, -1>(tw, nb * r, FftShape<NI>::NC));
+      v[n2] = cmul(ur, twf<T, NI, -1>(tw, nb * r, FftShape<T, NI>::NC));
     }
   }
   dft8<-1>(v);
@@ -352,7 +356,7 @@ __device__ inline void fft_b(const cx<T> *scr, cx<T> *v, int tt) {
 template <class T, int NI>
 __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, int M, int tt,
                              int mode) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   const cx<T> *tw = twiddles<T, NI>();
   const T half = (T)0.5;
   for (int k = tt; k < M; k += S::TPR) {
@@ -367,11 +371,11 @@ __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, i
 
 // Fourier (m <= M-1) -> grid; rows are j*C + col.  mode: fourier_scale.
 template <class T, int NI>
-__global__ __launch_bounds__(FftShape<NI>::THREADS) void k_ifft(const cx<T> *__restrict__ F,
+__global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T> *__restrict__ F,
                                                                 T *__restrict__ G, int C, ColMap cm,
                                                                 int J, int M, int mode,
                                                                 ActiveMask am) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
@@ -404,11 +408,11 @@ __global__ __launch_bounds__(FftShape<NI>::THREADS) void k_ifft(const cx<T> *__r
 
 // grid -> Fourier (m <= M-1).  mode: fourier_scale.
 template <class T, int NI>
-__global__ __launch_bounds__(FftShape<NI>::THREADS) void k_fft(const T *__restrict__ G,
+__global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_fft(const T *__restrict__ G,
                                                                cx<T> *__restrict__ F, int C,
                                                                ColMap cm, int J, int M, int mode,
                                                                ActiveMask am) {
-  using S = FftShape<NI>;
+  using S = FftShape<T, NI>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
