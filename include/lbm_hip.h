@@ -31,6 +31,10 @@
 
 #include <stdint.h>
 
+/* Version of the entry points' signatures below; a binding checks it
+ * against lbm_abi_version() of the library it loads. */
+#define LBM_ABI_VERSION 3
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -175,6 +179,61 @@ int lbm_set_lengths(lbm_plan *plan, const int32_t *h_nsteps, void *stream);
  * nsteps <= 0 uses the lengths of lbm_set_lengths. */
 int lbm_run(lbm_plan *plan, int nsteps, int use_graph, int check_every, void *stream);
 
+/* The same run in segments, for history output between them: runs (or
+ * continues) the run of lbm_run(plan, nsteps, ...) until full step stop is
+ * done (stop = 0: to the end), without restarting the start-up steps; the
+ * run continues with the next call for the same nsteps until its end or
+ * until a state is set.  Replaces the history branch of the LBM.run loop
+ * (model.py:255-277), which hands each output interval to the writer
+ * between steps. */
+int lbm_run_to(lbm_plan *plan, int nsteps, int stop, int use_graph, int check_every,
+               void *stream);
+
+/* Top-of-step history (model.py:150-166, 189-277).  interval_steps > 0
+ * accumulates, from the next run on, the trapezoid 0.5 (s_a + s_{a-1}) of
+ * the top-of-step samples s (psi, chi, u, v, w, t, z, p) over each output
+ * interval of interval_steps full steps: their mean when time_average = 1
+ * (Config.time_average), the last trapezoid of the interval otherwise;
+ * interval_steps = 0 stops it.  keep_last = 1 keeps the last top-of-step
+ * sample of each member's run for lbm_get_diagnostics. */
+int lbm_set_history(lbm_plan *plan, int interval_steps, int time_average, int keep_last,
+                    void *stream);
+
+/* Output interval d (1-based) of the current run, after lbm_run_to has
+ * passed full step d * interval_steps: psi, chi, u, v, w, t, z (nmem, J, I,
+ * L) and p (nmem, J, I) float64, the writer.add fields of model.py:265-272
+ * (z without the writer's surface-pressure term, output.py:62-63).  The
+ * interval's sums are cleared.  Any output may be NULL. */
+int lbm_get_history(lbm_plan *plan, int interval, double *d_psi, double *d_chi, double *d_u,
+                    double *d_v, double *d_w, double *d_t, double *d_z, double *d_p,
+                    void *stream);
+
+/* LBM.diagnostics after a run (model.py:150-166): psi, chi and w of the
+ * last step's tendencies (its top-of-step state) and z = WW T / g of the
+ * current state, (nmem, J, I, L) float64; u, v, t, p are lbm_get_state's.
+ * Needs keep_last of lbm_set_history before the run. */
+int lbm_get_diagnostics(lbm_plan *plan, double *d_psi, double *d_chi, double *d_w, double *d_z,
+                        void *stream);
+
+/* Time-varying MJO temperature forcing (forcing.py:99-161), interpolated on
+ * the device at every step: d_gtf (nf, ndays, J, I, L) float64 K/s, nf = 1
+ * (shared) or nmem; the call counter c = call0 + step (start-up steps
+ * included, 1-based) holds day 1 while c < spinup, then moves from day it to
+ * it + 1 over nt calls.  single = 1: the fields hold float32 values (a
+ * file's real*4) and the slope (b - a) / nt is rounded to float32, as the
+ * reference does with them.  lbm_set_forcing replaces it. */
+int lbm_set_mjo_forcing(lbm_plan *plan, const double *d_gtf, int nf, int ndays, int nt,
+                        int spinup, int call0, int single, void *stream);
+
+/* Full steps done by the run in progress (0 when none). */
+int lbm_run_segment_done(const lbm_plan *plan, int32_t *full_steps);
+
+/* Spectral resize (regrid.py:68-77): d_src (B, src_nmax+1, src_nmax+1, nk)
+ * -> d_dst (B, dst_nmax+1, dst_nmax+1, nk) complex128, zero-padded or with
+ * the high-n tail dropped.  No plan needed. */
+int lbm_spec_resize(const double *d_src, double *d_dst, int B, int src_nmax, int dst_nmax, int nk,
+                    void *stream);
+
 /* Current time level ga in grid space (state.py:94-99 of the last step). */
 int lbm_get_state(lbm_plan *plan, double *d_u, double *d_v, double *d_t, double *d_q,
                   double *d_ps, void *stream);
@@ -224,6 +283,10 @@ int lbm_adjoint_run(lbm_plan *plan, int nsteps, double *d_gvor, double *d_gdiv, 
 
 /* Copies the last error message into buf (always NUL terminated). */
 int lbm_last_error(char *buf, int64_t len);
+
+/* LBM_ABI_VERSION of the built library (no counterpart in the reference:
+ * lets a binding refuse a library built from other sources). */
+int lbm_abi_version(void);
 
 #ifdef __cplusplus
 }

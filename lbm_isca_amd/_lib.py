@@ -21,6 +21,9 @@ LBM_FP64 = 0
 LBM_FP32 = 1
 LBM_BF16X3 = 2
 
+# LBM_ABI_VERSION of include/lbm_hip.h these bindings are written for
+ABI_VERSION = 3
+
 # Every symbol include/lbm_hip.h declares.
 EXPORTS = (
     "lbm_plan_create", "lbm_plan_destroy", "lbm_plan_get",
@@ -31,6 +34,8 @@ EXPORTS = (
     "lbm_set_spectral_state", "lbm_set_response", "lbm_get_response", "lbm_adjoint_run",
     "lbm_grid_to_wave_adj", "lbm_wave_to_grid_adj", "lbm_uv_to_vordiv_adj",
     "lbm_psichi_to_uv_adj", "lbm_hgrad_adj",
+    "lbm_run_to", "lbm_set_history", "lbm_get_history", "lbm_get_diagnostics",
+    "lbm_set_mjo_forcing", "lbm_run_segment_done", "lbm_spec_resize", "lbm_abi_version",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -97,7 +102,22 @@ def load():
         "lbm_uv_to_vordiv_adj": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
         "lbm_psichi_to_uv_adj": ([P, P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
         "lbm_hgrad_adj": ([P, P, P, P, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_run_to": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_set_history": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_get_history": ([P, ctypes.c_int, P, P, P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_get_diagnostics": ([P, P, P, P, P, P], ctypes.c_int),
+        "lbm_set_mjo_forcing": ([P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_run_segment_done": ([P, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "lbm_spec_resize": ([P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
+                            ctypes.c_int),
+        "lbm_abi_version": ([], ctypes.c_int),
     }
+    if not hasattr(lib, "lbm_abi_version") or lib.lbm_abi_version() != ABI_VERSION:
+        found = lib.lbm_abi_version() if hasattr(lib, "lbm_abi_version") else "none"
+        raise RuntimeError(f"{LIB_PATH} has ABI version {found}, these bindings need "
+                           f"{ABI_VERSION}: rebuild it (python -c 'import __graft_entry__ as g; "
+                           "g.build()')")
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args

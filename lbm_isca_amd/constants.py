@@ -15,6 +15,9 @@ RAIR = RDGAS
 EPSV = RDGAS / RVGAS
 EPSVT = RVGAS / RDGAS - 1.0
 DAYLEN_SEC = 86400.0
+# history output: Z gets ps' [hPa] * 100 / (rho0 g) (output.py:11-15)
+RHO00 = 1.225
+GRAV_GT2GR = 9.80
 S2D = 86400.0
 
 IMAX = 128

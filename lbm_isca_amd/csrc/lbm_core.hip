@@ -19,6 +19,7 @@
 
 #include "../../include/lbm_hip.h"
 #include "lbm_adjoint.h"
+#include "lbm_history.h"
 
 using namespace lbm;
 
@@ -239,6 +240,18 @@ struct lbm_plan {
   double *jacc = nullptr;
   int nwstep = 0, wstep_cap = 0;
   bool resp_set = false;
+  // history: interval sums (two slots) and the last top sample
+  int hist_mode = 0, hist_K = 0;
+  bool keep_last = false;  // keep the last top sample (LBM.diagnostics)
+  void *hspec[2] = {nullptr, nullptr}, *hgrid[2] = {nullptr, nullptr};
+  void *hlast_spec = nullptr, *hlast_grid = nullptr;
+  // MJO forcing: daily fields [f][day][j][L][I] interpolated into frc per step
+  void *mjo = nullptr;
+  int mjo_nd = 0, mjo_nt = 0, mjo_spin = 0, mjo_call0 = 0, mjo_single = 0;
+  // a run in progress (lbm_run_to): per-member lengths and steps done
+  bool in_run = false;
+  std::vector<int> run_len;
+  int run_done = 0;
   Ctl *ctl = nullptr;
   int *pend = nullptr, *d_nsteps = nullptr;
   std::vector<int> nsteps_h;
@@ -358,6 +371,7 @@ int build_tabs(lbm_plan *p) {
     vc[vo.wP() + k] = p->wP[k];
     vc[vo.cpg() + k] = p->cpg[k];
     vc[vo.dlog3() + k] = p->dlog3[k];
+    vc[vo.sig() + k] = p->sig[k];
   }
   for (int k = 0; k <= L; ++k) vc[vo.sigm() + k] = p->sigm[k];
   CHK(upload(&t.vc, vc));
@@ -944,6 +958,22 @@ int get_state_t(lbm_plan *p, double *u, double *v, double *t, double *q, double 
 // ---------------------------------------------------------------------------
 // One time step: synthesis, grid tendencies, analysis, spectral step.
 // ---------------------------------------------------------------------------
+template <class T>
+HistArgs<T> hist_args(const lbm_plan *p) {
+  HistArgs<T> h;
+  h.mode = p->hist_mode;
+  h.K = std::max(p->hist_K, 1);
+  h.inistp = p->inistp;
+  h.nsteps = p->d_nsteps;
+  for (int k = 0; k < 2; ++k) {
+    h.spec[k] = (cx<T> *)p->hspec[k];
+    h.grid[k] = (T *)p->hgrid[k];
+  }
+  h.last_spec = p->keep_last ? (cx<T> *)p->hlast_spec : nullptr;
+  h.last_grid = p->keep_last ? (T *)p->hlast_grid : nullptr;
+  return h;
+}
+
 template <class T, int L>
 int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
@@ -1035,10 +1065,18 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   ta.linear = p->linear;
   ta.robert = (T)p->robert;
   ta.am = am;
+  ta.h = hist_args<T>(p);
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
   CHK(mark(p, 2, st));
   CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
+  if (p->mjo) {
+    // this call's MJO temperature forcing (forcing.py:113-156)
+    const size_t n = (size_t)p->J * L * p->I;
+    hipLaunchKernelGGL((k_mjo<T>), dim3(nblk((size_t)p->nfrc * n)), dim3(256), 0, st,
+                       (const T *)p->mjo, (T *)p->frc, (const Ctl *)p->ctl, p->nfrc, p->mjo_nd,
+                       p->mjo_nt, p->mjo_spin, p->mjo_call0, p->mjo_single, n);
+  }
   CHK(mark(p, 3, st));
   hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
@@ -1052,6 +1090,13 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   CHK(mark(p, 6, st));
   CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,
                                 nullptr, am)));
+  if (p->hist_mode != 0 || p->keep_last) {
+    const size_t nthr = (size_t)p->nspec * ne;
+    hipLaunchKernelGGL((k_hist_acc<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+                       (const cx<T> *)p->sa3, hist_args<T>(p), (const Ctl *)p->ctl, E, L, p->nspec,
+                       ne);
+    HIPCHK(hipGetLastError());
+  }
   CHK(mark(p, 7, st));
   if (L == 20)
     CHK((launch_step_kernel<T, 20>(p, st, ne)));
@@ -1421,12 +1466,70 @@ size_t alloc_bytes(const lbm_plan *p) {
   return sizeof(T);
 }
 
+// ---------------------------------------------------------------------------
+// History extraction: synthesis of the spectral sums, unpacking to the
+// reference's grid layout.
+// ---------------------------------------------------------------------------
+// (vor, div, t, z) spectral [s][4][L][E] in p->tend3 -> grid rows
+// (psi, chi, u, v, t, z) [e][j][6L][I] in p->gin -> the outputs that are
+// not null, (E, J, I, L) float64.
+template <class T>
+int hist_synth(lbm_plan *p, double *const out[6], hipStream_t st) {
+  const int E = p->E, L = p->L, J = p->J, I = p->I, EL = E * L;
+  const ColMap cm{E, L, 6, 0};
+  OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
+  CHK((launch_syn<T, SynHist>(p, p->tend3, p->fin, EL, cm.ncols(), sl, st, SynHist::NIN)));
+  CHK(launch_ifft<T>(p, p->fin, p->gin, cm, st));
+  const size_t n3 = (size_t)E * J * I * L;
+  for (int f = 0; f < 6; ++f)
+    if (out[f])
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, (const T *)p->gin,
+                         out[f], E, J, I, L, cm.rows(), f * L, 0, (const T *)nullptr, 0, 0, 1);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int get_history_t(lbm_plan *p, int slot, double *psi, double *chi, double *u, double *v, double *w,
+                  double *t, double *z, double *ps, hipStream_t st) {
+  const int E = p->E, L = p->L, J = p->J, I = p->I;
+  const T scale = p->hist_mode == 1 ? (T)1 / (T)p->hist_K : (T)1;
+  hipLaunchKernelGGL((k_hist_spec_out<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
+                     (cx<T> *)p->hspec[slot], (cx<T> *)p->tend3, p->tabs<T>().vc, E, L, p->nspec,
+                     scale, 1);
+  HIPCHK(hipGetLastError());
+  double *const out[6] = {psi, chi, u, v, t, z};
+  CHK(hist_synth<T>(p, out, st));
+  hipLaunchKernelGGL((k_hist_grid_out<T>), dim3(nblk((size_t)E * J * (L + 1) * I)), dim3(256), 0, st,
+                     (T *)p->hgrid[slot], w, ps, E, J, I, L, scale, 1);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int get_diagnostics_t(lbm_plan *p, double *psi, double *chi, double *w, double *z, hipStream_t st) {
+  const int E = p->E, L = p->L, J = p->J, I = p->I;
+  hipLaunchKernelGGL((k_diag_spec<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
+                     (const cx<T> *)p->hlast_spec, (const cx<T> *)p->sa3, (cx<T> *)p->tend3,
+                     p->tabs<T>().vc, E, L, p->nspec);
+  HIPCHK(hipGetLastError());
+  double *const out[6] = {psi, chi, nullptr, nullptr, nullptr, z};
+  CHK(hist_synth<T>(p, out, st));
+  if (w)
+    hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk((size_t)E * J * I * L)), dim3(256), 0, st,
+                       (const T *)p->hlast_grid, w, E, J, I, L, L, 0, 0, (const T *)nullptr, 0, 0, 1);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
 extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
 
 int lbm_last_error(char *buf, int64_t len) {
   if (!buf || len <= 0) return LBM_ERR_VALUE;
@@ -1442,7 +1545,9 @@ int lbm_plan_destroy(lbm_plan *p) {
   void *bufs[] = {p->d_n_of_s, p->d_mu, p->d_psbar, p->sa3, p->sa2, p->sb3, p->tend3, p->tend2,
                   p->fin, p->gin, p->gout, p->fout, p->raw, p->psa, p->psb, p->bsg, p->frc,
                   p->ctl, p->pend, p->d_nsteps, p->scratch, p->sb2, p->aa3, p->ab3, p->aa2,
-                  p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc};
+                  p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc,
+                  p->hspec[0], p->hspec[1], p->hgrid[0], p->hgrid[1], p->hlast_spec,
+                  p->hlast_grid, p->mjo};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
@@ -1658,6 +1763,12 @@ int lbm_set_forcing(lbm_plan *p, const double *frc, int nfrc, void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   hipStream_t st = p->strm(stream);
   destroy_graphs(p);
+  if (p->mjo) {
+    // a constant forcing replaces the MJO forcing
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipFree(p->mjo));
+    p->mjo = nullptr;
+  }
   if (!frc) {
     if (p->frc) {
       HIPCHK(hipStreamSynchronize(st));
@@ -1693,6 +1804,7 @@ int lbm_set_state(lbm_plan *p, const double *u, const double *v, const double *t
   if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before the perturbation");
   hipStream_t st = p->strm(stream);
   p->steps_total = 0;
+  p->in_run = false;
   Ctl c0{0, 0, INT_MAX, 0};
   int rc = p->prec == LBM_FP64 ? set_state_t<double>(p, u, v, t, q, ps, st)
                                : set_state_t<float>(p, u, v, t, q, ps, st);
@@ -1717,7 +1829,7 @@ int lbm_set_lengths(lbm_plan *p, const int32_t *h_nsteps, void *stream) {
   return LBM_OK;
 }
 
-int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *stream) {
+int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every, void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_run before lbm_set_basic_state");
   hipStream_t st = use_graph ? p->own : p->strm(stream);
@@ -1726,28 +1838,41 @@ int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *strea
   for (int e = 0; e < p->E; ++e) len[e] = nsteps > 0 ? nsteps : p->nsteps_h[e];
   const int maxlen = *std::max_element(len.begin(), len.end());
   const int total = p->inistp + maxlen;
-  if (maxlen <= 0) return LBM_OK;
-  HIPCHK(hipMemcpyAsync(p->d_nsteps, len.data(), sizeof(int) * p->E, hipMemcpyHostToDevice, st));
-  // every run restarts the start-up half steps (model.py:211-219)
-  {
+  if (maxlen <= 0) {
+    p->in_run = false;
+    return LBM_OK;
+  }
+  if (stop < 0 || stop > maxlen) return fail(LBM_ERR_VALUE, "stop=%d outside 0..%d", stop, maxlen);
+  const int target = stop == 0 ? total : p->inistp + stop;
+  if (p->in_run && len != p->run_len)
+    return fail(LBM_ERR_STATE, "a run of other lengths is in progress (finish it or set a state)");
+  if (p->in_run && target < p->run_done)
+    return fail(LBM_ERR_VALUE, "stop=%d is behind the %d full steps already run", stop,
+                p->run_done - p->inistp);
+  if (!p->in_run) {
+    HIPCHK(hipMemcpyAsync(p->d_nsteps, len.data(), sizeof(int) * p->E, hipMemcpyHostToDevice, st));
+    // every run restarts the start-up half steps (model.py:211-219)
     Ctl h;
     HIPCHK(hipMemcpyAsync(&h, p->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     h.step = 0;
     h.bad_step = INT_MAX;
     HIPCHK(hipMemcpyAsync(p->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, st));
+    p->in_run = true;
+    p->run_len = len;
+    p->run_done = 0;
   }
   const int chunk = 8;
-  int done = 0;
+  int done = p->run_done;
   int rc = LBM_OK;
-  if (p->raw_pending) {
+  if (done == 0 && p->raw_pending && done < target) {
     // the first step after a raw initial state reads the raw grid (unfused)
     CHK(step_dispatch(p, st, all_live(p, len, 0) ? 0 : live_members(p, len, 0, true), true));
     p->raw_pending = false;
     done = 1;
   }
-  while (done < total) {
-    int todo = total - done;
+  while (done < target) {
+    int todo = target - done;
     if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
     const int n = (use_graph && todo < chunk) ? 1 : std::min(todo, chunk);
     // the members still live at the first step of this stretch cover it;
@@ -1760,16 +1885,23 @@ int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *strea
     } else {
       for (int s = 0; s < n; ++s) CHK(step_dispatch(p, st, ecnt));
     }
-    todo = n;
-    done += todo;
+    done += n;
     if (check_every > 0 && done % check_every == 0 && (rc = check_bad(p, st, done)) != LBM_OK) break;
   }
-  p->steps_total += done;
+  p->steps_total += done - p->run_done;
+  p->run_done = done;
+  if (rc != LBM_OK || done == total) p->in_run = false;
   if (rc != LBM_OK) return rc;
-  CHK(p->prec == LBM_FP64 ? final_check<double>(p, st) : final_check<float>(p, st));
+  if (done == total) CHK(p->prec == LBM_FP64 ? final_check<double>(p, st) : final_check<float>(p, st));
   CHK(check_bad(p, st, done));
   if (use_graph && stream) HIPCHK(hipStreamSynchronize(st));
   return LBM_OK;
+}
+
+int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  p->in_run = false;
+  return lbm_run_to(p, nsteps, 0, use_graph, check_every, stream);
 }
 
 int lbm_get_state(lbm_plan *p, double *u, double *v, double *t, double *q, double *ps,
@@ -1843,6 +1975,7 @@ int lbm_set_spectral_state(lbm_plan *p, const double *vor, const double *div, co
   if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before the perturbation");
   hipStream_t st = p->strm(stream);
   p->steps_total = 0;
+  p->in_run = false;
   int rc = p->prec == LBM_FP64 ? set_spectral_t<double>(p, vor, div, t, q, pi, st)
                                : set_spectral_t<float>(p, vor, div, t, q, pi, st);
   if (rc != LBM_OK) return rc;
@@ -1882,6 +2015,9 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
   if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_adjoint_run before lbm_set_basic_state");
   if (!p->linear) return fail(LBM_ERR_STATE, "the adjoint needs a plan with tangent_linear = 1");
   if (!p->resp_set) return fail(LBM_ERR_STATE, "the adjoint needs a response (lbm_set_response)");
+  if (p->mjo)
+    return fail(LBM_ERR_STATE, "the adjoint is built for constant forcings (an MJO forcing is set)");
+  p->in_run = false;
   hipStream_t st = use_graph ? p->own : p->strm(stream);
   if (use_graph && stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   const int E = p->E, L = p->L, J = p->J, I = p->I;
@@ -1929,6 +2065,128 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
   // the forward loop control is left as it was
   HIPCHK(hipMemcpyAsync(p->ctl, &saved, sizeof(Ctl), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_run_segment_done(const lbm_plan *p, int32_t *full_steps) {
+  if (!p || !full_steps) return fail(LBM_ERR_VALUE, "null argument");
+  *full_steps = p->in_run ? std::max(0, p->run_done - p->inistp) : 0;
+  return LBM_OK;
+}
+
+int lbm_set_history(lbm_plan *p, int interval_steps, int time_average, int keep_last,
+                    void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (interval_steps < 0) return fail(LBM_ERR_VALUE, "interval_steps must be >= 0");
+  if (p->in_run) return fail(LBM_ERR_STATE, "history cannot change while a run is in progress");
+  hipStream_t st = p->strm(stream);
+  destroy_graphs(p);
+  const int E = p->E, L = p->L, J = p->J, I = p->I;
+  const size_t cs = 2 * p->rsize(), EL = (size_t)E * L;
+  const size_t bs = cs * p->nspec * 3 * EL, bg = p->rsize() * (size_t)E * J * (L + 1) * I;
+  p->keep_last = keep_last != 0;
+  if (p->keep_last && !p->hlast_spec) {
+    HIPCHK(hipMalloc(&p->hlast_spec, cs * p->nspec * 2 * EL));
+    HIPCHK(hipMalloc(&p->hlast_grid, p->rsize() * (size_t)E * J * L * I));
+    HIPCHK(hipMemsetAsync(p->hlast_spec, 0, cs * p->nspec * 2 * EL, st));
+    HIPCHK(hipMemsetAsync(p->hlast_grid, 0, p->rsize() * (size_t)E * J * L * I, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (interval_steps == 0) {
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < 2; ++k) {
+      if (p->hspec[k]) HIPCHK(hipFree(p->hspec[k]));
+      if (p->hgrid[k]) HIPCHK(hipFree(p->hgrid[k]));
+      p->hspec[k] = p->hgrid[k] = nullptr;
+    }
+    p->hist_mode = 0;
+    p->hist_K = 0;
+    return LBM_OK;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (!p->hspec[k]) HIPCHK(hipMalloc(&p->hspec[k], bs));
+    if (!p->hgrid[k]) HIPCHK(hipMalloc(&p->hgrid[k], bg));
+    HIPCHK(hipMemsetAsync(p->hspec[k], 0, bs, st));
+    HIPCHK(hipMemsetAsync(p->hgrid[k], 0, bg, st));
+  }
+  p->hist_mode = time_average ? 1 : 2;
+  p->hist_K = interval_steps;
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_get_history(lbm_plan *p, int interval, double *psi, double *chi, double *u, double *v,
+                    double *w, double *t, double *z, double *ps, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (p->hist_mode == 0) return fail(LBM_ERR_STATE, "no history set (lbm_set_history)");
+  if (interval < 1) return fail(LBM_ERR_VALUE, "intervals are numbered from 1");
+  hipStream_t st = p->strm(stream);
+  if (p->own != st) HIPCHK(hipStreamSynchronize(p->own));
+  const int slot = interval & 1;
+  return p->prec == LBM_FP64 ? get_history_t<double>(p, slot, psi, chi, u, v, w, t, z, ps, st)
+                             : get_history_t<float>(p, slot, psi, chi, u, v, w, t, z, ps, st);
+}
+
+int lbm_get_diagnostics(lbm_plan *p, double *psi, double *chi, double *w, double *z,
+                        void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "no basic state");
+  if (!p->hlast_spec)
+    return fail(LBM_ERR_STATE, "the last top-of-step sample is not kept (lbm_set_history keep_last)");
+  hipStream_t st = p->strm(stream);
+  if (p->own != st) HIPCHK(hipStreamSynchronize(p->own));
+  return p->prec == LBM_FP64 ? get_diagnostics_t<double>(p, psi, chi, w, z, st)
+                             : get_diagnostics_t<float>(p, psi, chi, w, z, st);
+}
+
+int lbm_set_mjo_forcing(lbm_plan *p, const double *gtf, int nf, int ndays, int nt, int spinup,
+                        int call0, int single, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!gtf) return fail(LBM_ERR_VALUE, "null forcing");
+  if (nf != 1 && nf != p->E) return fail(LBM_ERR_VALUE, "the forcing must have 1 or nmem members");
+  if (ndays < 1 || nt < 1 || spinup < 0 || call0 < 0)
+    return fail(LBM_ERR_VALUE, "ndays, nt >= 1 and spinup, call0 >= 0 are needed");
+  hipStream_t st = p->strm(stream);
+  destroy_graphs(p);
+  const int J = p->J, I = p->I, L = p->L;
+  const size_t n = (size_t)J * L * I, rs = p->rsize();
+  HIPCHK(hipStreamSynchronize(st));
+  if (p->mjo) HIPCHK(hipFree(p->mjo));
+  p->mjo = nullptr;
+  if (p->frc && p->nfrc != nf) {
+    HIPCHK(hipFree(p->frc));
+    p->frc = nullptr;
+  }
+  if (!p->frc) HIPCHK(hipMalloc(&p->frc, (size_t)nf * n * rs));
+  p->nfrc = nf;
+  HIPCHK(hipMalloc(&p->mjo, (size_t)nf * ndays * n * rs));
+  // (nf * ndays, J, I, L) float64 -> [f][day][j][L][I]
+  const int B = nf * ndays;
+  if (p->prec == LBM_FP64)
+    hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk((size_t)B * n)), dim3(256), 0, st, gtf,
+                       (double *)p->mjo, B, J, I, L, L, 0, (const double *)nullptr);
+  else
+    hipLaunchKernelGGL((k_pack_grid<float>), dim3(nblk((size_t)B * n)), dim3(256), 0, st, gtf,
+                       (float *)p->mjo, B, J, I, L, L, 0, (const double *)nullptr);
+  HIPCHK(hipGetLastError());
+  p->mjo_nd = ndays;
+  p->mjo_nt = nt;
+  p->mjo_spin = spinup;
+  p->mjo_call0 = call0;
+  p->mjo_single = single ? 1 : 0;
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_spec_resize(const double *src, double *dst, int B, int src_nmax, int dst_nmax, int nk,
+                    void *stream) {
+  if (!src || !dst) return fail(LBM_ERR_VALUE, "null array");
+  if (B < 1 || src_nmax < 0 || dst_nmax < 0 || nk < 1)
+    return fail(LBM_ERR_VALUE, "B, nk >= 1 and truncations >= 0 are needed");
+  const size_t tot = (size_t)B * (dst_nmax + 1) * (dst_nmax + 1) * nk;
+  hipLaunchKernelGGL(k_spec_resize, dim3(nblk(tot)), dim3(256), 0, (hipStream_t)stream,
+                     (const double2 *)src, (double2 *)dst, B, src_nmax + 1, dst_nmax + 1, nk);
+  HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 

@@ -482,6 +482,24 @@ struct SynSurf {  // (pi, sum dsig vor, sum dsig div) -> (d/dx pi, d/dy pi, pi, 
     A[5] = x[2];
   }
 };
+struct SynHist {  // (vor, div, t, z) -> (psi, chi, u, v, t, z): the history fields of model.py:159-165
+  static constexpr int NIN = 4, NOUT = 6;
+  static constexpr bool hasA(int) { return true; }
+  static constexpr bool hasB(int o) { return o == 2 || o == 3; }
+  static constexpr bool scaled(int o) { return o == 2 || o == 3; }
+  template <class T>
+  __device__ __forceinline__ static void terms(const cx<T> *x, T il, T fm, cx<T> *A, cx<T> *B) {
+    const cx<T> psi = il * x[0], chi = il * x[1];
+    A[0] = psi;
+    A[1] = chi;
+    A[2] = im_mul(fm, chi);  // u = P (im chi) - H psi  (vordiv_to_uv, spectral.py:255-276)
+    B[2] = -psi;
+    A[3] = im_mul(fm, psi);  // v = P (im psi) + H chi
+    B[3] = chi;
+    A[4] = x[2];
+    A[5] = x[3];
+  }
+};
 struct SynScalar {  // wave_to_grid (spectral.py:193-197)
   static constexpr int NIN = 1, NOUT = 1;
   static constexpr bool hasA(int) { return true; }

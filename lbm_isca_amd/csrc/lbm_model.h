@@ -9,6 +9,7 @@ namespace lbm {
 constexpr double R_DRY = 287.04;
 constexpr double KAPPA = 2.0 / 7.0;
 constexpr double EPSVT = 461.50 / 287.04 - 1.0;
+constexpr double GRAV = 9.80;  // gravity of the history's z (constants.py:39)
 constexpr double OMEGA = 7.2921150e-5;
 
 // Vertical constants, packed in one array (index helpers below).
@@ -21,7 +22,8 @@ struct VOff {
   __host__ __device__ int wP() const { return 4 * L + 1; }
   __host__ __device__ int cpg() const { return 5 * L + 1; }
   __host__ __device__ int dlog3() const { return 6 * L + 1; }
-  __host__ __device__ int size() const { return 7 * L + 1; }
+  __host__ __device__ int sig() const { return 7 * L + 1; }           // full levels
+  __host__ __device__ int size() const { return 8 * L + 1; }
 };
 
 // Rows of the basic-state grid block [eb][j][row][I].
@@ -94,7 +96,8 @@ template <class T>
 __device__ inline void column_level(int k, int L, const T *__restrict__ vc, const T *__restrict__ tref,
                                     const T *__restrict__ bs, const T *__restrict__ fr, int I,
                                     const ColConst<T> &cc, ColState<T> &cs, T up, T vp, T vor, T div,
-                                    T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out) {
+                                    T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out, T psbar, T pst,
+                                    T &omega) {
   const VOff vo{L};
   const BRow br{L};
   auto B = [&](int row) { return bs[(size_t)row * I]; };
@@ -139,6 +142,8 @@ __device__ inline void column_level(int k, int L, const T *__restrict__ vc, cons
   const T tv = tp + eps * (tp * bq + bt * qp);
   const T x5p = alpha * div + wP * padv + w3 * Sk1;
   const T adiab = -kap * (tv * bx5 + btv * x5p);
+  // omega diagnostic -sig (x5' ps_bar + x5_bar ps') [hPa/s] (dynamics.py:265-267)
+  omega = -vc[vo.sig() + k] * (x5p * psbar + bx5 * pst);
   const T tdev = bt - tr, tvdev = btv - tr;
   const T xu = up * tdev + bu * tp, xv = vp * tdev + bv * tp;
   const T hadv = tp * bdiv + tdev * div;
@@ -182,7 +187,7 @@ __device__ inline void column_level(int k, int L, const T *__restrict__ vc, cons
 // ln(ps_bar + ps'_b) - ln ps_bar = log1p(ps'_b/ps_bar) (state.py:77-79).
 template <class T>
 __device__ inline void column_surface(T *psa_p, T *psb_p, int mode, T pig, T psbar, T robert,
-                                      T S0, T D0, T &gt_pi, T &pi_prev, T &chk) {
+                                      T S0, T D0, T &gt_pi, T &pi_prev, T &chk, T &ps_t) {
   T psa = *psa_p, psb = *psb_p;
   if (mode != 0) {
     const T psn = psbar * expm1(pig);
@@ -191,9 +196,68 @@ __device__ inline void column_surface(T *psa_p, T *psb_p, int mode, T pig, T psb
     *psa_p = psa;
     *psb_p = psb;
   }
+  ps_t = psa;
   chk += psa;
   gt_pi = D0 - S0;
   pi_prev = log1p(psb / psbar);
+}
+
+// Top-of-step history (model.py:150-166, 189-277).  The reference samples
+// psi, chi, u, v, w, t, z, p at the top of every step and averages the
+// trapezoid 0.5 (sample_a + sample_{a-1}) over each output interval.  All
+// of them but w and p are linear in the spectral vorticity, divergence and
+// temperature, so those three are summed in spectral space and synthesised
+// once per interval; w (omega) and p (ps') are summed on the grid.  Two
+// slots alternate by interval parity: the sample that closes interval d
+// also opens d+1.  The last top sample of each member's run is kept for
+// LBM.diagnostics (psi, chi and w of the last step's tendencies).
+template <class T>
+struct HistArgs {
+  int mode;           // 0 off, 1 interval means, 2 instantaneous (time_average = false)
+  int K;              // full steps per output interval
+  int inistp;         // start-up steps: top sample a = step - inistp + 1
+  const int *nsteps;  // [E] full steps of each member's run
+  cx<T> *spec[2];     // [s][3][L][E] vor, div, t sums, or null
+  T *grid[2];         // [e][j][L+1][I] omega levels, then ps', or null
+  cx<T> *last_spec;   // [s][2][L][E] vor, div of the last top sample, or null
+  T *last_grid;       // [e][j][L][I] omega of the last top sample, or null
+};
+
+// (slot, weight) pairs of top sample a (0: the last start-up step; a >= 1:
+// full step a) of a member with nst full steps; returns their number.
+// mode 1: interval d = ceil(a/K) gets the trapezoid
+// 0.5 s_{(d-1)K} + s_{(d-1)K+1} + ... + 0.5 s_{dK}; mode 2: the
+// instantaneous 0.5 (s_{dK} + s_{dK-1}).
+template <class T>
+__device__ inline int hist_weights(int mode, int K, int a, int nst, int *slot, T *w) {
+  if (mode == 0 || a < 0 || a > nst) return 0;
+  if (mode == 1) {
+    if (a == 0) {
+      slot[0] = 1;
+      w[0] = (T)0.5;
+      return 1;
+    }
+    const int d = (a + K - 1) / K;
+    slot[0] = d & 1;
+    if (a % K != 0) {
+      w[0] = (T)1;
+      return 1;
+    }
+    w[0] = (T)0.5;
+    slot[1] = (d + 1) & 1;
+    w[1] = (T)0.5;
+    return 2;
+  }
+  int n = 0;
+  if ((a + 1) % K == 0) {
+    slot[n] = ((a + 1) / K) & 1;
+    w[n++] = (T)0.5;
+  }
+  if (a >= 1 && a % K == 0) {
+    slot[n] = (a / K) & 1;
+    w[n++] = (T)0.5;
+  }
+  return n;
 }
 
 template <class T>
@@ -213,6 +277,7 @@ struct TendArgs {
   int linear;       // tangent-linear mode: no ps' grid state (pi' is spectral)
   T robert;
   ActiveMask am;
+  HistArgs<T> h;
 };
 
 // Tendencies on grid rows, one thread per column, one pass over the levels
@@ -276,14 +341,24 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   ColState<T> cs;
   cs.bsd_lo = bs[(size_t)br.sdot() * I];
   const size_t pidx = (size_t)(e * J + j) * I + i;
-  T gpi, pib;
+  T gpi, pib, pst;
   if (a.linear) {
     gpi = cc.D0 - cc.S0;
     pib = 0;
+    pst = psbar * pig;
   } else {
     column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0,
-                      gpi, pib, cs.chk);
+                      gpi, pib, cs.chk, pst);
   }
+  // history samples of this top-of-step state (uniform per block)
+  int hslot[2];
+  T hw[2];
+  const int top_a = a.ctl->step - a.h.inistp + 1;
+  const int nst = a.h.nsteps ? a.h.nsteps[e] : 0;
+  const int hn = hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw);
+  const bool hlast = a.h.last_grid != nullptr && top_a == nst;
+  const size_t hrow = (size_t)(e * J + j) * (L + 1) * I + i;
+  for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
   go[(size_t)(9 * L) * I] = gpi;
   go[(size_t)(9 * L + 1) * I] = pib;
   const T *tref = a.tref + (size_t)eb * L;
@@ -292,11 +367,13 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     const bool top = (k + 1 == L);
     const T upn = top ? (T)0 : UP(k + 1), vpn = top ? (T)0 : VP(k + 1);
     const T tpn = top ? (T)0 : TP(k + 1), qpn = top ? (T)0 : QP(k + 1);
-    T out[9];
+    T out[9], omega;
     column_level<T>(k, L, a.vc, tref, bs, fr, I, cc, cs, up, vp, G(2 * L + k), G(3 * L + k), tp, qp,
-                    upn, vpn, tpn, qpn, out);
+                    upn, vpn, tpn, qpn, out, psbar, pst, omega);
 #pragma unroll
     for (int f = 0; f < 9; ++f) go[(size_t)(f * L + k) * I] = out[f];
+    for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
+    if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
     up = upn;
     vp = vpn;
     tp = tpn;
@@ -460,6 +537,32 @@ void k_step(StepArgs<T> a) {
   for (int l = 0; l < L; ++l)
     x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
   update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr);
+}
+
+// Samples of the top-of-step spectral state (the level the next k_step
+// replaces): the interval sums of vorticity, divergence and temperature and
+// the last sample (vorticity, divergence) of each member's run.  One thread
+// per (s, member), levels in a loop.
+template <class T>
+__global__ void k_hist_acc(const cx<T> *__restrict__ sa3, HistArgs<T> h, const Ctl *ctl, int E,
+                           int L, int nspec, int ecnt) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = idx / ecnt, e = idx - s * ecnt;
+  if (s >= (size_t)nspec) return;
+  const int top_a = ctl->step - h.inistp + 1;
+  int slot[2];
+  T w[2];
+  const int n = hist_weights<T>(h.mode, h.K, top_a, h.nsteps[e], slot, w);
+  const bool last = h.last_spec != nullptr && top_a == h.nsteps[e];
+  if (n == 0 && !last) return;
+  const size_t EL = (size_t)E * L;
+  for (int f = 0; f < 3; ++f)
+    for (int l = 0; l < L; ++l) {
+      const cx<T> x = sa3[(s * 4 + f) * EL + (size_t)l * E + e];
+      const size_t dst = (s * 3 + f) * EL + (size_t)l * E + e;
+      for (int q = 0; q < n; ++q) h.spec[slot[q]][dst] = h.spec[slot[q]][dst] + w[q] * x;
+      if (last && f < 2) h.last_spec[(s * 2 + f) * EL + (size_t)l * E + e] = x;
+    }
 }
 
 // Column sums sum_l dsig_l x_l of the spectral vorticity and divergence of

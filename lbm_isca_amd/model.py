@@ -3,7 +3,10 @@
 ``LBM.run`` executes the whole time loop on the GPU: the start-up half
 steps (dt/4, dt/2, clock not advanced), the fused step kernels and the
 Robert-Asselin filter, optionally replayed as a hipGraph, with an on-device
-finiteness flag instead of a host sync per step.
+finiteness flag instead of a host sync per step.  A time-varying MJO
+forcing is interpolated on the device inside the step graph, and the
+history (trapezoidal interval means of the top-of-step samples) is
+accumulated on the device and handed to the writer once per interval.
 """
 
 import time as _time
@@ -11,8 +14,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .backend import hip
+from .backend import asnumpy, hip
 from .constants import S2D
+from .forcing import MJOForcing
 from .plan import Plan
 from .state import State
 
@@ -168,6 +172,8 @@ class LBM:
         vertical.set_reference_temperature(tb[0])
         self.tbar = tb
         self.plan.set_state()
+        # keep the last top-of-step sample for diagnostics() after a run
+        self.plan.set_history(0, keep_last=True)
         self._ga = None
         self.time = 0.0
         self.istep = 0
@@ -207,37 +213,89 @@ class LBM:
         return self.plan.get_spectral(level)
 
     # -- time loop ---------------------------------------------------------
-    def run(self, mjo_forcing=None, steady_forcing=None, writer=None, verbose=True,
-            progress_every=240, use_graph=True, lengths=None, check_every=0):
-        """Integrate ``cfg.ndays`` (model.py:168-290).
+    def diagnostics(self):
+        """The eight history variables on the grid (model.py:150-166): psi,
+        chi and w from the last step's tendencies (its top-of-step state), u,
+        v, t, z and p from the current state.  Device tensors, with the
+        member axis for an ensemble."""
+        d = self.plan.get_diagnostics()
+        ga = self.ga
+        if not self.batched:
+            d = {k: v[0] for k, v in d.items()}
+        return {"psi": d["psi"], "chi": d["chi"], "u": ga.u, "v": ga.v, "w": d["w"],
+                "t": ga.t, "z": d["z"], "p": ga.ps}
 
-        ``lengths`` (extension): per-member numbers of full steps for an
-        uneven ensemble; members stop when they reach theirs.
-        """
-        if writer is not None:
-            raise NotImplementedError("history output (writer=) is not built on the hip backend")
-        c = self.cfg
-        frc = None
+    def _set_forcing(self, mjo_forcing, steady_forcing):
+        """Upload the run's temperature forcing; returns the time-varying
+        forcing object whose call counter the run advances, or None."""
         if mjo_forcing is not None:
+            if isinstance(mjo_forcing, MJOForcing):
+                self.plan.set_mjo_forcing(mjo_forcing.gtf, mjo_forcing.nt, mjo_forcing.spinup,
+                                          call0=mjo_forcing.counter)
+                return mjo_forcing
             if not getattr(mjo_forcing, "constant", False) and type(mjo_forcing).__name__ not in (
                     "EnsembleForcing", "ConstForcing"):
                 raise NotImplementedError(
-                    "time-varying forcing: only forcings whose step() is constant are built")
-            frc = mjo_forcing.step()
-        elif steady_forcing is not None:
+                    "time-varying forcing: MJOForcing or forcings whose step() is constant")
+            self.plan.set_forcing(mjo_forcing.step())
+            return None
+        frc = None
+        if steady_forcing is not None:
             for name in ("vor", "div", "ps"):
                 val = getattr(steady_forcing, name, None)
                 if val is not None and np.any(np.asarray(val) != 0):
                     raise NotImplementedError(f"non-zero steady {name} forcing is not built")
             frc = steady_forcing.t
         self.plan.set_forcing(frc)
-        nsteps = int(round(c.ndays * c.steps_per_day))
+        return None
+
+    def run(self, mjo_forcing=None, steady_forcing=None, writer=None, verbose=True,
+            progress_every=240, use_graph=True, lengths=None, check_every=0):
+        """Integrate ``cfg.ndays`` (model.py:168-290).
+
+        ``writer`` (``.add(day, fields)``, ``.close()``) receives, after each
+        ``out_interval_days``, the trapezoidal mean (``time_average``) or the
+        last trapezoid of the top-of-step history fields as host arrays, with
+        the member axis for an ensemble.  ``lengths`` (extension): per-member
+        numbers of full steps for an uneven ensemble; members stop when they
+        reach theirs (no writer then).
+        """
+        c = self.cfg
+        tv = self._set_forcing(mjo_forcing, steady_forcing)
+        spd = c.steps_per_day
+        nsteps = int(round(c.ndays * spd))
         if lengths is not None:
+            if writer is not None:
+                raise ValueError("history output needs one length for every member")
             self.plan.set_lengths(lengths)
             nsteps = 0
         t0 = _time.time()
-        self.plan.run(nsteps, use_graph=use_graph, check_every=check_every)
+        if writer is None:
+            self.plan.run(nsteps, use_graph=use_graph, check_every=check_every)
+        else:
+            K = int(round(c.out_interval_days * spd))
+            if K < 1:
+                raise ValueError("out_interval_days is shorter than one step")
+            self.plan.set_history(K, time_average=c.time_average, keep_last=True)
+            try:
+                day = 0
+                for stop in range(K, nsteps + 1, K):
+                    self.plan.run_to(nsteps, stop, use_graph=use_graph, check_every=check_every)
+                    day += 1
+                    fields = {k: asnumpy(v) for k, v in self.plan.get_history(day).items()}
+                    if not self.batched:
+                        fields = {k: v[0] for k, v in fields.items()}
+                    writer.add(day, fields)
+                    if verbose and progress_every and (stop % progress_every) < K:
+                        print(f"  day {stop / spd:6.2f}   [{_time.time() - t0:5.1f}s]", flush=True)
+                if nsteps % K:
+                    self.plan.run_to(nsteps, 0, use_graph=use_graph, check_every=check_every)
+            finally:
+                self.plan.set_history(0, keep_last=True)
+            writer.close()
         adv = nsteps if lengths is None else int(max(lengths))
+        if tv is not None:
+            tv.counter += c.inistp + adv
         self.time += adv * c.delt
         self.istep += adv
         self._ga = None

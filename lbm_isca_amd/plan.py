@@ -318,6 +318,61 @@ class Plan:
         _lib.check(self._lib.lbm_run(self._h, int(nsteps), int(bool(use_graph)),
                                      int(check_every), _stream()))
 
+    def run_to(self, nsteps, stop, use_graph=True, check_every=0):
+        """Run (or continue) the run of ``run(nsteps)`` up to full step
+        ``stop`` (0: its end) without restarting the start-up steps."""
+        _lib.check(self._lib.lbm_run_to(self._h, int(nsteps), int(stop), int(bool(use_graph)),
+                                        int(check_every), _stream()))
+
+    def set_history(self, interval_steps, time_average=True, keep_last=True):
+        """Accumulate the top-of-step history over intervals of
+        ``interval_steps`` full steps (0: off); ``keep_last`` keeps the last
+        top-of-step sample for :meth:`get_diagnostics`."""
+        _torch().cuda.current_stream().synchronize()
+        _lib.check(self._lib.lbm_set_history(self._h, int(interval_steps), int(bool(time_average)),
+                                             int(bool(keep_last)), _stream()))
+
+    def get_history(self, interval):
+        """Fields of output interval ``interval`` (1-based): dict of
+        psi, chi, u, v, w, t, z (E, J, I, L) and p (E, J, I) float64 tensors."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        out = {k: torch.empty((E, J, I, L), dtype=torch.float64, device="cuda")
+               for k in ("psi", "chi", "u", "v", "w", "t", "z")}
+        out["p"] = torch.empty((E, J, I), dtype=torch.float64, device="cuda")
+        _lib.check(self._lib.lbm_get_history(
+            self._h, int(interval), *[_lib.ptr(out[k]) for k in ("psi", "chi", "u", "v", "w",
+                                                                  "t", "z", "p")], _stream()))
+        return out
+
+    def get_diagnostics(self):
+        """psi, chi, w of the last step's tendencies and z of the current state
+        (E, J, I, L) float64 tensors."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        out = {k: torch.empty((E, J, I, L), dtype=torch.float64, device="cuda")
+               for k in ("psi", "chi", "w", "z")}
+        _lib.check(self._lib.lbm_get_diagnostics(
+            self._h, *[_lib.ptr(out[k]) for k in ("psi", "chi", "w", "z")], _stream()))
+        return out
+
+    def set_mjo_forcing(self, gtf, nt, spinup, call0=0):
+        """Daily temperature forcing fields (days, J, I, L) shared by every
+        member or (E, days, J, I, L) per member, interpolated on the device
+        (with the float32 slope of the reference when ``gtf`` is float32)."""
+        torch = _torch()
+        single = (gtf.dtype == torch.float32) if isinstance(gtf, torch.Tensor) else \
+            (np.asarray(gtf).dtype == np.float32)
+        g = _dev(gtf, torch.float64)
+        if g.dim() == 4:
+            g = g.unsqueeze(0)
+        if g.dim() != 5 or tuple(g.shape[2:]) != (self.nlat, self.nlon, self.nlev):
+            raise ValueError(f"MJO forcing must be (days, {self.nlat}, {self.nlon}, {self.nlev}) "
+                             "or (nmem, days, ...)")
+        _lib.check(self._lib.lbm_set_mjo_forcing(self._h, _lib.ptr(g), int(g.shape[0]),
+                                                 int(g.shape[1]), int(nt), int(spinup), int(call0),
+                                                 int(single), _stream()))
+
     def get_state(self):
         torch = _torch()
         E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev

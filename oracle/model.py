@@ -160,11 +160,15 @@ class OracleModel:
         self.ga.assign(self.gc)
         return new
 
-    def run(self, nsteps=None, frc_t=None, check_finite=True):
+    def run(self, nsteps=None, frc_t=None, check_finite=True, mjo=None, history=None):
         """Start-up half steps (dt/4, dt/2, clock not advanced) then ``nsteps``
         full steps (``model.py:211-253``).  ``frc_t`` is a constant (E|1, J, I, L)
-        temperature forcing in K/s.
+        temperature forcing in K/s; ``mjo`` an object whose ``step()`` gives
+        each call's forcing (replacing ``frc_t``); ``history`` an
+        ``oracle.history.History`` fed the top-of-step samples
+        (``model.py:233-277``).
         """
+        from .history import diagnostics
         c = self.cfg
         if nsteps is None:
             nsteps = int(round(c.ndays * c.steps_per_day))
@@ -176,12 +180,27 @@ class OracleModel:
                 oinit, dt = False, c.delt
             else:
                 oinit, dt = True, c.delt / (2.0 ** (c.inistp - istep + 1))
-            self.step(dt, oinit, frc_t)
+            frc = frc_t if mjo is None else np.asarray(mjo.step())
+            ga_top = None
+            if history is not None:
+                ga_top = GridState(self.ga.ps_bar, self.nmem, self.vg.kmax)
+                ga_top.assign(self.ga)
+            self.step(dt, oinit, frc)
+            top = None
+            if history is not None:
+                cur = self.ga
+                self.ga = ga_top
+                top = diagnostics(self)
+                self.ga = cur
             if check_finite and not self.ga.isfinite():
                 raise FloatingPointError(
                     f"model blew up at step {istep} (t={self.time / 86400:.2f} d)")
             if oinit:
+                if history is not None:
+                    history.start_step(top)
                 continue
             self.time += dt
             advanced += 1
+            if history is not None:
+                history.full_step(advanced, top)
         return self

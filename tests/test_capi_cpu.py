@@ -27,3 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_last_error_is_callable_without_device():
     assert isinstance(_lib.last_error(), str)
+
+
+def test_abi_version_of_header_binding_and_library_agree():
+    text = open(os.path.join(ROOT, "include", "lbm_hip.h")).read()
+    header = int(re.search(r"#define LBM_ABI_VERSION (\d+)", text).group(1))
+    assert _lib.ABI_VERSION == header
+    assert ctypes.CDLL(_lib.LIB_PATH).lbm_abi_version() == header
