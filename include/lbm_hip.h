@@ -33,7 +33,7 @@
 
 /* Version of the entry points' signatures below; a binding checks it
  * against lbm_abi_version() of the library it loads. */
-#define LBM_ABI_VERSION 3
+#define LBM_ABI_VERSION 4
 
 #ifdef __cplusplus
 extern "C" {
@@ -224,6 +224,15 @@ int lbm_get_diagnostics(lbm_plan *plan, double *d_psi, double *d_chi, double *d_
  * reference does with them.  lbm_set_forcing replaces it. */
 int lbm_set_mjo_forcing(lbm_plan *plan, const double *d_gtf, int nf, int ndays, int nt,
                         int spinup, int call0, int single, void *stream);
+
+/* Steady vorticity and divergence forcings d_vor, d_div (nf, J, I, L) and
+ * ln ps forcing d_ps (nf, J, I), float64 per second, nf = 1 or nmem, any of
+ * them NULL (all NULL: none): their analyses are added to the vorticity,
+ * divergence and pi tendencies of every step (model.py:202-205,
+ * dynamics.py:189-190,246-250).  The temperature forcing stays
+ * lbm_set_forcing's / lbm_set_mjo_forcing's. */
+int lbm_set_steady_forcing(lbm_plan *plan, const double *d_vor, const double *d_div,
+                           const double *d_ps, int nf, void *stream);
 
 /* Full steps done by the run in progress (0 when none). */
 int lbm_run_segment_done(const lbm_plan *plan, int32_t *full_steps);

@@ -22,7 +22,7 @@ LBM_FP32 = 1
 LBM_BF16X3 = 2
 
 # LBM_ABI_VERSION of include/lbm_hip.h these bindings are written for
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # Every symbol include/lbm_hip.h declares.
 EXPORTS = (
@@ -36,6 +36,7 @@ EXPORTS = (
     "lbm_psichi_to_uv_adj", "lbm_hgrad_adj",
     "lbm_run_to", "lbm_set_history", "lbm_get_history", "lbm_get_diagnostics",
     "lbm_set_mjo_forcing", "lbm_run_segment_done", "lbm_spec_resize", "lbm_abi_version",
+    "lbm_set_steady_forcing",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -112,6 +113,7 @@ def load():
         "lbm_spec_resize": ([P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
                             ctypes.c_int),
         "lbm_abi_version": ([], ctypes.c_int),
+        "lbm_set_steady_forcing": ([P, P, P, P, ctypes.c_int, P], ctypes.c_int),
     }
     if not hasattr(lib, "lbm_abi_version") or lib.lbm_abi_version() != ABI_VERSION:
         found = lib.lbm_abi_version() if hasattr(lib, "lbm_abi_version") else "none"

@@ -248,6 +248,9 @@ struct lbm_plan {
   // MJO forcing: daily fields [f][day][j][L][I] interpolated into frc per step
   void *mjo = nullptr;
   int mjo_nd = 0, mjo_nt = 0, mjo_spin = 0, mjo_call0 = 0, mjo_single = 0;
+  // steady vorticity/divergence [s][2][L][nsf] and ln ps [s][nsf] forcings (spectral)
+  void *sfv = nullptr, *sfp = nullptr;
+  int nsf = 0;
   // a run in progress (lbm_run_to): per-member lengths and steps done
   bool in_run = false;
   std::vector<int> run_len;
@@ -1090,6 +1093,13 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   CHK(mark(p, 6, st));
   CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,
                                 nullptr, am)));
+  if (p->sfv || p->sfp) {
+    const size_t nthr = (size_t)p->nspec * ne;
+    hipLaunchKernelGGL((k_add_steady<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+                       (cx<T> *)p->tend3, (cx<T> *)p->tend2, (const cx<T> *)p->sfv,
+                       (const cx<T> *)p->sfp, E, L, p->nspec, p->nsf, ne);
+    HIPCHK(hipGetLastError());
+  }
   if (p->hist_mode != 0 || p->keep_last) {
     const size_t nthr = (size_t)p->nspec * ne;
     hipLaunchKernelGGL((k_hist_acc<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
@@ -1200,6 +1210,9 @@ int set_spectral_t(lbm_plan *p, const double *vor, const double *div, const doub
   if (pi)
     hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st, pi,
                        (cx<T> *)p->sa2, E, M, 1, 3, 0, E);
+  // no global mean of vorticity or divergence (the reference's grid state
+  // holds winds: state.py:94-99); the tangent-linear model keeps its input
+  if (!p->linear) HIPCHK(hipMemsetAsync(p->sa3, 0, 2 * (size_t)EL * sizeof(cx<T>), st));
   hipLaunchKernelGGL((k_vsum<T>), dim3(nblk((size_t)p->nspec * E)), dim3(256), 0, st,
                      (const cx<T> *)p->sa3, (cx<T> *)p->sa2, p->tabs<T>().vc, E, L, p->nspec);
   HIPCHK(hipGetLastError());
@@ -1522,6 +1535,41 @@ int get_diagnostics_t(lbm_plan *p, double *psi, double *chi, double *w, double *
   return LBM_OK;
 }
 
+// Analyses of the steady forcings (nf members), packed for k_add_steady.
+template <class T>
+int set_steady_t(lbm_plan *p, const double *vor, const double *div, const double *ps, int nf,
+                 hipStream_t st) {
+  const int M = p->M, L = p->L;
+  const size_t nv = (size_t)p->nspec * 2 * L * nf, np = (size_t)p->nspec * nf;
+  double *w = nullptr;
+  HIPCHK(hipMalloc(&w, sizeof(double) * 2 * nf * M * M * L));
+  int rc = LBM_OK;
+  if (vor || div) {
+    rc = hipMalloc(&p->sfv, nv * sizeof(cx<T>)) == hipSuccess ? LBM_OK : fail(LBM_ERR_RUNTIME, "out of memory");
+    if (rc == LBM_OK && hipMemsetAsync(p->sfv, 0, nv * sizeof(cx<T>), st) != hipSuccess)
+      rc = fail(LBM_ERR_RUNTIME, "initialisation failed");
+    const double *src[2] = {vor, div};
+    for (int f = 0; f < 2 && rc == LBM_OK; ++f) {
+      if (!src[f]) continue;
+      rc = spectral_op<T>(p, OP_G2W, src[f], nullptr, w, nullptr, nf, L, st);
+      if (rc == LBM_OK)
+        hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk((size_t)p->nspec * nf * L)), dim3(256), 0,
+                           st, w, (cx<T> *)p->sfv, nf, M, L, 2, f, nf * L);
+    }
+  }
+  if (ps && rc == LBM_OK) {
+    rc = hipMalloc(&p->sfp, np * sizeof(cx<T>)) == hipSuccess ? LBM_OK : fail(LBM_ERR_RUNTIME, "out of memory");
+    if (rc == LBM_OK) rc = spectral_op<T>(p, OP_G2W, ps, nullptr, w, nullptr, nf, 1, st);
+    if (rc == LBM_OK)
+      hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(np)), dim3(256), 0, st, w, (cx<T> *)p->sfp,
+                         nf, M, 1, 1, 0, nf);
+  }
+  if (hipGetLastError() != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "kernel launch failed");
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(w);
+  return rc;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1547,7 +1595,7 @@ int lbm_plan_destroy(lbm_plan *p) {
                   p->ctl, p->pend, p->d_nsteps, p->scratch, p->sb2, p->aa3, p->ab3, p->aa2,
                   p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc,
                   p->hspec[0], p->hspec[1], p->hgrid[0], p->hgrid[1], p->hlast_spec,
-                  p->hlast_grid, p->mjo};
+                  p->hlast_grid, p->mjo, p->sfv, p->sfp};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
@@ -2017,6 +2065,10 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
   if (!p->resp_set) return fail(LBM_ERR_STATE, "the adjoint needs a response (lbm_set_response)");
   if (p->mjo)
     return fail(LBM_ERR_STATE, "the adjoint is built for constant forcings (an MJO forcing is set)");
+  if (p->sfv || p->sfp)
+    return fail(LBM_ERR_STATE,
+                "the adjoint is built for the linear model (steady vorticity, divergence or ln ps "
+                "forcing is set)");
   p->in_run = false;
   hipStream_t st = use_graph ? p->own : p->strm(stream);
   if (use_graph && stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
@@ -2188,6 +2240,23 @@ int lbm_spec_resize(const double *src, double *dst, int B, int src_nmax, int dst
                      (const double2 *)src, (double2 *)dst, B, src_nmax + 1, dst_nmax + 1, nk);
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+int lbm_set_steady_forcing(lbm_plan *p, const double *vor, const double *div, const double *ps,
+                           int nf, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  hipStream_t st = p->strm(stream);
+  destroy_graphs(p);
+  HIPCHK(hipStreamSynchronize(st));
+  if (p->sfv) HIPCHK(hipFree(p->sfv));
+  if (p->sfp) HIPCHK(hipFree(p->sfp));
+  p->sfv = p->sfp = nullptr;
+  p->nsf = 0;
+  if (!vor && !div && !ps) return LBM_OK;
+  if (nf != 1 && nf != p->E) return fail(LBM_ERR_VALUE, "the forcing must have 1 or nmem members");
+  p->nsf = nf;
+  return p->prec == LBM_FP64 ? set_steady_t<double>(p, vor, div, ps, nf, st)
+                             : set_steady_t<float>(p, vor, div, ps, nf, st);
 }
 
 }  // extern "C"

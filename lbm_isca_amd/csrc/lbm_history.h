@@ -137,6 +137,30 @@ __global__ void k_hist_grid_out(T *__restrict__ slot, double *__restrict__ w, do
   }
 }
 
+// Steady vorticity, divergence and ln ps forcings (model.py:202-205,
+// dynamics.py:189-190,246-250): their analyses, constant in time, added to
+// the analysed tendencies of vorticity, divergence and pi.  sfv [s][2][L][nf]
+// (or null), sfp [s][nf] (or null); one thread per (s, member).
+template <class T>
+__global__ void k_add_steady(cx<T> *__restrict__ tend3, cx<T> *__restrict__ tend2,
+                             const cx<T> *__restrict__ sfv, const cx<T> *__restrict__ sfp, int E,
+                             int L, int nspec, int nf, int ecnt) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = idx / ecnt, e = idx - s * ecnt;
+  if (s >= (size_t)nspec) return;
+  const size_t ef = nf > 1 ? e : 0, EL = (size_t)E * L, FL = (size_t)nf * L;
+  if (sfv)
+    for (int f = 0; f < 2; ++f)
+      for (int l = 0; l < L; ++l) {
+        cx<T> &x = tend3[(s * 4 + f) * EL + (size_t)l * E + e];
+        x = x + sfv[(s * 2 + f) * FL + (size_t)l * nf + ef];
+      }
+  if (sfp) {
+    cx<T> &x = tend2[s * 2 * E + e];
+    x = x + sfp[s * nf + ef];
+  }
+}
+
 // Spectral (B, Ms, Ms, K) -> (B, Md, Md, K) complex128: zero-padding or
 // dropping the high-n tail (regrid.py:68-77).
 __global__ void k_spec_resize(const double2 *__restrict__ src, double2 *__restrict__ dst, int B,

@@ -494,8 +494,13 @@ void k_step(StepArgs<T> a) {
     }
     s2[0] = pin;
   }
+  // The reference keeps winds, not vorticity and divergence, on its grid:
+  // their global means (n = 0) never survive a step (a forcing's mean
+  // still acts through dbar within the step).  Same here, outside the
+  // tangent-linear model.
+  const bool drop_mean = n == 0 && a.sb2 == nullptr;
   // row r of V (or the identity) applied to x, then the filter and the store
-  auto update = [&](int f, const T *V, bool apply, cx<T> *sum) __attribute__((always_inline)) {
+  auto update = [&](int f, const T *V, bool apply, cx<T> *sum, bool drop) __attribute__((always_inline)) {
     cx<T> acc = zero<T>();
 #pragma unroll
     for (int r = 0; r < L; ++r) {
@@ -505,6 +510,7 @@ void k_step(StepArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < L; ++c) fma_acc(y, V[r * L + c], x[c]);
       }
+      if (drop) y = zero<T>();
       const cx<T> cur = sa[at(f, r)];
       sb[at(f, r)] = oinit ? cur : cur + gam * (sb[at(f, r)] - (T)2 * cur + y);
       sa[at(f, r)] = y;
@@ -515,7 +521,7 @@ void k_step(StepArgs<T> a) {
   // divergence
 #pragma unroll
   for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[at(1, l)];
-  update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E);
+  update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E, drop_mean);
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
     cx<T> suf = zero<T>();
@@ -527,16 +533,16 @@ void k_step(StepArgs<T> a) {
       x[l] = (T)(1) / ft * (sb[at(2, l)] + dt2 * (tn[at(2, l)] - tmd));
     }
   }
-  update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr);
+  update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr, false);
   // vorticity
 #pragma unroll
   for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
-  update(0, vinv, (a.vmask & 1) && vd_low, s2 + E);
+  update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
   // humidity
 #pragma unroll
   for (int l = 0; l < L; ++l)
     x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
-  update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr);
+  update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
 }
 
 // Samples of the top-of-step spectral state (the level the next k_step

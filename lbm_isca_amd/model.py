@@ -226,8 +226,17 @@ class LBM:
                 "t": ga.t, "z": d["z"], "p": ga.ps}
 
     def _set_forcing(self, mjo_forcing, steady_forcing):
-        """Upload the run's temperature forcing; returns the time-varying
-        forcing object whose call counter the run advances, or None."""
+        """Upload the run's forcings (model.py:196-229): the temperature
+        forcing (MJO, else the steady one) and the steady vorticity,
+        divergence and ln ps forcings; returns the time-varying forcing
+        object whose call counter the run advances, or None."""
+        sv = {}
+        if steady_forcing is not None:
+            for name in ("vor", "div", "ps"):
+                val = getattr(steady_forcing, name, None)
+                if val is not None and np.any(np.asarray(asnumpy(val)) != 0):
+                    sv[name] = val
+        self.plan.set_steady_forcing(**sv)
         if mjo_forcing is not None:
             if isinstance(mjo_forcing, MJOForcing):
                 self.plan.set_mjo_forcing(mjo_forcing.gtf, mjo_forcing.nt, mjo_forcing.spinup,
@@ -239,13 +248,7 @@ class LBM:
                     "time-varying forcing: MJOForcing or forcings whose step() is constant")
             self.plan.set_forcing(mjo_forcing.step())
             return None
-        frc = None
-        if steady_forcing is not None:
-            for name in ("vor", "div", "ps"):
-                val = getattr(steady_forcing, name, None)
-                if val is not None and np.any(np.asarray(val) != 0):
-                    raise NotImplementedError(f"non-zero steady {name} forcing is not built")
-            frc = steady_forcing.t
+        frc = None if steady_forcing is None else steady_forcing.t
         self.plan.set_forcing(frc)
         return None
 

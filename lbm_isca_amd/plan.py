@@ -235,6 +235,27 @@ class Plan:
             f = f.unsqueeze(0)
         _lib.check(self._lib.lbm_set_forcing(self._h, _lib.ptr(f), int(f.shape[0]), _stream()))
 
+    def set_steady_forcing(self, vor=None, div=None, ps=None):
+        """Steady vorticity / divergence (J, I, L) and ln ps (J, I) forcings,
+        or with a leading member axis; all None removes them."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        args, nf = [], None
+        for x, nd in ((vor, 3), (div, 3), (ps, 2)):
+            if x is None:
+                args.append(None)
+                continue
+            x = _dev(x, torch.float64)
+            if x.dim() == nd:
+                x = x.unsqueeze(0)
+            if nf is not None and x.shape[0] != nf:
+                raise ValueError("steady forcings need the same member count")
+            nf = int(x.shape[0])
+            args.append(x.contiguous())
+        self._steady = args
+        _lib.check(self._lib.lbm_set_steady_forcing(self._h, *[_lib.ptr(x) for x in args],
+                                                    int(nf or 0), _stream()))
+
     def set_state(self, u=None, v=None, t=None, q=None, ps=None):
         torch = _torch()
         E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev

@@ -146,9 +146,12 @@ class OracleModel:
             setattr(self.ga, name, arr.copy())
             setattr(self.gb, name, arr.copy())
 
-    def step(self, dt, oinit, frc_t=None):
+    def step(self, dt, oinit, frc_t=None, steady=None):
+        """One step; ``steady`` = dict of frc_vor, frc_div (E|1, J, I, L) and
+        frc_ps (E|1, J, I) grid forcings (``model.py:202-226``)."""
         tend, diag = tendencies(self.sht, self.vg, self.bst, self.ga, frc_t=frc_t,
-                                strictly_linear=self.cfg.strictly_linear or self.cfg.linear)
+                                strictly_linear=self.cfg.strictly_linear or self.cfg.linear,
+                                **(steady or {}))
         self.last_diag = diag
         prev = self.gb.to_spectral(self.sht)
         new = self.stepper.step(dt, prev, tend)
@@ -160,7 +163,8 @@ class OracleModel:
         self.ga.assign(self.gc)
         return new
 
-    def run(self, nsteps=None, frc_t=None, check_finite=True, mjo=None, history=None):
+    def run(self, nsteps=None, frc_t=None, check_finite=True, mjo=None, history=None,
+            steady=None):
         """Start-up half steps (dt/4, dt/2, clock not advanced) then ``nsteps``
         full steps (``model.py:211-253``).  ``frc_t`` is a constant (E|1, J, I, L)
         temperature forcing in K/s; ``mjo`` an object whose ``step()`` gives
@@ -185,7 +189,7 @@ class OracleModel:
             if history is not None:
                 ga_top = GridState(self.ga.ps_bar, self.nmem, self.vg.kmax)
                 ga_top.assign(self.ga)
-            self.step(dt, oinit, frc)
+            self.step(dt, oinit, frc, steady)
             top = None
             if history is not None:
                 cur = self.ga

@@ -204,3 +204,31 @@ def test_regrid_matches_oracle():
     assert np.array_equal(L.asnumpy(R.resize_spec(up, 42)), xi)
     assert np.array_equal(L.asnumpy(R.resize_spec(xi[None, ..., 0], 21, batched=True)),
                           xi[None, :22, :22, 0])
+
+
+@pytest.mark.parametrize("with_mjo", [False, True])
+def test_steady_forcings_match_oracle(env, with_mjo):
+    import lbm_isca_amd as L
+    sht, vg, bs, heat = env
+    rng = np.random.default_rng(11)
+    smooth = lambda x: sht.wave_to_grid(sht.grid_to_wave(x[None]))[0]    # noqa: E731
+    fv = smooth(rng.standard_normal((64, 128, 20))) * 2e-11
+    fd = smooth(rng.standard_normal((64, 128, 20))) * 1e-11
+    fp = smooth(rng.standard_normal((64, 128, 1)))[..., 0] * 3e-9
+    sf = L.SteadyForcing(fv, fd, heat, fp)
+    m = hip_model(bs, ndays=10 / 72.0)
+    ref = OracleModel(bs, 1, OracleConfig())
+    steady = {"frc_vor": fv[None], "frc_div": fd[None], "frc_ps": fp[None]}
+    if with_mjo:
+        gtf = np.stack([0.5 * heat, 1.5 * np.roll(heat, 16, axis=1)])
+        m.run(mjo_forcing=L.MJOForcing(gtf=gtf, nt=4, spinup=3), steady_forcing=sf, verbose=False)
+        ref.run(nsteps=10, mjo=_Lead(H.MJOSequence(gtf, 4, 3)), steady=steady)
+    else:
+        m.run(steady_forcing=sf, verbose=False)
+        ref.run(nsteps=10, frc_t=heat[None], steady=steady)
+    for a, f in zip(m.plan.get_state(), ("u", "v", "t", "q", "ps")):
+        assert rel_err(a.cpu().numpy()[0], getattr(ref.ga, f)[0]) < 1e-10, f
+    # the vorticity forcing alone moves the state (the forcing is not lost)
+    m0 = hip_model(bs, ndays=10 / 72.0)
+    m0.run(steady_forcing=L.SteadyForcing(None, None, heat, None), verbose=False)
+    assert rel_err(m0.plan.get_state()[0].cpu().numpy(), m.plan.get_state()[0].cpu().numpy()) > 1e-6

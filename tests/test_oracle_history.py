@@ -125,3 +125,29 @@ def test_regrid_band_limited_round_trip_and_harmonic():
     u2, v2 = H.regrid_uv(hi, lo, *H.regrid_uv(lo, hi, u, v))
     np.testing.assert_allclose(u2, u, atol=1e-10 * np.abs(u).max())
     np.testing.assert_allclose(v2, v, atol=1e-10 * np.abs(v).max())
+
+
+def test_steady_forcings_enter_only_their_own_tendencies():
+    """frc_vor, frc_div and frc_ps change the vorticity, divergence and pi
+    tendencies by their analyses (+, in their own field) and nothing else."""
+    from oracle import BasicTerms
+    from oracle.dyn import tendencies
+    from oracle.model import GridState
+    sht, vg = SHT(), VerticalOps()
+    bs = cases.solid_body_state(sht.lat, sht.lon, vg.sig)
+    bst = BasicTerms(sht, vg, bs)
+    st = GridState(bs.ps, 1, vg.kmax)
+    rng = np.random.default_rng(4)
+    st.u, st.v, st.t = (sht.wave_to_grid(sht.grid_to_wave(rng.standard_normal((1, 64, 128, 20))))
+                        for _ in range(3))
+    fv, fd = rng.standard_normal((2, 1, 64, 128, 20)) * 1e-10
+    fp = rng.standard_normal((1, 64, 128)) * 1e-7
+    base, _ = tendencies(sht, vg, bst, st)
+    forced, _ = tendencies(sht, vg, bst, st, frc_vor=fv, frc_div=fd, frc_ps=fp)
+    want = {0: sht.grid_to_wave(fv), 1: sht.grid_to_wave(fd), 3: sht.grid_to_wave(fp[..., None])[..., 0]}
+    for k, (a, b) in enumerate(zip(base, forced)):
+        d = b - a
+        if k in want:
+            np.testing.assert_allclose(d, want[k], rtol=0, atol=1e-12 * np.abs(want[k]).max())
+        else:
+            assert np.abs(d).max() <= 1e-14 * max(np.abs(a).max(), 1e-300)
