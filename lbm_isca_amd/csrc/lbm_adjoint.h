@@ -23,8 +23,8 @@ namespace lbm {
 // Synthesis-kernel recipes (run with the analysis tables gw P, gw H and the
 // per-degree table n(n+1)/a^2).
 // ---------------------------------------------------------------------------
-// transpose of AnaTend: (vor, div, t, q)* -> Fourier (fu' fv' ke xu' xv' gt
-// yu' yv' g)*
+// transpose of the tendency analysis (AnaTendVD, AnaFluxDiv twice):
+// (vor, div, t, q)* -> Fourier (fu' fv' ke xu' xv' gt yu' yv' g)*
 struct AdjTend {
   static constexpr int NIN = 4, NOUT = 9;
   static constexpr bool hasA(int o) { return o != 4 && o != 7; }

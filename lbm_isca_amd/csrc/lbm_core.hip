@@ -1011,7 +1011,10 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   a.robert = (T)p->robert;
   a.ecnt = ecnt;
   const size_t nthr = (size_t)p->nspec * ecnt;
-  hipLaunchKernelGGL((k_step<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+  if (ecnt % 64 == 0 && p->nb == 1)
+    hipLaunchKernelGGL((k_step<T, L, true>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_step<T, L, false>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -1085,10 +1088,17 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
-  InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
   CHK(mark(p, 5, st));
-  CHK((launch_ana<T, AnaTend>(p, p->fout, p->tend3, EL, Co, sl, st, AnaTend::NOUT, 0, nullptr,
-                              nullptr, nullptr, am)));
+  {
+    // (FU, FV, KE) -> vor, div; (XU, XV, GT) -> t; (YU, YV, G) -> q
+    InSlots vd{{0, EL, 2 * EL}}, ft{{3 * EL, 4 * EL, 5 * EL}}, fq{{6 * EL, 7 * EL, 8 * EL}};
+    CHK((launch_ana<T, AnaTendVD>(p, p->fout, p->tend3, EL, Co, vd, st, 4, 0, nullptr, nullptr,
+                                  nullptr, am)));
+    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, ft, st, 4, 2, nullptr, nullptr,
+                                   nullptr, am)));
+    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, fq, st, 4, 3, nullptr, nullptr,
+                                   nullptr, am)));
+  }
   InSlots s2{{9 * EL}};
   CHK(mark(p, 6, st));
   CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,

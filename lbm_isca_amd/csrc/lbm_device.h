@@ -544,17 +544,29 @@ struct OutSlots {
 // Vector (u-like) inputs are pre-scaled in grid space by 1/(a cos lat), so
 // gw/(a(1-mu^2)) * (u cos lat) of spectral.py:236-243 becomes gw * u/(a cos).
 // ---------------------------------------------------------------------------
-struct AnaTend {  // (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t, q): dynamics.py:219,246-262
-  static constexpr int NIN = 9, NOUT = 4;
+// The tendency analysis (FU, FV, KE, XU, XV, GT, YU, YV, G) -> (vor, div, t,
+// q) of dynamics.py:219,246-262 in three groups that share no input: three
+// launches of three fields each need fewer registers per wave (more waves
+// and loads in flight) than one of nine.
+struct AnaTendVD {  // (FU, FV, KE) -> (vor, div): dynamics.py:246-250
+  static constexpr int NIN = 3, NOUT = 2;
   static constexpr bool ACC = false;
-  static constexpr bool needP(int f) { return f != 4 && f != 7; }
-  static constexpr bool needH(int f) { return f == 0 || f == 1 || f == 4 || f == 7; }
+  static constexpr bool needP(int) { return true; }
+  static constexpr bool needH(int f) { return f < 2; }
   template <class T>
   __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T nn1a2, cx<T> *o) {
-    o[0] = im_mul(fm, AP[1]) + AH[0];                                   // curl(fu, fv)
-    o[1] = im_mul(fm, AP[0]) - AH[1] + nn1a2 * AP[2];                   // div - lap(ke)
-    o[2] = AP[5] - im_mul(fm, AP[3]) + AH[4];                           // -div(xu,xv) + gt
-    o[3] = AP[8] - im_mul(fm, AP[6]) + AH[7];                           // -div(yu,yv) + g
+    o[0] = im_mul(fm, AP[1]) + AH[0];                  // curl(fu, fv)
+    o[1] = im_mul(fm, AP[0]) - AH[1] + nn1a2 * AP[2];  // div(fu, fv) - lap(ke)
+  }
+};
+struct AnaFluxDiv {  // (XU, XV, GT) -> t or (YU, YV, G) -> q: dynamics.py:219,258-262
+  static constexpr int NIN = 3, NOUT = 1;
+  static constexpr bool ACC = false;
+  static constexpr bool needP(int f) { return f != 1; }
+  static constexpr bool needH(int f) { return f == 1; }
+  template <class T>
+  __device__ __forceinline__ static void combine(const cx<T> *AP, const cx<T> *AH, T fm, T, cx<T> *o) {
+    o[0] = AP[2] - im_mul(fm, AP[0]) + AH[1];          // -div(xu, xv) + gt
   }
 };
 struct AnaState {  // (u', v', t, q) -> (vor, div, t, q): state.py:86-92

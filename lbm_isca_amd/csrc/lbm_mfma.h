@@ -134,17 +134,23 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
   int off[NIN];
 #pragma unroll
   for (int f = 0; f < NIN; ++f) off[f] = slots.off[f];
-  auto stage = [&](int jc, int f, int buf) __attribute__((always_inline)) {
+  // raw north/south values of this thread's (latitude pair, column) for
+  // every field: the loads of all fields are in flight together, and the
+  // registers of a field already staged take the next chunk's values
+  cx<T> rn[NIN], rs[NIN];
+  auto load = [&](int jc, int f) __attribute__((always_inline)) {
     const int jp = 32 * jc + lj;
-    cx<T> fn = zero<T>(), fs = zero<T>();
+    rn[f] = rs[f] = zero<T>();
     if (lcol_ok && jp < Jh) {
-      fn = in[((size_t)m * J + jp) * C + off[f] + col0 + lc];
-      fs = in[((size_t)m * J + (J - 1 - jp)) * C + off[f] + col0 + lc];
+      rn[f] = in[((size_t)m * J + jp) * C + off[f] + col0 + lc];
+      rs[f] = in[((size_t)m * J + (J - 1 - jp)) * C + off[f] + col0 + lc];
     }
-    sym[buf][lj][lc] = fn.x + fs.x;
-    sym[buf][lj][8 + lc] = fn.y + fs.y;
-    anti[buf][lj][lc] = fn.x - fs.x;
-    anti[buf][lj][8 + lc] = fn.y - fs.y;
+  };
+  auto store = [&](int f, int buf) __attribute__((always_inline)) {
+    sym[buf][lj][lc] = rn[f].x + rs[f].x;
+    sym[buf][lj][8 + lc] = rn[f].y + rs[f].y;
+    anti[buf][lj][lc] = rn[f].x - rs[f].x;
+    anti[buf][lj][8 + lc] = rn[f].y - rs[f].y;
   };
 
   // A operand lane (i = l%16 -> n row, k = l/16 -> latitude pair); B lane (j = l%16)
@@ -160,7 +166,9 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
   for (int f = 0; f < NIN; ++f) AP[f] = AH[f] = (v4){0, 0, 0, 0};
   constexpr bool anyP = any_p<R>(), anyH = any_h<R>();
 
-  stage(0, 0, 0);
+#pragma unroll
+  for (int f = 0; f < NIN; ++f) load(0, f);
+  store(0, 0);
   __syncthreads();
   int it = 0;
   for (int jc = 0; jc < nch; ++jc) {
@@ -186,10 +194,10 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
 #pragma unroll
     for (int f = 0; f < NIN; ++f, ++it) {
       const int buf = it & 1;
-      if (f + 1 < NIN)
-        stage(jc, f + 1, buf ^ 1);
-      else if (jc + 1 < nch)
-        stage(jc + 1, 0, buf ^ 1);
+      if (f + 1 < NIN) store(f + 1, buf ^ 1);
+      // field f of this chunk is in LDS: its registers load the next chunk's
+      if (jc + 1 < nch) load(jc + 1, f);
+      if (f + 1 == NIN && jc + 1 < nch) store(0, buf ^ 1);
       if (wave_live) {
         // the P rows of parity p contract with the symmetric sums when
         // p = 0, the H rows with the other one

@@ -406,8 +406,20 @@ struct StepArgs {
   T delt, robert;
 };
 
-template <class T, int L>
-__device__ inline void matvec(const T *__restrict__ A, const cx<T> *x, cx<T> *y) {
+// Read-only pointers: through the constant address space when the address
+// is wave-uniform, so the compiler may load through the scalar cache (the
+// tables are written only between kernels).
+template <class T, bool UNIFORM>
+struct RoPtr {
+  using type = const T *;
+};
+template <class T>
+struct RoPtr<T, true> {
+  using type = const T __attribute__((address_space(4))) *;
+};
+
+template <class T, int L, class P>
+__device__ inline void matvec(P A, const cx<T> *x, cx<T> *y) {
 #pragma unroll
   for (int r = 0; r < L; ++r) {
     cx<T> acc = zero<T>();
@@ -418,13 +430,18 @@ __device__ inline void matvec(const T *__restrict__ A, const cx<T> *x, cx<T> *y)
 }
 
 // One thread per (s, member), members fastest across lanes so every field
-// access of a wave is contiguous; all levels in registers.
-template <class T, int L>
+// access of a wave is contiguous; all levels in registers.  UNI: every wave
+// holds one coefficient s and the members share one basic state (launch
+// width a multiple of 64, nbasic = 1), so the degree, the damping rows and
+// the L x L matrices are wave-uniform and come through the scalar cache
+// instead of one vector load per element.
+template <class T, int L, bool UNI = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L <= 20 ? 2 : 1)))
 void k_step(StepArgs<T> a) {
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
-  const int s = idx / a.ecnt, e = idx - s * a.ecnt;
+  const int s = UNI ? __builtin_amdgcn_readfirstlane(idx / a.ecnt) : idx / a.ecnt;
+  const int e = idx - s * a.ecnt;
   if (s >= a.nspec) return;
   const int st = a.ctl->step;
   const bool active = st < a.inistp + a.nsteps[e];
@@ -434,13 +451,15 @@ void k_step(StepArgs<T> a) {
   const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
   const int set = oinit ? st : a.inistp;
   const T dt2 = (T)2 * dt;
-  const int n = a.n_of_s[s];
-  const int eb = a.nbasic > 1 ? e : 0;
+  const int n = UNI ? __builtin_amdgcn_readfirstlane(a.n_of_s[s]) : a.n_of_s[s];
+  const int eb = (!UNI && a.nbasic > 1) ? e : 0;
   const VOff vo{L};
-  const T *vc = a.vc;
-  const T *tref = a.tref + (size_t)eb * L;
-  const T *ktt = a.ktt + (size_t)n * L, *kd = a.kd + (size_t)n * L;
-  const T *kv = a.kv + (size_t)n * L, *kq = a.kq + (size_t)n * L;
+  using RU = typename RoPtr<T, true>::type;  // uniform in every launch
+  using RP = typename RoPtr<T, UNI>::type;   // uniform when one coefficient per wave
+  const RU vc = (RU)a.vc;
+  const RP tref = (RP)(a.tref + (size_t)eb * L);
+  const RP ktt = (RP)(a.ktt + (size_t)n * L), kd = (RP)(a.kd + (size_t)n * L);
+  const RP kv = (RP)(a.kv + (size_t)n * L), kq = (RP)(a.kq + (size_t)n * L);
   const size_t EL = (size_t)E * L;
   const size_t b3 = (size_t)s * 4 * EL + e;
   cx<T> *sa = a.sa3 + b3, *sb = a.sb3 + b3;
@@ -450,7 +469,7 @@ void k_step(StepArgs<T> a) {
   // pi at t-dt: the analysed log1p(ps'_b/ps_bar), or the spectral level
   // itself in tangent-linear mode
   const cx<T> pi_prev = a.sb2 ? a.sb2[(size_t)s * 3 * E + e] : a.tend2[(size_t)s * 2 * E + E + e];
-  const T lapn = a.lap[n];
+  const T lapn = ((RP)a.lap)[n];
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
   cx<T> *s2 = a.sa2 + (size_t)s * 3 * E + e;
 
@@ -478,8 +497,8 @@ void k_step(StepArgs<T> a) {
     x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
-  matvec<T, L>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, x, dbar);
-  const T *vinv = a.vinv + (size_t)set * 4 * L * L;
+  matvec<T, L>((RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L), x, dbar);
+  const RU vinv = (RU)(a.vinv + (size_t)set * 4 * L * L);
   const bool vd_low = !(a.skip_low_n && n <= 1);
 
   // surface pressure (integrator.py:301, 308)
@@ -500,7 +519,7 @@ void k_step(StepArgs<T> a) {
   // tangent-linear model.
   const bool drop_mean = n == 0 && a.sb2 == nullptr;
   // row r of V (or the identity) applied to x, then the filter and the store
-  auto update = [&](int f, const T *V, bool apply, cx<T> *sum, bool drop) __attribute__((always_inline)) {
+  auto update = [&](int f, RU V, bool apply, cx<T> *sum, bool drop) __attribute__((always_inline)) {
     cx<T> acc = zero<T>();
 #pragma unroll
     for (int r = 0; r < L; ++r) {
