@@ -52,28 +52,34 @@ def log(*a):
 
 
 # kernel of each profiling slot (FP64 names; FP32 replaces double by float)
-SLOT_KERNEL = {"syn_level": "k_syn_mfma<double, lbm::SynLevel>",
-               "syn_surf": "k_syn_mfma<double, lbm::SynSurf>",
-               "ifft": "k_ifft128<double>", "tend": "k_tend<double>", "fft": "k_fft128<double>",
-               "ana_level": "k_ana_mfma<double, lbm::AnaTend>",
-               "ana_surf": "k_ana_mfma<double, lbm::AnaScalar>", "step": "k_step<double, 20>"}
+# kernels of each profiling slot of one step, with their launches per step
+SLOT_KERNEL = {"syn_level": {"k_syn_mfma<double, lbm::SynLevel": 1},
+               "syn_surf": {"k_syn_mfma<double, lbm::SynSurf": 1},
+               "ifft": {"k_ifft<double, 128>": 1}, "tend": {"k_tend<double>": 1},
+               "fft": {"k_fft<double, 128>": 1},
+               "ana_level": {"k_ana_mfma<double, lbm::AnaTendVD": 1,
+                             "k_ana_mfma<double, lbm::AnaFluxDiv": 2},
+               "ana_surf": {"k_ana_mfma<double, lbm::AnaScalar": 1, "k_hist_acc<double>": 1},
+               "step": {"k_step<double, 20": 1}}
 
 
 def measured_traffic(slot, E, precision):
-    """HBM bytes per launch of a slot's kernel from the committed rocprofv3
+    """HBM bytes of a slot's kernels per step from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_T42L20_E{E}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         table = json.load(fh)["bytes_per_launch"]
-    name = SLOT_KERNEL[slot]
-    if precision == "fp32":
-        name = name.replace("double", "float")
-    for k, v in table.items():
-        if name in k:
-            return int(v)
-    return None
+    total = 0
+    for name, launches in SLOT_KERNEL[slot].items():
+        if precision == "fp32":
+            name = name.replace("double", "float")
+        hits = [v for k, v in table.items() if name in k]
+        if not hits:
+            return None
+        total += launches * int(hits[0])
+    return total
 
 
 def build_model(members, precision):
