@@ -327,7 +327,7 @@ unsigned tiles_x(int ncols, const ActiveMask &am) {
   return am.reduced() ? (unsigned)((ncols / am.E) * (am.ecnt / 8)) : (unsigned)((ncols + 7) / 8);
 }
 long fft_rows(long J, int C, const ActiveMask &am) {
-  return am.reduced() ? J * (C / am.E) * am.ecnt : J * C;
+  return am.reduced() ? J * (C / am.E) * am.ecnt : J * fpad(C);
 }
 
 // Tables of one precision from the FP64 host setup.
@@ -537,7 +537,7 @@ int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, do
     default: ngrid = 2; nspecf = 2; break;
   }
   const size_t gbytes = (size_t)B * J * ngrid * nk * I * sizeof(T);
-  const size_t fbytes = (size_t)M * J * ngrid * ncols * sizeof(cx<T>);
+  const size_t fbytes = (size_t)M * J * fpad(ngrid * ncols) * sizeof(cx<T>);
   const size_t sbytes = (size_t)p->nspec * nspecf * ncols * sizeof(cx<T>);
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
@@ -767,7 +767,7 @@ int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, co
   const ColMap cm{nb, L, 2, 1};  // u/(a cos), v/(a cos), ln ps
   const int C = cm.ncols();
   const size_t g1 = (size_t)nb * J * cm.rows() * I;
-  const size_t f1 = (size_t)M * J * C;              // complex
+  const size_t f1 = (size_t)M * J * fpad(C);        // complex
   const size_t s1 = (size_t)p->nspec * C;           // complex
   const size_t g2 = (size_t)nb * J * 2 * L * I, g3 = (size_t)nb * J * 2 * I;
   const size_t bs = (size_t)nb * J * br.rows() * I;
@@ -876,7 +876,7 @@ int set_state_t(lbm_plan *p, const double *u, const double *v, const double *t, 
   const ColMap cm{E, L, 4, 1};
   const int C = cm.ncols();
   const size_t gb = ((size_t)E * J * cm.rows() * I * sizeof(T) + 255) / 256 * 256;
-  const size_t fb = (size_t)M * J * C * sizeof(cx<T>);
+  const size_t fb = (size_t)M * J * fpad(C) * sizeof(cx<T>);
   void *base;
   CHK(scratch_get(p, gb + fb, &base, st));
   T *g = (T *)base;
@@ -1237,7 +1237,7 @@ int set_spectral_t(lbm_plan *p, const double *vor, const double *div, const doub
     const ColMap cm{E, 1, 0, 1};
     const size_t gb = (n2 * sizeof(T) + 255) / 256 * 256;
     void *base;
-    CHK(scratch_get(p, gb + (size_t)M * J * E * sizeof(cx<T>), &base, st));
+    CHK(scratch_get(p, gb + (size_t)M * J * fpad(E) * sizeof(cx<T>), &base, st));
     T *g = (T *)base;
     cx<T> *F = (cx<T> *)((char *)base + gb);
     OutSlots sl{{0}};
@@ -1279,7 +1279,7 @@ int set_response_t(lbm_plan *p, const double *wgrid, const double *wlev, const d
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
   void *base;
-  CHK(scratch_get(p, up(ng * sizeof(double)) + up(ng * sizeof(T)) + (size_t)M * J * sizeof(cx<T>),
+  CHK(scratch_get(p, up(ng * sizeof(double)) + up(ng * sizeof(T)) + (size_t)M * J * fpad(1) * sizeof(cx<T>),
                   &base, st));
   double *w64 = (double *)base;
   T *g = (T *)((char *)base + up(ng * sizeof(double)));
@@ -1707,10 +1707,10 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
       {&p->sa2, cs * p->nspec * 3 * E},
       {&p->tend3, cs * p->nspec * 4 * EL},
       {&p->tend2, cs * p->nspec * 2 * E},
-      {&p->fin, cs * M * J * cin.ncols()},
+      {&p->fin, cs * M * J * fpad(cin.ncols())},
       {&p->gin, rs * (size_t)E * J * cin.rows() * I},
       {&p->gout, rs * (size_t)E * J * cout.rows() * I},
-      {&p->fout, cs * M * J * cout.ncols()},
+      {&p->fout, cs * M * J * fpad(cout.ncols())},
       {&p->raw, rs * (size_t)E * J * 4 * L * I},
       {&p->psa, rs * (size_t)E * J * I},
       {&p->psb, rs * (size_t)E * J * I},

@@ -91,6 +91,15 @@ __host__ __device__ inline void dft8(cx<T> *v) {
 // Column-space map: Fourier/grid columns -> (member, grid row).  Columns of
 // level fields are (field, level, member) with the member fastest; surface
 // fields follow as (field, member).
+// Fourier arrays are [m][column tile of 8][j][8] complex, the C columns
+// padded to a multiple of 8: one m of a tile is J x 8 contiguous values
+// (what a Legendre block of 8 columns reads or writes), and 8 columns x 4
+// latitudes of an FFT block are 32 contiguous values.
+__host__ __device__ __forceinline__ int fpad(int C) { return (C + 7) & ~7; }
+__host__ __device__ __forceinline__ size_t fidx(int m, int j, int c, int J, int C) {
+  return (((size_t)m * (fpad(C) >> 3) + (c >> 3)) * J + j) * 8 + (c & 7);
+}
+
 struct ColMap {
   int E, L, nF3, nF2;  // members, levels, level fields, surface fields
   __host__ __device__ int ncols() const { return nF3 * E * L + nF2 * E; }
@@ -141,21 +150,26 @@ struct ActiveMask {
     return fl * E + (bx - fl * tpl) * 8;
   }
   // row of a transform over J latitudes and C columns -> (j, col); false
-  // past the last row of the launch
+  // past the last row of the launch and for the padding columns.  Rows run
+  // over 8 columns, then latitudes, then column tiles (the Fourier layout);
+  // a reduced launch covers the tiles of the first ecnt members of each
+  // field-level (E is then a multiple of 8).
   __device__ __forceinline__ bool row_col(long row, int J, int C, int &j, int &col) const {
+    const int c8 = (int)(row & 7);
+    const long r8 = row >> 3;
     if (!reduced()) {
-      if (row >= (long)J * C) return false;
-      j = (int)(row / C);
-      col = (int)(row - (long)j * C);
-      return true;
+      if (r8 >= (long)(fpad(C) >> 3) * J) return false;
+      const int tile = (int)(r8 / J);
+      j = (int)(r8 - (long)tile * J);
+      col = tile * 8 + c8;
+      return col < C;
     }
-    const int nfl = C / E;
-    if (row >= (long)J * nfl * ecnt) return false;
-    const long q = row / ecnt;
-    const int e = (int)(row - q * ecnt);
-    const int fl = (int)(q % nfl);
-    j = (int)(q / nfl);
-    col = fl * E + e;
+    const int nfl = C / E, tpl = ecnt >> 3;
+    if (r8 >= (long)nfl * tpl * J) return false;
+    const int t = (int)(r8 / J);
+    j = (int)(r8 - (long)t * J);
+    const int fl = t / tpl;
+    col = fl * E + (t - fl * tpl) * 8 + c8;
     return true;
   }
   // any member among columns col0 .. col0+7 (< ncols), member = col % E
@@ -385,7 +399,7 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
     cm.map(col, e, r);
     live = am(e);
   }
-  if (live) ifft_a<T, NI>(F + (size_t)j * C + col, (size_t)J * C, M, sh[rl], tt, mode);
+  if (live) ifft_a<T, NI>(F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, sh[rl], tt, mode);
   __syncthreads();
   cx<T> v[8];
   if (live) ifft_b<T, NI>(sh[rl], v, tt);
@@ -436,7 +450,7 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_fft(const T *__r
     for (int k1 = 0; k1 < 8; ++k1) sh[rl][rr + S::R * (tq + 8 * k1)] = v[k1];
   }
   __syncthreads();
-  if (live) fft_c<T, NI>(sh[rl], F + (size_t)j * C + col, (size_t)J * C, M, tt, mode);
+  if (live) fft_c<T, NI>(sh[rl], F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, tt, mode);
 }
 
 // ---------------------------------------------------------------------------

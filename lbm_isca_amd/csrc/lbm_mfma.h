@@ -142,8 +142,8 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
     const int jp = 32 * jc + lj;
     rn[f] = rs[f] = zero<T>();
     if (lcol_ok && jp < Jh) {
-      rn[f] = in[((size_t)m * J + jp) * C + off[f] + col0 + lc];
-      rs[f] = in[((size_t)m * J + (J - 1 - jp)) * C + off[f] + col0 + lc];
+      rn[f] = in[fidx(m, jp, off[f] + col0 + lc, J, C)];
+      rs[f] = in[fidx(m, J - 1 - jp, off[f] + col0 + lc, J, C)];
     }
   };
   auto store = [&](int f, int buf) __attribute__((always_inline)) {
@@ -423,8 +423,8 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
         vn *= sc;
         vs *= sc;
       }
-      reinterpret_cast<T *>(out + ((size_t)m * J + jp) * C + slots.off[o] + col)[part_b] = vn;
-      reinterpret_cast<T *>(out + ((size_t)m * J + (J - 1 - jp)) * C + slots.off[o] + col)[part_b] = vs;
+      reinterpret_cast<T *>(out + fidx(m, jp, slots.off[o] + col, J, C))[part_b] = vn;
+      reinterpret_cast<T *>(out + fidx(m, J - 1 - jp, slots.off[o] + col, J, C))[part_b] = vs;
     }
   }
 }
