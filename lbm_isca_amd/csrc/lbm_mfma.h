@@ -430,7 +430,7 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
 }
 
 template <class T, class R, bool SPLIT = false>
-__global__ __launch_bounds__(256) void k_syn_mfma(const cx<T> *__restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_syn_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Ps,
                                                   const T *__restrict__ Hs,
