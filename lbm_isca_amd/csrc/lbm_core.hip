@@ -1011,10 +1011,12 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   a.robert = (T)p->robert;
   a.ecnt = ecnt;
   const size_t nthr = (size_t)p->nspec * ecnt;
+  // two roles per (s, member): see k_step
+  const dim3 grid((unsigned)((nthr + 63) / 64), 2);
   if (ecnt % 64 == 0 && p->nb == 1)
-    hipLaunchKernelGGL((k_step<T, L, true>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((k_step<T, L, true>), grid, dim3(64), 0, st, a);
   else
-    hipLaunchKernelGGL((k_step<T, L, false>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((k_step<T, L, false>), grid, dim3(64), 0, st, a);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }

@@ -446,7 +446,11 @@ void k_step(StepArgs<T> a) {
   const int st = a.ctl->step;
   const bool active = st < a.inistp + a.nsteps[e];
   const bool oinit = st < a.inistp;
-  if (s == 0) a.pend[e] = active ? (oinit ? 1 : 2) : 0;
+  // blockIdx.y: 0 -> temperature, divergence, pi (the coupled solve);
+  // 1 -> vorticity and humidity (independent of it): twice the waves, each
+  // with half the dependent chain
+  const int role = blockIdx.y;
+  if (s == 0 && role == 0) a.pend[e] = active ? (oinit ? 1 : 2) : 0;
   if (!active) return;
   const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
   const int set = oinit ? st : a.inistp;
@@ -476,43 +480,8 @@ void k_step(StepArgs<T> a) {
   // two level arrays (x, dbar): every mat-vec row goes straight to the
   // Robert-Asselin update and to memory, which keeps two waves per SIMD
   cx<T> x[L], dbar[L];
-  // temperature estimate and its geopotential (integrator.py:287-292)
-#pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
-  {
-    cx<T> acc = zero<T>();
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const cx<T> wl = x[l];
-      x[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
-      acc = acc + (rg * vc[vo.dlog3() + l]) * wl;
-    }
-  }
-  const cx<T> wdpix = pi_prev + dt * wt_pi;
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const T ft = (T)1 + dt2 * ktt[l];
-    const T gg = rg * tref[l] * vc[vo.cpg() + l];
-    const cx<T> phy = x[l] + (ft * gg) * wdpix;
-    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
-  }
-  // dbar = M(n)^-1 rhs (integrator.py:165-172)
-  matvec<T, L>((RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L), x, dbar);
   const RU vinv = (RU)(a.vinv + (size_t)set * 4 * L * L);
   const bool vd_low = !(a.skip_low_n && n <= 1);
-
-  // surface pressure (integrator.py:301, 308)
-  {
-    cx<T> sdb = zero<T>();
-#pragma unroll
-    for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
-    const cx<T> pin = pi_prev + dt2 * (wt_pi - sdb);
-    if (a.sb2) {
-      const cx<T> cur = s2[0];
-      a.sb2[(size_t)s * 3 * E + e] = oinit ? cur : cur + gam * (pi_prev - (T)2 * cur + pin);
-    }
-    s2[0] = pin;
-  }
   // The reference keeps winds, not vorticity and divergence, on its grid:
   // their global means (n = 0) never survive a step (a forcing's mean
   // still acts through dbar within the step).  Same here, outside the
@@ -537,6 +506,54 @@ void k_step(StepArgs<T> a) {
     }
     if (sum) *sum = acc;
   };
+
+  if (role == 1) {
+    // vorticity
+#pragma unroll
+    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
+    update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
+    // humidity
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
+    update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
+    return;
+  }
+  // temperature estimate and its geopotential (integrator.py:287-292)
+#pragma unroll
+  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
+  {
+    cx<T> acc = zero<T>();
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const cx<T> wl = x[l];
+      x[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
+      acc = acc + (rg * vc[vo.dlog3() + l]) * wl;
+    }
+  }
+  const cx<T> wdpix = pi_prev + dt * wt_pi;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const T ft = (T)1 + dt2 * ktt[l];
+    const T gg = rg * tref[l] * vc[vo.cpg() + l];
+    const cx<T> phy = x[l] + (ft * gg) * wdpix;
+    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
+  }
+  // dbar = M(n)^-1 rhs (integrator.py:165-172)
+  matvec<T, L>((RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L), x, dbar);
+
+  // surface pressure (integrator.py:301, 308)
+  {
+    cx<T> sdb = zero<T>();
+#pragma unroll
+    for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
+    const cx<T> pin = pi_prev + dt2 * (wt_pi - sdb);
+    if (a.sb2) {
+      const cx<T> cur = s2[0];
+      a.sb2[(size_t)s * 3 * E + e] = oinit ? cur : cur + gam * (pi_prev - (T)2 * cur + pin);
+    }
+    s2[0] = pin;
+  }
   // divergence
 #pragma unroll
   for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[at(1, l)];
@@ -553,15 +570,6 @@ void k_step(StepArgs<T> a) {
     }
   }
   update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr, false);
-  // vorticity
-#pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
-  update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
-  // humidity
-#pragma unroll
-  for (int l = 0; l < L; ++l)
-    x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
-  update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
 }
 
 // Samples of the top-of-step spectral state (the level the next k_step
