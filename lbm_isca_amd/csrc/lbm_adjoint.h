@@ -352,8 +352,8 @@ struct StepAdjArgs {
   T delt, robert;
 };
 
-template <class T, int L>
-__device__ inline void matvec_t(const T *__restrict__ A, const cx<T> *x, cx<T> *y) {
+template <class T, int L, class P>
+__device__ inline void matvec_t(P A, const cx<T> *x, cx<T> *y) {
 #pragma unroll
   for (int c = 0; c < L; ++c) {
     cx<T> acc = zero<T>();
@@ -375,8 +375,12 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   cx<T> *A3 = a.aa3 + b3, *B3 = a.ab3 + b3, *T3 = a.at3 + b3;
   auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
   const VOff vo{L};
-  const T *vc = a.vc;
+  using RU = typename RoPtr<T, true>::type;  // wave-uniform: through the scalar cache
+  const RU vc = (RU)a.vc;
   const size_t i2 = (size_t)s * E + e;
+  // blockIdx.y: 0 -> pi, divergence, temperature (the coupled solve);
+  // 1 -> vorticity and humidity (independent of it), as in k_step
+  const int role = blockIdx.y;
   const cx<T> spi = a.s3a[(size_t)s * 3 * E + e];
   const cx<T> svor = a.s3a[(size_t)s * 3 * E + E + e];
   const cx<T> sdiv = a.s3a[(size_t)s * 3 * E + 2 * E + e];
@@ -384,12 +388,18 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const bool active = st < a.inistp + a.nsteps[e];
   if (!active) {
     // a member past its length: the step is the identity
+    if (role == 1) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        A3[at(0, l)] = A3[at(0, l)] + vc[vo.dsig() + l] * svor;
+        T3[at(0, l)] = T3[at(3, l)] = zero<T>();
+      }
+      return;
+    }
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      A3[at(0, l)] = A3[at(0, l)] + vc[vo.dsig() + l] * svor;
       A3[at(1, l)] = A3[at(1, l)] + vc[vo.dsig() + l] * sdiv;
-#pragma unroll
-      for (int f = 0; f < 4; ++f) T3[at(f, l)] = zero<T>();
+      T3[at(1, l)] = T3[at(2, l)] = zero<T>();
     }
     a.aa2[i2] = a.aa2[i2] + spi;
     *at2 = zero<T>();
@@ -406,7 +416,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const T *kv = a.kv + (size_t)n * L, *kq = a.kq + (size_t)n * L;
   const T lapn = a.lap[n];
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
-  const T *vinv = a.vinv + (size_t)set * 4 * L * L;
+  const RU vinv = (RU)(a.vinv + (size_t)set * 4 * L * L);
   const bool vd_low = !(a.skip_low_n && n <= 1);
   const bool resp = a.obs != nullptr;
   const T wr = (resp && st < a.nwstep) ? a.wstep[st] : (T)0;
@@ -418,6 +428,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const T cn = oinit ? (T)0 : gam;
   cx<T> x[L], y[L], dbar[L];
 
+  if (role == 1) {
   // vorticity: vor = Vv (vor_b + 2dt wt_vor) / (1 + 2dt kv)
 #pragma unroll
   for (int l = 0; l < L; ++l)
@@ -453,6 +464,8 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     B3[at(3, l)] = cb * bb + yl;
     A3[at(3, l)] = ca * bb;
     T3[at(3, l)] = dt2 * yl;
+  }
+  return;
   }
   // pi: pi = pi_b + 2dt (wt_pi - sum dsig dbar)
   const cx<T> bpi = a.ab2[i2];

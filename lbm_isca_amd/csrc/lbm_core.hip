@@ -1343,7 +1343,8 @@ int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   a.robert = (T)p->robert;
   a.ecnt = ecnt;
   const size_t nthr = (size_t)p->nspec * ecnt;
-  hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64)), dim3(64), 0, st, a);
+  // two roles per (s, member): see k_step_adj
+  hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64), 2), dim3(64), 0, st, a);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
