@@ -37,6 +37,21 @@ __host__ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
 }
 template <class T>
 __host__ __device__ inline cx<T> conj(cx<T> a) { return mk(a.x, -a.y); }
+
+// Streaming stores of data read back only by the next kernel (from HBM):
+// non-temporal, so they do not displace reused lines from the caches.
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(cx<T> *p, cx<T> v) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2 *>(p));
+}
 // (i*m) * a
 template <class T>
 __host__ __device__ inline cx<T> im_mul(T m, cx<T> a) { return mk(-m * a.y, m * a.x); }
@@ -320,7 +335,7 @@ __device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
 #pragma unroll
     for (int q = 1; q < R; ++q)
       if (q == na) o = v[q];
-    out[nb + 64 * na] = o;
+    st_nt(out + nb + 64 * na, o);
   }
 }
 
@@ -407,7 +422,7 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
   if constexpr (S::R == 1) {
     if (live) {
 #pragma unroll
-      for (int k1 = 0; k1 < 8; ++k1) out[tt + 8 * k1] = v[k1];
+      for (int k1 = 0; k1 < 8; ++k1) st_nt(out + tt + 8 * k1, v[k1]);
     }
   } else {
     __syncthreads();  // scr is reused as the exchange buffer

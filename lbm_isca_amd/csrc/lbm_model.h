@@ -371,7 +371,7 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     column_level<T>(k, L, a.vc, tref, bs, fr, I, cc, cs, up, vp, G(2 * L + k), G(3 * L + k), tp, qp,
                     upn, vpn, tpn, qpn, out, psbar, pst, omega);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) go[(size_t)(f * L + k) * I] = out[f];
+    for (int f = 0; f < 9; ++f) st_nt(go + (size_t)(f * L + k) * I, out[f]);
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
     if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
     up = upn;
