@@ -142,8 +142,8 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
     const int jp = 32 * jc + lj;
     rn[f] = rs[f] = zero<T>();
     if (lcol_ok && jp < Jh) {
-      rn[f] = in[fidx(m, jp, off[f] + col0 + lc, J, C)];
-      rs[f] = in[fidx(m, J - 1 - jp, off[f] + col0 + lc, J, C)];
+      rn[f] = ld_nt(in + fidx(m, jp, off[f] + col0 + lc, J, C));
+      rs[f] = ld_nt(in + fidx(m, J - 1 - jp, off[f] + col0 + lc, J, C));
     }
   };
   auto store = [&](int f, int buf) __attribute__((always_inline)) {

@@ -306,7 +306,7 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const T *__restrict__ fr =
       a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
   T *__restrict__ go = a.gout + (size_t)(e * J + j) * Rout * I + i;
-  auto G = [&](int row) { return gi[(size_t)row * I]; };
+  auto G = [&](int row) { return ld_nt(gi + (size_t)row * I); };
   auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : G(k); };
   auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : G(L + k); };
   auto TP = [&](int k) { return use_raw ? rw[(size_t)(2 * L + k) * I] : G(4 * L + k); };
