@@ -371,9 +371,18 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   if (s >= a.nspec) return;
   const int st = a.ctl->step;
   const size_t EL = (size_t)E * L;
-  const size_t b3 = (size_t)s * 4 * EL + e;
-  cx<T> *A3 = a.aa3 + b3, *B3 = a.ab3 + b3, *T3 = a.at3 + b3;
-  auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
+  // level arrays through buffer views based at the wave's first coefficient
+  // (per-lane offset + wave-uniform level offset), as in k_step
+  const int s_w = __builtin_amdgcn_readfirstlane(s);
+  const size_t wbase = (size_t)s_w * 4 * EL;
+  const unsigned voff = (unsigned)(((size_t)(s - s_w) * 4 * EL + e) * sizeof(cx<T>));
+  const size_t wbytes = (size_t)(a.nspec - s_w) * 4 * EL * sizeof(cx<T>);
+  const unsigned wrange = wbytes < 0xffffffffull ? (unsigned)wbytes : 0xffffffffu;
+  const BufView bA(a.aa3 + wbase, wrange), bB(a.ab3 + wbase, wrange), bT(a.at3 + wbase, wrange);
+  auto lo = [&](int f, int l) __attribute__((always_inline)) {
+    return (unsigned)(((size_t)f * EL + (size_t)l * E) * sizeof(cx<T>));
+  };
+  const cx<T> *const tag = nullptr;  // selects the precision of BufView::ld
   const VOff vo{L};
   using RU = typename RoPtr<T, true>::type;  // wave-uniform: through the scalar cache
   const RU vc = (RU)a.vc;
@@ -391,15 +400,17 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     if (role == 1) {
 #pragma unroll
       for (int l = 0; l < L; ++l) {
-        A3[at(0, l)] = A3[at(0, l)] + vc[vo.dsig() + l] * svor;
-        T3[at(0, l)] = T3[at(3, l)] = zero<T>();
+        bA.st(voff, lo(0, l), bA.ld(voff, lo(0, l), tag) + vc[vo.dsig() + l] * svor);
+        bT.st(voff, lo(0, l), zero<T>());
+        bT.st(voff, lo(3, l), zero<T>());
       }
       return;
     }
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      A3[at(1, l)] = A3[at(1, l)] + vc[vo.dsig() + l] * sdiv;
-      T3[at(1, l)] = T3[at(2, l)] = zero<T>();
+      bA.st(voff, lo(1, l), bA.ld(voff, lo(1, l), tag) + vc[vo.dsig() + l] * sdiv);
+      bT.st(voff, lo(1, l), zero<T>());
+      bT.st(voff, lo(2, l), zero<T>());
     }
     a.aa2[i2] = a.aa2[i2] + spi;
     *at2 = zero<T>();
@@ -432,8 +443,8 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   // vorticity: vor = Vv (vor_b + 2dt wt_vor) / (1 + 2dt kv)
 #pragma unroll
   for (int l = 0; l < L; ++l)
-    x[l] = A3[at(0, l)] + vc[vo.dsig() + l] * svor + (resp ? wr * a.wlev[l] : (T)0) * ob +
-           cn * B3[at(0, l)];
+    x[l] = bA.ld(voff, lo(0, l), tag) + vc[vo.dsig() + l] * svor + (resp ? wr * a.wlev[l] : (T)0) * ob +
+           cn * bB.ld(voff, lo(0, l), tag);
   if ((a.vmask & 1) && vd_low) {
     matvec_t<T, L>(vinv, x, y);
   } else {
@@ -443,14 +454,14 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const cx<T> yl = ((T)1 / ((T)1 + dt2 * kv[l])) * y[l];
-    const cx<T> bb = B3[at(0, l)];
-    B3[at(0, l)] = cb * bb + yl;
-    A3[at(0, l)] = ca * bb;
-    T3[at(0, l)] = dt2 * yl;
+    const cx<T> bb = bB.ld(voff, lo(0, l), tag);
+    bB.st(voff, lo(0, l), cb * bb + yl);
+    bA.st(voff, lo(0, l), ca * bb);
+    bT.st(voff, lo(0, l), dt2 * yl);
   }
   // humidity
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = A3[at(3, l)] + cn * B3[at(3, l)];
+  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(3, l), tag) + cn * bB.ld(voff, lo(3, l), tag);
   if (a.vmask & 8) {
     matvec_t<T, L>(vinv + 3 * L * L, x, y);
   } else {
@@ -460,10 +471,10 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const cx<T> yl = ((T)1 / ((T)1 + dt2 * kq[l])) * y[l];
-    const cx<T> bb = B3[at(3, l)];
-    B3[at(3, l)] = cb * bb + yl;
-    A3[at(3, l)] = ca * bb;
-    T3[at(3, l)] = dt2 * yl;
+    const cx<T> bb = bB.ld(voff, lo(3, l), tag);
+    bB.st(voff, lo(3, l), cb * bb + yl);
+    bA.st(voff, lo(3, l), ca * bb);
+    bT.st(voff, lo(3, l), dt2 * yl);
   }
   return;
   }
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   cx<T> pib_a = cb * bpi + pin_a;
   // divergence: D = Vd (2 dbar - D_b)
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = A3[at(1, l)] + vc[vo.dsig() + l] * sdiv + cn * B3[at(1, l)];
+  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(1, l), tag) + vc[vo.dsig() + l] * sdiv + cn * bB.ld(voff, lo(1, l), tag);
   if ((a.vmask & 2) && vd_low) {
     matvec_t<T, L>(vinv + 1 * L * L, x, y);
   } else {
@@ -483,14 +494,14 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   }
 #pragma unroll
   for (int l = 0; l < L; ++l) {
-    const cx<T> bb = B3[at(1, l)];
-    A3[at(1, l)] = ca * bb;
-    B3[at(1, l)] = cb * bb - y[l];
+    const cx<T> bb = bB.ld(voff, lo(1, l), tag);
+    bA.st(voff, lo(1, l), ca * bb);
+    bB.st(voff, lo(1, l), cb * bb - y[l]);
     dbar[l] = (T)2 * y[l];
   }
   // temperature: T = Vt (T_b + 2dt (wt_t - tmtx dbar)) / ft
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = A3[at(2, l)] + cn * B3[at(2, l)];
+  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(2, l), tag) + cn * bB.ld(voff, lo(2, l), tag);
   if (a.vmask & 4) {
     matvec_t<T, L>(vinv + 2 * L * L, x, y);
   } else {
@@ -501,11 +512,11 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   for (int l = 0; l < L; ++l) {
     const T ft = (T)1 + dt2 * ktt[l];
     const cx<T> yl = ((T)1 / ft) * y[l];
-    const cx<T> bb = B3[at(2, l)];
-    A3[at(2, l)] = ca * bb;
-    B3[at(2, l)] = cb * bb + yl;
+    const cx<T> bb = bB.ld(voff, lo(2, l), tag);
+    bA.st(voff, lo(2, l), ca * bb);
+    bB.st(voff, lo(2, l), cb * bb + yl);
     y[l] = dt2 * yl;  // wt_t* so far, and -(tmtx dbar)*
-    T3[at(2, l)] = y[l];
+    bT.st(voff, lo(2, l), y[l]);
   }
   // (tmtx dbar)_k = kappa Tr_k (alpha_k dbar_k + w3_k sum_{l>k} dsig_l dbar_l)
   {
@@ -527,8 +538,8 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
 #pragma unroll
   for (int l = 0; l < L; ++l) {
     const T ft = (T)1 + dt2 * ktt[l];
-    B3[at(1, l)] = B3[at(1, l)] + (((T)1 + dt * kd[l]) * ft) * x[l];
-    T3[at(1, l)] = (dt * ft) * x[l];
+    bB.st(voff, lo(1, l), bB.ld(voff, lo(1, l), tag) + (((T)1 + dt * kd[l]) * ft) * x[l]);
+    bT.st(voff, lo(1, l), (dt * ft) * x[l]);
     y[l] = (-dt * lapn) * x[l];
     wdpix_a = wdpix_a + (ft * (rg * tref[l] * vc[vo.cpg() + l])) * y[l];
   }
@@ -542,8 +553,8 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     for (int l = L - 1; l >= 0; --l) {
       const cx<T> w_a = (rg * vc[vo.alpha() + l]) * y[l] + (rg * vc[vo.dlog3() + l]) * suf;
       suf = suf + y[l];
-      B3[at(2, l)] = B3[at(2, l)] + ((T)1 + dt * ktt[l]) * w_a;
-      T3[at(2, l)] = T3[at(2, l)] + dt * w_a;
+      bB.st(voff, lo(2, l), bB.ld(voff, lo(2, l), tag) + ((T)1 + dt * ktt[l]) * w_a);
+      bT.st(voff, lo(2, l), bT.ld(voff, lo(2, l), tag) + dt * w_a);
     }
   }
 }

@@ -51,6 +51,37 @@ __device__ __forceinline__ cx<T> ld_nt(const cx<T> *p) {
   return mk(w.x, w.y);
 }
 
+// Raw buffer view of complex values: the resource (base, byte range) sits in
+// scalar registers, each access adds a 32-bit per-lane byte offset and a
+// wave-uniform byte offset (soffset).  A kernel that walks one array at a
+// runtime stride (the level stride of the spectral state) then needs one
+// offset register per lane instead of a 64-bit address per level.  Accesses
+// at or past the range are dropped by the hardware (loads return zero).
+struct BufView {
+  __amdgpu_buffer_rsrc_t r;
+  // 0x00020000: the raw (untyped, 32-bit data format) buffer word of gfx9
+  __device__ __forceinline__ BufView(const void *p, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ cx<double> ld(unsigned voff, unsigned soff, const cx<double> *) const {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    const i4 w = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(cx<double>, w);
+  }
+  __device__ __forceinline__ cx<float> ld(unsigned voff, unsigned soff, const cx<float> *) const {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    const i2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(cx<float>, w);
+  }
+  __device__ __forceinline__ void st(unsigned voff, unsigned soff, cx<double> v) const {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i4, v), r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void st(unsigned voff, unsigned soff, cx<float> v) const {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2, v), r, (int)voff, (int)soff, 0);
+  }
+};
+
 // Streaming stores of data read back only by the next kernel (from HBM):
 // non-temporal, so they do not displace reused lines from the caches.
 template <class T>

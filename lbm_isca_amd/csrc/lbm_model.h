@@ -465,10 +465,22 @@ void k_step(StepArgs<T> a) {
   const RP ktt = (RP)(a.ktt + (size_t)n * L), kd = (RP)(a.kd + (size_t)n * L);
   const RP kv = (RP)(a.kv + (size_t)n * L), kq = (RP)(a.kq + (size_t)n * L);
   const size_t EL = (size_t)E * L;
-  const size_t b3 = (size_t)s * 4 * EL + e;
-  cx<T> *sa = a.sa3 + b3, *sb = a.sb3 + b3;
-  const cx<T> *tn = a.tend3 + b3;
-  auto at = [&](int f, int l) { return (size_t)f * EL + (size_t)l * E; };
+  // Level l of field f of this (s, member) through buffer views based at the
+  // wave's first coefficient s_w: a per-lane offset (coefficient within the
+  // wave, member) and a wave-uniform level offset, since the level stride E
+  // is not a compile-time constant (64-bit addresses per level would not
+  // fit in registers).  A wave spans at most 64 coefficients, so the offsets
+  // stay below 2^32 bytes.
+  const int s_w = __builtin_amdgcn_readfirstlane(s);
+  const size_t wbase = (size_t)s_w * 4 * EL;
+  const unsigned voff = (unsigned)(((size_t)(s - s_w) * 4 * EL + e) * sizeof(cx<T>));
+  const size_t wbytes = (size_t)(a.nspec - s_w) * 4 * EL * sizeof(cx<T>);
+  const unsigned wrange = wbytes < 0xffffffffull ? (unsigned)wbytes : 0xffffffffu;
+  const BufView bsa(a.sa3 + wbase, wrange), bsb(a.sb3 + wbase, wrange), btn(a.tend3 + wbase, wrange);
+  auto lo = [&](int f, int l) __attribute__((always_inline)) {
+    return (unsigned)(((size_t)f * EL + (size_t)l * E) * sizeof(cx<T>));
+  };
+  const cx<T> *const tag = nullptr;  // selects the precision of BufView::ld
   const cx<T> wt_pi = a.tend2[(size_t)s * 2 * E + e];
   // pi at t-dt: the analysed log1p(ps'_b/ps_bar), or the spectral level
   // itself in tangent-linear mode
@@ -499,9 +511,9 @@ void k_step(StepArgs<T> a) {
         for (int c = 0; c < L; ++c) fma_acc(y, V[r * L + c], x[c]);
       }
       if (drop) y = zero<T>();
-      const cx<T> cur = sa[at(f, r)];
-      sb[at(f, r)] = oinit ? cur : cur + gam * (sb[at(f, r)] - (T)2 * cur + y);
-      sa[at(f, r)] = y;
+      const cx<T> cur = bsa.ld(voff, lo(f, r), tag);
+      bsb.st(voff, lo(f, r), oinit ? cur : cur + gam * (bsb.ld(voff, lo(f, r), tag) - (T)2 * cur + y));
+      bsa.st(voff, lo(f, r), y);
       if (sum) acc = acc + vc[vo.dsig() + r] * y;
     }
     if (sum) *sum = acc;
@@ -510,18 +522,18 @@ void k_step(StepArgs<T> a) {
   if (role == 1) {
     // vorticity
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (sb[at(0, l)] + dt2 * tn[at(0, l)]);
+    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (bsb.ld(voff, lo(0, l), tag) + dt2 * btn.ld(voff, lo(0, l), tag));
     update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
     // humidity
 #pragma unroll
     for (int l = 0; l < L; ++l)
-      x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (sb[at(3, l)] + dt2 * tn[at(3, l)]);
+      x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (bsb.ld(voff, lo(3, l), tag) + dt2 * btn.ld(voff, lo(3, l), tag));
     update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
     return;
   }
   // temperature estimate and its geopotential (integrator.py:287-292)
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * sb[at(2, l)] + dt * tn[at(2, l)];
+  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * bsb.ld(voff, lo(2, l), tag) + dt * btn.ld(voff, lo(2, l), tag);
   {
     cx<T> acc = zero<T>();
 #pragma unroll
@@ -537,7 +549,7 @@ void k_step(StepArgs<T> a) {
     const T ft = (T)1 + dt2 * ktt[l];
     const T gg = rg * tref[l] * vc[vo.cpg() + l];
     const cx<T> phy = x[l] + (ft * gg) * wdpix;
-    x[l] = (((T)1 + dt * kd[l]) * ft) * sb[at(1, l)] + dt * (ft * tn[at(1, l)] - lapn * phy);
+    x[l] = (((T)1 + dt * kd[l]) * ft) * bsb.ld(voff, lo(1, l), tag) + dt * (ft * btn.ld(voff, lo(1, l), tag) - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
   matvec<T, L>((RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L), x, dbar);
@@ -556,7 +568,7 @@ void k_step(StepArgs<T> a) {
   }
   // divergence
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - sb[at(1, l)];
+  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - bsb.ld(voff, lo(1, l), tag);
   update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E, drop_mean);
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
@@ -566,7 +578,7 @@ void k_step(StepArgs<T> a) {
       const T ft = (T)1 + dt2 * ktt[l];
       const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * dbar[l] + vc[vo.w3() + l] * suf);
       suf = suf + vc[vo.dsig() + l] * dbar[l];
-      x[l] = (T)(1) / ft * (sb[at(2, l)] + dt2 * (tn[at(2, l)] - tmd));
+      x[l] = (T)(1) / ft * (bsb.ld(voff, lo(2, l), tag) + dt2 * (btn.ld(voff, lo(2, l), tag) - tmd));
     }
   }
   update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr, false);
