@@ -7,7 +7,8 @@
 // Internal layouts (E members, L levels, M = nmax+1, J lats, Jh = J/2):
 //   spectral  [s][f][ncols]  complex, s = packed (m, n>=m) index,
 //             ncols = E*L for level fields, E for surface fields
-//   Fourier   [m][j][C]      complex, C = nF3*E*L + nF2*E columns
+//   Fourier   [m][C/8][j][8] complex, C = nF3*E*L + nF2*E columns padded
+//             to a multiple of 8, in tiles of 8 columns (fidx)
 //   grid      [e][j][r][I]   real, rows r = f*L + l (level fields) then
 //             nF3*L + g (surface fields); longitude fastest
 #pragma once
@@ -70,6 +71,17 @@ struct BufView {
   __device__ __forceinline__ cx<float> ld(unsigned voff, unsigned soff, const cx<float> *) const {
     typedef int i2 __attribute__((ext_vector_type(2)));
     const i2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(cx<float>, w);
+  }
+  // non-temporal loads (aux bit 1: the data is not expected to be reused)
+  __device__ __forceinline__ cx<double> ld_nt(unsigned voff, unsigned soff, const cx<double> *) const {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    const i4 w = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 2);
+    return __builtin_bit_cast(cx<double>, w);
+  }
+  __device__ __forceinline__ cx<float> ld_nt(unsigned voff, unsigned soff, const cx<float> *) const {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    const i2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2);
     return __builtin_bit_cast(cx<float>, w);
   }
   __device__ __forceinline__ void st(unsigned voff, unsigned soff, cx<double> v) const {

@@ -138,12 +138,17 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
   // every field: the loads of all fields are in flight together, and the
   // registers of a field already staged take the next chunk's values
   cx<T> rn[NIN], rs[NIN];
+  // the Fourier slab of m as a buffer (32-bit per-lane byte offsets)
+  const size_t slab = (size_t)J * fpad(C);
+  const BufView bin(in + (size_t)m * slab, (unsigned)(slab * sizeof(cx<T>)));
+  const cx<T> *const tag = nullptr;
   auto load = [&](int jc, int f) __attribute__((always_inline)) {
     const int jp = 32 * jc + lj;
     rn[f] = rs[f] = zero<T>();
     if (lcol_ok && jp < Jh) {
-      rn[f] = ld_nt(in + fidx(m, jp, off[f] + col0 + lc, J, C));
-      rs[f] = ld_nt(in + fidx(m, J - 1 - jp, off[f] + col0 + lc, J, C));
+      const int c = off[f] + col0 + lc;
+      rn[f] = bin.ld_nt((unsigned)(fidx(0, jp, c, J, C) * sizeof(cx<T>)), 0u, tag);
+      rs[f] = bin.ld_nt((unsigned)(fidx(0, J - 1 - jp, c, J, C) * sizeof(cx<T>)), 0u, tag);
     }
   };
   auto store = [&](int f, int buf) __attribute__((always_inline)) {
