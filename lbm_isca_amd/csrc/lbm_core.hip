@@ -526,6 +526,10 @@ int spectral_op(lbm_plan *p, SpecOp op, const double *in0, const double *in1, do
   if (B <= 0 || nk <= 0) return fail(LBM_ERR_VALUE, "batch and column counts must be positive");
   const int J = p->J, I = p->I, M = p->M;
   const int ncols = B * nk;
+  // two fields of Fourier columns per wavenumber, 32-bit offsets in the analysis
+  if ((size_t)J * ((2 * (size_t)ncols + 7) & ~(size_t)7) * sizeof(cx<T>) >= ((size_t)1 << 32))
+    return fail(LBM_ERR_VALUE, "batch %d x %d columns: one wavenumber of the Fourier columns exceeds 4 GiB",
+                B, nk);
   // grid -> spectral (analysis kernels) or spectral -> grid (synthesis kernels)
   const bool to_spec = op == OP_G2W || op == OP_UV2VD || op == OP_W2G_ADJ || op == OP_PC2UV_ADJ ||
                        op == OP_HGRAD_ADJ;
@@ -1636,6 +1640,16 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   if (!d->h_sigm || !d->h_kv || !d->h_kd || !d->h_ktt || !d->h_kq)
     return fail(LBM_ERR_VALUE, "sigm and damping arrays are required");
   if (d->inistp < 0 || d->inistp > 8) return fail(LBM_ERR_VALUE, "inistp out of range");
+  // the Legendre analysis addresses one m of the Fourier columns with 32-bit
+  // byte offsets: the largest (9L+2 analysed columns per member) must stay
+  // below 4 GiB
+  {
+    const size_t cx_bytes = d->precision == LBM_FP64 ? 16 : 8;
+    const size_t slab = (size_t)d->nlat * (((size_t)(9 * d->nlev + 2) * d->nmem + 7) & ~(size_t)7) * cx_bytes;
+    if (slab >= ((size_t)1 << 32))
+      return fail(LBM_ERR_VALUE, "nmem=%d: one wavenumber of the analysed Fourier columns (%zu bytes) "
+                  "exceeds 4 GiB; use fewer members per plan", d->nmem, slab);
+  }
   lbm_plan *p = new lbm_plan();
   p->J = d->nlat;
   p->I = d->nlon;

@@ -34,3 +34,25 @@ def test_abi_version_of_header_binding_and_library_agree():
     header = int(re.search(r"#define LBM_ABI_VERSION (\d+)", text).group(1))
     assert _lib.ABI_VERSION == header
     assert ctypes.CDLL(_lib.LIB_PATH).lbm_abi_version() == header
+
+
+def test_plan_create_rejects_a_fourier_slab_beyond_4gib():
+    """Validation precedes any device call: one wavenumber of the analysed
+    Fourier columns (nlat x (9 nlev + 2) nmem complex) must stay below
+    4 GiB, since the analysis addresses it with 32-bit offsets."""
+    import numpy as np
+    lib = _lib.load()
+    nmax, nlev = 42, 20
+    sig = np.linspace(1.0, 0.0, nlev + 1)
+    damp = np.zeros((nmax + 1) * nlev)
+    d = _lib.LbmDesc()
+    d.nlon, d.nlat, d.nmax, d.nlev = 128, 64, nmax, nlev
+    d.nbasic, d.precision, d.inistp = 1, 0, 2
+    d.radius, d.delt, d.robert = 6.371e6, 1200.0, 0.05
+    d.h_sigm = _lib.dptr(sig)
+    d.h_kv = d.h_kd = d.h_ktt = d.h_kq = _lib.dptr(damp)
+    d.nmem = 23100                   # 64 x 182 x 23100 x 16 bytes > 4 GiB
+    h = ctypes.c_void_p()
+    assert lib.lbm_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert "exceeds 4 GiB" in _lib.last_error()
+    assert not h.value
