@@ -84,6 +84,25 @@ struct BufView {
     const i2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2);
     return __builtin_bit_cast(cx<float>, w);
   }
+  // real values; aux 2 = non-temporal
+  template <int AUX = 0>
+  __device__ __forceinline__ double ld(unsigned voff, unsigned soff, const double *) const {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
+  }
+  template <int AUX = 0>
+  __device__ __forceinline__ float ld(unsigned voff, unsigned soff, const float *) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+  }
+  template <int AUX = 0>
+  __device__ __forceinline__ void st(unsigned voff, unsigned soff, double v) const {
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i2, v), r, (int)voff, (int)soff, AUX);
+  }
+  template <int AUX = 0>
+  __device__ __forceinline__ void st(unsigned voff, unsigned soff, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, AUX);
+  }
   __device__ __forceinline__ void st(unsigned voff, unsigned soff, cx<double> v) const {
     typedef int i4 __attribute__((ext_vector_type(4)));
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i4, v), r, (int)voff, (int)soff, 0);

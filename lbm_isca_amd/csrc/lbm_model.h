@@ -89,18 +89,20 @@ __device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig
   return (flux_hi - flux_lo - f * (s_hi - s_lo)) * rdsig;
 }
 
-// One level k of the column.  bs: basic-state column (row stride I); fr:
-// forcing column (row stride I) or null.  (up..qp) at level k, (upn..qpn)
-// at k+1 (ignored at the top).
+// One level k of the column.  bs: basic-state rows of the column's block,
+// fr: forcing rows (when has_fr), both read at the lane offset voff plus
+// row * rowb bytes.  (up..qp) at level k, (upn..qpn) at k+1 (ignored at the
+// top).
 template <class T>
 __device__ inline void column_level(int k, int L, const T *__restrict__ vc, const T *__restrict__ tref,
-                                    const T *__restrict__ bs, const T *__restrict__ fr, int I,
-                                    const ColConst<T> &cc, ColState<T> &cs, T up, T vp, T vor, T div,
-                                    T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out, T psbar, T pst,
-                                    T &omega) {
+                                    const BufView &bs, const BufView &fr, bool has_fr, unsigned voff,
+                                    unsigned rowb, const ColConst<T> &cc, ColState<T> &cs, T up, T vp,
+                                    T vor, T div, T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out,
+                                    T psbar, T pst, T &omega) {
   const VOff vo{L};
   const BRow br{L};
-  auto B = [&](int row) { return bs[(size_t)row * I]; };
+  const T *const tag = nullptr;
+  auto B = [&](int row) { return bs.ld(voff, (unsigned)row * rowb, tag); };
   const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY, h = (T)0.5;
   const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
   const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
@@ -149,7 +151,7 @@ __device__ inline void column_level(int k, int L, const T *__restrict__ vc, cons
   const T hadv = tp * bdiv + tdev * div;
   const T tm = kap * tr * (alpha * div + w3 * (cc.D0 - PDk1));
   T gt = adiab + hadv + va_tp + va_bt + tm;
-  if (fr) gt += fr[(size_t)k * I];
+  if (has_fr) gt += fr.ld(voff, (unsigned)k * rowb, tag);
   // momentum (dynamics.py:224-244)
   const T fu = vp * babsv + bv * vor - rcpg * (tv * cc.bpix + tvdev * cc.pix) + va_up + va_bu;
   const T fv = -up * babsv - bu * vor - rcpg * (tv * cc.bpiy + tvdev * cc.piy) + va_vp + va_bv;
@@ -300,32 +302,39 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const BRow br{L};
   const int Rin = 6 * L + 6, Rout = 9 * L + 2;
   const bool use_raw = a.raw != nullptr && a.ctl->raw;
-  const T *__restrict__ gi = a.gin + (size_t)(e * J + j) * Rin * I + i;
-  const T *__restrict__ rw = use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I + i : nullptr;
-  const T *__restrict__ bs = a.bsg + (size_t)(eb * J + j) * br.rows() * I + i;
-  const T *__restrict__ fr =
-      a.frc ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I + i : nullptr;
-  T *__restrict__ go = a.gout + (size_t)(e * J + j) * Rout * I + i;
-  auto G = [&](int row) { return ld_nt(gi + (size_t)row * I); };
-  auto UP = [&](int k) { return use_raw ? rw[(size_t)k * I] : G(k); };
-  auto VP = [&](int k) { return use_raw ? rw[(size_t)(L + k) * I] : G(L + k); };
-  auto TP = [&](int k) { return use_raw ? rw[(size_t)(2 * L + k) * I] : G(4 * L + k); };
-  auto QP = [&](int k) { return use_raw ? rw[(size_t)(3 * L + k) * I] : G(5 * L + k); };
+  // the rows of this block's (member, latitude) through buffer views: the
+  // row bases sit in scalar registers, every lane adds its longitude
+  const unsigned rowb = (unsigned)(I * sizeof(T)), voff = (unsigned)(i * sizeof(T));
+  const T *const tag = nullptr;
+  const BufView gi(a.gin + (size_t)(e * J + j) * Rin * I, (unsigned)Rin * rowb);
+  const BufView rw(use_raw ? a.raw + (size_t)(e * J + j) * 4 * L * I : nullptr, use_raw ? 4u * L * rowb : 0u);
+  const BufView bs(a.bsg + (size_t)(eb * J + j) * br.rows() * I, (unsigned)br.rows() * rowb);
+  const bool has_fr = a.frc != nullptr;
+  const BufView fr(has_fr ? a.frc + (size_t)((a.nfrc > 1 ? e : 0) * J + j) * L * I : nullptr,
+                   has_fr ? (unsigned)L * rowb : 0u);
+  const BufView go(a.gout + (size_t)(e * J + j) * Rout * I, (unsigned)Rout * rowb);
+  auto G = [&](int row) { return gi.template ld<2>(voff, (unsigned)row * rowb, tag); };
+  auto RW = [&](int row) { return rw.ld(voff, (unsigned)row * rowb, tag); };
+  auto UP = [&](int k) { return use_raw ? RW(k) : G(k); };
+  auto VP = [&](int k) { return use_raw ? RW(L + k) : G(L + k); };
+  auto TP = [&](int k) { return use_raw ? RW(2 * L + k) : G(4 * L + k); };
+  auto QP = [&](int k) { return use_raw ? RW(3 * L + k) : G(5 * L + k); };
+  auto BS = [&](int row) { return bs.ld(voff, (unsigned)row * rowb, tag); };
   ColConst<T> cc;
   cc.pix = G(6 * L);
   cc.piy = G(6 * L + 1);
   const T pig = G(6 * L + 2);
-  cc.bpix = bs[(size_t)br.pix() * I];
-  cc.bpiy = bs[(size_t)br.piy() * I];
-  const T psbar = bs[(size_t)br.psbar() * I];
+  cc.bpix = BS(br.pix());
+  cc.bpiy = BS(br.piy());
+  const T psbar = BS(br.psbar());
   cc.rc = a.rcos[j];
   cc.full_ke = !a.strictly_linear;
   if (use_raw) {
     T S0 = 0, D0 = 0;
     for (int k = 0; k < L; ++k) {
       const T d = G(3 * L + k);
-      const T padv = UP(k) * cc.bpix + bs[(size_t)(br.u() + k) * I] * cc.pix + VP(k) * cc.bpiy +
-                     bs[(size_t)(br.v() + k) * I] * cc.piy;
+      const T padv = UP(k) * cc.bpix + BS(br.u() + k) * cc.pix + VP(k) * cc.bpiy +
+                     BS(br.v() + k) * cc.piy;
       const T ds = a.vc[vo.dsig() + k];
       S0 += ds * (d + padv);
       D0 += ds * d;
@@ -335,11 +344,10 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   } else {
     const T su = G(6 * L + 3), sv = G(6 * L + 4), sd = G(6 * L + 5);
     cc.D0 = sd;
-    cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * bs[(size_t)br.sbu() * I] +
-            cc.piy * bs[(size_t)br.sbv() * I];
+    cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * BS(br.sbu()) + cc.piy * BS(br.sbv());
   }
   ColState<T> cs;
-  cs.bsd_lo = bs[(size_t)br.sdot() * I];
+  cs.bsd_lo = BS(br.sdot());
   const size_t pidx = (size_t)(e * J + j) * I + i;
   T gpi, pib, pst;
   if (a.linear) {
@@ -359,8 +367,8 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   const bool hlast = a.h.last_grid != nullptr && top_a == nst;
   const size_t hrow = (size_t)(e * J + j) * (L + 1) * I + i;
   for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
-  go[(size_t)(9 * L) * I] = gpi;
-  go[(size_t)(9 * L + 1) * I] = pib;
+  go.st(voff, (unsigned)(9 * L) * rowb, gpi);
+  go.st(voff, (unsigned)(9 * L + 1) * rowb, pib);
   const T *tref = a.tref + (size_t)eb * L;
   T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
   for (int k = 0; k < L; ++k) {
@@ -368,10 +376,10 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     const T upn = top ? (T)0 : UP(k + 1), vpn = top ? (T)0 : VP(k + 1);
     const T tpn = top ? (T)0 : TP(k + 1), qpn = top ? (T)0 : QP(k + 1);
     T out[9], omega;
-    column_level<T>(k, L, a.vc, tref, bs, fr, I, cc, cs, up, vp, G(2 * L + k), G(3 * L + k), tp, qp,
-                    upn, vpn, tpn, qpn, out, psbar, pst, omega);
+    column_level<T>(k, L, a.vc, tref, bs, fr, has_fr, voff, rowb, cc, cs, up, vp, G(2 * L + k),
+                    G(3 * L + k), tp, qp, upn, vpn, tpn, qpn, out, psbar, pst, omega);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) st_nt(go + (size_t)(f * L + k) * I, out[f]);
+    for (int f = 0; f < 9; ++f) go.template st<2>(voff, (unsigned)(f * L + k) * rowb, out[f]);
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
     if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
     up = upn;
