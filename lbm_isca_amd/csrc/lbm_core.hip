@@ -263,6 +263,11 @@ struct lbm_plan {
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   hipStream_t own = nullptr;
+  // second stream of the time step (launches beside the level kernels) and
+  // its fork/join events: fft done, surface synthesis, MJO, history sums,
+  // surface analysis, step start
+  hipStream_t side = nullptr;
+  hipEvent_t fj[6] = {};
   // graphs keyed by (kind, member count of the launches): kind 0 forward
   // chunk, 1 forward single step, 2 adjoint chunk, 3 adjoint single step
   std::map<std::pair<int, int>, hipGraphExec_t> graphs;
@@ -1044,6 +1049,49 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   const int Ci = cmi.ncols();
   const ActiveMask am = loop_mask(p, true, ecnt);
   const int ne = ecnt > 0 ? ecnt : E;  // members launched
+  // The surface synthesis, the MJO forcing of this call, the history sums of
+  // the top-of-step state and the surface analysis run on a second stream
+  // beside the level kernels (none reads what the other side writes), each
+  // joined before its first consumer: the inverse FFT, k_tend, k_step and
+  // the steady-forcing / step kernels.  Profiling keeps one stream, so its
+  // per-slot events time each launch.
+  const bool fork = !p->prof && p->side != nullptr;
+  hipStream_t sd = fork ? p->side : st;
+  enum { F_FFT, F_SURF, F_MJO, F_HIST, F_ANAS, F_START };
+  auto fork_at = [&](int k) -> int {
+    if (fork) {
+      HIPCHK(hipEventRecord(p->fj[k], st));
+      HIPCHK(hipStreamWaitEvent(sd, p->fj[k], 0));
+    }
+    return LBM_OK;
+  };
+  auto side_done = [&](int k) -> int {
+    if (fork) HIPCHK(hipEventRecord(p->fj[k], sd));
+    return LBM_OK;
+  };
+  auto join = [&](int k) -> int {
+    if (fork) HIPCHK(hipStreamWaitEvent(st, p->fj[k], 0));
+    return LBM_OK;
+  };
+  const bool hist = p->hist_mode != 0 || p->keep_last;
+  auto launch_mjo = [&]() -> int {
+    // this call's MJO temperature forcing (forcing.py:113-156)
+    const size_t n = (size_t)p->J * L * p->I;
+    hipLaunchKernelGGL((k_mjo<T>), dim3(nblk((size_t)p->nfrc * n)), dim3(256), 0, sd,
+                       (const T *)p->mjo, (T *)p->frc, (const Ctl *)p->ctl, p->nfrc, p->mjo_nd,
+                       p->mjo_nt, p->mjo_spin, p->mjo_call0, p->mjo_single, n);
+    HIPCHK(hipGetLastError());
+    return side_done(F_MJO);
+  };
+  auto launch_hist = [&]() -> int {
+    const size_t nthr = (size_t)p->nspec * ne;
+    hipLaunchKernelGGL((k_hist_acc<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, sd,
+                       (const cx<T> *)p->sa3, hist_args<T>(p), (const Ctl *)p->ctl, E, L, p->nspec,
+                       ne);
+    HIPCHK(hipGetLastError());
+    return side_done(F_HIST);
+  };
+  CHK(fork_at(F_START));
   {
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
     CHK(mark(p, 0, st));
@@ -1051,8 +1099,11 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
                                  nullptr, am)));
     OutSlots s2{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
     CHK(mark(p, 1, st));
-    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, Ci, s2, st, SynSurf::NIN, nullptr, nullptr,
+    CHK((launch_syn<T, SynSurf>(p, p->sa2, p->fin, E, Ci, s2, sd, SynSurf::NIN, nullptr, nullptr,
                                 nullptr, am)));
+    CHK(side_done(F_SURF));
+    if (fork && p->mjo) CHK(launch_mjo());
+    if (fork && hist) CHK(launch_hist());
   }
   TendArgs<T> ta;
   ta.gin = (const T *)p->gin;
@@ -1080,20 +1131,17 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   ta.h = hist_args<T>(p);
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
+  CHK(join(F_SURF));
   CHK(mark(p, 2, st));
   CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
-  if (p->mjo) {
-    // this call's MJO temperature forcing (forcing.py:113-156)
-    const size_t n = (size_t)p->J * L * p->I;
-    hipLaunchKernelGGL((k_mjo<T>), dim3(nblk((size_t)p->nfrc * n)), dim3(256), 0, st,
-                       (const T *)p->mjo, (T *)p->frc, (const Ctl *)p->ctl, p->nfrc, p->mjo_nd,
-                       p->mjo_nt, p->mjo_spin, p->mjo_call0, p->mjo_single, n);
-  }
+  if (!fork && p->mjo) CHK(launch_mjo());
+  if (p->mjo) CHK(join(F_MJO));
   CHK(mark(p, 3, st));
   hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
+  CHK(fork_at(F_FFT));
   CHK(mark(p, 5, st));
   {
     // (FU, FV, KE) -> vor, div; (XU, XV, GT) -> t; (YU, YV, G) -> q
@@ -1107,8 +1155,10 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
   }
   InSlots s2{{9 * EL}};
   CHK(mark(p, 6, st));
-  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,
+  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, sd, 1, 0, nullptr, nullptr,
                                 nullptr, am)));
+  CHK(side_done(F_ANAS));
+  CHK(join(F_ANAS));
   if (p->sfv || p->sfp) {
     const size_t nthr = (size_t)p->nspec * ne;
     hipLaunchKernelGGL((k_add_steady<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
@@ -1116,13 +1166,8 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
                        (const cx<T> *)p->sfp, E, L, p->nspec, p->nsf, ne);
     HIPCHK(hipGetLastError());
   }
-  if (p->hist_mode != 0 || p->keep_last) {
-    const size_t nthr = (size_t)p->nspec * ne;
-    hipLaunchKernelGGL((k_hist_acc<T>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
-                       (const cx<T> *)p->sa3, hist_args<T>(p), (const Ctl *)p->ctl, E, L, p->nspec,
-                       ne);
-    HIPCHK(hipGetLastError());
-  }
+  if (!fork && hist) CHK(launch_hist());
+  if (hist) CHK(join(F_HIST));
   CHK(mark(p, 7, st));
   if (L == 20)
     CHK((launch_step_kernel<T, 20>(p, st, ne)));
@@ -1617,6 +1662,9 @@ int lbm_plan_destroy(lbm_plan *p) {
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : p->fj)
+    if (e) (void)hipEventDestroy(e);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->own) (void)hipStreamDestroy(p->own);
   delete p;
   return LBM_OK;
@@ -1703,6 +1751,12 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
     return LBM_OK;
   };
   if (hipStreamCreate(&p->own) != hipSuccess) return bail(fail(LBM_ERR_RUNTIME, "stream create failed"));
+  // default priority, like every stream of the library
+  if (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(LBM_ERR_RUNTIME, "stream create failed"));
+  for (auto &e : p->fj)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(LBM_ERR_RUNTIME, "event create failed"));
   if ((rc = build_tabs<double>(p)) != LBM_OK) return bail(rc);
   if (p->prec != LBM_FP64 && (rc = build_tabs<float>(p)) != LBM_OK) return bail(rc);
   if ((rc = set_twiddles<double>()) != LBM_OK) return bail(rc);
