@@ -92,13 +92,14 @@ __device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig
 // One level k of the column.  bs: basic-state rows of the column's block,
 // fr: forcing rows (when has_fr), both read at the lane offset voff plus
 // row * rowb bytes.  (up..qp) at level k, (upn..qpn) at k+1 (ignored at the
-// top).
+// top).  buvtq: the basic-state u, v, T, q of level k on entry, of level
+// k+1 on return (each level's values are loaded once).
 template <class T>
 __device__ inline void column_level(int k, int L, const T *__restrict__ vc, const T *__restrict__ tref,
                                     const BufView &bs, const BufView &fr, bool has_fr, unsigned voff,
                                     unsigned rowb, const ColConst<T> &cc, ColState<T> &cs, T up, T vp,
                                     T vor, T div, T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out,
-                                    T psbar, T pst, T &omega) {
+                                    T psbar, T pst, T &omega, T (&buvtq)[4]) {
   const VOff vo{L};
   const BRow br{L};
   const T *const tag = nullptr;
@@ -108,12 +109,16 @@ __device__ inline void column_level(int k, int L, const T *__restrict__ vc, cons
   const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
   const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
   const T tr = tref[k];
-  const T bu = B(br.u() + k), bv = B(br.v() + k), bt = B(br.t() + k), bq = B(br.q() + k);
+  const T bu = buvtq[0], bv = buvtq[1], bt = buvtq[2], bq = buvtq[3];
   const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
   const T bx5 = B(br.x5() + k);
   const bool top = (k + 1 == L);
   const T bun = top ? (T)0 : B(br.u() + k + 1), bvn = top ? (T)0 : B(br.v() + k + 1);
   const T btn = top ? (T)0 : B(br.t() + k + 1), bqn = top ? (T)0 : B(br.q() + k + 1);
+  buvtq[0] = bun;
+  buvtq[1] = bvn;
+  buvtq[2] = btn;
+  buvtq[3] = bqn;
   const T bsd_hi = B(br.sdot() + k + 1);
   if (top) upn = vpn = tpn = qpn = 0;
   const T padv = up * cc.bpix + bu * cc.pix + vp * cc.bpiy + bv * cc.piy;
@@ -371,13 +376,19 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   go.st(voff, (unsigned)(9 * L + 1) * rowb, pib);
   const T *tref = a.tref + (size_t)eb * L;
   T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
+  T vor = G(2 * L), dv = G(3 * L);
+  T buvtq[4] = {BS(br.u()), BS(br.v()), BS(br.t()), BS(br.q())};
   for (int k = 0; k < L; ++k) {
     const bool top = (k + 1 == L);
+    // level k+1 in flight while level k is formed
     const T upn = top ? (T)0 : UP(k + 1), vpn = top ? (T)0 : VP(k + 1);
     const T tpn = top ? (T)0 : TP(k + 1), qpn = top ? (T)0 : QP(k + 1);
+    const T vorn = top ? (T)0 : G(2 * L + k + 1), dvn = top ? (T)0 : G(3 * L + k + 1);
     T out[9], omega;
-    column_level<T>(k, L, a.vc, tref, bs, fr, has_fr, voff, rowb, cc, cs, up, vp, G(2 * L + k),
-                    G(3 * L + k), tp, qp, upn, vpn, tpn, qpn, out, psbar, pst, omega);
+    column_level<T>(k, L, a.vc, tref, bs, fr, has_fr, voff, rowb, cc, cs, up, vp, vor, dv, tp, qp,
+                    upn, vpn, tpn, qpn, out, psbar, pst, omega, buvtq);
+    vor = vorn;
+    dv = dvn;
 #pragma unroll
     for (int f = 0; f < 9; ++f) go.template st<2>(voff, (unsigned)(f * L + k) * rowb, out[f]);
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
