@@ -89,37 +89,34 @@ __device__ inline T vadv_term(T flux_hi, T flux_lo, T f, T s_hi, T s_lo, T rdsig
   return (flux_hi - flux_lo - f * (s_hi - s_lo)) * rdsig;
 }
 
-// One level k of the column.  bs: basic-state rows of the column's block,
-// fr: forcing rows (when has_fr), both read at the lane offset voff plus
-// row * rowb bytes.  (up..qp) at level k, (upn..qpn) at k+1 (ignored at the
-// top).  buvtq: the basic-state u, v, T, q of level k on entry, of level
-// k+1 on return (each level's values are loaded once).
+// Basic-state values of one level k: u, v, T, q, Tv, absolute vorticity,
+// divergence, x5 and sigma-dot of half level k+1; zero above the top.
+template <class T>
+struct BLev {
+  T u = 0, v = 0, t = 0, q = 0, tv = 0, av = 0, dv = 0, x5 = 0, sd = 0;
+};
+
+// One level k of the column.  b: the basic state of level k, bn: that of
+// level k+1 (u, v, T, q; zero at the top), both loaded a level ahead by the
+// caller.  fk: the forcing of level k (0 without one).  (up..qp) at level
+// k, (upn..qpn) at k+1 (ignored at the top).
 template <class T>
 __device__ inline void column_level(int k, int L, const T *__restrict__ vc, const T *__restrict__ tref,
-                                    const BufView &bs, const BufView &fr, bool has_fr, unsigned voff,
-                                    unsigned rowb, const ColConst<T> &cc, ColState<T> &cs, T up, T vp,
+                                    const BLev<T> &b, const BLev<T> &bn, T fk,
+                                    const ColConst<T> &cc, ColState<T> &cs, T up, T vp,
                                     T vor, T div, T tp, T qp, T upn, T vpn, T tpn, T qpn, T *out,
-                                    T psbar, T pst, T &omega, T (&buvtq)[4]) {
+                                    T psbar, T pst, T &omega) {
   const VOff vo{L};
-  const BRow br{L};
-  const T *const tag = nullptr;
-  auto B = [&](int row) { return bs.ld(voff, (unsigned)row * rowb, tag); };
   const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY, h = (T)0.5;
   const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
   const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
   const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
   const T tr = tref[k];
-  const T bu = buvtq[0], bv = buvtq[1], bt = buvtq[2], bq = buvtq[3];
-  const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
-  const T bx5 = B(br.x5() + k);
+  const T bu = b.u, bv = b.v, bt = b.t, bq = b.q;
+  const T btv = b.tv, babsv = b.av, bdiv = b.dv, bx5 = b.x5;
   const bool top = (k + 1 == L);
-  const T bun = top ? (T)0 : B(br.u() + k + 1), bvn = top ? (T)0 : B(br.v() + k + 1);
-  const T btn = top ? (T)0 : B(br.t() + k + 1), bqn = top ? (T)0 : B(br.q() + k + 1);
-  buvtq[0] = bun;
-  buvtq[1] = bvn;
-  buvtq[2] = btn;
-  buvtq[3] = bqn;
-  const T bsd_hi = B(br.sdot() + k + 1);
+  const T bun = bn.u, bvn = bn.v, btn = bn.t, bqn = bn.q;
+  const T bsd_hi = b.sd;
   if (top) upn = vpn = tpn = qpn = 0;
   const T padv = up * cc.bpix + bu * cc.pix + vp * cc.bpiy + bv * cc.piy;
   cs.chk += up + vp + tp + qp;
@@ -156,7 +153,7 @@ __device__ inline void column_level(int k, int L, const T *__restrict__ vc, cons
   const T hadv = tp * bdiv + tdev * div;
   const T tm = kap * tr * (alpha * div + w3 * (cc.D0 - PDk1));
   T gt = adiab + hadv + va_tp + va_bt + tm;
-  if (has_fr) gt += fr.ld(voff, (unsigned)k * rowb, tag);
+  gt += fk;
   // momentum (dynamics.py:224-244)
   const T fu = vp * babsv + bv * vor - rcpg * (tv * cc.bpix + tvdev * cc.pix) + va_up + va_bu;
   const T fv = -up * babsv - bu * vor - rcpg * (tv * cc.bpiy + tvdev * cc.piy) + va_vp + va_bv;
@@ -375,20 +372,39 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   go.st(voff, (unsigned)(9 * L) * rowb, gpi);
   go.st(voff, (unsigned)(9 * L + 1) * rowb, pib);
   const T *tref = a.tref + (size_t)eb * L;
+  // the state, basic state and forcing of level k+1 are in flight while
+  // level k is formed
+  auto blev = [&](int k) {
+    BLev<T> b;
+    b.u = BS(br.u() + k);
+    b.v = BS(br.v() + k);
+    b.t = BS(br.t() + k);
+    b.q = BS(br.q() + k);
+    b.tv = BS(br.tv() + k);
+    b.av = BS(br.absvor() + k);
+    b.dv = BS(br.div() + k);
+    b.x5 = BS(br.x5() + k);
+    b.sd = BS(br.sdot() + k + 1);
+    return b;
+  };
+  auto FR = [&](int k) { return has_fr ? fr.ld(voff, (unsigned)k * rowb, tag) : (T)0; };
   T up = UP(0), vp = VP(0), tp = TP(0), qp = QP(0);
-  T vor = G(2 * L), dv = G(3 * L);
-  T buvtq[4] = {BS(br.u()), BS(br.v()), BS(br.t()), BS(br.q())};
+  T vor = G(2 * L), dv = G(3 * L), fk = FR(0);
+  BLev<T> bc = blev(0);
   for (int k = 0; k < L; ++k) {
     const bool top = (k + 1 == L);
-    // level k+1 in flight while level k is formed
     const T upn = top ? (T)0 : UP(k + 1), vpn = top ? (T)0 : VP(k + 1);
     const T tpn = top ? (T)0 : TP(k + 1), qpn = top ? (T)0 : QP(k + 1);
     const T vorn = top ? (T)0 : G(2 * L + k + 1), dvn = top ? (T)0 : G(3 * L + k + 1);
+    const T fkn = top ? (T)0 : FR(k + 1);
+    const BLev<T> bn = top ? BLev<T>{} : blev(k + 1);
     T out[9], omega;
-    column_level<T>(k, L, a.vc, tref, bs, fr, has_fr, voff, rowb, cc, cs, up, vp, vor, dv, tp, qp,
-                    upn, vpn, tpn, qpn, out, psbar, pst, omega, buvtq);
+    column_level<T>(k, L, a.vc, tref, bc, bn, fk, cc, cs, up, vp, vor, dv, tp, qp, upn, vpn, tpn,
+                    qpn, out, psbar, pst, omega);
     vor = vorn;
     dv = dvn;
+    fk = fkn;
+    bc = bn;
 #pragma unroll
     for (int f = 0; f < 9; ++f) go.template st<2>(voff, (unsigned)(f * L + k) * rowb, out[f]);
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
