@@ -416,13 +416,13 @@ __device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
 
 // Forward, stage a (thread (r, t)): radix-R across n_a of z[n_b + 64 n_a]
 // for n_b = t + 8 n2, keep output r, twiddle exp(-2 pi i n_b r / NC), first
-// radix-8 pass -> scr.
+// radix-8 pass; v[k2] (twiddled) goes to scr by fft_a_store.  in: the row in
+// global memory (R = 1) or staged in LDS (R > 1).
 template <class T, int NI>
-__device__ inline void fft_a(const cx<T> *in, cx<T> *scr, int tt) {
+__device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt) {
   constexpr int R = FftShape<T, NI>::R;
   const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
-  cx<T> v[8];
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
     const int nb = t + 8 * n2;
@@ -445,9 +445,15 @@ __device__ inline void fft_a(const cx<T> *in, cx<T> *scr, int tt) {
     }
   }
   dft8<-1>(v);
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) v[k2] = cmul(v[k2], twf<T, NI, -1>(tw, t * k2, 64));
+}
+template <class T, int NI>
+__device__ inline void fft_a_store(const cx<T> *v, cx<T> *scr, int tt) {
+  const int r = tt >> 3, t = tt & 7;
   cx<T> *sc = scr + r * 72 + t * 9;
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = cmul(v[k2], twf<T, NI, -1>(tw, t * k2, 64));
+  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = v[k2];
 }
 
 // Forward, stage b: second radix-8 pass; v[k1] = Z[r + R (t + 8 k1)].
@@ -530,12 +536,27 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_fft(const T *__r
     cm.map(col, e, r);
     live = am(e);
   }
-  if (live) {
-    fft_a<T, NI>(reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI),
-                 sh[rl], tt);
+  const cx<T> *row_in = reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+  cx<T> v[8];
+  if constexpr (S::R == 1) {
+    if (live) {
+      fft_a<T, NI>(row_in, v, tt);
+      fft_a_store<T, NI>(v, sh[rl], tt);
+    }
+  } else {
+    // the row's 64 R samples staged once in LDS (each of the R threads of a
+    // column needs all R of its inputs)
+    if (live) {
+      const int ra = tt >> 3, ta = tt & 7;
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) sh[rl][ta + 8 * n2 + 64 * ra] = row_in[ta + 8 * n2 + 64 * ra];
+    }
+    __syncthreads();
+    if (live) fft_a<T, NI>(sh[rl], v, tt);
+    __syncthreads();  // the staged row is overwritten by the first pass
+    if (live) fft_a_store<T, NI>(v, sh[rl], tt);
   }
   __syncthreads();
-  cx<T> v[8];
   if (live) fft_b<T, NI>(sh[rl], v, tt);
   __syncthreads();  // scr is reused for Z
   if (live) {
