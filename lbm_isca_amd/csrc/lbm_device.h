@@ -348,34 +348,43 @@ __device__ __forceinline__ T fourier_scale(int mode, int m) {
   return (T)1;
 }
 
-// Inverse, stage a (thread (r, t) of the row): X_m (m < M, stride ms) ->
-// Z_k for k = r + R(t + 8 n2) -> first radix-8 pass -> scr.
+// First-pass outputs v[k2] of thread (r, t) into the row's scratch: the
+// 8 x 8 transpose of sub-transform r, rows padded to 9 (distinct banks for
+// the 8 lanes of a row in both directions).
+template <class T, int NI>
+__device__ inline void fft_a_store(const cx<T> *v, cx<T> *scr, int tt) {
+  const int r = tt >> 3, t = tt & 7;
+  cx<T> *sc = scr + r * 72 + t * 9;
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = v[k2];
+}
+
+// Inverse, stage a (thread (r, t) of the row): X_m = X(m) (m < M) -> Z_k
+// for k = r + R(t + 8 n2) -> first radix-8 pass, twiddled, in v (to scr by
+// fft_a_store).
 // g = X0.re + 2 sum_m Re(X_m e^{i m lam}) = irfft(zero-filled * nlon)
 // (spectral.py:166-172); the imaginary part of X0 is discarded.
-template <class T, int NI>
-__device__ inline void ifft_a(const cx<T> *__restrict__ F, size_t ms, int M, cx<T> *scr, int tt,
-                              int mode) {
+template <class T, int NI, class XF>
+__device__ inline void ifft_a(XF X, int M, cx<T> *v, int tt, int mode) {
   using S = FftShape<T, NI>;
   const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
-  cx<T> v[8];
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
     const int k = r + S::R * (t + 8 * n2);
     cx<T> A = zero<T>(), B = zero<T>();
     if (k < M) {
-      A = fourier_scale<T, NI>(mode, k) * F[(size_t)k * ms];
+      A = fourier_scale<T, NI>(mode, k) * X(k);
       if (k == 0) A.y = 0;
     }
     const int kb = S::NC - k;
-    if (kb < M) B = conj(fourier_scale<T, NI>(mode, kb) * F[(size_t)kb * ms]);
+    if (kb < M) B = conj(fourier_scale<T, NI>(mode, kb) * X(kb));
     const cx<T> w = conj(tw[k]);  // exp(+2 pi i k / NI)
     v[n2] = (A + B) + cmul(mk((T)0, (T)1), cmul(w, A - B));
   }
   dft8<+1>(v);
-  cx<T> *sc = scr + r * 72 + t * 9;
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = cmul(v[k2], twf<T, NI, +1>(tw, t * k2, 64));
+  for (int k2 = 0; k2 < 8; ++k2) v[k2] = cmul(v[k2], twf<T, NI, +1>(tw, t * k2, 64));
 }
 
 // Inverse, stage b: second radix-8 pass of sub-transform r, then the
@@ -448,14 +457,6 @@ __device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt) {
 #pragma unroll
   for (int k2 = 0; k2 < 8; ++k2) v[k2] = cmul(v[k2], twf<T, NI, -1>(tw, t * k2, 64));
 }
-template <class T, int NI>
-__device__ inline void fft_a_store(const cx<T> *v, cx<T> *scr, int tt) {
-  const int r = tt >> 3, t = tt & 7;
-  cx<T> *sc = scr + r * 72 + t * 9;
-#pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) sc[k2] = v[k2];
-}
-
 // Forward, stage b: second radix-8 pass; v[k1] = Z[r + R (t + 8 k1)].
 template <class T, int NI>
 __device__ inline void fft_b(const cx<T> *scr, cx<T> *v, int tt) {
@@ -499,9 +500,26 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
     cm.map(col, e, r);
     live = am(e);
   }
-  if (live) ifft_a<T, NI>(F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, sh[rl], tt, mode);
-  __syncthreads();
+  // The row's M coefficients staged in LDS once (ifft_a reads X_m and
+  // X_{NC-m} of each m); M <= NC (nmax + 1 <= nlon / 2, checked by the
+  // plan).  384 FP32 points read them from memory (measured: staging loses).
+  constexpr bool STAGE = S::R == 1 || sizeof(T) == 8;
+  static_assert(S::LDS >= S::NC, "the row scratch holds the staged coefficients");
+  const cx<T> *Fr = F + fidx(0, j, col, J, C);
+  const size_t ms = (size_t)J * fpad(C);
   cx<T> v[8];
+  if constexpr (STAGE) {
+    if (live)
+      for (int m = tt; m < M; m += S::TPR) sh[rl][m] = Fr[(size_t)m * ms];
+    __syncthreads();
+    const cx<T> *X = sh[rl];
+    if (live) ifft_a<T, NI>([&](int k) { return X[k]; }, M, v, tt, mode);
+    __syncthreads();  // the staged coefficients are overwritten by the first pass
+  } else {
+    if (live) ifft_a<T, NI>([&](int k) { return Fr[(size_t)k * ms]; }, M, v, tt, mode);
+  }
+  if (live) fft_a_store<T, NI>(v, sh[rl], tt);
+  __syncthreads();
   if (live) ifft_b<T, NI>(sh[rl], v, tt);
   cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
   if constexpr (S::R == 1) {
