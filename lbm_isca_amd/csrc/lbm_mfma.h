@@ -114,7 +114,9 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   int out_f0, ActiveMask am) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
-  __shared__ T sym[2][32][16], anti[2][32][16];
+  // one symmetric/antisymmetric buffer per field: a chunk is stored whole,
+  // then contracted (two barriers per chunk)
+  __shared__ T sym[NIN][32][16], anti[NIN][32][16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int p = w & 1, tile = (w >> 1) + 2 * blockIdx.z;
   const int m = blockIdx.y, col0 = am.tile_col0(blockIdx.x);
@@ -173,10 +175,16 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
 
 #pragma unroll
   for (int f = 0; f < NIN; ++f) load(0, f);
-  store(0, 0);
-  __syncthreads();
-  int it = 0;
   for (int jc = 0; jc < nch; ++jc) {
+    if (jc > 0) __syncthreads();  // the previous chunk's contractions are done
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) store(f, f);
+    __syncthreads();
+    // this chunk is in LDS: the registers load the next one's
+    if (jc + 1 < nch) {
+#pragma unroll
+      for (int f = 0; f < NIN; ++f) load(jc + 1, f);
+    }
     // Legendre operands of this chunk, shared by every field: lane row n_a,
     // latitude pair 32 jc + 4 q + k_a (FP64/FP32 K = 4 steps) or
     // 32 jc + 16 (q / 4) + 4 k_a + q % 4 (bf16 K = 16 steps)
@@ -197,12 +205,8 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
       }
     }
 #pragma unroll
-    for (int f = 0; f < NIN; ++f, ++it) {
-      const int buf = it & 1;
-      if (f + 1 < NIN) store(f + 1, buf ^ 1);
-      // field f of this chunk is in LDS: its registers load the next chunk's
-      if (jc + 1 < nch) load(jc + 1, f);
-      if (f + 1 == NIN && jc + 1 < nch) store(0, buf ^ 1);
+    for (int f = 0; f < NIN; ++f) {
+      const int buf = f;
       if (wave_live) {
         // the P rows of parity p contract with the symmetric sums when
         // p = 0, the H rows with the other one
@@ -237,7 +241,6 @@ __global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
           }
         }
       }
-      __syncthreads();
     }
   }
   if (!wave_live) return;
