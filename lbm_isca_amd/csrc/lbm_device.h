@@ -522,10 +522,32 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
   __syncthreads();
   if (live) ifft_b<T, NI>(sh[rl], v, tt);
   cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
-  if constexpr (S::R == 1) {
+  if constexpr (S::R == 1 && sizeof(T) == 8) {
     if (live) {
 #pragma unroll
       for (int k1 = 0; k1 < 8; ++k1) st_nt(out + tt + 8 * k1, v[k1]);
+    }
+  } else if constexpr (S::R == 1) {
+    // FP32: the row back through LDS in natural order, so that each lane
+    // stores two consecutive values (full 128-byte lines per row and store)
+    __syncthreads();  // scr is reused
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) sh[rl][tt + 8 * k1] = v[k1];
+    }
+    __syncthreads();
+    if (live) {
+      typedef T t4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const cx<T> a0 = sh[rl][2 * tt + 16 * q], a1 = sh[rl][2 * tt + 16 * q + 1];
+        t4 w;
+        w.x = a0.x;
+        w.y = a0.y;
+        w.z = a1.x;
+        w.w = a1.y;
+        __builtin_nontemporal_store(w, reinterpret_cast<t4 *>(out + 2 * tt + 16 * q));
+      }
     }
   } else {
     __syncthreads();  // scr is reused as the exchange buffer
