@@ -578,18 +578,30 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_fft(const T *__r
   }
   const cx<T> *row_in = reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
   cx<T> v[8];
-  if constexpr (S::R == 1) {
+  if constexpr (S::R == 1 && sizeof(T) == 8) {
     if (live) {
       fft_a<T, NI>(row_in, v, tt);
       fft_a_store<T, NI>(v, sh[rl], tt);
     }
   } else {
-    // the row's 64 R samples staged once in LDS (each of the R threads of a
-    // column needs all R of its inputs)
+    // the row's 64 R samples staged once in LDS: each of the R threads of a
+    // column needs all R of its inputs, and FP32 lanes load two consecutive
+    // values (full 128-byte lines)
     if (live) {
-      const int ra = tt >> 3, ta = tt & 7;
+      if constexpr (sizeof(T) == 4) {
+        typedef T t4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-      for (int n2 = 0; n2 < 8; ++n2) sh[rl][ta + 8 * n2 + 64 * ra] = row_in[ta + 8 * n2 + 64 * ra];
+        for (int q = 0; q < 4; ++q) {
+          const int i = tt + S::TPR * q;  // pair of samples 2i, 2i + 1
+          const t4 w = *reinterpret_cast<const t4 *>(row_in + 2 * i);
+          sh[rl][2 * i] = mk(w.x, w.y);
+          sh[rl][2 * i + 1] = mk(w.z, w.w);
+        }
+      } else {
+        const int ra = tt >> 3, ta = tt & 7;
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) sh[rl][ta + 8 * n2 + 64 * ra] = row_in[ta + 8 * n2 + 64 * ra];
+      }
     }
     __syncthreads();
     if (live) fft_a<T, NI>(sh[rl], v, tt);
