@@ -403,9 +403,9 @@ __device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt) {
 }
 
 // Inverse, stage c (R > 1): radix-R across r from ex[r][n_b] for output
-// block n_a = r of this thread: z[n_b + 64 n_a], n_b = t + 8 k1.
+// block n_a = r of this thread: o[k1] = z[n_b + 64 n_a], n_b = t + 8 k1.
 template <class T, int NI>
-__device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
+__device__ inline void ifft_c(const cx<T> *ex, cx<T> *o, int tt) {
   constexpr int R = FftShape<T, NI>::R;
   const int na = tt >> 3, t = tt & 7;
 #pragma unroll
@@ -415,11 +415,29 @@ __device__ inline void ifft_c(const cx<T> *ex, cx<T> *out, int tt) {
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = ex[r * 64 + nb];
     dftR<R, +1>(v);
-    cx<T> o = v[0];
+    o[k1] = v[0];
 #pragma unroll
     for (int q = 1; q < R; ++q)
-      if (q == na) o = v[q];
-    st_nt(out + nb + 64 * na, o);
+      if (q == na) o[k1] = v[q];
+  }
+}
+
+// A row of NC complex samples from LDS to memory, two per lane with
+// non-temporal 16-byte stores (FP32: full 128-byte lines).
+template <class T, int NI>
+__device__ inline void store_row_pairs(const cx<T> *z, cx<T> *out, int tt) {
+  using S = FftShape<T, NI>;
+  typedef T t4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = tt + S::TPR * q;  // samples 2i, 2i + 1 (NC / 2 = 4 TPR pairs)
+    const cx<T> a0 = z[2 * i], a1 = z[2 * i + 1];
+    t4 w;
+    w.x = a0.x;
+    w.y = a0.y;
+    w.z = a1.x;
+    w.w = a1.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<t4 *>(out + 2 * i));
   }
 }
 
@@ -536,19 +554,7 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
       for (int k1 = 0; k1 < 8; ++k1) sh[rl][tt + 8 * k1] = v[k1];
     }
     __syncthreads();
-    if (live) {
-      typedef T t4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const cx<T> a0 = sh[rl][2 * tt + 16 * q], a1 = sh[rl][2 * tt + 16 * q + 1];
-        t4 w;
-        w.x = a0.x;
-        w.y = a0.y;
-        w.z = a1.x;
-        w.w = a1.y;
-        __builtin_nontemporal_store(w, reinterpret_cast<t4 *>(out + 2 * tt + 16 * q));
-      }
-    }
+    if (live) store_row_pairs<T, NI>(sh[rl], out, tt);
   } else {
     __syncthreads();  // scr is reused as the exchange buffer
     if (live) {
@@ -556,7 +562,23 @@ __global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T>
       for (int k1 = 0; k1 < 8; ++k1) sh[rl][(tt >> 3) * 64 + (tt & 7) + 8 * k1] = v[k1];
     }
     __syncthreads();
-    if (live) ifft_c<T, NI>(sh[rl], out, tt);
+    cx<T> o[8];
+    if (live) ifft_c<T, NI>(sh[rl], o, tt);
+    const int na = tt >> 3, t8 = tt & 7;
+    if constexpr (sizeof(T) == 8) {
+      if (live) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) st_nt(out + t8 + 8 * k1 + 64 * na, o[k1]);
+      }
+    } else {
+      __syncthreads();  // the exchange buffer takes the row in natural order
+      if (live) {
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) sh[rl][t8 + 8 * k1 + 64 * na] = o[k1];
+      }
+      __syncthreads();
+      if (live) store_row_pairs<T, NI>(sh[rl], out, tt);
+    }
   }
 }
 
