@@ -203,21 +203,50 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
   };
   T Pst = 0, PDst = 0, SKE = 0, S0st = 0, D0st = 0, pixst = 0, piyst = 0;
   T wk[4], wn[4] = {0, 0, 0, 0}, wp[4] = {0, 0, 0, 0};
+  // the basic state and tendency adjoints of level k-1 are loaded while
+  // level k is formed; level k+1's basic u, v, T, q and sigma-dot come from
+  // the previous iteration (levels run top-down)
+  struct ALev {
+    T u = 0, v = 0, t = 0, q = 0, tv = 0, av = 0, dv = 0, x5 = 0, sdlo = 0;
+    T ke = 0, xu = 0, xv = 0, yu = 0, yv = 0;
+  };
+  auto alev = [&](int k) {
+    ALev b;
+    b.u = B(br.u() + k);
+    b.v = B(br.v() + k);
+    b.t = B(br.t() + k);
+    b.q = B(br.q() + k);
+    b.tv = B(br.tv() + k);
+    b.av = B(br.absvor() + k);
+    b.dv = B(br.div() + k);
+    b.x5 = B(br.x5() + k);
+    b.sdlo = B(br.sdot() + k);
+    b.ke = O(2, k);
+    b.xu = O(3, k) * rc;
+    b.xv = O(4, k) * rc;
+    b.yu = O(6, k) * rc;
+    b.yv = O(7, k) * rc;
+    return b;
+  };
   W(L - 1, wk);
+  ALev cur = alev(L - 1);
+  T bsd_hi = B(br.sdot() + L);
+  T bun = 0, bvn = 0, btn = 0, bqn = 0;
   for (int k = L - 1; k >= 0; --k) {
     if (k > 0) W(k - 1, wp);
+    const ALev nxt = k > 0 ? alev(k - 1) : ALev{};
     const T ds = vc[vo.dsig() + k], rds = (T)1 / ds;
     const T alpha = vc[vo.alpha() + k], w3 = vc[vo.w3() + k], wP = vc[vo.wP() + k];
     const T rcpg = rgas * vc[vo.cpg() + k], dlog3 = vc[vo.dlog3() + k];
     const T tr = tref[k];
-    const T bu = B(br.u() + k), bv = B(br.v() + k), bt = B(br.t() + k), bq = B(br.q() + k);
-    const T btv = B(br.tv() + k), babsv = B(br.absvor() + k), bdiv = B(br.div() + k);
-    const T bx5 = B(br.x5() + k);
-    const T bsd_lo = B(br.sdot() + k), bsd_hi = B(br.sdot() + k + 1);
+    const T bu = cur.u, bv = cur.v, bt = cur.t, bq = cur.q;
+    const T btv = cur.tv, babsv = cur.av, bdiv = cur.dv;
+    const T bx5 = cur.x5;
+    const T bsd_lo = cur.sdlo;
     const bool top = (k + 1 == L);
     const T fu_a = wk[0], fv_a = wk[1], gt_a = wk[2], g_a = wk[3];
-    const T ke_a = O(2, k), xu_a = O(3, k) * rc, xv_a = O(4, k) * rc;
-    const T yu_a = O(6, k) * rc, yv_a = O(7, k) * rc;
+    const T ke_a = cur.ke, xu_a = cur.xu, xv_a = cur.xv;
+    const T yu_a = cur.yu, yv_a = cur.yv;
     T up_a = 0, vp_a = 0, vor_a = 0, d_a = 0, tp_a = 0, qp_a = 0, tv_a = 0, padv_a = 0;
     const T tdev = bt - tr, tvdev = btv - tr;
     // temperature: adiabatic term -kappa (tv x5 + tv_bar x5'), energy
@@ -278,8 +307,6 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
     // levels k and k+1
     T sd_a = 0;
     if (!top) {
-      const T bun = B(br.u() + k + 1), bvn = B(br.v() + k + 1);
-      const T btn = B(br.t() + k + 1), bqn = B(br.q() + k + 1);
       const T r1 = (T)1 / vc[vo.dsig() + k + 1];
       sd_a = h * ((bun - bu) * (wk[0] * rds + wn[0] * r1) + (bvn - bv) * (wk[1] * rds + wn[1] * r1) +
                   (btn - bt) * (wk[2] * rds + wn[2] * r1) + (bqn - bq) * (wk[3] * rds + wn[3] * r1));
@@ -310,6 +337,12 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
       wn[q] = wk[q];
       wk[q] = wp[q];
     }
+    bun = bu;
+    bvn = bv;
+    btn = bt;
+    bqn = bq;
+    bsd_hi = bsd_lo;
+    cur = nxt;
   }
   // gt_pi = D_0 - S_0;  S_0 = sum D + bpix sum u + bpiy sum v + pix sbu + piy sbv
   const T gpi_a = go[(size_t)(9 * L) * I];
