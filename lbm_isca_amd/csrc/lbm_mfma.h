@@ -417,29 +417,42 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
   const int j_b = lane & 15;
   const int c_b = j_b & 7, part_b = j_b >> 3;
   const int col = col0 + c_b;
-  if (col >= ncols) return;
+  if (col >= ncols) return;  // the partner lane (lane ^ 8) has the same column
+  // Lanes j and j ^ 8 hold the real and imaginary parts of one column: they
+  // swap halves so that each stores whole complex values, the real-part
+  // lanes rows v0, the imaginary-part lanes rows v0 + 1.
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int jp = 32 * blockIdx.z + 16 * rt + Mfma<T>::row(lane, v);
-    if (jp >= Jh) continue;
-    const T sc = rcos[jp];
+  for (int v0 = 0; v0 < 4; v0 += 2) {
+    const int jp0 = 32 * blockIdx.z + 16 * rt + Mfma<T>::row(lane, v0);
+    const int jp1 = 32 * blockIdx.z + 16 * rt + Mfma<T>::row(lane, v0 + 1);
+    const T sc0 = jp0 < Jh ? rcos[jp0] : (T)0, sc1 = jp1 < Jh ? rcos[jp1] : (T)0;
+    const int jp = part_b == 0 ? jp0 : jp1;  // the row this lane stores
 #pragma unroll
     for (int o = G; o < NOUT; o += 2) {
-      T vn = EA[o][v] + OA[o][v] + EB[o][v] + OB[o][v];
-      T vs = EA[o][v] - OA[o][v] + OB[o][v] - EB[o][v];
+      T n0 = EA[o][v0] + OA[o][v0] + EB[o][v0] + OB[o][v0];
+      T s0 = EA[o][v0] - OA[o][v0] + OB[o][v0] - EB[o][v0];
+      T n1 = EA[o][v0 + 1] + OA[o][v0 + 1] + EB[o][v0 + 1] + OB[o][v0 + 1];
+      T s1 = EA[o][v0 + 1] - OA[o][v0 + 1] + OB[o][v0 + 1] - EB[o][v0 + 1];
       if (R::scaled(o)) {
-        vn *= sc;
-        vs *= sc;
+        n0 *= sc0;
+        s0 *= sc0;
+        n1 *= sc1;
+        s1 *= sc1;
       }
-      T *pn = reinterpret_cast<T *>(out + fidx(m, jp, slots.off[o] + col, J, C)) + part_b;
-      T *ps = reinterpret_cast<T *>(out + fidx(m, J - 1 - jp, slots.off[o] + col, J, C)) + part_b;
+      const T gn = __shfl_xor(part_b == 0 ? n1 : n0, 8);
+      const T gs = __shfl_xor(part_b == 0 ? s1 : s0, 8);
+      if (jp >= Jh) continue;
+      const cx<T> zn = part_b == 0 ? mk(n0, gn) : mk(gn, n1);
+      const cx<T> zs = part_b == 0 ? mk(s0, gs) : mk(gs, s1);
+      cx<T> *pn = out + fidx(m, jp, slots.off[o] + col, J, C);
+      cx<T> *ps = out + fidx(m, J - 1 - jp, slots.off[o] + col, J, C);
       if constexpr (sizeof(T) == 8) {
-        // 8-byte halves stream past the caches (measured: 4-byte ones lose)
-        st_nt(pn, vn);
-        st_nt(ps, vs);
+        // streaming past the caches (measured: FP32 loses)
+        st_nt(pn, zn);
+        st_nt(ps, zs);
       } else {
-        *pn = vn;
-        *ps = vs;
+        *pn = zn;
+        *ps = zs;
       }
     }
   }
