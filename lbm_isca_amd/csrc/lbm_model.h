@@ -524,9 +524,13 @@ void k_step(StepArgs<T> a) {
   const T rg = (T)R_DRY, kap = (T)KAPPA, gam = a.robert;
   cx<T> *s2 = a.sa2 + (size_t)s * 3 * E + e;
 
-  // two level arrays (x, dbar): every mat-vec row goes straight to the
-  // Robert-Asselin update and to memory, which keeps two waves per SIMD
-  cx<T> x[L], dbar[L];
+  // Level arrays in registers: x, and dbar unless DB_MEM.  Every mat-vec
+  // row goes straight to the Robert-Asselin update and to memory.  DB_MEM
+  // (FP32, 40 levels, measured faster there only): dbar goes row by row into
+  // the divergence-tendency slot (its last read is the rhs) and is read back
+  // where used, one level array fewer for a round trip through the cache.
+  constexpr bool DB_MEM = sizeof(T) == 4 && L == 40;
+  cx<T> x[L], dbar[DB_MEM ? 1 : L];
   const RU vinv = (RU)(a.vinv + (size_t)set * 4 * L * L);
   const bool vd_low = !(a.skip_low_n && n <= 1);
   // The reference keeps winds, not vorticity and divergence, on its grid:
@@ -587,13 +591,30 @@ void k_step(StepArgs<T> a) {
     x[l] = (((T)1 + dt * kd[l]) * ft) * bsb.ld(voff, lo(1, l), tag) + dt * (ft * btn.ld(voff, lo(1, l), tag) - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
-  matvec<T, L>((RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L), x, dbar);
+  cx<T> sdb = zero<T>();
+  {
+    const RP mi = (RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      cx<T> acc = zero<T>();
+#pragma unroll
+      for (int c = 0; c < L; ++c) fma_acc(acc, mi[r * L + c], x[c]);
+      if constexpr (DB_MEM)
+        btn.st(voff, lo(1, r), acc);
+      else
+        dbar[r] = acc;
+      sdb = sdb + vc[vo.dsig() + r] * acc;
+    }
+  }
+  auto DB = [&](int l) __attribute__((always_inline)) {
+    if constexpr (DB_MEM)
+      return btn.ld(voff, lo(1, l), tag);
+    else
+      return dbar[l];
+  };
 
   // surface pressure (integrator.py:301, 308)
   {
-    cx<T> sdb = zero<T>();
-#pragma unroll
-    for (int l = 0; l < L; ++l) sdb = sdb + vc[vo.dsig() + l] * dbar[l];
     const cx<T> pin = pi_prev + dt2 * (wt_pi - sdb);
     if (a.sb2) {
       const cx<T> cur = s2[0];
@@ -603,7 +624,7 @@ void k_step(StepArgs<T> a) {
   }
   // divergence
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)2 * dbar[l] - bsb.ld(voff, lo(1, l), tag);
+  for (int l = 0; l < L; ++l) x[l] = (T)2 * DB(l) - bsb.ld(voff, lo(1, l), tag);
   update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E, drop_mean);
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
@@ -611,8 +632,9 @@ void k_step(StepArgs<T> a) {
 #pragma unroll
     for (int l = L - 1; l >= 0; --l) {
       const T ft = (T)1 + dt2 * ktt[l];
-      const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * dbar[l] + vc[vo.w3() + l] * suf);
-      suf = suf + vc[vo.dsig() + l] * dbar[l];
+      const cx<T> db = DB(l);
+      const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * db + vc[vo.w3() + l] * suf);
+      suf = suf + vc[vo.dsig() + l] * db;
       x[l] = (T)(1) / ft * (bsb.ld(voff, lo(2, l), tag) + dt2 * (btn.ld(voff, lo(2, l), tag) - tmd));
     }
   }
