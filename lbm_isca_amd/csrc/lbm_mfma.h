@@ -105,7 +105,7 @@ constexpr bool any_h() {
 }
 
 template <class T, class R, bool SPLIT = false>
-__global__ __launch_bounds__(256) void k_ana_mfma(const cx<T> *__restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Pa,
                                                   const T *__restrict__ Ha,
