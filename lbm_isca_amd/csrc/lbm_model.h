@@ -516,6 +516,7 @@ void k_step(StepArgs<T> a) {
     return (unsigned)(((size_t)f * EL + (size_t)l * E) * sizeof(cx<T>));
   };
   const cx<T> *const tag = nullptr;  // selects the precision of BufView::ld
+  // tendencies: non-temporal at their last read (streamed once per step)
   const cx<T> wt_pi = a.tend2[(size_t)s * 2 * E + e];
   // pi at t-dt: the analysed log1p(ps'_b/ps_bar), or the spectral level
   // itself in tangent-linear mode
@@ -561,12 +562,12 @@ void k_step(StepArgs<T> a) {
   if (role == 1) {
     // vorticity
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (bsb.ld(voff, lo(0, l), tag) + dt2 * btn.ld(voff, lo(0, l), tag));
+    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (bsb.ld(voff, lo(0, l), tag) + dt2 * btn.ld_nt(voff, lo(0, l), tag));
     update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
     // humidity
 #pragma unroll
     for (int l = 0; l < L; ++l)
-      x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (bsb.ld(voff, lo(3, l), tag) + dt2 * btn.ld(voff, lo(3, l), tag));
+      x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (bsb.ld(voff, lo(3, l), tag) + dt2 * btn.ld_nt(voff, lo(3, l), tag));
     update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
     return;
   }
@@ -588,7 +589,7 @@ void k_step(StepArgs<T> a) {
     const T ft = (T)1 + dt2 * ktt[l];
     const T gg = rg * tref[l] * vc[vo.cpg() + l];
     const cx<T> phy = x[l] + (ft * gg) * wdpix;
-    x[l] = (((T)1 + dt * kd[l]) * ft) * bsb.ld(voff, lo(1, l), tag) + dt * (ft * btn.ld(voff, lo(1, l), tag) - lapn * phy);
+    x[l] = (((T)1 + dt * kd[l]) * ft) * bsb.ld(voff, lo(1, l), tag) + dt * (ft * btn.ld_nt(voff, lo(1, l), tag) - lapn * phy);
   }
   // dbar = M(n)^-1 rhs (integrator.py:165-172)
   cx<T> sdb = zero<T>();
@@ -635,7 +636,7 @@ void k_step(StepArgs<T> a) {
       const cx<T> db = DB(l);
       const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * db + vc[vo.w3() + l] * suf);
       suf = suf + vc[vo.dsig() + l] * db;
-      x[l] = (T)(1) / ft * (bsb.ld(voff, lo(2, l), tag) + dt2 * (btn.ld(voff, lo(2, l), tag) - tmd));
+      x[l] = (T)(1) / ft * (bsb.ld(voff, lo(2, l), tag) + dt2 * (btn.ld_nt(voff, lo(2, l), tag) - tmd));
     }
   }
   update(2, vinv + 2 * L * L, (a.vmask & 4) != 0, nullptr, false);
