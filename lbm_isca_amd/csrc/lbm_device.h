@@ -290,10 +290,12 @@ struct FftShape {
   static constexpr int NC = NI / 2;
   static constexpr int TPR = 8 * R;             // threads per row
   // rows per block: consecutive rows are consecutive Fourier columns, so a
-  // block's accesses to one coefficient m form ROWS x sizeof(cx<T>) bytes;
-  // at 384 points the LDS per row (3.4 KB FP64) bounds the blocks per CU
-  // (measured best: 20 rows FP64, 32 rows FP32)
-  static constexpr int ROWS = R == 1 ? 32 : (sizeof(T) == 8 ? 20 : 32);
+  // block's accesses to one coefficient m form runs of 8 x sizeof(cx<T>)
+  // bytes; the LDS per row (1.2 KB FP64 at 128 points, 3.4 KB at 384)
+  // bounds the blocks per CU (measured best: 128 points 16 rows FP64 (fewer
+  // waves per barrier at the same occupancy), 32 FP32; 384 points 20 rows
+  // FP64, 32 FP32)
+  static constexpr int ROWS = R == 1 ? (sizeof(T) == 8 ? 16 : 32) : (sizeof(T) == 8 ? 20 : 32);
   static constexpr int THREADS = ROWS * TPR;
   static constexpr int LDS = 8 * 9 * R;         // complex scratch per row (>= NC)
 };
