@@ -468,7 +468,7 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
 template <class T, int NI>
 int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
                   ActiveMask am) {
-  using S = FftShape<T, NI>;
+  using S = FftShape<T, NI, true>;
   const int C = cm.ncols();
   const long rows = fft_rows(p->J, C, am);
   hipLaunchKernelGGL((k_ifft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),

@@ -284,18 +284,18 @@ struct ActiveMask {
 // uses 8R threads; then the real split to/from X_m = rfft/nlon.  Stages are
 // separated by block barriers placed by the kernels.
 // ---------------------------------------------------------------------------
-template <class T, int NI>
+template <class T, int NI, bool INV = false>
 struct FftShape {
   static constexpr int R = NI / 128;
   static constexpr int NC = NI / 2;
   static constexpr int TPR = 8 * R;             // threads per row
-  // rows per block: consecutive rows are consecutive Fourier columns, so a
-  // block's accesses to one coefficient m form runs of 8 x sizeof(cx<T>)
-  // bytes; the LDS per row (1.2 KB FP64 at 128 points, 3.4 KB at 384)
-  // bounds the blocks per CU (measured best: 128 points 16 rows FP64 (fewer
-  // waves per barrier at the same occupancy), 32 FP32; 384 points 20 rows
-  // FP64, 32 FP32)
-  static constexpr int ROWS = R == 1 ? (sizeof(T) == 8 ? 16 : 32) : (sizeof(T) == 8 ? 20 : 32);
+  // rows per block (INV: the inverse transform): consecutive rows are
+  // consecutive Fourier columns, so a block's accesses to one coefficient m
+  // form runs of 8 x sizeof(cx<T>) bytes; the LDS per row (1.2 KB FP64 at
+  // 128 points, 3.4 KB at 384) bounds the blocks per CU.  Measured best:
+  // FP64 16 rows at 128 points (fewer waves per barrier at the same
+  // occupancy), 10 (inverse) and 20 (forward) at 384; FP32 32 rows.
+  static constexpr int ROWS = sizeof(T) == 4 ? 32 : (R == 1 ? 16 : (INV ? 10 : 20));
   static constexpr int THREADS = ROWS * TPR;
   static constexpr int LDS = 8 * 9 * R;         // complex scratch per row (>= NC)
 };
@@ -506,11 +506,11 @@ __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, i
 
 // Fourier (m <= M-1) -> grid; rows are j*C + col.  mode: fourier_scale.
 template <class T, int NI>
-__global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_ifft(const cx<T> *__restrict__ F,
+__global__ __launch_bounds__((FftShape<T, NI, true>::THREADS)) void k_ifft(const cx<T> *__restrict__ F,
                                                                 T *__restrict__ G, int C, ColMap cm,
                                                                 int J, int M, int mode,
                                                                 ActiveMask am) {
-  using S = FftShape<T, NI>;
+  using S = FftShape<T, NI, true>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
