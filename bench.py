@@ -307,6 +307,20 @@ def measure_t127(E=32, days=20.0):
                     "dense peak of the precision"}
 
 
+def calibrate_copy(mib, reps=3):
+    """A plain device copy of known size: each repetition reads and writes
+    ``mib`` MiB once, so FETCH_SIZE / WRITE_SIZE of these dispatches give the
+    factors that turn the counters into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x {reps}")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +343,9 @@ def main():
                     help="response averaging window in days")
     ap.add_argument("--t127-members", type=int, default=32)
     ap.add_argument("--t127-days", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
+                    help="only copy a float64 buffer of MIB mebibytes three times (the "
+                         "reference kernel that scales the HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); members are "
@@ -339,6 +356,9 @@ def main():
     from lbm_isca_amd import roofline
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a HIP device")
+    if args.calibrate_copy is not None:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.only is not None:
         if args.only == "uneven":
             res = measure_fp32_vs_fp64(args.uneven_members, args.max_days)

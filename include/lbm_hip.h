@@ -33,7 +33,7 @@
 
 /* Version of the entry points' signatures below; a binding checks it
  * against lbm_abi_version() of the library it loads. */
-#define LBM_ABI_VERSION 4
+#define LBM_ABI_VERSION 5
 
 #ifdef __cplusplus
 extern "C" {
@@ -95,8 +95,10 @@ int lbm_plan_destroy(lbm_plan *plan);
 /* Host copies of setup arrays: "mu", "gw", "coslat", "lat", "lon", "sig",
  * "dsig", "alpha", "cpg", "w3", "wP", "dsigm", "ww" (nlev*nlev), "dlog3",
  * "sigm", "tbar" (nbasic*nlev, after lbm_set_basic_state), "minv"
- * ((inistp+1)*nbasic*(nmax+1)*nlev*nlev: one block per start-up dt, then
- * delt), "vinv" ((inistp+1)*4*nlev*nlev), "lop" (nlev*nlev).
+ * ((inistp+2)*nbasic*(nmax+1)*nlev*nlev: one block per start-up dt, then
+ * delt, then the free dt of lbm_prepare_dt), "vinv" ((inistp+2)*4*nlev*nlev),
+ * "mmat" (nbasic*(nmax+1)*nlev*nlev: M(n) of the free dt before inversion,
+ * SemiImplicit.m of integrator.py:149-158), "lop" (nlev*nlev).
  * n is the capacity of h_out in doubles; returns the count written. */
 int lbm_plan_get(const lbm_plan *plan, const char *name, double *h_out, int64_t n);
 
@@ -167,6 +169,75 @@ int lbm_set_spectral_state(lbm_plan *plan, const double *d_vor, const double *d_
                            const double *d_t, const double *d_q, const double *d_pi,
                            void *stream);
 
+/* set_initial_perturbation exactly (model.py:142-148): each given field
+ * replaces both time levels, NULL fields keep each level's current values
+ * (all NULL: nothing changes).  Resets the clock like lbm_set_state. */
+int lbm_set_state_keep(lbm_plan *plan, const double *d_u, const double *d_v, const double *d_t,
+                       const double *d_q, const double *d_ps, void *stream);
+
+/* The filtered previous time level gb (model.py:128-131) on the grid, in
+ * lbm_get_state's layouts: ps' as stored, u, v, t, q synthesised from the
+ * level's spectral form (the raw grid of lbm_set_state while both levels
+ * still hold it). */
+int lbm_get_prev_state(lbm_plan *plan, double *d_u, double *d_v, double *d_t, double *d_q,
+                       double *d_ps, void *stream);
+
+/* Dynamics.tendencies (dynamics.py:156-271) of a grid state u, v, t, q
+ * (nmem,J,I,L) and ps' (nmem,J,I) hPa, any NULL = zero, with the plan's
+ * basic state and forcings (lbm_set_forcing's temperature forcing, the
+ * steady vorticity / divergence / ln ps forcings; with an MJO forcing the
+ * field of the next dynamics call): the spectral tendencies of vor, div,
+ * t, q (nmem,M,M,L) and pi (nmem,M,M) complex128, any output NULL.  The
+ * run's state is untouched.  Not in the tangent-linear model. */
+int lbm_tendencies(lbm_plan *plan, const double *d_u, const double *d_v, const double *d_t,
+                   const double *d_q, const double *d_ps, double *d_tvor, double *d_tdiv,
+                   double *d_tt, double *d_tq, double *d_tpi, void *stream);
+
+/* SemiImplicit.prepare + VerticalDiffusion.prepare (integrator.py:130-163,
+ * 216-228) for a free time step dt: FP64 M(n) and its inverse per basic
+ * state, and the vertical-diffusion inverses, in the plan's free matrix
+ * set.  Needs the basic state. */
+int lbm_prepare_dt(lbm_plan *plan, double dt);
+
+/* Integrator.step (integrator.py:264-315): from the spectral level at t-dt
+ * (d_p*), the explicit tendencies (d_t*) and dt, the level at t+dt (d_n*);
+ * vor, div, t, q (nmem,M,M,L), pi (nmem,M,M) complex128.  A dt that is none
+ * of the start-up steps, delt or the last prepared one is prepared first.
+ * No Robert-Asselin filter (the caller's State does that, state.py:110-128);
+ * the run's state is untouched. */
+int lbm_step(lbm_plan *plan, double dt, const double *d_pvor, const double *d_pdiv,
+             const double *d_pt, const double *d_pq, const double *d_ppi, const double *d_tvor,
+             const double *d_tdiv, const double *d_tt, const double *d_tq, const double *d_tpi,
+             double *d_nvor, double *d_ndiv, double *d_nt, double *d_nq, double *d_npi,
+             void *stream);
+
+/* Element-wise operators of State / Spectral / VerticalGrid on caller
+ * buffers (float64):
+ *   lbm_robert_asselin : prev <- cur + gamma (prev - 2 cur + new), n values
+ *                        (State.robert_asselin, state.py:110-128)
+ *   lbm_lnps           : inverse = 0: out = ln(ps_bar + in) - ln ps_bar;
+ *                        1: out = ps_bar (exp(in) - 1); ps_bar has nper values
+ *                        repeated along the leading axis (state.py:77-83)
+ *   lbm_grid_stats     : h_out[0] = max |x| over finite values, h_out[1] = the
+ *                        number of non-finite values (State.max_abs /
+ *                        isfinite, state.py:131-142); synchronises the stream
+ *   lbm_spec_scale     : (B,M,M,nk) complex times -n(n+1)/a^2 (kind 0) or
+ *                        -a^2/(n(n+1)), 0 at n = 0 (kind 1) (spectral.py:202-208)
+ *   lbm_hydrostatic    : phi = x WW^T along the level axis of (npts, nlev,
+ *                        inner) values, inner = 1 real or 2 complex, with
+ *                        WW = R (diag alpha + strictly lower dlog3 columns)
+ *                        from d_alpha, d_dlog3 (nlev each, device)
+ *                        (VerticalGrid.hydrostatic, vertical.py:196-203) */
+int lbm_robert_asselin(int64_t n, double *d_prev, const double *d_cur, const double *d_new,
+                       double gamma, void *stream);
+int lbm_lnps(int64_t n, int64_t nper, const double *d_in, const double *d_psbar, double *d_out,
+             int inverse, void *stream);
+int lbm_grid_stats(int64_t n, const double *d_x, double *h_out, void *stream);
+int lbm_spec_scale(lbm_plan *plan, const double *d_in, double *d_out, int B, int nk, int kind,
+                   void *stream);
+int lbm_hydrostatic(const double *d_in, double *d_out, int64_t npts, int nlev, int inner,
+                    const double *d_alpha, const double *d_dlog3, void *stream);
+
 /* Per-member number of full steps (excluding start-up), NULL = all equal to
  * nsteps.  Members with fewer steps stop advancing when they reach theirs. */
 int lbm_set_lengths(lbm_plan *plan, const int32_t *h_nsteps, void *stream);
@@ -233,6 +304,17 @@ int lbm_set_mjo_forcing(lbm_plan *plan, const double *d_gtf, int nf, int ndays, 
  * lbm_set_forcing's / lbm_set_mjo_forcing's. */
 int lbm_set_steady_forcing(lbm_plan *plan, const double *d_vor, const double *d_div,
                            const double *d_ps, int nf, void *stream);
+
+/* The same run advanced by count executed steps (start-up steps included),
+ * for a forcing whose .step() changes every call: the caller replaces the
+ * forcing (lbm_set_forcing, same member count) before each step, as the
+ * LBM.run loop calls mjo_forcing.step() (model.py:222-229). */
+int lbm_run_steps(lbm_plan *plan, int nsteps, int count, int use_graph, int check_every,
+                  void *stream);
+
+/* Ends the run in progress without finishing it (e.g. after a writer
+ * failed between segments); the next run restarts the start-up steps. */
+int lbm_run_abort(lbm_plan *plan);
 
 /* Full steps done by the run in progress (0 when none). */
 int lbm_run_segment_done(const lbm_plan *plan, int32_t *full_steps);

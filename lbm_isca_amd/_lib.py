@@ -22,7 +22,7 @@ LBM_FP32 = 1
 LBM_BF16X3 = 2
 
 # LBM_ABI_VERSION of include/lbm_hip.h these bindings are written for
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # Every symbol include/lbm_hip.h declares.
 EXPORTS = (
@@ -36,7 +36,9 @@ EXPORTS = (
     "lbm_psichi_to_uv_adj", "lbm_hgrad_adj",
     "lbm_run_to", "lbm_set_history", "lbm_get_history", "lbm_get_diagnostics",
     "lbm_set_mjo_forcing", "lbm_run_segment_done", "lbm_spec_resize", "lbm_abi_version",
-    "lbm_set_steady_forcing",
+    "lbm_set_steady_forcing", "lbm_set_state_keep", "lbm_get_prev_state", "lbm_tendencies",
+    "lbm_prepare_dt", "lbm_step", "lbm_robert_asselin", "lbm_lnps", "lbm_grid_stats",
+    "lbm_spec_scale", "lbm_hydrostatic", "lbm_run_steps", "lbm_run_abort",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -114,6 +116,20 @@ def load():
                             ctypes.c_int),
         "lbm_abi_version": ([], ctypes.c_int),
         "lbm_set_steady_forcing": ([P, P, P, P, ctypes.c_int, P], ctypes.c_int),
+        "lbm_set_state_keep": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_get_prev_state": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "lbm_tendencies": ([P] + [P] * 5 + [P] * 5 + [P], ctypes.c_int),
+        "lbm_prepare_dt": ([P, ctypes.c_double], ctypes.c_int),
+        "lbm_step": ([P, ctypes.c_double] + [P] * 15 + [P], ctypes.c_int),
+        "lbm_robert_asselin": ([ctypes.c_int64, P, P, P, ctypes.c_double, P], ctypes.c_int),
+        "lbm_lnps": ([ctypes.c_int64, ctypes.c_int64, P, P, P, ctypes.c_int, P], ctypes.c_int),
+        "lbm_grid_stats": ([ctypes.c_int64, P, _dp, P], ctypes.c_int),
+        "lbm_spec_scale": ([P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
+        "lbm_hydrostatic": ([P, P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P],
+                            ctypes.c_int),
+        "lbm_run_steps": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
+                          ctypes.c_int),
+        "lbm_run_abort": ([P], ctypes.c_int),
     }
     if not hasattr(lib, "lbm_abi_version") or lib.lbm_abi_version() != ABI_VERSION:
         found = lib.lbm_abi_version() if hasattr(lib, "lbm_abi_version") else "none"
@@ -153,6 +169,12 @@ def ptr(t):
     if t is None:
         return None
     return ctypes.c_void_p(t.data_ptr())
+
+
+def stream(s=None):
+    """``hipStream_t`` of torch's current stream as a ctypes pointer."""
+    import torch
+    return ctypes.c_void_p((s or torch.cuda.current_stream()).cuda_stream)
 
 
 def dptr(a):

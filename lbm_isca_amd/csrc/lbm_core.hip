@@ -20,6 +20,7 @@
 #include "../../include/lbm_hip.h"
 #include "lbm_adjoint.h"
 #include "lbm_history.h"
+#include "lbm_util.h"
 
 using namespace lbm;
 
@@ -219,6 +220,9 @@ struct lbm_plan {
   std::vector<double> kv, kd, ktt, kq;
   std::vector<double> vdif[4];
   std::vector<double> tbar, minv, vinv;
+  // M(n) before inversion for the free time step (set inistp + 1), per basic state
+  std::vector<double> mmat;
+  double user_dt = 0;  // time step of matrix set inistp + 1 (lbm_prepare_dt)
   std::vector<short> n_of_s;
   // device
   DevTabs<double> d64;
@@ -259,9 +263,22 @@ struct lbm_plan {
   int *pend = nullptr, *d_nsteps = nullptr;
   std::vector<int> nsteps_h;
   bool basic_set = false, have_raw = false, raw_pending = false;
+  // both time levels hold the same state (set by lbm_set_state or
+  // lbm_set_spectral_state, until a step or a partial update separates them)
+  bool levels_equal = true;
   long long steps_total = 0;
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // a second copy of the prognostic state for the stand-alone tendency and
+  // step entry points, which must leave the run's state untouched
+  void *x_sa3 = nullptr, *x_sa2 = nullptr, *x_sb3 = nullptr, *x_sb2 = nullptr;
+  void *x_psa = nullptr, *x_psb = nullptr, *x_raw = nullptr;
+  // raw grid (u, v, t, q) of the previous level after a partial update
+  // (lbm_set_state_keep) that left the two levels different
+  void *rawb = nullptr;
+  int *x_pend = nullptr;
+  Ctl *x_ctl = nullptr;
+
   hipStream_t own = nullptr;
   // second stream of the time step (launches beside the level kernels) and
   // its fork/join events: fft done, surface synthesis, MJO, history sums,
@@ -693,13 +710,17 @@ int build_vertical(lbm_plan *p, const double *sigm) {
 // operators and the FP64 semi-implicit inverses on the host.
 // ---------------------------------------------------------------------------
 double step_dt(const lbm_plan *p, int set) {
+  if (set > p->inistp) return p->user_dt;
   return set < p->inistp ? p->delt / std::ldexp(1.0, p->inistp - set) : p->delt;
 }
 
-int build_matrices(lbm_plan *p) {
+// M(n) and its FP64 inverse for matrix sets lo..hi-1 (all by default)
+int build_matrices(lbm_plan *p, int lo = 0, int hi = -1) {
   const int M = p->M, L = p->L, nb = p->nb, nset = p->nset;
-  p->minv.assign((size_t)nset * nb * M * L * L, 0.0);
-  for (int set = 0; set < nset; ++set) {
+  if (hi < 0) hi = nset;
+  p->minv.resize((size_t)nset * nb * M * L * L, 0.0);
+  p->mmat.resize((size_t)nb * M * L * L, 0.0);
+  for (int set = lo; set < hi; ++set) {
     const double dt = step_dt(p, set), dt2 = 2.0 * dt;
     for (int b = 0; b < nb; ++b) {
       const double *tb = &p->tbar[(size_t)b * L];
@@ -727,6 +748,8 @@ int build_matrices(lbm_plan *p) {
             mat[(size_t)r * L + c] = -dt * dt * lap * (wh[(size_t)r * L + c] + gg[r] * p->dsig[c] * ft);
           mat[(size_t)r * L + r] += fd * ft;
         }
+        if (set == nset - 1)
+          std::copy(mat.begin(), mat.end(), p->mmat.begin() + ((size_t)b * M + n) * L * L);
         if (!invert(mat, L))
           return fail(LBM_ERR_VALUE, "singular semi-implicit matrix (n=%d, basic state %d)", n, b);
         std::copy(mat.begin(), mat.end(), p->minv.begin() + (((size_t)set * nb + b) * M + n) * L * L);
@@ -737,13 +760,14 @@ int build_matrices(lbm_plan *p) {
 }
 
 // (I - dt nu lop)^-1 per variable and dt (integrator.py:216-228)
-int build_vdiff(lbm_plan *p) {
+int build_vdiff(lbm_plan *p, int lo = 0, int hi = -1) {
   const int L = p->L, nset = p->nset;
-  p->vinv.assign((size_t)nset * 4 * L * L, 0.0);
+  if (hi < 0) hi = nset;
+  p->vinv.resize((size_t)nset * 4 * L * L, 0.0);
   p->vmask = 0;
   for (int f = 0; f < 4; ++f)
     if (!p->vdif[f].empty()) p->vmask |= 1 << f;
-  for (int set = 0; set < nset; ++set) {
+  for (int set = lo; set < hi; ++set) {
     const double dt = step_dt(p, set);
     for (int f = 0; f < 4; ++f) {
       std::vector<double> mat((size_t)L * L, 0.0);
@@ -986,8 +1010,8 @@ HistArgs<T> hist_args(const lbm_plan *p) {
   return h;
 }
 
-template <class T, int L>
-int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
+template <class T>
+StepArgs<T> step_args(lbm_plan *p, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
   StepArgs<T> a;
   a.sa3 = (cx<T> *)p->sa3;
@@ -1019,6 +1043,14 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
   a.ecnt = ecnt;
+  a.fixed_set = -1;
+  a.fixed_dt = 0;
+  return a;
+}
+
+template <class T, int L>
+int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
+  const StepArgs<T> a = step_args<T>(p, ecnt);
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step
   const dim3 grid((unsigned)((nthr + 63) / 64), 2);
@@ -1041,8 +1073,10 @@ inline int mark(lbm_plan *p, int k, hipStream_t st) {
 // state (k_tend reads the raw grid).
 // ecnt: leading members covered by the launches, or 0 when every member is
 // live through the step (no activity mask)
+// tend_only: stop after the analysed tendencies (tend3, tend2) of the
+// current state, with no history sums and no spectral step.
 template <class T>
-int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
+int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_only = false) {
   const int E = p->E, L = p->L, EL = E * L;
   DevTabs<T> &t = p->tabs<T>();
   const ColMap cmi{E, L, 6, 6};
@@ -1073,7 +1107,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
     if (fork) HIPCHK(hipStreamWaitEvent(st, p->fj[k], 0));
     return LBM_OK;
   };
-  const bool hist = p->hist_mode != 0 || p->keep_last;
+  const bool hist = !tend_only && (p->hist_mode != 0 || p->keep_last);
   auto launch_mjo = [&]() -> int {
     // this call's MJO temperature forcing (forcing.py:113-156)
     const size_t n = (size_t)p->J * L * p->I;
@@ -1166,6 +1200,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt) {
                        (const cx<T> *)p->sfp, E, L, p->nspec, p->nsf, ne);
     HIPCHK(hipGetLastError());
   }
+  if (tend_only) return LBM_OK;
   if (!fork && hist) CHK(launch_hist());
   if (hist) CHK(join(F_HIST));
   CHK(mark(p, 7, st));
@@ -1248,6 +1283,266 @@ int final_check(lbm_plan *p, hipStream_t st) {
                      (const cx<T> *)p->sa2, n2, p->ctl);
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stand-alone tendencies and step (Dynamics.tendencies, Integrator.step) on
+// caller buffers, the previous time level on the grid and the partial state
+// update of set_initial_perturbation.
+// ---------------------------------------------------------------------------
+int ensure_aux(lbm_plan *p) {
+  if (p->x_sa3) return LBM_OK;
+  const size_t rs = p->rsize(), cs = 2 * rs, E = p->E, L = p->L, J = p->J, I = p->I;
+  const size_t EL = E * L, ns = p->nspec;
+  struct A {
+    void **ptr;
+    size_t bytes;
+  } al[] = {{&p->x_sa3, cs * ns * 4 * EL}, {&p->x_sb3, cs * ns * 4 * EL},
+            {&p->x_sa2, cs * ns * 3 * E},  {&p->x_psa, rs * E * J * I},
+            {&p->x_psb, rs * E * J * I},   {&p->x_raw, rs * E * J * 4 * L * I},
+            {(void **)&p->x_pend, sizeof(int) * E}, {(void **)&p->x_ctl, sizeof(Ctl)}};
+  for (auto &a : al) HIPCHK(hipMalloc(a.ptr, std::max<size_t>(a.bytes, 16)));
+  if (p->linear) HIPCHK(hipMalloc(&p->x_sb2, cs * ns * 3 * E));
+  return LBM_OK;
+}
+
+// exchanges the run's prognostic buffers with the second copy
+void swap_state(lbm_plan *p) {
+  std::swap(p->sa3, p->x_sa3);
+  std::swap(p->sb3, p->x_sb3);
+  std::swap(p->sa2, p->x_sa2);
+  std::swap(p->sb2, p->x_sb2);
+  std::swap(p->psa, p->x_psa);
+  std::swap(p->psb, p->x_psb);
+  std::swap(p->raw, p->x_raw);
+  std::swap(p->pend, p->x_pend);
+  std::swap(p->ctl, p->x_ctl);
+}
+
+// (vor, div, t, q) of a [s][nf3][L][E] array and pi (field f2 of a
+// [s][nf2][E] array) to the reference's spectral layout; null outputs skipped
+template <class T>
+int unpack_spec5(lbm_plan *p, const void *s3, int nf3, const void *s2, int nf2, int f2,
+                 double *const out[5], hipStream_t st) {
+  const int E = p->E, M = p->M, L = p->L;
+  const size_t n3 = (size_t)E * M * M * L, n2 = (size_t)E * M * M;
+  for (int f = 0; f < 4; ++f)
+    if (out[f])
+      hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk(n3)), dim3(256), 0, st, (const cx<T> *)s3,
+                         out[f], E, M, L, nf3, f, E * L);
+  if (out[4])
+    hipLaunchKernelGGL((k_unpack_spec<T>), dim3(nblk(n2)), dim3(256), 0, st, (const cx<T> *)s2,
+                       out[4], E, M, 1, nf2, f2, E);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+template <class T>
+int pack_spec5(lbm_plan *p, const double *const in[5], void *s3, int nf3, void *s2, int nf2, int f2,
+               hipStream_t st) {
+  const int E = p->E, M = p->M, L = p->L;
+  const size_t n3 = (size_t)p->nspec * E * L, n2 = (size_t)p->nspec * E;
+  for (int f = 0; f < 4; ++f)
+    hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(n3)), dim3(256), 0, st, in[f], (cx<T> *)s3, E, M,
+                       L, nf3, f, E * L);
+  hipLaunchKernelGGL((k_pack_spec<T>), dim3(nblk(n2)), dim3(256), 0, st, in[4], (cx<T> *)s2, E, M, 1,
+                     nf2, f2, E);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// Dynamics.tendencies (dynamics.py:156-271) of a caller grid state with the
+// plan's forcings: the same set-up and kernels as the first step after a
+// raw initial perturbation, on the second copy of the state.
+template <class T>
+int tendencies_t(lbm_plan *p, const double *u, const double *v, const double *t, const double *q,
+                 const double *ps, double *const out[5], hipStream_t st) {
+  CHK(ensure_aux(p));
+  const bool have_raw = p->have_raw, raw_pending = p->raw_pending, keep_last = p->keep_last;
+  const int hist_mode = p->hist_mode;
+  swap_state(p);
+  p->hist_mode = 0;
+  p->keep_last = false;
+  int rc = set_state_t<T>(p, u, v, t, q, ps, st);
+  Ctl c0{0, p->have_raw ? 1 : 0, INT_MAX, 0};
+  if (rc == LBM_OK && (hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st) != hipSuccess ||
+                       hipStreamSynchronize(st) != hipSuccess))
+    rc = fail(LBM_ERR_RUNTIME, "control block upload failed");
+  if (rc == LBM_OK) rc = one_step<T>(p, st, true, 0, true);
+  if (rc == LBM_OK) rc = unpack_spec5<T>(p, p->tend3, 4, p->tend2, 2, 0, out, st);
+  swap_state(p);
+  p->have_raw = have_raw;
+  p->raw_pending = raw_pending;
+  p->hist_mode = hist_mode;
+  p->keep_last = keep_last;
+  return rc;
+}
+
+// matrix set of a time step: a start-up dt, delt or the free dt; -1 if none
+int find_set(const lbm_plan *p, double dt) {
+  for (int set = 0; set < p->nset; ++set)
+    if (std::fabs(step_dt(p, set) - dt) <= 1e-12 * std::fabs(dt)) return set;
+  return -1;
+}
+
+// SemiImplicit.prepare / VerticalDiffusion.prepare (integrator.py:130-163,
+// 216-228) for the free matrix set
+int prepare_dt(lbm_plan *p, double dt) {
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(LBM_ERR_VALUE, "dt must be positive, got %g", dt);
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before preparing a time step");
+  p->user_dt = dt;
+  CHK(build_matrices(p, p->nset - 1, p->nset));
+  CHK(build_vdiff(p, p->nset - 1, p->nset));
+  CHK(upload_operators<double>(p));
+  if (p->prec != LBM_FP64) CHK(upload_operators<float>(p));
+  return LBM_OK;
+}
+
+// Integrator.step (integrator.py:264-315) on caller buffers: k_step with a
+// fixed dt on the second copy of the spectral state
+template <class T>
+int step_t(lbm_plan *p, double dt, const double *const prev[5], const double *const tend[5],
+           double *const out[5], hipStream_t st) {
+  CHK(ensure_aux(p));
+  int set = find_set(p, dt);
+  if (set < 0) {
+    CHK(prepare_dt(p, dt));
+    set = p->nset - 1;
+  }
+  // previous level: (vor, div, t, q) as the filtered level, pi where the step
+  // reads pi at t-dt (tendency field 1); tendencies in their usual slots
+  CHK(pack_spec5<T>(p, prev, p->x_sb3, 4, p->tend2, 2, 1, st));
+  CHK(pack_spec5<T>(p, tend, p->tend3, 4, p->tend2, 2, 0, st));
+  StepArgs<T> a = step_args<T>(p, p->E);
+  a.sa3 = (cx<T> *)p->x_sa3;
+  a.sb3 = (cx<T> *)p->x_sb3;
+  a.sa2 = (cx<T> *)p->x_sa2;
+  a.sb2 = nullptr;
+  a.pend = p->x_pend;
+  a.fixed_set = set;
+  a.fixed_dt = (T)dt;
+  const dim3 grid((unsigned)(((size_t)p->nspec * p->E + 63) / 64), 2);
+  if (p->L == 20)
+    hipLaunchKernelGGL((k_step<T, 20, false>), grid, dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
+  HIPCHK(hipGetLastError());
+  return unpack_spec5<T>(p, p->x_sa3, 4, p->x_sa2, 3, 0, out, st);
+}
+
+// The deferred ps' update of the last step applied now (the next step then
+// reads psa / psb as they are).
+template <class T>
+int settle_ps(lbm_plan *p, hipStream_t st) {
+  if (p->linear || p->steps_total == 0) return LBM_OK;
+  CHK(synthesize_state<T>(p, st));
+  const int E = p->E, J = p->J, I = p->I, L = p->L;
+  const BRow br{L};
+  hipLaunchKernelGGL((k_settle_ps<T>), dim3(nblk((size_t)E * J * I)), dim3(256), 0, st,
+                     (const T *)p->gin, 6 * L + 6, 6 * L + 2, (const T *)p->bsg, br.rows(), br.psbar(),
+                     (T *)p->psa, (T *)p->psb, (const int *)p->pend, (T)p->robert, E, J, I, p->nb);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(p->pend, 0, E * sizeof(int), st));
+  return LBM_OK;
+}
+
+// grid fields of time level 0 (ga) or 1 (gb, the filtered previous level)
+template <class T>
+int get_level_t(lbm_plan *p, int level, double *u, double *v, double *t, double *q, double *ps,
+                hipStream_t st) {
+  if (level == 0 || (p->steps_total == 0 && p->levels_equal))
+    return get_state_t<T>(p, u, v, t, q, ps, st);
+  const int J = p->J, I = p->I, L = p->L, E = p->E;
+  if (p->steps_total == 0 && p->rawb) {
+    // the previous level as the partial update left it: its raw grid and ps'
+    const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
+    double *dst[4] = {u, v, t, q};
+    for (int f = 0; f < 4; ++f)
+      if (dst[f])
+        hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, (const T *)p->rawb,
+                           dst[f], E, J, I, L, 4 * L, f * L, 0, (const T *)nullptr, 0, 0, 1);
+    if (ps)
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->psb, ps,
+                         E, J, I, 1, 1, 0, 0, (const T *)nullptr, 0, 0, 1);
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+  }
+  CHK(settle_ps<T>(p, st));
+  std::swap(p->sa3, p->sb3);
+  if (p->linear) std::swap(p->sa2, p->sb2);
+  const int rc = synthesize_state<T>(p, st);
+  std::swap(p->sa3, p->sb3);
+  if (p->linear) std::swap(p->sa2, p->sb2);
+  CHK(rc);
+  const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
+  const int rows = 6 * L + 6;
+  const int r0[4] = {0, L, 4 * L, 5 * L};
+  double *dst[4] = {u, v, t, q};
+  for (int f = 0; f < 4; ++f)
+    if (dst[f])
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n3)), dim3(256), 0, st, (const T *)p->gin,
+                         dst[f], E, J, I, L, rows, r0[f], 0, (const T *)nullptr, 0, 0, 1);
+  if (ps) {
+    const BRow br{L};
+    if (p->linear)
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->gin, ps,
+                         E, J, I, 1, rows, 6 * L + 2, 2, (const T *)p->bsg, br.rows(), br.psbar(), p->nb);
+    else
+      hipLaunchKernelGGL((k_unpack_grid<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->psb, ps,
+                         E, J, I, 1, 1, 0, 0, (const T *)nullptr, 0, 0, 1);
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// set_initial_perturbation (model.py:142-148): the given fields replace both
+// time levels, the others keep each level's values.
+template <class T>
+int set_state_keep_t(lbm_plan *p, const double *const in[5], hipStream_t st) {
+  const int J = p->J, I = p->I, L = p->L, E = p->E;
+  const size_t n3 = (size_t)E * J * I * L, n2 = (size_t)E * J * I;
+  const bool two = p->steps_total > 0 || !p->levels_equal;
+  double *buf = nullptr;
+  HIPCHK(hipMalloc(&buf, (two ? 2 : 1) * (4 * n3 + n2) * sizeof(double)));
+  auto level_ptrs = [&](double *base, double *o[5]) {
+    for (int f = 0; f < 4; ++f) o[f] = base + (size_t)f * n3;
+    o[4] = base + 4 * n3;
+  };
+  double *ga[5], *gb[5];
+  level_ptrs(buf, ga);
+  if (two) level_ptrs(buf + 4 * n3 + n2, gb);
+  int rc = get_level_t<T>(p, 0, ga[0], ga[1], ga[2], ga[3], ga[4], st);
+  if (rc == LBM_OK && two) rc = get_level_t<T>(p, 1, gb[0], gb[1], gb[2], gb[3], gb[4], st);
+  const double *a[5], *b[5];
+  for (int f = 0; f < 5; ++f) {
+    a[f] = in[f] ? in[f] : ga[f];
+    b[f] = in[f] ? in[f] : (two ? gb[f] : ga[f]);
+  }
+  if (rc == LBM_OK && two) {
+    // the previous level through the second copy, then into sb3 / psb
+    if ((rc = ensure_aux(p)) == LBM_OK) {
+      const bool have_raw = p->have_raw, raw_pending = p->raw_pending;
+      swap_state(p);
+      rc = set_state_t<T>(p, b[0], b[1], b[2], b[3], b[4], st);
+      swap_state(p);
+      p->have_raw = have_raw;
+      p->raw_pending = raw_pending;
+    }
+  }
+  if (rc == LBM_OK) rc = set_state_t<T>(p, a[0], a[1], a[2], a[3], a[4], st);
+  if (rc == LBM_OK && two) {
+    const size_t cs = 2 * p->rsize(), rawbytes = p->rsize() * 4 * n3;
+    if (!p->rawb && hipMalloc(&p->rawb, rawbytes) != hipSuccess) rc = fail(LBM_ERR_RUNTIME, "allocation failed");
+    if (rc == LBM_OK &&
+        (hipMemcpyAsync(p->rawb, p->x_raw, rawbytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(p->sb3, p->x_sa3, cs * p->nspec * 4 * E * L, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(p->psb, p->x_psa, p->rsize() * n2, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+         (p->sb2 && hipMemcpyAsync(p->sb2, p->x_sa2, cs * p->nspec * 3 * E, hipMemcpyDeviceToDevice, st) != hipSuccess)))
+      rc = fail(LBM_ERR_RUNTIME, "previous-level copy failed");
+  }
+  if (hipStreamSynchronize(st) != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "sync failed");
+  (void)hipFree(buf);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1657,7 +1952,8 @@ int lbm_plan_destroy(lbm_plan *p) {
                   p->ctl, p->pend, p->d_nsteps, p->scratch, p->sb2, p->aa3, p->ab3, p->aa2,
                   p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc,
                   p->hspec[0], p->hspec[1], p->hgrid[0], p->hgrid[1], p->hlast_spec,
-                  p->hlast_grid, p->mjo, p->sfv, p->sfp};
+                  p->hlast_grid, p->mjo, p->sfv, p->sfp, p->x_sa3, p->x_sa2, p->x_sb3,
+                  p->x_sb2, p->x_psa, p->x_psb, p->x_raw, p->x_pend, p->x_ctl, p->rawb};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
@@ -1711,7 +2007,10 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->nspec = p->M * (p->M + 1) / 2;
   p->Jh = p->J / 2;
   p->inistp = d->inistp;
-  p->nset = p->inistp + 1;
+  // one matrix set per start-up dt, one for delt and one for a free dt
+  // (Integrator.step with any dt, lbm_prepare_dt)
+  p->nset = p->inistp + 2;
+  p->user_dt = d->delt;
   p->strictly_linear = d->strictly_linear;
   p->linear = d->tangent_linear ? 1 : 0;
   p->skip_low_n = d->skip_low_n_vdiff;
@@ -1837,6 +2136,7 @@ int lbm_plan_get(const lbm_plan *p, const char *name, double *h_out, int64_t n) 
   else if (s == "tbar") v = &p->tbar;
   else if (s == "minv") v = &p->minv;
   else if (s == "vinv") v = &p->vinv;
+  else if (s == "mmat") v = &p->mmat;
   if (!v) return fail(LBM_ERR_VALUE, "unknown plan array '%s'", name);
   const int64_t cnt = (int64_t)v->size();
   if (n < cnt) return fail(LBM_ERR_VALUE, "'%s' needs %lld doubles", name, (long long)cnt);
@@ -1891,17 +2191,20 @@ int lbm_set_basic_state(lbm_plan *p, const double *u, const double *v, const dou
 int lbm_set_forcing(lbm_plan *p, const double *frc, int nfrc, void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   hipStream_t st = p->strm(stream);
-  destroy_graphs(p);
+  // the captured graphs stay valid while the forcing buffer keeps its
+  // address (a new forcing of the same member count between steps)
   if (p->mjo) {
     // a constant forcing replaces the MJO forcing
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipFree(p->mjo));
     p->mjo = nullptr;
+    destroy_graphs(p);
   }
   if (!frc) {
     if (p->frc) {
       HIPCHK(hipStreamSynchronize(st));
       HIPCHK(hipFree(p->frc));
+      destroy_graphs(p);
     }
     p->frc = nullptr;
     p->nfrc = 0;
@@ -1915,7 +2218,12 @@ int lbm_set_forcing(lbm_plan *p, const double *frc, int nfrc, void *stream) {
     HIPCHK(hipFree(p->frc));
     p->frc = nullptr;
   }
-  if (!p->frc) HIPCHK(hipMalloc(&p->frc, n * p->rsize()));
+  if (!p->frc) {
+    HIPCHK(hipMalloc(&p->frc, n * p->rsize()));
+    destroy_graphs(p);
+  }
+  // a graph of the previous run may still read the buffer
+  if (p->own != st) HIPCHK(hipStreamSynchronize(p->own));
   p->nfrc = nfrc;
   if (p->prec == LBM_FP64)
     hipLaunchKernelGGL((k_pack_grid<double>), dim3(nblk(n)), dim3(256), 0, st, frc, (double *)p->frc,
@@ -1934,6 +2242,7 @@ int lbm_set_state(lbm_plan *p, const double *u, const double *v, const double *t
   hipStream_t st = p->strm(stream);
   p->steps_total = 0;
   p->in_run = false;
+  p->levels_equal = true;
   Ctl c0{0, 0, INT_MAX, 0};
   int rc = p->prec == LBM_FP64 ? set_state_t<double>(p, u, v, t, q, ps, st)
                                : set_state_t<float>(p, u, v, t, q, ps, st);
@@ -1941,6 +2250,143 @@ int lbm_set_state(lbm_plan *p, const double *u, const double *v, const double *t
   c0.raw = p->have_raw ? 1 : 0;
   HIPCHK(hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_set_state_keep(lbm_plan *p, const double *u, const double *v, const double *t,
+                       const double *q, const double *ps, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "set the basic state before the perturbation");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  if (!(u || v || t || q || ps)) return LBM_OK;
+  hipStream_t st = p->strm(stream);
+  const double *in[5] = {u, v, t, q, ps};
+  const bool equal = p->steps_total == 0 && p->levels_equal;
+  int rc = p->prec == LBM_FP64 ? set_state_keep_t<double>(p, in, st) : set_state_keep_t<float>(p, in, st);
+  if (rc != LBM_OK) return rc;
+  p->levels_equal = equal;
+  p->steps_total = 0;
+  Ctl c0{0, p->have_raw ? 1 : 0, INT_MAX, 0};
+  HIPCHK(hipMemcpyAsync(p->ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return LBM_OK;
+}
+
+int lbm_get_prev_state(lbm_plan *p, double *u, double *v, double *t, double *q, double *ps,
+                       void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "no basic state");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  hipStream_t st = p->strm(stream);
+  return p->prec == LBM_FP64 ? get_level_t<double>(p, 1, u, v, t, q, ps, st)
+                             : get_level_t<float>(p, 1, u, v, t, q, ps, st);
+}
+
+int lbm_tendencies(lbm_plan *p, const double *u, const double *v, const double *t, const double *q,
+                   const double *ps, double *tvor, double *tdiv, double *tt, double *tq,
+                   double *tpi, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_tendencies before lbm_set_basic_state");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  if (p->linear) return fail(LBM_ERR_STATE, "the tangent-linear model has no grid-state tendencies");
+  hipStream_t st = p->strm(stream);
+  double *out[5] = {tvor, tdiv, tt, tq, tpi};
+  return p->prec == LBM_FP64 ? tendencies_t<double>(p, u, v, t, q, ps, out, st)
+                             : tendencies_t<float>(p, u, v, t, q, ps, out, st);
+}
+
+int lbm_prepare_dt(lbm_plan *p, double dt) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  return prepare_dt(p, dt);
+}
+
+int lbm_step(lbm_plan *p, double dt, const double *pvor, const double *pdiv, const double *pt,
+             const double *pq, const double *ppi, const double *tvor, const double *tdiv,
+             const double *tt, const double *tq, const double *tpi, double *nvor, double *ndiv,
+             double *nt, double *nq, double *npi, void *stream) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_step before lbm_set_basic_state");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  const double *prev[5] = {pvor, pdiv, pt, pq, ppi}, *tend[5] = {tvor, tdiv, tt, tq, tpi};
+  for (int f = 0; f < 5; ++f)
+    if (!prev[f] || !tend[f]) return fail(LBM_ERR_VALUE, "lbm_step needs every previous field and tendency");
+  hipStream_t st = p->strm(stream);
+  double *out[5] = {nvor, ndiv, nt, nq, npi};
+  return p->prec == LBM_FP64 ? step_t<double>(p, dt, prev, tend, out, st)
+                             : step_t<float>(p, dt, prev, tend, out, st);
+}
+
+int lbm_robert_asselin(int64_t n, double *prev, const double *cur, const double *nxt, double gamma,
+                       void *stream) {
+  if (n < 0 || (n > 0 && (!prev || !cur || !nxt))) return fail(LBM_ERR_VALUE, "bad arguments");
+  if (n == 0) return LBM_OK;
+  hipLaunchKernelGGL(k_robert_asselin, dim3(nblk((size_t)n)), dim3(256), 0, (hipStream_t)stream, prev,
+                     cur, nxt, gamma, (size_t)n);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_lnps(int64_t n, int64_t nper, const double *in, const double *psbar, double *out,
+             int inverse, void *stream) {
+  if (n < 0 || nper <= 0 || n % nper || (n > 0 && (!in || !psbar || !out)))
+    return fail(LBM_ERR_VALUE, "bad arguments (n must be a multiple of nper)");
+  if (n == 0) return LBM_OK;
+  hipLaunchKernelGGL(k_lnps, dim3(nblk((size_t)n)), dim3(256), 0, (hipStream_t)stream, in, psbar, out,
+                     (size_t)n, (size_t)nper, inverse);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_grid_stats(int64_t n, const double *x, double *h_out, void *stream) {
+  if (!h_out || n < 0 || (n > 0 && !x)) return fail(LBM_ERR_VALUE, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long *d = nullptr, h[2] = {0, 0};
+  HIPCHK(hipMalloc((void **)&d, sizeof(h)));
+  int rc = LBM_OK;
+  if (hipMemsetAsync(d, 0, sizeof(h), st) != hipSuccess) rc = fail(LBM_ERR_RUNTIME, "memset failed");
+  if (rc == LBM_OK && n > 0) {
+    const unsigned nb = (unsigned)std::min<size_t>(((size_t)n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_grid_stats, dim3(nb), dim3(256), 0, st, x, (size_t)n, d);
+    if (hipGetLastError() != hipSuccess) rc = fail(LBM_ERR_RUNTIME, "launch failed");
+  }
+  if (rc == LBM_OK && (hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                       hipStreamSynchronize(st) != hipSuccess))
+    rc = fail(LBM_ERR_RUNTIME, "statistics copy failed");
+  (void)hipFree(d);
+  if (rc != LBM_OK) return rc;
+  double mx;
+  std::memcpy(&mx, &h[0], sizeof(double));
+  h_out[0] = mx;
+  h_out[1] = (double)h[1];
+  return LBM_OK;
+}
+
+int lbm_spec_scale(lbm_plan *p, const double *in, double *out, int B, int nk, int kind,
+                   void *stream) {
+  if (!p || !in || !out || B < 0 || nk < 1) return fail(LBM_ERR_VALUE, "bad arguments");
+  if (kind != 0 && kind != 1) return fail(LBM_ERR_VALUE, "kind must be 0 (laplacian) or 1 (inverse)");
+  const size_t n = (size_t)B * p->M * p->M * nk * 2;
+  if (n == 0) return LBM_OK;
+  hipLaunchKernelGGL(k_spec_scale, dim3(nblk(n)), dim3(256), 0, p->strm(stream), in, out,
+                     kind == 0 ? p->d64.lap : p->d64.inv_lap, n, p->M, nk);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_hydrostatic(const double *in, double *out, int64_t npts, int nlev, int inner,
+                    const double *alpha, const double *dlog3, void *stream) {
+  if (!in || !out || !alpha || !dlog3 || npts < 0 || nlev < 1 || (inner != 1 && inner != 2))
+    return fail(LBM_ERR_VALUE, "bad arguments (inner is 1 for real, 2 for complex)");
+  if (npts == 0) return LBM_OK;
+  hipLaunchKernelGGL(k_hydrostatic, dim3(nblk((size_t)npts * inner)), dim3(256), 0, (hipStream_t)stream,
+                     in, out, alpha, dlog3, (size_t)npts, nlev, inner);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_run_abort(lbm_plan *p) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  p->in_run = false;
   return LBM_OK;
 }
 
@@ -1958,7 +2404,11 @@ int lbm_set_lengths(lbm_plan *p, const int32_t *h_nsteps, void *stream) {
   return LBM_OK;
 }
 
-int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every, void *stream) {
+// The run loop behind lbm_run, lbm_run_to and lbm_run_steps: runs (or
+// continues) the run of nsteps up to full step stop (0: its end), or, with
+// count > 0, by count more executed steps (start-up steps included).
+int run_core(lbm_plan *p, int nsteps, int stop, int count, int use_graph, int check_every,
+             void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_run before lbm_set_basic_state");
   hipStream_t st = use_graph ? p->own : p->strm(stream);
@@ -1972,10 +2422,10 @@ int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every
     return LBM_OK;
   }
   if (stop < 0 || stop > maxlen) return fail(LBM_ERR_VALUE, "stop=%d outside 0..%d", stop, maxlen);
-  const int target = stop == 0 ? total : p->inistp + stop;
+  const int target_stop = stop == 0 ? total : p->inistp + stop;
   if (p->in_run && len != p->run_len)
     return fail(LBM_ERR_STATE, "a run of other lengths is in progress (finish it or set a state)");
-  if (p->in_run && target < p->run_done)
+  if (p->in_run && target_stop < p->run_done)
     return fail(LBM_ERR_VALUE, "stop=%d is behind the %d full steps already run", stop,
                 p->run_done - p->inistp);
   if (!p->in_run) {
@@ -1991,6 +2441,7 @@ int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every
     p->run_len = len;
     p->run_done = 0;
   }
+  const int target = count > 0 ? std::min(total, p->run_done + count) : target_stop;
   const int chunk = 8;
   int done = p->run_done;
   int rc = LBM_OK;
@@ -2025,6 +2476,16 @@ int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every
   CHK(check_bad(p, st, done));
   if (use_graph && stream) HIPCHK(hipStreamSynchronize(st));
   return LBM_OK;
+}
+
+int lbm_run_to(lbm_plan *p, int nsteps, int stop, int use_graph, int check_every, void *stream) {
+  return run_core(p, nsteps, stop, 0, use_graph, check_every, stream);
+}
+
+int lbm_run_steps(lbm_plan *p, int nsteps, int count, int use_graph, int check_every,
+                  void *stream) {
+  if (count < 1) return fail(LBM_ERR_VALUE, "count must be >= 1");
+  return run_core(p, nsteps, 0, count, use_graph, check_every, stream);
 }
 
 int lbm_run(lbm_plan *p, int nsteps, int use_graph, int check_every, void *stream) {
@@ -2105,6 +2566,7 @@ int lbm_set_spectral_state(lbm_plan *p, const double *vor, const double *div, co
   hipStream_t st = p->strm(stream);
   p->steps_total = 0;
   p->in_run = false;
+  p->levels_equal = true;
   int rc = p->prec == LBM_FP64 ? set_spectral_t<double>(p, vor, div, t, q, pi, st)
                                : set_spectral_t<float>(p, vor, div, t, q, pi, st);
   if (rc != LBM_OK) return rc;

@@ -439,6 +439,12 @@ struct StepArgs {
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
   int ecnt;                      // members covered by the launch (<= E)
   T delt, robert;
+  // >= 0: one Integrator.step of the given dt with matrix set fixed_set on
+  // caller buffers (integrator.py:264-315): every member advances, no
+  // start-up schedule, no Robert-Asselin filter, no deferred ps' update and
+  // the global means are kept
+  int fixed_set;
+  T fixed_dt;
 };
 
 // Read-only pointers: through the constant address space when the address
@@ -478,17 +484,18 @@ void k_step(StepArgs<T> a) {
   const int s = UNI ? __builtin_amdgcn_readfirstlane(idx / a.ecnt) : idx / a.ecnt;
   const int e = idx - s * a.ecnt;
   if (s >= a.nspec) return;
-  const int st = a.ctl->step;
-  const bool active = st < a.inistp + a.nsteps[e];
+  const bool pure = a.fixed_set >= 0;
+  const int st = pure ? a.inistp : a.ctl->step;
+  const bool active = pure || st < a.inistp + a.nsteps[e];
   const bool oinit = st < a.inistp;
   // blockIdx.y: 0 -> temperature, divergence, pi (the coupled solve);
   // 1 -> vorticity and humidity (independent of it): twice the waves, each
   // with half the dependent chain
   const int role = blockIdx.y;
-  if (s == 0 && role == 0) a.pend[e] = active ? (oinit ? 1 : 2) : 0;
+  if (!pure && s == 0 && role == 0) a.pend[e] = active ? (oinit ? 1 : 2) : 0;
   if (!active) return;
-  const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
-  const int set = oinit ? st : a.inistp;
+  const T dt = pure ? a.fixed_dt : oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
+  const int set = pure ? a.fixed_set : oinit ? st : a.inistp;
   const T dt2 = (T)2 * dt;
   const int n = UNI ? __builtin_amdgcn_readfirstlane(a.n_of_s[s]) : a.n_of_s[s];
   const int eb = (!UNI && a.nbasic > 1) ? e : 0;
@@ -538,7 +545,7 @@ void k_step(StepArgs<T> a) {
   // their global means (n = 0) never survive a step (a forcing's mean
   // still acts through dbar within the step).  Same here, outside the
   // tangent-linear model.
-  const bool drop_mean = n == 0 && a.sb2 == nullptr;
+  const bool drop_mean = n == 0 && a.sb2 == nullptr && !pure;
   // row r of V (or the identity) applied to x, then the filter and the store
   auto update = [&](int f, RU V, bool apply, cx<T> *sum, bool drop) __attribute__((always_inline)) {
     cx<T> acc = zero<T>();
@@ -551,8 +558,10 @@ void k_step(StepArgs<T> a) {
         for (int c = 0; c < L; ++c) fma_acc(y, V[r * L + c], x[c]);
       }
       if (drop) y = zero<T>();
-      const cx<T> cur = bsa.ld(voff, lo(f, r), tag);
-      bsb.st(voff, lo(f, r), oinit ? cur : cur + gam * (bsb.ld(voff, lo(f, r), tag) - (T)2 * cur + y));
+      if (!pure) {
+        const cx<T> cur = bsa.ld(voff, lo(f, r), tag);
+        bsb.st(voff, lo(f, r), oinit ? cur : cur + gam * (bsb.ld(voff, lo(f, r), tag) - (T)2 * cur + y));
+      }
       bsa.st(voff, lo(f, r), y);
       if (sum) acc = acc + vc[vo.dsig() + r] * y;
     }
@@ -810,6 +819,31 @@ __global__ void k_pack_pi(const double *__restrict__ ps, T *__restrict__ dst, in
     const double pb = psbar[((size_t)bb * J + j) * I + i];
     const double v = ps ? ps[q] : 0.0;
     dst[(((size_t)b * J + j) * R + r0) * I + i] = (T)(linear ? v / pb : log1p(v / pb));
+  }
+}
+
+// The deferred ps' update of the last step applied now (the same
+// arithmetic as column_surface, so the next step sees bit-identical values):
+// mode 1 assign (start-up), 2 Robert-Asselin filter, from the synthesised
+// pi row r0 of g.  The caller clears pend afterwards.
+template <class T>
+__global__ void k_settle_ps(const T *__restrict__ g, int R, int r0, const T *__restrict__ bsg,
+                            int brows, int brow, T *__restrict__ psa, T *__restrict__ psb,
+                            const int *__restrict__ pend, T robert, int B, int J, int I, int nbasic) {
+  const size_t tot = (size_t)B * J * I;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    const size_t bj = q / I;
+    const int j = (int)(bj % J), b = (int)(bj / J);
+    const int mode = pend[b];
+    if (mode == 0) continue;
+    const int bb = nbasic > 1 ? b : 0;
+    const T pb = bsg[(((size_t)bb * J + j) * brows + brow) * I + i];
+    const T psn = pb * expm1(g[(bj * R + r0) * I + i]);
+    const T pa = psa[q];
+    psb[q] = (mode == 1) ? pa : pa + robert * (psb[q] - (T)2 * pa + psn);
+    psa[q] = psn;
   }
 }
 

@@ -20,24 +20,27 @@ _REC = ">f4"
 
 
 def read_fortran_records(path, nlon=IMAX, nlat=JMAX, count=None, offset=0):
-    """Fortran sequential records of ``nlon * nlat`` big-endian real*4 values
-    (``forcing.py:45-65``): returns ``(nrec, nlat, nlon)`` float32 after
-    checking every record marker."""
-    npts = nlon * nlat
-    raw = np.fromfile(path, dtype=_REC)
-    per = npts + 2
-    if raw.size % per:
-        raise ValueError(f"{path}: size {raw.size * 4} is not a whole number of "
-                         f"{npts}-value Fortran records")
-    rec = raw.reshape(-1, per)
-    marks = rec[:, [0, -1]].copy().view(">i4")
-    bad = np.nonzero(np.any(marks != npts * 4, axis=1))[0]
-    if bad.size:
-        b = int(bad[0])
-        raise ValueError(f"{path}: bad record marker at record {b} ({int(marks[b, 0])} != "
-                         f"{npts * 4}); wrong endianness or record length?")
-    stop = None if count is None else offset + count
-    return rec[offset:stop, 1:-1].reshape(-1, nlat, nlon)
+    """Planes of a file of Fortran sequential records, each record one
+    ``nlat x nlon`` big-endian real*4 field between two 4-byte length words
+    (the layout ``forcing.py:45-65`` reads).  Returns ``(nrec, nlat, nlon)``
+    float32 from record ``offset`` on (``count`` of them, or all).  Every
+    length word is checked, so a file of another grid or byte order raises
+    ValueError instead of being misread."""
+    words = nlon * nlat + 2                      # length, field, length
+    flat = np.fromfile(path, dtype=_REC)
+    nrec, extra = divmod(flat.size, words)
+    if extra:
+        raise ValueError(f"{path}: {flat.size * 4} bytes do not split into records of "
+                         f"{nlon}x{nlat} real*4 values")
+    table = flat.reshape(nrec, words)
+    lengths = table[:, [0, -1]].copy().view(">i4")
+    wrong = np.argwhere(lengths != 4 * (words - 2))
+    if wrong.size:
+        r = int(wrong[0, 0])
+        raise ValueError(f"{path}: record {r} has length word {int(lengths[r, wrong[0, 1]])}, "
+                         f"expected {4 * (words - 2)} (another grid, or little-endian data?)")
+    sel = table[offset:] if count is None else table[offset:offset + count]
+    return sel[:, 1:-1].reshape(-1, nlat, nlon)
 
 
 def _orient(a, yflip):
@@ -45,27 +48,36 @@ def _orient(a, yflip):
 
 
 class SteadyForcing:
-    """vorticity / divergence / temperature / ln(Ps) forcing, time constant."""
+    """Time-constant forcing of four prognostic variables (the
+    ``SteadyForcing(vor, div, t, ps)`` container of ``forcing.py:71-96``):
+    vorticity and divergence (J, I, L) in s^-2, temperature (J, I, L) in
+    K/s and ln ps (J, I) in s^-1."""
+
+    FIELDS = ("vor", "div", "t", "ps")
 
     def __init__(self, vor, div, t, ps):
         self.vor, self.div, self.t, self.ps = vor, div, t, ps
 
     @classmethod
     def zeros(cls, nlat=JMAX, nlon=IMAX, nlev=KMAX):
-        z3 = np.zeros((nlat, nlon, nlev))
-        return cls(z3.copy(), z3.copy(), z3.copy(), np.zeros((nlat, nlon)))
+        """No forcing: three zero 3-D fields and a zero surface field."""
+        shapes = [(nlat, nlon, nlev)] * 3 + [(nlat, nlon)]
+        return cls(*(np.zeros(shp) for shp in shapes))
 
     @classmethod
     def from_file(cls, path, nlev=KMAX, ncase=1, yflip=False, nlon=IMAX, nlat=JMAX):
-        """Case ``ncase`` of a steady-forcing file: ``3 nlev + 1`` records
-        (vorticity, divergence, temperature levels, then ln Ps), ``forcing.py:78-96``."""
-        per = 3 * nlev + 1
-        rec = read_fortran_records(path, nlon, nlat, count=per, offset=(ncase - 1) * per)
-        if rec.shape[0] != per:
-            raise ValueError(f"{path}: expected {per} records, got {rec.shape[0]}")
-        rec = _orient(rec, yflip)
-        lev = lambda k: np.moveaxis(rec[k * nlev:(k + 1) * nlev], 0, -1).copy()  # noqa: E731
-        return cls(lev(0), lev(1), lev(2), rec[3 * nlev].copy())
+        """Case ``ncase`` (1-based) of a steady-forcing file, whose cases are
+        consecutive blocks of ``3 nlev + 1`` records: the vorticity, then
+        divergence, then temperature levels, then ln ps
+        (``forcing.py:78-96``)."""
+        block = 3 * nlev + 1
+        planes = read_fortran_records(path, nlon, nlat, count=block, offset=(ncase - 1) * block)
+        if planes.shape[0] != block:
+            raise ValueError(f"{path}: case {ncase} needs {block} records, the file has "
+                             f"{planes.shape[0]} from there")
+        planes = _orient(planes, yflip)
+        cube = np.moveaxis(planes[:3 * nlev].reshape(3, nlev, nlat, nlon), 1, -1)
+        return cls(cube[0].copy(), cube[1].copy(), cube[2].copy(), planes[3 * nlev].copy())
 
 
 class ConstForcing:
@@ -98,8 +110,8 @@ class MJOForcing:
         if gtf is None:
             rec = read_fortran_records(path, nlon, nlat)
             if rec.shape[0] != ndays * nlev:
-                raise ValueError(f"{path}: expected {ndays * nlev} records ({ndays} days x {nlev} "
-                                 f"levels), got {rec.shape[0]}.  Check ndays/nlev.")
+                raise ValueError(f"{path}: {rec.shape[0]} records do not hold {ndays} days of "
+                                 f"{nlev} levels; pass the file's ndays and nlev")
             # real*4 stays float32, scale included (numpy keeps float32 for a
             # Python float factor), as forcing.py:131 leaves it
             rec = _orient(rec, yflip).astype(np.float32) * scale

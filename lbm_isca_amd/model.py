@@ -16,7 +16,9 @@ import numpy as np
 
 from .backend import asnumpy, hip
 from .constants import S2D
+from .dynamics import BasicStateTerms, Dynamics
 from .forcing import MJOForcing
+from .integrator import Damping, Integrator, SemiImplicit, VerticalDiffusion
 from .plan import Plan
 from .state import State
 
@@ -60,66 +62,13 @@ class Config:
         return int(round(86400.0 / self.delt))
 
 
-_UNIT = {"SEC": 1.0, "MIN": 60.0, "HOU": 3600.0, "DAY": 86400.0, "MON": 30 * 86400.0}
-
-
-def _seconds(value, unit):
-    return float(value) * _UNIT[unit.strip().upper()[:3]]
-
-
-def _drag(profile, kmax, unit):
-    """Per-level timescale -> s^-1, <= 0 disables (integrator.py:101-112)."""
-    out = np.zeros(kmax)
-    if profile is None:
-        return out
-    d = np.asarray(profile, dtype=np.float64)
-    if d.size != kmax:
-        raise ValueError(f"damping profile must have {kmax} entries")
-    sec = np.array([_seconds(x, unit) for x in d])
-    out[sec > 0] = 1.0 / sec[sec > 0]
-    return out
-
-
 def damping_coefficients(cfg, nmax, sig):
-    """(kv, kd, ktt, kq), each (nmax+1, nlev) in s^-1: implicit hyperdiffusion
-    [n(n+1)/(N(N+1))]^p / tau, tanh top sponge and Rayleigh/Newtonian drag
-    (integrator.py:41-98)."""
-    kmax = sig.size
-    n = np.arange(nmax + 1, dtype=np.float64)
-    p = cfg.hdif_order // 2
-    tau = _seconds(cfg.hdif_tefold, cfg.hdif_tunit)
-    difmax = (nmax * (nmax + 1.0) / nmax) ** p
-    dift = (n * (n + 1.0) / nmax) ** p
-    difv = dift - (2.0 / nmax) ** p
-    kt = dift / difmax / tau
-    kv = difv / difmax / tau
-    kt[0] = 0.0
-    kv[0] = 0.0
-    kv[1] = max(kv[1], 0.0)
-    sponge = np.zeros(kmax)
-    if cfg.sponge:
-        zsclh, rzsclh = 8.0e3, 7.0e3
-        z = -zsclh * np.log(sig)
-        zofs = -zsclh * np.log(float(sig[-1]))
-        sponge = np.maximum(1.0 + np.tanh((z - zofs) / rzsclh), 1e-20) / _seconds(cfg.ray0, cfg.hdif_tunit)
-    dv = _drag(cfg.ddragv, kmax, cfg.damp_tunit)
-    dd = _drag(cfg.ddragd, kmax, cfg.damp_tunit)
-    dt_ = _drag(cfg.ddragt, kmax, cfg.damp_tunit)
-    dq = _drag(cfg.ddragq, kmax, cfg.damp_tunit)
-    return (kv[:, None] + sponge + dv, kv[:, None] + sponge + dd,
-            kt[:, None] + dt_, kt[:, None] + dq)
-
-
-def _vdif_rate(v, kmax):
-    """1/(86400 v) per level; None or any v <= 0 disables (integrator.py:243-253)."""
-    if v is None:
-        return None
-    v = np.atleast_1d(np.asarray(v, dtype=np.float64))
-    if v.size == 1:
-        v = np.full(kmax, v[0])
-    if np.any(v <= 0):
-        return None
-    return 1.0 / (S2D * v)
+    """(kv, kd, ktt, kq), each (nmax+1, nlev) in s^-1, of a Config
+    (``integrator.py:41-98``)."""
+    import types
+    sp = types.SimpleNamespace(nmax=nmax)
+    vg = types.SimpleNamespace(sig=np.asarray(sig), kmax=np.asarray(sig).size)
+    return Damping.from_config(sp, vg, cfg).arrays()
 
 
 def _field(obj, name):
@@ -154,23 +103,30 @@ class LBM:
         self.batched = bool(getattr(spectral, "batched", False)) or c.nmem > 1
         nmem = max(int(c.nmem), 1)
         self.nmem = nmem
-        damping = damping_coefficients(c, spectral.nmax, vertical.sig)
-        kmax = vertical.kmax
-        vdif = tuple(_vdif_rate(v, kmax) for v in (c.vdifv, c.vdifd, c.vdift, c.vdifq))
+        self.damping = Damping.from_config(spectral, vertical, c)
+        self.vdiff = VerticalDiffusion(vertical, c.vdifv, c.vdifd, c.vdift, c.vdifq)
         nbasic = nmem if per_member_basic else 1
         self.plan = Plan(spectral.nlon, spectral.nlat, spectral.nmax, spectral.a,
                          vertical.sigm, nmem=nmem, nbasic=nbasic, precision=c.precision,
-                         damping=damping, vdif=vdif, delt=c.delt, robert=c.robert,
-                         inistp=c.inistp, strictly_linear=c.strictly_linear,
+                         damping=self.damping.arrays(), vdif=self.vdiff.rates(), delt=c.delt,
+                         robert=c.robert, inistp=c.inistp, strictly_linear=c.strictly_linear,
                          tangent_linear=c.tangent_linear)
         fields = [np.asarray(_field(basic_state, k), dtype=np.float64) for k in ("u", "v", "t", "q", "ps")]
         if not per_member_basic:
             fields = [f[None] for f in fields]
         self.plan.set_basic_state(*fields)
         self.ps_bar = fields[4] if per_member_basic else fields[4][0]
+        kmax = vertical.kmax
         tb = self.plan.get("tbar", nbasic * kmax).reshape(nbasic, kmax)
         vertical.set_reference_temperature(tb[0])
         self.tbar = tb
+        # the reference's component objects (model.py:104-124) on this plan
+        self.bst = BasicStateTerms(spectral, vertical, basic_state, plan=self.plan)
+        self.dyn = Dynamics(spectral, vertical, self.bst, strictly_linear=c.strictly_linear,
+                            plan=self.plan)
+        self.semi = SemiImplicit(spectral, vertical, self.damping, plan=self.plan)
+        self.integ = Integrator(spectral, vertical, self.damping, self.semi, self.vdiff,
+                                plan=self.plan)
         self.plan.set_state()
         # keep the last top-of-step sample for diagnostics() after a run
         self.plan.set_history(0, keep_last=True)
@@ -184,28 +140,37 @@ class LBM:
 
     # -- state -----------------------------------------------------------
     def set_initial_perturbation(self, u=None, v=None, t=None, q=None, ps=None):
-        """Sets both time levels (model.py:142-148).  Fields left out keep the
-        current values of the present level."""
+        """Sets the given fields on both time levels (model.py:142-148); the
+        fields left out keep each level's current values."""
         given = dict(u=u, v=v, t=t, q=q, ps=ps)
-        if any(x is None for x in given.values()) and self.plan.steps_taken() > 0:
-            cur = dict(zip(("u", "v", "t", "q", "ps"), self.plan.get_state()))
-            if not self.batched:
-                cur = {k: x[0] for k, x in cur.items()}
-            given = {k: (cur[k] if x is None else x) for k, x in given.items()}
         if not self.batched:
             given = {k: _lead(x) for k, x in given.items()}
-        self.plan.set_state(**given)
+        self.plan.set_state_keep(**given)
         self._ga = None
+
+    def _state(self, fields):
+        u, v, t, q, ps = fields
+        if not self.batched:
+            u, v, t, q, ps = (x[0] for x in (u, v, t, q, ps))
+        return State(self.ps_bar, u, v, t, q, ps)
 
     @property
     def ga(self):
-        """Current time level as a grid :class:`State` (device tensors)."""
+        """Current time level t as a grid :class:`State` (device tensors)."""
         if self._ga is None:
-            u, v, t, q, ps = self.plan.get_state()
-            if not self.batched:
-                u, v, t, q, ps = (x[0] for x in (u, v, t, q, ps))
-            self._ga = State(self.ps_bar, u, v, t, q, ps)
+            self._ga = self._state(self.plan.get_state())
         return self._ga
+
+    @property
+    def gb(self):
+        """The filtered previous level t - dt as a grid :class:`State`."""
+        return self._state(self.plan.get_prev_state())
+
+    @property
+    def gc(self):
+        """The t + dt scratch level of the last step, which the reference's
+        loop leaves equal to ga (model.py:237-243)."""
+        return self.ga.copy()
 
     def spectral_state(self, level=0):
         """(vor, div, t, q, pi) spectral fields of the current (0) or
@@ -228,8 +193,10 @@ class LBM:
     def _set_forcing(self, mjo_forcing, steady_forcing):
         """Upload the run's forcings (model.py:196-229): the temperature
         forcing (MJO, else the steady one) and the steady vorticity,
-        divergence and ln ps forcings; returns the time-varying forcing
-        object whose call counter the run advances, or None."""
+        divergence and ln ps forcings.  Returns ``(interpolated, stepped)``:
+        the MJOForcing whose call counter the run advances on the device, or
+        the general forcing object whose ``.step()`` the run calls before
+        every step (start-up steps included), or None."""
         sv = {}
         if steady_forcing is not None:
             for name in ("vor", "div", "ps"):
@@ -241,16 +208,44 @@ class LBM:
             if isinstance(mjo_forcing, MJOForcing):
                 self.plan.set_mjo_forcing(mjo_forcing.gtf, mjo_forcing.nt, mjo_forcing.spinup,
                                           call0=mjo_forcing.counter)
-                return mjo_forcing
-            if not getattr(mjo_forcing, "constant", False) and type(mjo_forcing).__name__ not in (
-                    "EnsembleForcing", "ConstForcing"):
-                raise NotImplementedError(
-                    "time-varying forcing: MJOForcing or forcings whose step() is constant")
-            self.plan.set_forcing(mjo_forcing.step())
-            return None
+                return mjo_forcing, None
+            if getattr(mjo_forcing, "constant", False):
+                # declared constant: one upload serves the whole run
+                self.plan.set_forcing(mjo_forcing.step())
+                return None, None
+            if not callable(getattr(mjo_forcing, "step", None)):
+                raise TypeError("a temperature forcing needs a .step() method (model.py:224)")
+            return None, mjo_forcing
         frc = None if steady_forcing is None else steady_forcing.t
         self.plan.set_forcing(frc)
-        return None
+        return None, None
+
+    def _step_forcing(self, obj):
+        frc = obj.step()
+        self.plan.set_forcing(frc)
+
+    def _write(self, writer, day):
+        fields = {k: asnumpy(v) for k, v in self.plan.get_history(day).items()}
+        if not self.batched:
+            fields = {k: v[0] for k, v in fields.items()}
+        writer.add(day, fields)
+
+    def _run_stepped(self, obj, nsteps, lengths, writer, K, use_graph, check_every, verbose,
+                     progress_every, t0):
+        """The run with a forcing whose ``.step()`` changes every call: the
+        field of each call is uploaded before its step (model.py:222-229),
+        one executed step at a time."""
+        c = self.cfg
+        total = c.inistp + (nsteps if lengths is None else int(max(lengths)))
+        for k in range(total):
+            self._step_forcing(obj)
+            self.plan.run_steps(nsteps, 1, use_graph=use_graph, check_every=check_every)
+            done = k + 1 - c.inistp           # full steps done
+            if writer is not None and done > 0 and done % K == 0:
+                self._write(writer, done // K)
+            if verbose and progress_every and done and done % progress_every == 0:
+                print(f"  day {done / c.steps_per_day:6.2f}   [{_time.time() - t0:5.1f}s]",
+                      flush=True)
 
     def run(self, mjo_forcing=None, steady_forcing=None, writer=None, verbose=True,
             progress_every=240, use_graph=True, lengths=None, check_every=0):
@@ -264,7 +259,7 @@ class LBM:
         reach theirs (no writer then).
         """
         c = self.cfg
-        tv = self._set_forcing(mjo_forcing, steady_forcing)
+        tv, stepped = self._set_forcing(mjo_forcing, steady_forcing)
         spd = c.steps_per_day
         nsteps = int(round(c.ndays * spd))
         if lengths is not None:
@@ -273,28 +268,34 @@ class LBM:
             self.plan.set_lengths(lengths)
             nsteps = 0
         t0 = _time.time()
-        if writer is None:
-            self.plan.run(nsteps, use_graph=use_graph, check_every=check_every)
-        else:
+        K = 0
+        if writer is not None:
             K = int(round(c.out_interval_days * spd))
             if K < 1:
                 raise ValueError("out_interval_days is shorter than one step")
             self.plan.set_history(K, time_average=c.time_average, keep_last=True)
-            try:
+        try:
+            if stepped is not None:
+                self._run_stepped(stepped, nsteps, lengths, writer, K, use_graph, check_every,
+                                  verbose, progress_every, t0)
+            elif writer is None:
+                self.plan.run(nsteps, use_graph=use_graph, check_every=check_every)
+            else:
                 day = 0
                 for stop in range(K, nsteps + 1, K):
                     self.plan.run_to(nsteps, stop, use_graph=use_graph, check_every=check_every)
                     day += 1
-                    fields = {k: asnumpy(v) for k, v in self.plan.get_history(day).items()}
-                    if not self.batched:
-                        fields = {k: v[0] for k, v in fields.items()}
-                    writer.add(day, fields)
+                    self._write(writer, day)
                     if verbose and progress_every and (stop % progress_every) < K:
                         print(f"  day {stop / spd:6.2f}   [{_time.time() - t0:5.1f}s]", flush=True)
                 if nsteps % K:
                     self.plan.run_to(nsteps, 0, use_graph=use_graph, check_every=check_every)
-            finally:
+        finally:
+            # a failed writer or an interrupt leaves no half-finished run behind
+            self.plan.run_abort()
+            if writer is not None:
                 self.plan.set_history(0, keep_last=True)
+        if writer is not None:
             writer.close()
         adv = nsteps if lengths is None else int(max(lengths))
         if tv is not None:

@@ -45,8 +45,59 @@ def _host_fields(fields, member, apply_gt2gr_z):
     return f
 
 
+def fortran_records(planes):
+    """Bytes of one big-endian Fortran sequential record per 2-D plane of
+    ``planes`` (n, nlat, nlon): a 4-byte length, the plane as real*4, the
+    length again -- packed in one go through a structured dtype."""
+    planes = np.asarray(planes)
+    n, npts = planes.shape[0], int(np.prod(planes.shape[1:]))
+    rec = np.empty(n, dtype=[("head", ">i4"), ("body", ">f4", (npts,)), ("tail", ">i4")])
+    rec["head"] = rec["tail"] = 4 * npts
+    rec["body"] = planes.reshape(n, npts)
+    return rec.tobytes()
+
+
+def history_planes(f, nlev):
+    """The (n, nlat, nlon) planes of one output time in file order: each
+    variable of ``VARS`` level by level, 2-D variables as one plane."""
+    out = []
+    for name in VARS:
+        a = f[name]
+        out.append(a[None] if a.ndim == 2 else np.moveaxis(a, -1, 0))
+    planes = np.concatenate(out, axis=0)
+    if planes.shape[0] != 7 * nlev + 1:
+        raise ValueError(f"history fields give {planes.shape[0]} planes, expected {7 * nlev + 1}")
+    return planes
+
+
+def _rows(values, fmt, per=6):
+    """Descriptor continuation lines: ``per`` formatted values a line."""
+    return [" " + " ".join(fmt.format(x) for x in values[i:i + per])
+            for i in range(0, len(values), per)]
+
+
+def grads_descriptor(data_name, title, nlon, lat_deg, sig, ntime):
+    """Text of the GrADS ``.ctl`` descriptor of a history file: sequential
+    big-endian records, north-first latitudes declared with YREV (listed
+    south to north), sigma levels, daily steps, the eight variables."""
+    lat = np.asarray(lat_deg, dtype=np.float64)[::-1]
+    sig = np.asarray(sig, dtype=np.float64)
+    head = [("DSET", f"^{data_name}"), ("OPTIONS", "SEQUENTIAL YREV BIG_ENDIAN"),
+            ("TITLE", title), ("UNDEF", "-999."),
+            ("XDEF", f"{nlon} LINEAR 0. {360.0 / nlon:.6g}")]
+    lines = [f"{key} {val}" for key, val in head]
+    lines += [f"YDEF {lat.size} LEVELS"] + _rows(lat, "{:9.4f}")
+    lines += [f"ZDEF {sig.size} LEVELS"] + _rows(sig, "{:.6g}")
+    lines += [f"TDEF {max(ntime, 1)} LINEAR 01jan0001 1dy", f"VARS {len(VARS)}"]
+    lines += [f"{v:4s} {0 if v == 'p' else sig.size:3d} 99 {VAR_INFO[v][0]} [{VAR_INFO[v][1]}]"
+              for v in VARS]
+    return "\n".join(lines + ["ENDVARS"]) + "\n"
+
+
 class GradsWriter:
-    """Writes ``out.grd`` + ``out.ctl`` GrADS flat binary."""
+    """History as a GrADS flat binary file and its ``.ctl`` descriptor
+    (the byte layout of ``output.py:38-102``): every ``add`` appends the
+    7 L + 1 records of one output time, ``close`` writes the descriptor."""
 
     def __init__(self, path, spectral, vertical, ctl_path=None, apply_gt2gr_z=True,
                  title="PyLBM-Isca T42L20", member=None):
@@ -60,21 +111,9 @@ class GradsWriter:
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
         self.fh = open(path, "wb")
 
-    def _record(self, a2d):
-        """One Fortran sequential record of a (nlat, nlon) real*4 field."""
-        data = np.ascontiguousarray(a2d, dtype=">f4")
-        mark = np.array([data.size * 4], dtype=">i4").tobytes()
-        self.fh.write(mark + data.tobytes() + mark)
-
     def add(self, day, fields):
         f = _host_fields(fields, self.member, self.apply_gt2gr_z)
-        for name in VARS:
-            a = f[name]
-            if a.ndim == 2:
-                self._record(a)
-            else:
-                for k in range(a.shape[-1]):
-                    self._record(a[..., k])
+        self.fh.write(fortran_records(history_planes(f, int(self.vg.kmax))))
         self.ntime += 1
 
     def close(self):
@@ -83,26 +122,11 @@ class GradsWriter:
         self.write_ctl()
 
     def write_ctl(self):
-        lat = np.asarray(asnumpy(self.sp.lat_deg))[::-1]    # listed south -> north with YREV
-        sig = np.asarray(asnumpy(self.vg.sig))
-        nlon, kmax = int(self.sp.nlon), int(self.vg.kmax)
-        lines = [f"DSET ^{os.path.basename(self.path)}",
-                 "OPTIONS SEQUENTIAL YREV BIG_ENDIAN",
-                 f"TITLE {self.title}",
-                 "UNDEF -999.",
-                 f"XDEF {nlon} LINEAR 0. {360.0 / nlon:.6g}",
-                 f"YDEF {lat.size} LEVELS"]
-        lines += [" " + " ".join(f"{x:9.4f}" for x in lat[i:i + 6]) for i in range(0, lat.size, 6)]
-        lines.append(f"ZDEF {kmax} LEVELS")
-        lines += [" " + " ".join(f"{x:.6g}" for x in sig[i:i + 6]) for i in range(0, kmax, 6)]
-        lines.append(f"TDEF {max(self.ntime, 1)} LINEAR 01jan0001 1dy")
-        lines.append(f"VARS {len(VARS)}")
-        for name in VARS:
-            desc, unit = VAR_INFO[name]
-            lines.append(f"{name:4s} {0 if name == 'p' else kmax:3d} 99 {desc} [{unit}]")
-        lines.append("ENDVARS")
+        text = grads_descriptor(os.path.basename(self.path), self.title, int(self.sp.nlon),
+                                np.asarray(asnumpy(self.sp.lat_deg)),
+                                np.asarray(asnumpy(self.vg.sig)), self.ntime)
         with open(self.ctl_path, "w") as fh:
-            fh.write("\n".join(lines) + "\n")
+            fh.write(text)
 
 
 class NetCDFWriter:
@@ -157,7 +181,8 @@ class NetCDFWriter:
 
 
 class MultiWriter:
-    """Fan a history stream out to several writers."""
+    """Hands every output time to each of several writers in turn
+    (``output.py:155-165``); ``None`` entries are dropped."""
 
     def __init__(self, *writers):
         self.writers = [w for w in writers if w is not None]

@@ -29,10 +29,37 @@ def _dev(x, dtype=None):
     if isinstance(x, torch.Tensor):
         t = x.to(device="cuda")
     else:
-        t = torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+        a = np.ascontiguousarray(x)
+        t = torch.as_tensor(a if a.flags.writeable else a.copy(), device="cuda")
     if dtype is not None and t.dtype != dtype:
         t = t.to(dtype)
     return t.contiguous()
+
+
+def _shape(x):
+    """Shape of an array-like without moving it."""
+    return tuple(int(n) for n in (x.shape if hasattr(x, "shape") else np.shape(x)))
+
+
+def _lead_axis(x, nd):
+    """``x`` with a leading member axis when it has ``nd`` dimensions."""
+    if len(_shape(x)) != nd:
+        return x
+    return x.unsqueeze(0) if hasattr(x, "unsqueeze") else np.asarray(x)[None]
+
+
+def _expand(x, shp, what, dtype):
+    """``x`` as a contiguous device tensor of shape ``shp`` (broadcast);
+    ValueError, before any device transfer, if it does not broadcast."""
+    have = _shape(x)
+    try:
+        ok = np.broadcast_shapes(have, tuple(shp)) == tuple(shp)
+    except ValueError:
+        ok = False
+    if not ok:
+        raise ValueError(f"{what}: shape {have} does not broadcast to {tuple(shp)}")
+    t = _dev(x, dtype)
+    return t if tuple(t.shape) == tuple(shp) else t.expand(shp).contiguous()
 
 
 def zero_damping(nmax, nlev):
@@ -67,6 +94,8 @@ class Plan:
         d.tangent_linear = int(bool(tangent_linear))
         self.tangent_linear = bool(tangent_linear)
         d.inistp = int(inistp)
+        self.inistp, self.delt = int(inistp), float(delt)
+        self.strictly_linear = bool(strictly_linear)
         d.skip_low_n_vdiff = 1
         d.radius, d.delt, d.robert = float(radius), float(delt), float(robert)
         d.h_sigm = _lib.dptr(self.sigm)
@@ -109,13 +138,65 @@ class Plan:
         n = _lib.check(self._lib.lbm_plan_get(self._h, name.encode(), _lib.dptr(out), count))
         return out[:n]
 
+    # -- shape checks (before any device call) -----------------------------
+    def _grid(self, x, what, nk=None, lead=None):
+        """(B, nlat, nlon, K) float64 device tensor of a grid argument; raises
+        ValueError, before touching the device, when its latitude/longitude
+        (or level, member) extents are not the plan's."""
+        self._grid_check(x, what, nk, lead)
+        return _dev(x, _torch().float64)
+
+    def _grid_check(self, x, what, nk=None, lead=None):
+        shp = _shape(x)
+        if len(shp) != 4 or shp[1:3] != (self.nlat, self.nlon):
+            raise ValueError(f"{what}: grid field must be (B, {self.nlat}, {self.nlon}, K), "
+                             f"got {shp}")
+        if nk is not None and shp[3] != nk:
+            raise ValueError(f"{what}: expected {nk} levels, got {shp[3]}")
+        if lead is not None and shp[0] not in lead:
+            raise ValueError(f"{what}: leading axis must be one of {sorted(set(lead))}, "
+                             f"got {shp[0]}")
+
+    def _surf(self, x, what, lead):
+        """(B, nlat, nlon) float64 device tensor of a surface argument."""
+        self._surf_check(x, what, lead)
+        return _dev(x, _torch().float64)
+
+    def _surf_check(self, x, what, lead):
+        shp = _shape(x)
+        if len(shp) != 3 or shp[1:] != (self.nlat, self.nlon) or shp[0] not in lead:
+            raise ValueError(f"{what}: surface field must be ({'|'.join(map(str, sorted(set(lead))))}"
+                             f", {self.nlat}, {self.nlon}), got {shp}")
+
+    def _spec(self, x, what):
+        """(B, nmax+1, nmax+1, K) complex128 device tensor of a spectral argument."""
+        self._spec_check(x, what)
+        return _dev(x, _torch().complex128)
+
+    def _spec_check(self, x, what):
+        shp = _shape(x)
+        M = self.nmax + 1
+        if len(shp) != 4 or shp[1:3] != (M, M):
+            raise ValueError(f"{what}: spectral field must be (B, {M}, {M}, K), got {shp}")
+
+    def _pair(self, a, b, what, spectral):
+        """Both arguments of a two-field transform, checked (same shape, the
+        plan's extents) before either moves to the device."""
+        chk = self._spec_check if spectral else self._grid_check
+        chk(a, what)
+        chk(b, what)
+        if _shape(a) != _shape(b):
+            raise ValueError(f"{what}: the two fields differ in shape")
+        dt = _torch().complex128 if spectral else _torch().float64
+        return _dev(a, dt), _dev(b, dt)
+
     # -- transforms --------------------------------------------------------
     def _call(self, fn, *ptrs, B, nk):
         _lib.check(fn(self._h, *ptrs, int(B), int(nk), _stream()))
 
     def grid_to_wave(self, g):
         torch = _torch()
-        g = _dev(g, torch.float64)
+        g = self._grid(g, "grid_to_wave")
         B, J, I, K = g.shape
         w = torch.empty((B, self.nmax + 1, self.nmax + 1, K), dtype=torch.complex128, device="cuda")
         self._call(self._lib.lbm_grid_to_wave, _lib.ptr(g), _lib.ptr(w), B=B, nk=K)
@@ -123,7 +204,7 @@ class Plan:
 
     def wave_to_grid(self, w):
         torch = _torch()
-        w = _dev(w, torch.complex128)
+        w = self._spec(w, "wave_to_grid")
         B, M, _, K = w.shape
         g = torch.empty((B, self.nlat, self.nlon, K), dtype=torch.float64, device="cuda")
         self._call(self._lib.lbm_wave_to_grid, _lib.ptr(w), _lib.ptr(g), B=B, nk=K)
@@ -131,8 +212,7 @@ class Plan:
 
     def uv_to_vordiv(self, u, v):
         torch = _torch()
-        u = _dev(u, torch.float64)
-        v = _dev(v, torch.float64)
+        u, v = self._pair(u, v, "uv_to_vordiv", False)
         B, J, I, K = u.shape
         shp = (B, self.nmax + 1, self.nmax + 1, K)
         vor = torch.empty(shp, dtype=torch.complex128, device="cuda")
@@ -143,8 +223,7 @@ class Plan:
 
     def psichi_to_uv(self, psi, chi):
         torch = _torch()
-        psi = _dev(psi, torch.complex128)
-        chi = _dev(chi, torch.complex128)
+        psi, chi = self._pair(psi, chi, "psichi_to_uv", True)
         B, M, _, K = psi.shape
         shp = (B, self.nlat, self.nlon, K)
         u = torch.empty(shp, dtype=torch.float64, device="cuda")
@@ -155,7 +234,7 @@ class Plan:
 
     def hgrad(self, w):
         torch = _torch()
-        w = _dev(w, torch.complex128)
+        w = self._spec(w, "hgrad")
         B, M, _, K = w.shape
         shp = (B, self.nlat, self.nlon, K)
         gx = torch.empty(shp, dtype=torch.float64, device="cuda")
@@ -176,7 +255,7 @@ class Plan:
 
     def grid_to_wave_adj(self, w):
         """Transpose of grid_to_wave: spectral (B, M, M, K) -> grid."""
-        w = _dev(w, _torch().complex128)
+        w = self._spec(w, "grid_to_wave_adj")
         B, _, _, K = w.shape
         (g,) = self._grid_out(B, K, 1)
         self._call(self._lib.lbm_grid_to_wave_adj, _lib.ptr(w), _lib.ptr(g), B=B, nk=K)
@@ -184,7 +263,7 @@ class Plan:
 
     def wave_to_grid_adj(self, g):
         """Transpose of wave_to_grid: grid (B, J, I, K) -> spectral."""
-        g = _dev(g, _torch().float64)
+        g = self._grid(g, "wave_to_grid_adj")
         B, _, _, K = g.shape
         (w,) = self._spec_out(B, K, 1)
         self._call(self._lib.lbm_wave_to_grid_adj, _lib.ptr(g), _lib.ptr(w), B=B, nk=K)
@@ -193,7 +272,7 @@ class Plan:
     def uv_to_vordiv_adj(self, vor, div):
         """Transpose of uv_to_vordiv: spectral (vor, div) -> grid (u, v)."""
         torch = _torch()
-        vor, div = _dev(vor, torch.complex128), _dev(div, torch.complex128)
+        vor, div = self._pair(vor, div, "uv_to_vordiv_adj", True)
         B, _, _, K = vor.shape
         u, v = self._grid_out(B, K, 2)
         self._call(self._lib.lbm_uv_to_vordiv_adj, _lib.ptr(vor), _lib.ptr(div), _lib.ptr(u),
@@ -203,7 +282,7 @@ class Plan:
     def psichi_to_uv_adj(self, u, v):
         """Transpose of psichi_to_uv: grid (u, v) -> spectral (psi, chi)."""
         torch = _torch()
-        u, v = _dev(u, torch.float64), _dev(v, torch.float64)
+        u, v = self._pair(u, v, "psichi_to_uv_adj", False)
         B, _, _, K = u.shape
         psi, chi = self._spec_out(B, K, 2)
         self._call(self._lib.lbm_psichi_to_uv_adj, _lib.ptr(u), _lib.ptr(v), _lib.ptr(psi),
@@ -213,7 +292,7 @@ class Plan:
     def hgrad_adj(self, gx, gy):
         """Transpose of hgrad: grid (gx, gy) -> spectral."""
         torch = _torch()
-        gx, gy = _dev(gx, torch.float64), _dev(gy, torch.float64)
+        gx, gy = self._pair(gx, gy, "hgrad_adj", False)
         B, _, _, K = gx.shape
         (w,) = self._spec_out(B, K, 1)
         self._call(self._lib.lbm_hgrad_adj, _lib.ptr(gx), _lib.ptr(gy), _lib.ptr(w), B=B, nk=K)
@@ -221,8 +300,15 @@ class Plan:
 
     # -- model -------------------------------------------------------------
     def set_basic_state(self, u, v, t, q, ps):
-        torch = _torch()
-        f = [_dev(x, torch.float64) for x in (u, v, t, q, ps)]
+        """nbasic basic states: u, v, t, q (nbasic, J, I, L), ps (nbasic, J, I) hPa."""
+        nb = (self.nbasic,)
+        for k, x in (("u", u), ("v", v), ("t", t), ("q", q)):
+            self._grid_check(x, f"basic state {k}", self.nlev, nb)
+        shp = _shape(ps)
+        if len(shp) != 3 or shp != (self.nbasic, self.nlat, self.nlon):
+            raise ValueError(f"basic state ps: must be ({self.nbasic}, {self.nlat}, {self.nlon}), "
+                             f"got {shp}")
+        f = [_dev(x, _torch().float64) for x in (u, v, t, q, ps)]
         _lib.check(self._lib.lbm_set_basic_state(self._h, *[_lib.ptr(x) for x in f], _stream()))
 
     def set_forcing(self, frc):
@@ -230,9 +316,7 @@ class Plan:
         if frc is None:
             _lib.check(self._lib.lbm_set_forcing(self._h, None, 0, _stream()))
             return
-        f = _dev(frc, torch.float64)
-        if f.dim() == 3:
-            f = f.unsqueeze(0)
+        f = self._grid(_lead_axis(frc, 3), "temperature forcing", self.nlev, (1, self.nmem))
         _lib.check(self._lib.lbm_set_forcing(self._h, _lib.ptr(f), int(f.shape[0]), _stream()))
 
     def set_steady_forcing(self, vor=None, div=None, ps=None):
@@ -240,18 +324,21 @@ class Plan:
         or with a leading member axis; all None removes them."""
         torch = _torch()
         E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
-        args, nf = [], None
+        given, nf = [], None
         for x, nd in ((vor, 3), (div, 3), (ps, 2)):
             if x is None:
-                args.append(None)
+                given.append(None)
                 continue
-            x = _dev(x, torch.float64)
-            if x.dim() == nd:
-                x = x.unsqueeze(0)
-            if nf is not None and x.shape[0] != nf:
+            x = _lead_axis(x, nd)
+            if nd == 3:
+                self._grid_check(x, "steady forcing", self.nlev, (1, self.nmem))
+            else:
+                self._surf_check(x, "steady ln ps forcing", (1, self.nmem))
+            if nf is not None and _shape(x)[0] != nf:
                 raise ValueError("steady forcings need the same member count")
-            nf = int(x.shape[0])
-            args.append(x.contiguous())
+            nf = _shape(x)[0]
+            given.append(x)
+        args = [None if x is None else _dev(x, torch.float64) for x in given]
         self._steady = args
         _lib.check(self._lib.lbm_set_steady_forcing(self._h, *[_lib.ptr(x) for x in args],
                                                     int(nf or 0), _stream()))
@@ -265,8 +352,7 @@ class Plan:
             if x is None:
                 args.append(None)
                 continue
-            x = _dev(x, torch.float64)
-            x = x.expand(shp).contiguous() if tuple(x.shape) != shp else x
+            x = _expand(x, shp, "initial perturbation", torch.float64)
             args.append(x)
         self._state_inputs = args
         _lib.check(self._lib.lbm_set_state(self._h, *[_lib.ptr(x) for x in args], _stream()))
@@ -282,8 +368,7 @@ class Plan:
             if x is None:
                 args.append(None)
                 continue
-            x = _dev(x, torch.complex128)
-            x = x.expand(shp).contiguous() if tuple(x.shape) != shp else x
+            x = _expand(x, shp, "spectral initial state", torch.complex128)
             args.append(x)
         self._state_inputs = args
         _lib.check(self._lib.lbm_set_spectral_state(self._h, *[_lib.ptr(x) for x in args],
@@ -384,12 +469,14 @@ class Plan:
         torch = _torch()
         single = (gtf.dtype == torch.float32) if isinstance(gtf, torch.Tensor) else \
             (np.asarray(gtf).dtype == np.float32)
-        g = _dev(gtf, torch.float64)
-        if g.dim() == 4:
-            g = g.unsqueeze(0)
-        if g.dim() != 5 or tuple(g.shape[2:]) != (self.nlat, self.nlon, self.nlev):
+        shp = _shape(gtf)
+        if len(shp) == 4:
+            shp = (1,) + shp
+        if len(shp) != 5 or shp[2:] != (self.nlat, self.nlon, self.nlev) or \
+                shp[0] not in (1, self.nmem):
             raise ValueError(f"MJO forcing must be (days, {self.nlat}, {self.nlon}, {self.nlev}) "
                              "or (nmem, days, ...)")
+        g = _dev(gtf, torch.float64).reshape(shp)
         _lib.check(self._lib.lbm_set_mjo_forcing(self._h, _lib.ptr(g), int(g.shape[0]),
                                                  int(g.shape[1]), int(nt), int(spinup), int(call0),
                                                  int(single), _stream()))
@@ -411,6 +498,108 @@ class Plan:
         _lib.check(self._lib.lbm_get_spectral(self._h, int(level), *[_lib.ptr(x) for x in out],
                                               _lib.ptr(pi), _stream()))
         return out + [pi]
+
+    def set_state_keep(self, u=None, v=None, t=None, q=None, ps=None):
+        """set_initial_perturbation (``model.py:142-148``): given fields
+        replace both time levels, omitted ones keep each level's values."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        args = []
+        for x, shp in ((u, (E, J, I, L)), (v, (E, J, I, L)), (t, (E, J, I, L)),
+                       (q, (E, J, I, L)), (ps, (E, J, I))):
+            args.append(None if x is None else
+                        _expand(x, shp, "initial perturbation", torch.float64))
+        self._state_inputs = args
+        _lib.check(self._lib.lbm_set_state_keep(self._h, *[_lib.ptr(x) for x in args],
+                                                _stream()))
+
+    def get_prev_state(self):
+        """Grid fields (u, v, t, q, ps') of the filtered previous level."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        out = [torch.empty((E, J, I, L), dtype=torch.float64, device="cuda") for _ in range(4)]
+        ps = torch.empty((E, J, I), dtype=torch.float64, device="cuda")
+        _lib.check(self._lib.lbm_get_prev_state(self._h, *[_lib.ptr(x) for x in out],
+                                                _lib.ptr(ps), _stream()))
+        return out + [ps]
+
+    def _spec5(self, fields, what):
+        """(vor, div, t, q, pi) complex128 device tensors of the plan's shapes."""
+        torch = _torch()
+        E, M, L = self.nmem, self.nmax + 1, self.nlev
+        for x, shp in zip(fields, [(E, M, M, L)] * 4 + [(E, M, M)]):
+            if x is None:
+                raise ValueError(f"{what}: every field is needed")
+            if _shape(x) != shp:
+                raise ValueError(f"{what}: expected {shp}, got {_shape(x)}")
+        return [_dev(x, torch.complex128) for x in fields]
+
+    def _spec5_check(self, fields, what):
+        E, M, L = self.nmem, self.nmax + 1, self.nlev
+        if len(fields) != 5:
+            raise ValueError(f"{what}: five fields (vor, div, t, q, pi)")
+        for x, shp in zip(fields, [(E, M, M, L)] * 4 + [(E, M, M)]):
+            if x is None or _shape(x) != shp:
+                raise ValueError(f"{what}: expected {shp}, got {None if x is None else _shape(x)}")
+
+    def _spec5_out(self):
+        torch = _torch()
+        E, M, L = self.nmem, self.nmax + 1, self.nlev
+        out = [torch.empty((E, M, M, L), dtype=torch.complex128, device="cuda") for _ in range(4)]
+        return out + [torch.empty((E, M, M), dtype=torch.complex128, device="cuda")]
+
+    def tendencies(self, u=None, v=None, t=None, q=None, ps=None):
+        """Explicit spectral tendencies (vor, div, t, q, pi) of a grid state
+        with the plan's basic state and forcings (``dynamics.py:156-271``)."""
+        torch = _torch()
+        E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
+        args = []
+        for x, shp in ((u, (E, J, I, L)), (v, (E, J, I, L)), (t, (E, J, I, L)),
+                       (q, (E, J, I, L)), (ps, (E, J, I))):
+            args.append(None if x is None else _expand(x, shp, "state", torch.float64))
+        out = self._spec5_out()
+        _lib.check(self._lib.lbm_tendencies(self._h, *[_lib.ptr(x) for x in args],
+                                            *[_lib.ptr(x) for x in out], _stream()))
+        return out
+
+    def prepare_dt(self, dt):
+        _torch().cuda.current_stream().synchronize()
+        _lib.check(self._lib.lbm_prepare_dt(self._h, float(dt)))
+
+    def step(self, dt, prev, tend):
+        """One semi-implicit leapfrog step (``integrator.py:264-315``): prev and
+        tend are (vor, div, t, q, pi) spectral tensors; returns the new level."""
+        self._spec5_check(prev, "previous level")
+        self._spec5_check(tend, "tendencies")
+        p = self._spec5(prev, "previous level")
+        t = self._spec5(tend, "tendencies")
+        out = self._spec5_out()
+        _lib.check(self._lib.lbm_step(self._h, float(dt), *[_lib.ptr(x) for x in p],
+                                      *[_lib.ptr(x) for x in t], *[_lib.ptr(x) for x in out],
+                                      _stream()))
+        return out
+
+    def run_steps(self, nsteps, count, use_graph=True, check_every=0):
+        """Advance the run of ``run(nsteps)`` by ``count`` executed steps."""
+        _lib.check(self._lib.lbm_run_steps(self._h, int(nsteps), int(count),
+                                           int(bool(use_graph)), int(check_every), _stream()))
+
+    def run_abort(self):
+        _lib.check(self._lib.lbm_run_abort(self._h))
+
+    def run_segment_done(self):
+        n = ctypes.c_int32()
+        _lib.check(self._lib.lbm_run_segment_done(self._h, ctypes.byref(n)))
+        return n.value
+
+    def spec_scale(self, xi, kind):
+        """(B, M, M, K) complex times -n(n+1)/a^2 (kind 0) or its inverse (1)."""
+        t = self._spec(xi, "spectral scaling")
+        out = _torch().empty_like(t)
+        B, _, _, K = t.shape
+        _lib.check(self._lib.lbm_spec_scale(self._h, _lib.ptr(t), _lib.ptr(out), int(B), int(K),
+                                            int(kind), _stream()))
+        return out
 
     def steps_taken(self):
         n = ctypes.c_int64()

@@ -20,19 +20,26 @@ from .backend import asnumpy
 
 
 def detect_grid(path):
-    """``(nlon, nlat)`` of a Fortran sequential field file from its first
-    record marker, assuming ``nlon = 2 nlat`` (``regrid.py:37-58``)."""
+    """Native ``(nlon, nlat)`` of a big-endian Fortran sequential field file.
+
+    The file's first 4-byte record marker is the byte count of one real*4
+    field; with two longitudes per latitude (every Gaussian grid this model
+    reads) the point count fixes the grid.  Behaviour of
+    ``regrid.py:37-58``.
+    """
     with open(path, "rb") as fh:
-        marker = int(np.frombuffer(fh.read(4), dtype=">i4")[0])
-    if marker <= 0 or marker % 4:
-        raise ValueError(f"{path}: unreadable record marker {marker}")
-    npts = marker // 4
-    nlat = int(round((npts / 2.0) ** 0.5))
-    nlon = 2 * nlat
-    if nlon * nlat != npts:
-        raise ValueError(f"{path}: {npts} points per record is not a 2:1 Gaussian grid; "
-                         "pass the source grid explicitly")
-    return nlon, nlat
+        head = fh.read(4)
+    if len(head) < 4:
+        raise ValueError(f"{path}: shorter than one record marker")
+    nbytes = int.from_bytes(head, "big", signed=True)
+    npts, rem = divmod(nbytes, 4)
+    if nbytes <= 0 or rem:
+        raise ValueError(f"{path}: first record marker {nbytes} is not a real*4 field size")
+    nlat = int(np.sqrt(npts / 2.0) + 0.5)
+    if 2 * nlat * nlat != npts:
+        raise ValueError(f"{path}: {npts} values per field do not make an nlon = 2 nlat grid; "
+                         "give the source grid instead of detecting it")
+    return 2 * nlat, nlat
 
 
 def resize_spec(xi, dst_nmax, batched=False):

@@ -103,21 +103,15 @@ class Spectral:
         return back(self.plan.wave_to_grid(w))
 
     # -- differential operators (spectral.py:202-276) -----------------------
-    def _nscale(self, xi, fac):
-        torch = _torch()
-        t = torch.as_tensor(xi, device="cuda") if not isinstance(xi, torch.Tensor) else xi
-        nd = t.dim()
-        lead = 1 if self.batched else 0
-        # n is axis lead+1; trailing level axis optional
-        shape = [1] * nd
-        shape[lead + 1] = -1
-        return t * fac.reshape(shape)
-
     def laplacian(self, xi):
-        return self._nscale(xi, self.laplace)
+        """xi times -n(n+1)/a^2 (``spectral.py:202-204``), on the device."""
+        w, back = self._spec4(xi)
+        return back(self.plan.spec_scale(w, 0))
 
     def inverse_laplacian(self, xi):
-        return self._nscale(xi, self.inv_laplace)
+        """xi times -a^2/(n(n+1)), zero at n = 0 (``spectral.py:206-208``)."""
+        w, back = self._spec4(xi)
+        return back(self.plan.spec_scale(w, 1))
 
     def hgrad(self, xi):
         w, back = self._spec4(xi)

@@ -41,6 +41,10 @@ class VerticalGrid:
         self.ww = np.tril(np.ones((k, k)), -1) * (RAIR * self.dlog3)[None, :]
         self.ww[np.arange(k), np.arange(k)] = RAIR * self.alpha
         self.tbar = self.gg = self.tmtx = self.whmtx = self.gcmtx = None
+        self._ww_dev = None
+        # the BasicStateTerms that last set the reference temperature (the
+        # semi-implicit operators of an Integrator built on this grid use it)
+        self.basic_terms = None
 
     def set_reference_temperature(self, tbar):
         """gg, tmtx, whmtx, gcmtx about a reference Tv profile (vertical.py:139-193)."""
@@ -57,11 +61,31 @@ class VerticalGrid:
         self.gcmtx = np.outer(self.gg, self.dsig)
 
     def hydrostatic(self, tv):
-        """phi = Tv @ WW^T over the last axis (vertical.py:196-203)."""
+        """phi = Tv @ WW^T over the last (level) axis of a grid or spectral
+        array (``vertical.py:196-203``): WW is R alpha on the diagonal and
+        R dlog3_l below it, so phi_k is a running sum up the column, formed
+        on the device (``lbm_hydrostatic``)."""
+        import ctypes
         import torch
-        ww = torch.as_tensor(self.ww, device="cuda")
-        t = torch.as_tensor(tv, device="cuda") if not isinstance(tv, torch.Tensor) else tv
-        return t.to(torch.float64) @ ww.T
+        from . import _lib
+        t = tv if isinstance(tv, torch.Tensor) else torch.as_tensor(np.asarray(tv))
+        t = t.to("cuda")
+        if not t.is_complex():
+            t = t.to(torch.float64)
+        else:
+            t = t.to(torch.complex128)
+        t = t.contiguous()
+        if t.shape[-1] != self.kmax:
+            raise ValueError(f"hydrostatic: the last axis must hold the {self.kmax} levels")
+        if getattr(self, "_ww_dev", None) is None:
+            self._ww_dev = (torch.as_tensor(self.alpha, device="cuda"),
+                            torch.as_tensor(self.dlog3, device="cuda"))
+        al, dl = self._ww_dev
+        out = torch.empty_like(t)
+        _lib.check(_lib.load().lbm_hydrostatic(
+            _lib.ptr(t), _lib.ptr(out), t.numel() // self.kmax, self.kmax, 2 if t.is_complex() else 1,
+            _lib.ptr(al), _lib.ptr(dl), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
 
 
 def compute_tbar(gft):

@@ -3,7 +3,8 @@ diffusion rates) against the oracle's restatement."""
 
 import numpy as np
 
-from lbm_isca_amd.model import Config, _vdif_rate, damping_coefficients
+from lbm_isca_amd.integrator import VerticalDiffusion, vdif_rate as _vdif_rate
+from lbm_isca_amd.model import Config, damping_coefficients
 from oracle.integ import Damping, vdif_rate
 from oracle.vgrid import VerticalOps
 
@@ -29,3 +30,22 @@ def test_bad_drag_profile_raises():
     cfg = Config(ddragv=[1.0] * 19)
     with pytest.raises(ValueError):
         damping_coefficients(cfg, 42, VerticalOps().sig)
+
+
+def test_vertical_diffusion_operator_matches_oracle():
+    """The host sigma Laplacian and (I - dt nu lop)^-1 of the reference API
+    (integrator.py:178-240) against the oracle's restatement."""
+    import types
+    from oracle.integ import Stepper
+    from oracle.sht import SHT
+    vg = VerticalOps()
+    ours = VerticalDiffusion(types.SimpleNamespace(kmax=vg.kmax, dsig=vg.dsig, dsigm=vg.dsigm),
+                             1e3, 1e3, 1e3, None)
+    vg.set_reference_temperature(np.full(vg.kmax, 250.0))
+    ref = Stepper(SHT(), vg, Damping(42, vg), 1e3, 1e3, 1e3, None)
+    np.testing.assert_allclose(ours.lop, ref.lop, rtol=1e-14, atol=0)
+    ours.prepare(1200.0)
+    mats = ref.matrices(1200.0)
+    for name in ("vor", "div", "t"):
+        np.testing.assert_allclose(ours.minv[name], mats["vinv"][name], rtol=1e-13, atol=1e-15)
+    assert ours.minv["q"] is None
