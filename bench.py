@@ -9,12 +9,13 @@ amplitude U(0.5,3) K/day, 10 x 30 deg e-folding, sin(pi sigma) profile,
 seed 0), FP64, default T42 dt = 1200 s and damping.  One step advances all
 members by one leapfrog step; ``--steps 2160`` is the 30-day run.
 
-Prints one JSON line: member-days/s (full steps only; the two start-up
-half steps of each run are executed inside the timed region but not
-counted as days), the dominant kernel's achieved HBM bandwidth against the
-8 TB/s peak (HIP events on the plan's stream, eager replay of the same
-steps) and the numpy FP64 oracle timed on the host cores on a bounded
-single-member sample of the same workload.
+Prints one JSON line: member-days/s over exactly --steps full steps (the
+run's two start-up half steps execute before the timed region), the
+dominant kernel's achieved HBM bandwidth against the 8 TB/s peak (HIP events
+on the plan's stream, eager replay of the same steps) and the numpy FP64
+oracle timed on the host cores the reference's way (all threads, one
+thread, and a node of concurrent single-thread processes) on the
+single-member solid-body workload.
 
 Two more parts of the metric ride along in the same line:
 
@@ -40,6 +41,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -96,22 +99,90 @@ def build_model(members, precision):
     return m, bs, frc, vg
 
 
-def cpu_baseline(bs, frc, sigm, seconds=15.0):
-    """numpy FP64 oracle on one member of the same workload, bounded."""
-    from oracle import OracleConfig, OracleModel
+def _oracle_member_days(steps):
+    """member-days/s of the numpy FP64 oracle on the single-member
+    solid-body workload (2 K/day at 0N 180E, 20 equal sigma levels): the two
+    start-up half steps warm the caches, then ``steps`` full steps are timed
+    (analysis/benchmark_perf_isca.py:78-106 times a warmed run the same way)."""
+    from lbm_isca_amd import cases
+    from oracle import OracleConfig, OracleModel, SHT, VerticalOps
+    sigm = cases.EQUAL_SIGM_L20
+    sht, vg = SHT(), VerticalOps(sigm)
+    bs = cases.solid_body_state(sht.lat, sht.lon, vg.sig)
+    heat = cases.heating(sht.lat, sht.lon, vg.sig, 0.0, 180.0, 2.0)[None]
     m = OracleModel(bs, 1, OracleConfig(), sigm=sigm)
-    t0 = time.time()
-    m.step(1200.0 / 4, True, frc[:1])        # start-up step, warm caches
-    n = 0
-    t0 = time.time()
-    while time.time() - t0 < seconds:
-        m.step(1200.0, False, frc[:1])
-        n += 1
-    dt = time.time() - t0
-    return {"value": n / 72.0 / dt, "unit": "member-days/s", "cores": os.cpu_count(),
-            "kind": "port",
-            "sample": f"numpy FP64 oracle, member 0 of the workload, {n} full steps "
-                      f"({dt:.1f} s) after one start-up step, BLAS threads = host default"}
+    m.step(300.0, True, heat)
+    m.step(600.0, True, heat)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.step(1200.0, False, heat)
+    dt = time.perf_counter() - t0
+    return steps / 72.0 / dt
+
+
+def cpu_worker(steps):
+    """One single-thread oracle run (a child process of the node figure)."""
+    print(f"member_days_per_s={_oracle_member_days(steps):.9g}", flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """CPU cores this process may use, at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(steps=16, node_steps=12):
+    """The numpy FP64 oracle timed on the host the way the reference times its
+    CPU path: a warmed single-member run with BLAS/FFT threads = the cores
+    and = 1 (analysis/benchmark_perf_isca.py:78-106), and the fully loaded
+    node -- N concurrent single-thread processes, each one member
+    (analysis/benchmark_ensemble_isca.py:225-244).  ``value`` is the node's
+    aggregate member-days/s; ``cores`` the processes used for it."""
+    import subprocess
+    from threadpoolctl import threadpool_limits
+    ncores = host_cores()
+    with threadpool_limits(ncores):
+        all_thr = _oracle_member_days(steps)
+    with threadpool_limits(1):
+        one_thr = _oracle_member_days(steps)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               NUMEXPR_NUM_THREADS="1")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", str(node_steps)]
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, text=True)
+             for _ in range(ncores)]
+    outs = [p.communicate()[0] for p in procs]
+    wall = time.perf_counter() - t0
+    per = [float(line.split("=")[1]) for o in outs for line in o.splitlines()
+           if line.startswith("member_days_per_s=")]
+    if len(per) != ncores:
+        raise RuntimeError(f"{ncores - len(per)} CPU worker processes failed")
+    node = float(sum(per))
+    return {"value": round(node, 4), "unit": "member-days/s", "cores": ncores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "threads_all": {"threads": ncores, "member_days_per_s": round(all_thr, 4)},
+            "threads_1": {"threads": 1, "member_days_per_s": round(one_thr, 4)},
+            "node": {"processes": ncores, "member_days_per_s": round(node, 4),
+                     "per_process_median": round(float(np.median(per)), 4),
+                     "wall_s": round(wall, 1)},
+            "sample": f"numpy FP64 oracle, single-member solid-body workload (2 K/day at 0N "
+                      f"180E, 20 equal sigma levels), two start-up steps then {steps} timed full "
+                      f"steps with BLAS threads = {ncores} and = 1; node = {ncores} concurrent "
+                      f"single-thread processes of {node_steps} timed steps each (value)"}
 
 
 def uneven_ensemble(E, seed=1, max_days=None):
@@ -329,7 +400,8 @@ def main():
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"))
     ap.add_argument("--profile-steps", type=int, default=24)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-steps", type=int, default=16,
+                    help="timed oracle steps of the CPU baseline's thread figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP32-vs-FP64 uneven ensemble and the adjoint timing")
@@ -343,14 +415,27 @@ def main():
                     help="response averaging window in days")
     ap.add_argument("--t127-members", type=int, default=32)
     ap.add_argument("--t127-days", type=float, default=20.0)
+    ap.add_argument("--cpu-worker", type=int, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
                     help="only copy a float64 buffer of MIB mebibytes three times (the "
                          "reference kernel that scales the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.cpu_worker is not None:
+        cpu_worker(args.cpu_worker)         # host only: no GPU is touched
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); members are "
               "independent, so several GPUs would each run their own share", file=sys.stderr)
         sys.exit(2)
+
+    # the host baseline first: its worker processes start before this
+    # process initialises the GPU
+    cpu = None
+    if args.only is None and args.calibrate_copy is None and not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']} member-days/s on {cpu['cores']} cores "
+            f"({cpu['cpu_model']})")
 
     import torch
     from lbm_isca_amd import roofline
@@ -374,10 +459,13 @@ def main():
     log(f"warmup {args.warmup} steps ...")
     plan.run(args.warmup, use_graph=True)
     torch.cuda.synchronize()
-    log(f"timed {args.steps} steps ...")
+    # a run's two start-up half steps go before the timed region, which then
+    # holds exactly --steps full steps (model.py:211-251)
+    plan.run_steps(args.steps, m.cfg.inistp, use_graph=True)
     torch.cuda.synchronize()
+    log(f"timed {args.steps} steps ...")
     t0 = time.perf_counter()
-    plan.run(args.steps, use_graph=True)
+    plan.run_to(args.steps, 0, use_graph=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = E * args.steps / 72.0 / elapsed
@@ -420,10 +508,6 @@ def main():
                 log(f"{key} failed: {exc}")
             torch.cuda.empty_cache()
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(bs, frc, vg.sigm, args.cpu_seconds)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "member-days/s", "n_gpus": 1,
@@ -434,7 +518,7 @@ def main():
         "config": {"workload": f"T42L20 {E}-member heating ensemble, solid-body basic state, "
                                f"{args.precision.upper()}, 20 equal sigma levels",
                    "members": E, "resolution": "T42L20", "dt_s": 1200.0,
-                   "graph": True, "startup_steps_in_timed_region": 2},
+                   "graph": True, "startup_steps_in_timed_region": 0},
         "roofline": rl, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(out), flush=True)

@@ -181,7 +181,24 @@ struct DevTabs {
   T *tref = nullptr;                                             // [nb][L]
   T *minv = nullptr;                                             // [nset][nb][M][L][L]
   T *vinv = nullptr;                                             // [nset][4][L][L]
+  // split-BF16x3 plans: Ps, Hs, Pa, Ha as BF16 high / low parts
+  short *split_hi[4] = {}, *split_lo[4] = {};
 };
+
+// hi = bf16(x), lo = bf16(x - hi), both round to nearest even, of the FP32
+// table value x (the rounding the kernels apply to their data operands)
+inline short bf16_rne_host(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+inline float bf16_up_host(short h) {
+  const unsigned u = ((unsigned)(unsigned short)h) << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
 
 template <class T>
 int upload(T **dst, const std::vector<double> &src) {
@@ -197,6 +214,10 @@ void free_tabs(DevTabs<T> &t) {
              t.kv, t.kd, t.ktt, t.kq, t.tref, t.minv, t.vinv};
   for (T *p : ps)
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 4; ++k) {
+    if (t.split_hi[k]) (void)hipFree(t.split_hi[k]);
+    if (t.split_lo[k]) (void)hipFree(t.split_lo[k]);
+  }
   t = DevTabs<T>();
 }
 
@@ -375,6 +396,21 @@ int build_tabs(lbm_plan *p) {
   CHK(upload(&t.Hs, Hs));
   CHK(upload(&t.Pa, Pa));
   CHK(upload(&t.Ha, Ha));
+  if (std::is_same<T, float>::value && p->split) {
+    const std::vector<double> *tabs4[4] = {&Ps, &Hs, &Pa, &Ha};
+    for (int k = 0; k < 4; ++k) {
+      std::vector<short> hi(tabs4[k]->size()), lo(hi.size());
+      for (size_t i = 0; i < hi.size(); ++i) {
+        const float x = (float)(*tabs4[k])[i];
+        hi[i] = bf16_rne_host(x);
+        lo[i] = bf16_rne_host(x - bf16_up_host(hi[i]));
+      }
+      HIPCHK(hipMalloc((void **)&t.split_hi[k], hi.size() * sizeof(short)));
+      HIPCHK(hipMalloc((void **)&t.split_lo[k], lo.size() * sizeof(short)));
+      HIPCHK(hipMemcpy(t.split_hi[k], hi.data(), hi.size() * sizeof(short), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(t.split_lo[k], lo.data(), lo.size() * sizeof(short), hipMemcpyHostToDevice));
+    }
+  }
   std::vector<double> il(M), nn(M), lp(M), rc(J);
   for (int n = 0; n < M; ++n) {
     const double nn1 = double(n) * (n + 1.0);
@@ -432,24 +468,48 @@ int set_twiddles() {
 // ---------------------------------------------------------------------------
 // Synthesis with the synthesis tables P, H and -a^2/(n(n+1)) by default; the
 // transposed analysis passes the analysis tables instead.
+// The pre-split halves of a plan table (Ps, Hs, Pa or Ha) pair.
+template <class T>
+SplitTabs split_of(DevTabs<T> &t, const T *P, const T *H) {
+  SplitTabs sp;
+  const T *tabs4[4] = {t.Ps, t.Hs, t.Pa, t.Ha};
+  for (int k = 0; k < 4; ++k) {
+    if (tabs4[k] == P) sp.Ph = t.split_hi[k], sp.Pl = t.split_lo[k];
+    if (tabs4[k] == H) sp.Hh = t.split_hi[k], sp.Hl = t.split_lo[k];
+  }
+  return sp;
+}
+
+// Recipes whose fields stay in plain FP32 in split-BF16x3 plans: the
+// surface fields (pi, its gradient and the column sums) and every scalar
+// analysis.  pi is analysed again from the ps' grid at every step, so its
+// transform error accumulates over the run instead of staying a per-step
+// rounding; these contractions are a few columns per member.
+template <class R>
+constexpr bool split_allowed() {
+  return !std::is_same<R, SynSurf>::value && !std::is_same<R, AnaScalar>::value &&
+         !std::is_same<R, AdjSynSurf>::value && !std::is_same<R, AdjScalarFft>::value;
+}
+
 template <class T, class R>
 int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
                hipStream_t st, int in_nf = R::NIN, const T *P = nullptr, const T *H = nullptr,
                const T *per_n = nullptr, ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid(tiles_x(ncols, am), p->M, (p->Jh + 31) / 32);
-  if constexpr (std::is_same<T, float>::value) {
+  const T *tP = P ? P : t.Ps, *tH = H ? H : t.Hs;
+  if constexpr (std::is_same<T, float>::value && split_allowed<R>()) {
     if (p->split) {
       hipLaunchKernelGGL((k_syn_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
-                         (cx<T> *)out, P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap,
-                         t.rcos, ncols, C, sl, p->J, p->M, in_nf, am);
+                         (cx<T> *)out, tP, tH, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
+                         p->J, p->M, in_nf, am, split_of<T>(t, tP, tH));
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
   hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     P ? P : t.Ps, H ? H : t.Hs, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
-                     p->J, p->M, in_nf, am);
+                     tP, tH, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M, in_nf, am,
+                     SplitTabs{});
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -464,18 +524,19 @@ int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots
   DevTabs<T> &t = p->tabs<T>();
   // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
   dim3 grid(tiles_x(ncols, am), p->M, (p->M + 63) / 64);
-  if constexpr (std::is_same<T, float>::value) {
+  const T *tP = P ? P : t.Pa, *tH = H ? H : t.Ha;
+  if constexpr (std::is_same<T, float>::value && split_allowed<R>()) {
     if (p->split) {
       hipLaunchKernelGGL((k_ana_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
-                         (cx<T> *)out, P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols,
-                         C, sl, p->J, p->M, out_nf, out_f0, am);
+                         (cx<T> *)out, tP, tH, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
+                         out_nf, out_f0, am, split_of<T>(t, tP, tH));
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
   hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     P ? P : t.Pa, H ? H : t.Ha, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
-                     out_nf, out_f0, am);
+                     tP, tH, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M, out_nf, out_f0, am,
+                     SplitTabs{});
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -1977,6 +2038,8 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
     return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=%d, nlat=%d", d->nmax, d->nlon,
                 d->nlat);
   if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
+  if (d->precision == LBM_BF16X3 && d->nlat % 8)
+    return fail(LBM_ERR_VALUE, "split-BF16x3 needs nlat a multiple of 8 (got %d)", d->nlat);
   if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
   if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");
   if (d->precision != LBM_FP64 && d->precision != LBM_FP32 && d->precision != LBM_BF16X3)

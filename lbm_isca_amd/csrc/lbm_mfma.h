@@ -16,7 +16,9 @@
 //   A[i][k]: lane i + 16(k/4), element k%4;  B[k][j]: lane j + 16(k/4),
 //   element k%4;  D as f32.  Each operand x = hi + lo (hi = bf16(x), lo =
 //   bf16(x - hi)) and a b ~ hi_a hi_b + hi_a lo_b + lo_a hi_b in FP32
-//   accumulation (the lo_a lo_b term, ~2^-16 relative, is dropped).
+//   accumulation (the lo_a lo_b term, ~2^-16 relative, is dropped).  The
+//   Legendre tables come pre-split from the plan (SplitTabs); the data
+//   operand is split as it is staged.
 #pragma once
 #include "lbm_device.h"
 
@@ -104,6 +106,12 @@ constexpr bool any_h() {
   return false;
 }
 
+// Tables pre-split into BF16 high and low parts (split-BF16x3 plans only):
+// the same [s][Jh] layout as the FP32 table they come from.
+struct SplitTabs {
+  const short *Ph = nullptr, *Pl = nullptr, *Hh = nullptr, *Hl = nullptr;
+};
+
 template <class T, class R, bool SPLIT = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
@@ -111,7 +119,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                   const T *__restrict__ Ha,
                                                   const T *__restrict__ nn1a2, int ncols, int C,
                                                   InSlots slots, int J, int M, int out_nf,
-                                                  int out_f0, ActiveMask am) {
+                                                  int out_f0, ActiveMask am, SplitTabs sp) {
   typedef typename Mfma<T>::v4 v4;
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
   // one symmetric/antisymmetric buffer per field: a chunk is stored whole,
@@ -186,22 +194,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int f = 0; f < NIN; ++f) load(jc + 1, f);
     }
     // Legendre operands of this chunk, shared by every field: lane row n_a,
-    // latitude pair 32 jc + 4 q + k_a (FP64/FP32 K = 4 steps) or
-    // 32 jc + 16 (q / 4) + 4 k_a + q % 4 (bf16 K = 16 steps)
+    // latitude pair 32 jc + 4 q + k_a (FP64/FP32 K = 4 steps), or, split,
+    // the pre-split halves of pairs 32 jc + 16 h + 4 k_a .. +3 (bf16 K = 16
+    // steps; Jh is a multiple of 4, so a group is all in range or all out)
     T pr[8], hr[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int jp = 32 * jc + (SPLIT ? 16 * (q >> 2) + 4 * k_a + (q & 3) : 4 * q + k_a);
-      const bool ok = wave_live && row_ok && jp < Jh;
-      pr[q] = (anyP && ok) ? Pa[trow + jp] : (T)0;
-      hr[q] = (anyH && ok) ? Ha[trow + jp] : (T)0;
-    }
     Bf16x3::s4 prh[2], prl[2], hrh[2], hrl[2];
     if constexpr (SPLIT) {
+      const Bf16x3::s4 z4 = {0, 0, 0, 0};
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        if (anyP) Bf16x3::split(reinterpret_cast<const float *>(pr) + 4 * h, prh[h], prl[h]);
-        if (anyH) Bf16x3::split(reinterpret_cast<const float *>(hr) + 4 * h, hrh[h], hrl[h]);
+        const int jp0 = 32 * jc + 16 * h + 4 * k_a;
+        const bool ok = wave_live && row_ok && jp0 < Jh;
+        const size_t ti = trow + (ok ? jp0 : 0);
+        prh[h] = (anyP && ok) ? *reinterpret_cast<const Bf16x3::s4 *>(sp.Ph + ti) : z4;
+        prl[h] = (anyP && ok) ? *reinterpret_cast<const Bf16x3::s4 *>(sp.Pl + ti) : z4;
+        hrh[h] = (anyH && ok) ? *reinterpret_cast<const Bf16x3::s4 *>(sp.Hh + ti) : z4;
+        hrl[h] = (anyH && ok) ? *reinterpret_cast<const Bf16x3::s4 *>(sp.Hl + ti) : z4;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int jp = 32 * jc + 4 * q + k_a;
+        const bool ok = wave_live && row_ok && jp < Jh;
+        pr[q] = (anyP && ok) ? Pa[trow + jp] : (T)0;
+        hr[q] = (anyH && ok) ? Ha[trow + jp] : (T)0;
       }
     }
 #pragma unroll
@@ -284,7 +300,7 @@ template <class T, class R, int G, bool SPLIT>
 __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_chunk, int r0, int nn, int m,
                                  int s0, int Jh, int jp_a, bool jp_ok, int lane,
                                  const T *__restrict__ Ps, const T *__restrict__ Hs,
-                                 const T *__restrict__ inv_lap,
+                                 const SplitTabs &sp, const T *__restrict__ inv_lap,
                                  typename Mfma<T>::v4 (&EA)[R::NOUT], typename Mfma<T>::v4 (&OA)[R::NOUT],
                                  typename Mfma<T>::v4 (&EB)[R::NOUT], typename Mfma<T>::v4 (&OB)[R::NOUT]) {
   constexpr int NIN = R::NIN, NOUT = R::NOUT;
@@ -295,15 +311,17 @@ __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_
     // one K = 16 step per parity: rows p + 2 (4 (lane/16) + q), q < 4
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-      float pa[4], ha[4];
+      Bf16x3::s4 ph, pl, hh, hl, bh, bl;
       float bA[NOUT][4], bB[NOUT][4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int ra = p + 2 * (4 * k_a + q);
         const bool a_ok = jp_ok && (r0 + ra) < nn;
         const size_t ti = (size_t)(s0 + (a_ok ? m + r0 + ra : m)) * Jh + (jp_ok ? jp_a : 0);
-        pa[q] = a_ok ? (float)Ps[ti] : 0.f;
-        ha[q] = a_ok ? (float)Hs[ti] : 0.f;
+        ph[q] = a_ok ? sp.Ph[ti] : (short)0;
+        pl[q] = a_ok ? sp.Pl[ti] : (short)0;
+        hh[q] = a_ok ? sp.Hh[ti] : (short)0;
+        hl[q] = a_ok ? sp.Hl[ti] : (short)0;
         cx<T> x[NIN], A[NOUT], B[NOUT];
 #pragma unroll
         for (int o = 0; o < NOUT; ++o) A[o] = B[o] = zero<T>();
@@ -318,9 +336,6 @@ __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_
           bB[o][q] = (float)pick<T>(B[o].x, B[o].y, part_b);
         }
       }
-      Bf16x3::s4 ph, pl, hh, hl, bh, bl;
-      Bf16x3::split(pa, ph, pl);
-      Bf16x3::split(ha, hh, hl);
 #pragma unroll
       for (int o = G; o < NOUT; o += 2) {
         if (R::hasA(o)) {
@@ -385,7 +400,7 @@ __device__ __forceinline__ void syn_group(const cx<T> (*xin)[32][8], bool first_
 template <class T, class R, int G, bool SPLIT>
 __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                 const T *__restrict__ Ps, const T *__restrict__ Hs,
-                                const T *__restrict__ inv_lap, const T *__restrict__ rcos, int ncols,
+                                const SplitTabs &sp, const T *__restrict__ inv_lap, const T *__restrict__ rcos, int ncols,
                                 int C, const OutSlots &slots, int J, int M, int in_nf,
                                 cx<T> (*xin)[32][8], int col0) {
   typedef typename Mfma<T>::v4 v4;
@@ -411,8 +426,8 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
         xin[f][r][c] = ok ? in[((size_t)(s0 + m + r0 + r) * in_nf + f) * ncols + col0 + c] : zero<T>();
     }
     __syncthreads();
-    syn_group<T, R, G, SPLIT>(xin, r0 == 0, r0, nn, m, s0, Jh, jp_a, jp_ok, lane, Ps, Hs, inv_lap,
-                              EA, OA, EB, OB);
+    syn_group<T, R, G, SPLIT>(xin, r0 == 0, r0, nn, m, s0, Jh, jp_a, jp_ok, lane, Ps, Hs, sp,
+                              inv_lap, EA, OA, EB, OB);
   }
   const int j_b = lane & 15;
   const int c_b = j_b & 7, part_b = j_b >> 3;
@@ -466,15 +481,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                   const T *__restrict__ inv_lap,
                                                   const T *__restrict__ rcos, int ncols, int C,
                                                   OutSlots slots, int J, int M, int in_nf,
-                                                  ActiveMask am) {
+                                                  ActiveMask am, SplitTabs sp) {
   __shared__ cx<T> xin[R::NIN][32][8];
   const int col0 = am.tile_col0(blockIdx.x);
   if (!am.any8(col0, ncols)) return;
   if ((threadIdx.x >> 7) == 0)
-    syn_body<T, R, 0, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
+    syn_body<T, R, 0, SPLIT>(in, out, Ps, Hs, sp, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
                              col0);
   else
-    syn_body<T, R, 1, SPLIT>(in, out, Ps, Hs, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
+    syn_body<T, R, 1, SPLIT>(in, out, Ps, Hs, sp, inv_lap, rcos, ncols, C, slots, J, M, in_nf, xin,
                              col0);
 }
 

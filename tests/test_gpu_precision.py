@@ -74,3 +74,29 @@ def test_t127l40_bf16x3_against_oracle():
         e = rel_err(a, getattr(ref.ga, f))
         print(f"T127L40 bf16x3 {f}: {e:.2e}")
         assert e < 1e-4, (f, e)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_t127l40_days_low_precision_against_fp64(precision):
+    """Five days of two T127 L40 balanced-jet members (their own basic
+    states, n^-1.5 vorticity perturbation) in FP32 and split-BF16x3 against
+    the same run in FP64 (itself within 1e-9 of the oracle over its first
+    steps, test_gpu_t127.py): every field within 1e-4."""
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases as C
+    sp = L.BatchedSpectral(384, 192, 127)
+    vg = L.VerticalGrid(C.EQUAL_SIGM_L40)
+    bs, _ = C.jet_states(sp, vg.sig, 2, seed=1)
+    vor = C.vorticity_perturbation(sp, vg.kmax, 2, seed=4)
+    out = {}
+    for prec in ("fp64", precision):
+        cfg = L.Config(**C.t127l40_config(precision=prec))
+        m, _, _ = L.make_ensemble_model(bs, 2, cfg, sigm=C.EQUAL_SIGM_L40, per_member_basic=True,
+                                        spectral=sp)
+        m.plan.set_spectral_state(vor=vor)
+        m.plan.run(5 * 192)
+        out[prec] = [x.cpu().numpy() for x in m.plan.get_state()]
+    errs = {f: rel_err(a, b) for f, a, b in zip(("u", "v", "t", "q", "ps"), out[precision],
+                                                out["fp64"]) if np.abs(b).max() > 0}
+    print(precision, {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) < 1e-4, errs
