@@ -214,22 +214,35 @@ def measure_fp32_vs_fp64(E=256, max_days=None):
     import numpy as np
     import torch
     import lbm_isca_amd as L
+    from lbm_isca_amd import tuning
     bs, frc, lengths, sigm = uneven_ensemble(E, max_days=max_days)
     out, states = {}, {}
-    for prec in ("fp64", "fp32"):
-        cfg = L.Config(precision=prec)
-        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
-        m.plan.set_forcing(frc)
-        m.plan.set_lengths(lengths)
+    days = float(lengths.sum()) / 72.0
+
+    def timed_run(m):
         m.plan.run(8, use_graph=True)               # capture + warm caches
         m.plan.set_state()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         m.plan.run(0, use_graph=True)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out[prec] = {"member_days_per_s": round(float(lengths.sum()) / 72.0 / dt, 2),
-                     "seconds": round(dt, 3)}
+        return time.perf_counter() - t0
+
+    for prec in ("fp64", "fp32"):
+        cfg = L.Config(precision=prec)
+        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
+        m.plan.set_forcing(frc)
+        m.plan.set_lengths(lengths)
+        # built-in launch variants, then the autotuned ones (Legendre tiles'
+        # register budget and FFT rows per block timed on this ensemble)
+        dt_builtin = timed_run(m)
+        tuned = tuning.autotune(m.plan, True)
+        m.plan.set_forcing(frc)
+        dt = timed_run(m)
+        out[prec] = {"member_days_per_s": round(days / dt, 2), "seconds": round(dt, 3),
+                     "builtin_member_days_per_s": round(days / dt_builtin, 2),
+                     "autotuned": dict(zip(m.plan.TUNED, tuned["choice"])),
+                     "autotune_report": tuned["report"]}
         states[prec] = [x.cpu().numpy() for x in m.plan.get_state()]
         log(f"uneven ensemble {prec}: {out[prec]}")
         del m
@@ -239,7 +252,8 @@ def measure_fp32_vs_fp64(E=256, max_days=None):
         den = np.sqrt((b ** 2).sum())
         err[f] = float(np.sqrt(((a - b) ** 2).sum()) / den) if den > 0 else 0.0
     return {"workload": f"T42L20 {E}-member uneven ensemble, per-member jet basic states, "
-                        "heatings of the main workload, lengths U(5,60) days, one graph replay",
+                        "heatings of the main workload, lengths U(5,60) days, one graph replay, "
+                        "autotuned launches (built-in variants timed too)",
             "members": E, "member_days": round(float(lengths.sum()) / 72.0, 2),
             "fp64": out["fp64"], "fp32": out["fp32"],
             "speedup_fp32_over_fp64": round(out["fp32"]["member_days_per_s"]

@@ -309,6 +309,11 @@ struct lbm_plan {
   // graphs keyed by (kind, member count of the launches): kind 0 forward
   // chunk, 1 forward single step, 2 adjoint chunk, 3 adjoint single step
   std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  // launch variants chosen by lbm_autotune (0 = the built-in choice):
+  // inverse-FFT rows per block, forward-FFT rows per block, occupancy target
+  // of the level synthesis and of the level tendency analyses
+  static constexpr int NTUNE = 4;
+  int tune[NTUNE] = {0, 0, 0, 0};
   // kernel timing (lbm_profile)
   static constexpr int NPROF = 8;
   bool prof = false;
@@ -491,80 +496,145 @@ constexpr bool split_allowed() {
          !std::is_same<R, AdjSynSurf>::value && !std::is_same<R, AdjScalarFft>::value;
 }
 
-template <class T, class R>
-int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
-               hipStream_t st, int in_nf = R::NIN, const T *P = nullptr, const T *H = nullptr,
-               const T *per_n = nullptr, ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+// Occupancy targets (waves per SIMD) of the autotuned Legendre kernels:
+// variant 0 is the built-in one.
+constexpr int SYN_OCC[3] = {3, 2, 4};
+constexpr int ANA_OCC[3] = {4, 3, 2};
+
+template <class T, class R, int OCC>
+int launch_syn_k(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
+                 hipStream_t st, int in_nf, const T *P, const T *H, const T *per_n, ActiveMask am) {
   DevTabs<T> &t = p->tabs<T>();
   dim3 grid(tiles_x(ncols, am), p->M, (p->Jh + 31) / 32);
   const T *tP = P ? P : t.Ps, *tH = H ? H : t.Hs;
   if constexpr (std::is_same<T, float>::value && split_allowed<R>()) {
     if (p->split) {
-      hipLaunchKernelGGL((k_syn_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
+      hipLaunchKernelGGL((k_syn_mfma<T, R, true, OCC>), grid, dim3(256), 0, st, (const cx<T> *)in,
                          (cx<T> *)out, tP, tH, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl,
                          p->J, p->M, in_nf, am, split_of<T>(t, tP, tH));
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
-  hipLaunchKernelGGL((k_syn_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     tP, tH, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl, p->J, p->M, in_nf, am,
-                     SplitTabs{});
+  hipLaunchKernelGGL((k_syn_mfma<T, R, false, OCC>), grid, dim3(256), 0, st, (const cx<T> *)in,
+                     (cx<T> *)out, tP, tH, per_n ? per_n : t.inv_lap, t.rcos, ncols, C, sl, p->J,
+                     p->M, in_nf, am, SplitTabs{});
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
-// Analysis with the analysis tables gw P, gw H and n(n+1)/a^2 by default;
-// the transposed synthesis passes the synthesis tables instead.
+// Synthesis with the synthesis tables P, H and -a^2/(n(n+1)) by default; the
+// transposed analysis passes the analysis tables instead.  The level
+// synthesis of the time step runs the autotuned occupancy variant.
 template <class T, class R>
-int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
-               hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0, const T *P = nullptr,
-               const T *H = nullptr, const T *per_n = nullptr,
-               ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+int launch_syn(lbm_plan *p, const void *in, void *out, int ncols, int C, OutSlots sl,
+               hipStream_t st, int in_nf = R::NIN, const T *P = nullptr, const T *H = nullptr,
+               const T *per_n = nullptr, ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  if constexpr (std::is_same<R, SynLevel>::value) {
+    if (p->tune[2] == 1)
+      return launch_syn_k<T, R, SYN_OCC[1]>(p, in, out, ncols, C, sl, st, in_nf, P, H, per_n, am);
+    if (p->tune[2] == 2)
+      return launch_syn_k<T, R, SYN_OCC[2]>(p, in, out, ncols, C, sl, st, in_nf, P, H, per_n, am);
+  }
+  return launch_syn_k<T, R, SYN_OCC[0]>(p, in, out, ncols, C, sl, st, in_nf, P, H, per_n, am);
+}
+
+template <class T, class R, int OCC>
+int launch_ana_k(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
+                 hipStream_t st, int out_nf, int out_f0, const T *P, const T *H, const T *per_n,
+                 ActiveMask am) {
   DevTabs<T> &t = p->tabs<T>();
   // each block covers 2 n-tiles of 16 per parity (64 degrees); more for large N
   dim3 grid(tiles_x(ncols, am), p->M, (p->M + 63) / 64);
   const T *tP = P ? P : t.Pa, *tH = H ? H : t.Ha;
   if constexpr (std::is_same<T, float>::value && split_allowed<R>()) {
     if (p->split) {
-      hipLaunchKernelGGL((k_ana_mfma<T, R, true>), grid, dim3(256), 0, st, (const cx<T> *)in,
+      hipLaunchKernelGGL((k_ana_mfma<T, R, true, OCC>), grid, dim3(256), 0, st, (const cx<T> *)in,
                          (cx<T> *)out, tP, tH, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
                          out_nf, out_f0, am, split_of<T>(t, tP, tH));
       HIPCHK(hipGetLastError());
       return LBM_OK;
     }
   }
-  hipLaunchKernelGGL((k_ana_mfma<T, R>), grid, dim3(256), 0, st, (const cx<T> *)in, (cx<T> *)out,
-                     tP, tH, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M, out_nf, out_f0, am,
-                     SplitTabs{});
+  hipLaunchKernelGGL((k_ana_mfma<T, R, false, OCC>), grid, dim3(256), 0, st, (const cx<T> *)in,
+                     (cx<T> *)out, tP, tH, per_n ? per_n : t.nn1a2, ncols, C, sl, p->J, p->M,
+                     out_nf, out_f0, am, SplitTabs{});
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
-// FFTs over the rows j*C + col of a column map; mode: fourier_scale (0 for
-// the model, 1/2 for the transposed transforms of the adjoint)
-template <class T, int NI>
-int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
-                  ActiveMask am) {
-  using S = FftShape<T, NI, true>;
+// Analysis with the analysis tables gw P, gw H and n(n+1)/a^2 by default;
+// the transposed synthesis passes the synthesis tables instead.  The level
+// tendency analyses of the time step run the autotuned occupancy variant.
+template <class T, class R>
+int launch_ana(lbm_plan *p, const void *in, void *out, int ncols, int C, InSlots sl,
+               hipStream_t st, int out_nf = R::NOUT, int out_f0 = 0, const T *P = nullptr,
+               const T *H = nullptr, const T *per_n = nullptr,
+               ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  if constexpr (std::is_same<R, AnaTendVD>::value || std::is_same<R, AnaFluxDiv>::value) {
+    if (p->tune[3] == 1)
+      return launch_ana_k<T, R, ANA_OCC[1]>(p, in, out, ncols, C, sl, st, out_nf, out_f0, P, H,
+                                            per_n, am);
+    if (p->tune[3] == 2)
+      return launch_ana_k<T, R, ANA_OCC[2]>(p, in, out, ncols, C, sl, st, out_nf, out_f0, P, H,
+                                            per_n, am);
+  }
+  return launch_ana_k<T, R, ANA_OCC[0]>(p, in, out, ncols, C, sl, st, out_nf, out_f0, P, H, per_n,
+                                        am);
+}
+
+// Rows per block of the autotuned FFT variants 1 and 2 (variant 0: the
+// FftShape default): the LDS of a row bounds the blocks per CU, the rows
+// per block the waves per barrier.
+template <class T, int NI, bool INV>
+constexpr int fft_alt_rows(int v) {
+  if (NI == 128) return sizeof(T) == 8 ? (v == 1 ? 8 : 32) : (v == 1 ? 16 : 64);
+  if (sizeof(T) == 4) return v == 1 ? 16 : 20;
+  return INV ? (v == 1 ? 16 : 20) : (v == 1 ? 10 : 16);
+}
+
+template <class T, int NI, int RW>
+int ifft_rows_launch(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
+                     ActiveMask am) {
+  using S = FftShape<T, NI, true, RW>;
   const int C = cm.ncols();
   const long rows = fft_rows(p->J, C, am);
-  hipLaunchKernelGGL((k_ifft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
+  hipLaunchKernelGGL((k_ifft<T, NI, RW>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
                      dim3(S::THREADS), 0, st, (const cx<T> *)F, (T *)G, C, cm, p->J, p->M, mode, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
-template <class T, int NI>
-int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode,
-                 ActiveMask am) {
-  using S = FftShape<T, NI>;
+template <class T, int NI, int RW>
+int fft_rows_launch(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode,
+                    ActiveMask am) {
+  using S = FftShape<T, NI, false, RW>;
   const int C = cm.ncols();
   const long rows = fft_rows(p->J, C, am);
-  hipLaunchKernelGGL((k_fft<T, NI>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
+  hipLaunchKernelGGL((k_fft<T, NI, RW>), dim3((unsigned)((rows + S::ROWS - 1) / S::ROWS)),
                      dim3(S::THREADS), 0, st, (const T *)G, (cx<T> *)F, C, cm, p->J, p->M, mode, am);
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+template <class T, int NI>
+int launch_ifft_n(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode,
+                  ActiveMask am) {
+  if (p->tune[0] == 1)
+    return ifft_rows_launch<T, NI, fft_alt_rows<T, NI, true>(1)>(p, F, G, cm, st, mode, am);
+  if (p->tune[0] == 2)
+    return ifft_rows_launch<T, NI, fft_alt_rows<T, NI, true>(2)>(p, F, G, cm, st, mode, am);
+  return ifft_rows_launch<T, NI, 0>(p, F, G, cm, st, mode, am);
+}
+
+template <class T, int NI>
+int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode,
+                 ActiveMask am) {
+  if (p->tune[1] == 1)
+    return fft_rows_launch<T, NI, fft_alt_rows<T, NI, false>(1)>(p, G, F, cm, st, mode, am);
+  if (p->tune[1] == 2)
+    return fft_rows_launch<T, NI, fft_alt_rows<T, NI, false>(2)>(p, G, F, cm, st, mode, am);
+  return fft_rows_launch<T, NI, 0>(p, G, F, cm, st, mode, am);
 }
 
 template <class T>
@@ -1343,6 +1413,89 @@ int final_check(lbm_plan *p, hipStream_t st) {
   hipLaunchKernelGGL((k_check_spec<T>), dim3(256), dim3(256), 0, st, (const cx<T> *)p->sa3, n3,
                      (const cx<T> *)p->sa2, n2, p->ctl);
   HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: each tunable launch of the time step is timed in its variants on
+// the plan's own buffers and shapes (HIP events on the plan's stream, the
+// best of three trials of `reps` launches), and the fastest is kept.  The
+// variants differ only in block shape and register budget, never in
+// arithmetic, so results are bit-identical whichever is chosen.
+// ---------------------------------------------------------------------------
+template <class T>
+int autotune_t(lbm_plan *p, int reps, std::string &rep) {
+  const int E = p->E, L = p->L, EL = E * L;
+  hipStream_t st = p->own;
+  const ColMap cmi{E, L, 6, 6}, cmo{E, L, 9, 2};
+  const int Ci = cmi.ncols(), Co = cmo.ncols();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto timed = [&](auto launch, double *ms) -> int {
+    CHK(launch());  // warm (first launch of a variant loads its code object)
+    double best = 1e300;
+    for (int trial = 0; trial < 3; ++trial) {
+      HIPCHK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) CHK(launch());
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      best = std::min(best, (double)t / reps);
+    }
+    *ms = best;
+    return LBM_OK;
+  };
+  const char *names[lbm_plan::NTUNE] = {"ifft_rows", "fft_rows", "syn_waves", "ana_waves"};
+  char buf[256];
+  for (int knob = 0; knob < lbm_plan::NTUNE; ++knob) {
+    double ms[3];
+    for (int v = 0; v < 3; ++v) {
+      p->tune[knob] = v;
+      int rc = LBM_OK;
+      if (knob == 0)
+        rc = timed([&] { return launch_ifft<T>(p, p->fin, p->gin, cmi, st); }, &ms[v]);
+      else if (knob == 1)
+        rc = timed([&] { return launch_fft<T>(p, p->gout, p->fout, cmo, st); }, &ms[v]);
+      else if (knob == 2)
+        rc = timed([&] {
+          OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
+          return launch_syn<T, SynLevel>(p, p->sa3, p->fin, EL, Ci, sl, st);
+        }, &ms[v]);
+      else
+        rc = timed([&] {
+          InSlots vd{{0, EL, 2 * EL}}, ft{{3 * EL, 4 * EL, 5 * EL}}, fq{{6 * EL, 7 * EL, 8 * EL}};
+          CHK((launch_ana<T, AnaTendVD>(p, p->fout, p->tend3, EL, Co, vd, st, 4, 0)));
+          CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, ft, st, 4, 2)));
+          return launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, fq, st, 4, 3);
+        }, &ms[v]);
+      if (rc != LBM_OK) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+      }
+    }
+    int best = 0;
+    for (int v = 1; v < 3; ++v)
+      if (ms[v] < ms[best]) best = v;
+    p->tune[knob] = best;
+    auto label = [&](int v) -> int {
+      if (knob == 2) return SYN_OCC[v];
+      if (knob == 3) return ANA_OCC[v];
+      const bool inv = knob == 0;
+      if (v == 0)
+        return p->I == 128 ? (inv ? FftShape<T, 128, true>::ROWS : FftShape<T, 128>::ROWS)
+                           : (inv ? FftShape<T, 384, true>::ROWS : FftShape<T, 384>::ROWS);
+      return p->I == 128 ? (inv ? fft_alt_rows<T, 128, true>(v) : fft_alt_rows<T, 128, false>(v))
+                         : (inv ? fft_alt_rows<T, 384, true>(v) : fft_alt_rows<T, 384, false>(v));
+    };
+    std::snprintf(buf, sizeof(buf), "%s%s: %d=%.4f %d=%.4f %d=%.4f ms -> %d", knob ? "; " : "",
+                  names[knob], label(0), ms[0], label(1), ms[1], label(2), ms[2], label(best));
+    rep += buf;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return LBM_OK;
 }
 
@@ -2445,6 +2598,37 @@ int lbm_hydrostatic(const double *in, double *out, int64_t npts, int nlev, int i
                      in, out, alpha, dlog3, (size_t)npts, nlev, inner);
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+int lbm_autotune(lbm_plan *p, int reps, char *report, int64_t len) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (reps < 1) return fail(LBM_ERR_VALUE, "reps must be >= 1");
+  if (!p->basic_set) return fail(LBM_ERR_STATE, "autotune after lbm_set_basic_state");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  if (p->prof) return fail(LBM_ERR_STATE, "stop profiling before autotuning");
+  destroy_graphs(p);
+  std::string rep;
+  const int rc = p->prec == LBM_FP64 ? autotune_t<double>(p, reps, rep) : autotune_t<float>(p, reps, rep);
+  if (rc != LBM_OK) return rc;
+  if (report && len > 0) std::snprintf(report, (size_t)len, "%s", rep.c_str());
+  return LBM_OK;
+}
+
+int lbm_set_tuning(lbm_plan *p, const int32_t *h_choice, int n) {
+  if (!p || (n > 0 && !h_choice)) return fail(LBM_ERR_VALUE, "null argument");
+  if (n < 0 || n > lbm_plan::NTUNE) return fail(LBM_ERR_VALUE, "at most %d choices", lbm_plan::NTUNE);
+  for (int k = 0; k < n; ++k)
+    if (h_choice[k] < 0 || h_choice[k] > 2) return fail(LBM_ERR_VALUE, "choice %d out of range", k);
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  destroy_graphs(p);
+  for (int k = 0; k < lbm_plan::NTUNE; ++k) p->tune[k] = k < n ? h_choice[k] : 0;
+  return LBM_OK;
+}
+
+int lbm_get_tuning(const lbm_plan *p, int32_t *h_choice, int n) {
+  if (!p || !h_choice || n < lbm_plan::NTUNE) return fail(LBM_ERR_VALUE, "need %d entries", lbm_plan::NTUNE);
+  for (int k = 0; k < lbm_plan::NTUNE; ++k) h_choice[k] = p->tune[k];
+  return lbm_plan::NTUNE;
 }
 
 int lbm_run_abort(lbm_plan *p) {

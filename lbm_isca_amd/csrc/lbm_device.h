@@ -284,7 +284,7 @@ struct ActiveMask {
 // uses 8R threads; then the real split to/from X_m = rfft/nlon.  Stages are
 // separated by block barriers placed by the kernels.
 // ---------------------------------------------------------------------------
-template <class T, int NI, bool INV = false>
+template <class T, int NI, bool INV = false, int RW = 0>
 struct FftShape {
   static constexpr int R = NI / 128;
   static constexpr int NC = NI / 2;
@@ -295,7 +295,8 @@ struct FftShape {
   // 128 points, 3.4 KB at 384) bounds the blocks per CU.  Measured best:
   // FP64 16 rows at 128 points (fewer waves per barrier at the same
   // occupancy), 10 (inverse) and 20 (forward) at 384; FP32 32 rows.
-  static constexpr int ROWS = sizeof(T) == 4 ? 32 : (R == 1 ? 16 : (INV ? 10 : 20));
+  // (RW > 0: an autotuner variant with RW rows per block)
+  static constexpr int ROWS = RW > 0 ? RW : sizeof(T) == 4 ? 32 : (R == 1 ? 16 : (INV ? 10 : 20));
   static constexpr int THREADS = ROWS * TPR;
   static constexpr int LDS = 8 * 9 * R;         // complex scratch per row (>= NC)
 };
@@ -505,12 +506,12 @@ __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, i
 }
 
 // Fourier (m <= M-1) -> grid; rows are j*C + col.  mode: fourier_scale.
-template <class T, int NI>
-__global__ __launch_bounds__((FftShape<T, NI, true>::THREADS)) void k_ifft(const cx<T> *__restrict__ F,
+template <class T, int NI, int RW = 0>
+__global__ __launch_bounds__((FftShape<T, NI, true, RW>::THREADS)) void k_ifft(const cx<T> *__restrict__ F,
                                                                 T *__restrict__ G, int C, ColMap cm,
                                                                 int J, int M, int mode,
                                                                 ActiveMask am) {
-  using S = FftShape<T, NI, true>;
+  using S = FftShape<T, NI, true, RW>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;
@@ -585,12 +586,12 @@ __global__ __launch_bounds__((FftShape<T, NI, true>::THREADS)) void k_ifft(const
 }
 
 // grid -> Fourier (m <= M-1).  mode: fourier_scale.
-template <class T, int NI>
-__global__ __launch_bounds__((FftShape<T, NI>::THREADS)) void k_fft(const T *__restrict__ G,
+template <class T, int NI, int RW = 0>
+__global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(const T *__restrict__ G,
                                                                cx<T> *__restrict__ F, int C,
                                                                ColMap cm, int J, int M, int mode,
                                                                ActiveMask am) {
-  using S = FftShape<T, NI>;
+  using S = FftShape<T, NI, false, RW>;
   __shared__ cx<T> sh[S::ROWS][S::LDS];
   const int t = threadIdx.x, rl = t / S::TPR, tt = t - rl * S::TPR;
   const long row = (long)blockIdx.x * S::ROWS + rl;

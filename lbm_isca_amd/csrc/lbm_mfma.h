@@ -112,8 +112,8 @@ struct SplitTabs {
   const short *Ph = nullptr, *Pl = nullptr, *Hh = nullptr, *Hl = nullptr;
 };
 
-template <class T, class R, bool SPLIT = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ana_mfma(const cx<T> *__restrict__ in,
+template <class T, class R, bool SPLIT = false, int OCC = 4>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void k_ana_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Pa,
                                                   const T *__restrict__ Ha,
@@ -473,8 +473,8 @@ __device__ __forceinline__ void syn_body(const cx<T> *__restrict__ in, cx<T> *__
   }
 }
 
-template <class T, class R, bool SPLIT = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_syn_mfma(const cx<T> *__restrict__ in,
+template <class T, class R, bool SPLIT = false, int OCC = 3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void k_syn_mfma(const cx<T> *__restrict__ in,
                                                   cx<T> *__restrict__ out,
                                                   const T *__restrict__ Ps,
                                                   const T *__restrict__ Hs,

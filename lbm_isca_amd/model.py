@@ -14,6 +14,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import tuning as _tuning
 from .backend import asnumpy, hip
 from .constants import S2D
 from .dynamics import BasicStateTerms, Dynamics
@@ -55,7 +56,10 @@ class Config:
     tangent_linear: bool = False
     nmem: int = 1
     device: str = "hip"
-    precision: str = "fp64"      # "fp64" or "fp32" arithmetic on the GPU
+    precision: str = "fp64"      # "fp64", "fp32" or "bf16x3" arithmetic on the GPU
+    # launch variants of the step: False built-in, True timed at set-up,
+    # "cache" reuse a stored pick (lbm_isca_amd.tuning)
+    autotune: object = False
 
     @property
     def steps_per_day(self):
@@ -128,6 +132,7 @@ class LBM:
         self.integ = Integrator(spectral, vertical, self.damping, self.semi, self.vdiff,
                                 plan=self.plan)
         self.plan.set_state()
+        self.tuning = _tuning.autotune(self.plan, c.autotune)
         # keep the last top-of-step sample for diagnostics() after a run
         self.plan.set_history(0, keep_last=True)
         self._ga = None

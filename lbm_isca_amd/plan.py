@@ -601,6 +601,32 @@ class Plan:
                                             int(kind), _stream()))
         return out
 
+    # -- autotuner -----------------------------------------------------------
+    TUNED = ("ifft_rows", "fft_rows", "syn_waves", "ana_waves")
+
+    def autotune(self, reps=10):
+        """Time the variants of the step's tuned launches on this plan and
+        keep the fastest (``lbm_autotune``); returns ``{"choice": [...],
+        "report": text}``."""
+        _torch().cuda.synchronize()
+        buf = ctypes.create_string_buffer(2048)
+        _lib.check(self._lib.lbm_autotune(self._h, int(reps), buf, len(buf)))
+        return {"choice": self.get_tuning(), "report": buf.value.decode()}
+
+    def set_tuning(self, choice):
+        arr = (ctypes.c_int32 * len(choice))(*[int(c) for c in choice])
+        _lib.check(self._lib.lbm_set_tuning(self._h, arr, len(choice)))
+
+    def get_tuning(self):
+        arr = (ctypes.c_int32 * len(self.TUNED))()
+        _lib.check(self._lib.lbm_get_tuning(self._h, arr, len(self.TUNED)))
+        return [int(x) for x in arr]
+
+    def tuning_key(self):
+        """Cache key: grid, levels, members, basic states and precision."""
+        return (f"{self.nlon}x{self.nlat} N{self.nmax} L{self.nlev} E{self.nmem} "
+                f"B{self.nbasic} {self.precision}")
+
     def steps_taken(self):
         n = ctypes.c_int64()
         _lib.check(self._lib.lbm_steps_taken(self._h, ctypes.byref(n)))

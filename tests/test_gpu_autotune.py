@@ -1,0 +1,44 @@
+"""The autotuner of the step's launches: it times every variant on the
+plan, keeps the fastest, and every variant computes the same numbers (the
+variants differ in FFT rows per block and in the register budget of the
+Legendre kernels only), so a run under any tuning is bit-identical."""
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import SHT, VerticalOps
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_every_tuning_gives_identical_results(precision, tmp_path, monkeypatch):
+    import lbm_isca_amd as L
+    from lbm_isca_amd import tuning
+    sht, vg = SHT(), VerticalOps()
+    bs = cases.jet_basic_state(sht.lat, sht.lon, vg.sig)
+    heat = np.stack([cases.heating(sht.lat, sht.lon, vg.sig, lon0=30.0 * e) for e in range(16)])
+    m, _, _ = L.make_ensemble_model(bs, 16, L.Config(ndays=6 / 72.0, precision=precision))
+    res = m.plan.autotune(reps=3)
+    print(precision, res["report"])
+    assert all(c in (0, 1, 2) for c in res["choice"]) and "ifft_rows" in res["report"]
+    out = {}
+    for choice in ([0, 0, 0, 0], [1, 1, 1, 1], [2, 2, 2, 2], res["choice"]):
+        m.plan.set_tuning(choice)
+        m.plan.set_state()                      # every run from rest
+        m.run(mjo_forcing=L.EnsembleForcing(heat), verbose=False)
+        out[tuple(choice)] = [x.cpu().numpy() for x in m.plan.get_state()]
+    ref = out[(0, 0, 0, 0)]
+    assert np.abs(ref[2]).max() > 0
+    for k, v in out.items():
+        for a, b in zip(v, ref):
+            assert np.array_equal(a, b), (precision, k)
+    # the cache: a stored pick is applied without timing
+    monkeypatch.setenv("LBM_TUNING_CACHE", str(tmp_path / "tune.json"))
+    first = tuning.autotune(m.plan, "cache", reps=2)
+    assert first["source"] == "timed"
+    m.plan.set_tuning([0, 0, 0, 0])
+    again = tuning.autotune(m.plan, "cache")
+    assert again["source"].endswith("tune.json") and again["choice"] == first["choice"]
+    assert m.plan.get_tuning() == first["choice"]
