@@ -331,6 +331,15 @@ int lbm_autotune(lbm_plan *plan, int reps, char *report, int64_t len);
 int lbm_set_tuning(lbm_plan *plan, const int32_t *h_choice, int n);
 int lbm_get_tuning(const lbm_plan *plan, int32_t *h_choice, int n);
 
+/* Checkpoints for the adjoint of the default model (no counterpart in the
+ * reference): every > 0 makes the following runs keep the whole model state
+ * (both time levels, ps' of both, the pending update, loop control) before
+ * every `every`-th executed step, in device memory (about
+ * 2.6 MB x nmem per checkpoint at T42L20 FP64); lbm_adjoint_run recomputes
+ * one segment at a time from them, storing `every` steps of trajectory.
+ * 0 turns it off and frees the buffers. */
+int lbm_set_checkpoints(lbm_plan *plan, int every);
+
 /* Ends the run in progress without finishing it (e.g. after a writer
  * failed between segments); the next run restarts the start-up steps. */
 int lbm_run_abort(lbm_plan *plan);
@@ -379,15 +388,21 @@ int lbm_set_response(lbm_plan *plan, const double *h_wgrid, const double *h_wlev
                      const double *h_wstep, int nstep);
 int lbm_get_response(lbm_plan *plan, double *h_j);
 
-/* Adjoint of the tangent-linear run (no counterpart in the reference): one
- * backward sweep over the schedule lbm_run(plan, nsteps, ...) follows
- * (start-up half steps, per-member lengths) gives, for every member e, the
- * gradient of J_e of lbm_set_response with respect to that member's
- * spectral initial state of lbm_set_spectral_state (d_gvor, d_gdiv, d_gt,
- * d_gq (nmem,M,M,L), d_gpi (nmem,M,M); complex128 dJ/dRe + i dJ/dIm) and
- * to its temperature forcing (d_gfrc (nmem,J,I,L), K/s).  Any output may be
- * NULL.  Needs tangent_linear = 1 and a response; the forward state is left
- * untouched (the model is linear, so no trajectory is stored). */
+/* Adjoint (no counterpart in the reference): one backward sweep over the
+ * schedule lbm_run(plan, nsteps, ...) follows (start-up half steps,
+ * per-member lengths) gives, for every member e, the gradient of J_e of
+ * lbm_set_response with respect to that member's spectral initial state of
+ * lbm_set_spectral_state (d_gvor, d_gdiv, d_gt, d_gq (nmem,M,M,L), d_gpi
+ * (nmem,M,M); complex128 dJ/dRe + i dJ/dIm) and to its temperature forcing
+ * (d_gfrc (nmem,J,I,L), K/s).  Any output may be NULL.  Needs a response.
+ * tangent_linear = 1: the exactly linear model, nothing stored.  Otherwise
+ * (the default model, whose kinetic energy u'u'/2 + v'v'/2 and exp / log
+ * pair ps' <-> pi' are not linear, dynamics.py:243-244, state.py:77-83):
+ * the gradient at the last run's trajectory, from the checkpoints that run
+ * kept (lbm_set_checkpoints before it; the run starts from
+ * lbm_set_spectral_state): each segment is recomputed from its checkpoint
+ * and swept backward with the transposed kernels about it.  The forward
+ * state is left as the run left it. */
 int lbm_adjoint_run(lbm_plan *plan, int nsteps, double *d_gvor, double *d_gdiv, double *d_gt,
                     double *d_gq, double *d_gpi, double *d_gfrc, int use_graph, void *stream);
 

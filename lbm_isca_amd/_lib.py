@@ -39,7 +39,7 @@ EXPORTS = (
     "lbm_set_steady_forcing", "lbm_set_state_keep", "lbm_get_prev_state", "lbm_tendencies",
     "lbm_prepare_dt", "lbm_step", "lbm_robert_asselin", "lbm_lnps", "lbm_grid_stats",
     "lbm_spec_scale", "lbm_hydrostatic", "lbm_run_steps", "lbm_run_abort",
-    "lbm_autotune", "lbm_set_tuning", "lbm_get_tuning",
+    "lbm_autotune", "lbm_set_tuning", "lbm_get_tuning", "lbm_set_checkpoints",
 )
 
 PROFILE_SLOTS = ("syn_level", "syn_surf", "ifft", "tend", "fft", "ana_level", "ana_surf", "step")
@@ -134,6 +134,7 @@ def load():
         "lbm_autotune": ([P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
         "lbm_set_tuning": ([P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int], ctypes.c_int),
         "lbm_get_tuning": ([P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int], ctypes.c_int),
+        "lbm_set_checkpoints": ([P, ctypes.c_int], ctypes.c_int),
     }
     if not hasattr(lib, "lbm_abi_version") or lib.lbm_abi_version() != ABI_VERSION:
         found = lib.lbm_abi_version() if hasattr(lib, "lbm_abi_version") else "none"

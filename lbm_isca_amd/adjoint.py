@@ -1,18 +1,24 @@
-"""Sensitivity of a linear response to the forcing and the initial state.
+"""Sensitivity of a response to the forcing and the initial state.
 
-The reference has no adjoint.  Here the tangent-linear model
-(``Config(tangent_linear=True)``: strictly linear tendencies,
-``ps' = ps_bar pi'``) is exactly linear, so a response that is linear in the
-states,
+The reference has no adjoint.  A response that is linear in the states,
 
     J_e = sum_k w_k sum_l w_l sum_{j,i} W_{j,i} psi_l(j, i; step k),
 
-is ``J_e = <g_x, x0_e> + <g_f, f_e>`` and one backward sweep of the
-hand-written adjoint kernels gives ``g_x`` and ``g_f`` for every member at
-once.  The adjoint does not depend on the forward state, so no trajectory
-(and no checkpoint of leapfrog pairs) is stored.
+gets its gradient with respect to member e's spectral initial state and
+temperature forcing from one backward sweep of the hand-written transposed
+kernels, for every member at once:
 
-``LinearResponse`` wraps a forward run and the adjoint sweep as a
+* the tangent-linear model (``Config(tangent_linear=True)``: strictly linear
+  tendencies, ``ps' = ps_bar pi'``) is exactly linear, so
+  ``J_e = <g_x, x0_e> + <g_f, f_e>`` and the sweep needs no trajectory;
+* the default model keeps u'u'/2 + v'v'/2 and the exact ps' <-> pi' pair, so
+  its gradient is taken about the trajectory of the run: the run keeps the
+  whole state every K steps (``plan.set_checkpoints(K)``, 48 by default
+  here), and the sweep recomputes one K-step segment at a time from its
+  checkpoint, storing the per-step u, v and ps' it needs, before running
+  the transposed kernels about them backward through that segment.
+
+``LinearResponse`` wraps a forward run and the sweep as a
 ``torch.autograd.Function``; complex gradients follow torch's convention
 ``dJ/dRe + i dJ/dIm``.
 """
@@ -97,11 +103,15 @@ def _autograd_function():
     class LinearResponse(torch.autograd.Function):
         """J = LinearResponse.apply(plan, nsteps, use_graph, frc, vor, div, t, q, pi).
 
-        ``plan`` is a tangent-linear :class:`lbm_isca_amd.plan.Plan` with a
-        response installed; ``frc`` (E|1, J, I, L) float64 K/s or None; the
-        spectral initial state (E, M, M, L) / (E, M, M) complex128 or None.
-        ``nsteps`` <= 0 runs each member for its own length
-        (``plan.set_lengths``).  Returns J (E,) float64.
+        ``plan`` is a :class:`lbm_isca_amd.plan.Plan` (tangent-linear, or
+        the default model with checkpoints, which ``response`` turns on)
+        with a response installed; ``frc`` (E|1, J, I, L) float64 K/s or
+        None; the spectral initial state (E, M, M, L) / (E, M, M) complex128
+        or None.  ``nsteps`` <= 0 runs each member for its own length
+        (``plan.set_lengths``).  Returns J (E,) float64.  The backward sweep
+        refuses a plan whose response, lengths, forcing or basic state
+        changed after the forward run (the gradient would belong to another
+        function).
         """
 
         @staticmethod
@@ -111,6 +121,7 @@ def _autograd_function():
             plan.run(nsteps, use_graph=use_graph)
             J = torch.as_tensor(plan.get_response(), dtype=torch.float64, device="cuda")
             ctx.plan, ctx.nsteps, ctx.use_graph = plan, nsteps, use_graph
+            ctx.generation = plan.generation
             ctx.frc_shape = None if frc is None else tuple(frc.shape)
             ctx.present = [x is not None for x in (vor, div, t, q, pi)]
             return J
@@ -118,6 +129,9 @@ def _autograd_function():
         @staticmethod
         def backward(ctx, gJ):
             plan = ctx.plan
+            if plan.generation != ctx.generation:
+                raise RuntimeError("the plan's response, lengths, forcing or basic state changed "
+                                   "after the forward run; run it again before the backward")
             gx, gf = plan.adjoint_run(ctx.nsteps, use_graph=ctx.use_graph)
             w = gJ.to(torch.float64)
             E = plan.nmem
@@ -138,13 +152,27 @@ def _autograd_function():
 _FN = None
 
 
-def linear_response(plan, nsteps, frc=None, vor=None, div=None, t=None, q=None, pi=None,
-                    use_graph=True):
+CHECKPOINT_EVERY = 48
+
+
+def response(plan, nsteps, frc=None, vor=None, div=None, t=None, q=None, pi=None,
+             use_graph=True, checkpoint_every=CHECKPOINT_EVERY):
     """J (E,) of the installed response, differentiable with respect to
-    ``frc`` and the spectral initial state through the adjoint sweep."""
+    ``frc`` and the spectral initial state through the adjoint sweep: of the
+    tangent-linear model, or of the default model about its trajectory
+    (checkpoints every ``checkpoint_every`` steps)."""
     global _FN
     if _FN is None:
         _FN = _autograd_function()
     if not plan.tangent_linear:
-        raise RuntimeError("linear_response needs a plan built with tangent_linear=True")
+        plan.set_checkpoints(checkpoint_every)
     return _FN.apply(plan, int(nsteps), bool(use_graph), frc, vor, div, t, q, pi)
+
+
+def linear_response(plan, nsteps, frc=None, vor=None, div=None, t=None, q=None, pi=None,
+                    use_graph=True):
+    """``response`` of a tangent-linear plan."""
+    if not plan.tangent_linear:
+        raise RuntimeError("linear_response needs a plan built with tangent_linear=True "
+                           "(use response() for the default model)")
+    return response(plan, nsteps, frc, vor, div, t, q, pi, use_graph)

@@ -169,6 +169,16 @@ struct TendAdjArgs {
   const T *vc, *tref, *rcos;
   int E, J, I, L, nbasic;
   ActiveMask am;
+  // the default model about a stored trajectory (null: the tangent-linear
+  // model about rest): this step's trajectory [e][j][2L+2][I] (u, v per
+  // level, ps' of both levels after the deferred update), the carried grid
+  // adjoints of ps' at both levels, and what the mode of the deferred
+  // update needs
+  const T *traj;
+  T *psa_a, *psb_a;
+  const int *nsteps;
+  int inistp, step, full_ke;
+  T robert;
 };
 
 template <class T>
@@ -193,6 +203,7 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
   const T rc = a.rcos[j];
   const T eps = (T)EPSVT, kap = (T)KAPPA, rgas = (T)R_DRY, h = (T)0.5;
   const T bpix = B(br.pix()), bpiy = B(br.piy());
+  const T *trj = a.traj ? a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i : nullptr;
 
   // adjoints of the four tendencies carrying vertical advection (u, v, T, q)
   auto W = [&](int k, T *w) {
@@ -274,9 +285,15 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
     vor_a += -bu * fv_a;
     tv_a += -rcpg * bpiy * fv_a;
     piyst += -rcpg * tvdev * fv_a;
-    // kinetic energy u_bar u' + v_bar v' and the virtual geopotential
+    // kinetic energy u_bar u' + v_bar v' (+ u' u'/2 + v' v'/2 of the
+    // default model: about the trajectory, u_traj u' + v_traj v') and the
+    // virtual geopotential
     up_a += bu * ke_a;
     vp_a += bv * ke_a;
+    if (trj && a.full_ke) {
+      up_a += trj[(size_t)k * I] * ke_a;
+      vp_a += trj[(size_t)(L + k) * I] * ke_a;
+    }
     const T x_a = rgas * alpha * ke_a + rgas * dlog3 * SKE;
     SKE += ke_a;
     tv_a += x_a;
@@ -350,9 +367,35 @@ __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
   S0st -= gpi_a;
   pixst += B(br.sbu()) * S0st;
   piyst += B(br.sbv()) * S0st;
+  // the default model: transposes of pi_prev = log1p(ps'_b / ps_bar) and of
+  // the deferred update at the top of this step (ps'_a = ps_bar expm1(pi),
+  // ps'_b from the start-up assignment or the Robert-Asselin filter of the
+  // previous levels), carrying the grid adjoints of ps' to the step before
+  T pig_a = 0;
+  if (trj) {
+    const size_t pidx = (size_t)(e * J + j) * I + i;
+    const T psbar = B(br.psbar());
+    const T ca = psbar + trj[(size_t)(2 * L) * I], cb = psbar + trj[(size_t)(2 * L + 1) * I];
+    const T psa = a.psa_a[pidx];
+    const T psb = a.psb_a[pidx] + go[(size_t)(9 * L + 1) * I] / cb;
+    const int k = a.step;
+    const bool upd = k >= 1 && (k - 1) < a.inistp + a.nsteps[e];
+    const int mode = upd ? ((k - 1) < a.inistp ? 1 : 2) : 0;
+    if (mode == 2) {
+      pig_a = ca * (psa + a.robert * psb);
+      a.psa_a[pidx] = ((T)1 - (T)2 * a.robert) * psb;
+      a.psb_a[pidx] = a.robert * psb;
+    } else if (mode == 1) {
+      pig_a = ca * psa;
+      a.psa_a[pidx] = psb;
+      a.psb_a[pidx] = 0;
+    } else {
+      a.psb_a[pidx] = psb;
+    }
+  }
   gi[(size_t)(6 * L) * I] = pixst * rc;
   gi[(size_t)(6 * L + 1) * I] = piyst * rc;
-  gi[(size_t)(6 * L + 2) * I] = 0;
+  gi[(size_t)(6 * L + 2) * I] = pig_a;
   gi[(size_t)(6 * L + 3) * I] = bpix * S0st * rc;
   gi[(size_t)(6 * L + 4) * I] = bpiy * S0st * rc;
   gi[(size_t)(6 * L + 5) * I] = S0st + D0st;
@@ -383,6 +426,9 @@ struct StepAdjArgs {
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
   int ecnt;  // members covered by the launch (<= E)
   T delt, robert;
+  // 1: the default model (pi at t-dt is the analysed ps'_b: its adjoint goes
+  // to the tendency field 1, and pi has no spectral Robert-Asselin filter)
+  int traj;
 };
 
 template <class T, int L, class P>
@@ -447,6 +493,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     }
     a.aa2[i2] = a.aa2[i2] + spi;
     *at2 = zero<T>();
+    if (a.traj) at2[E] = zero<T>();
     return;
   }
   const bool oinit = st < a.inistp;
@@ -512,7 +559,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   return;
   }
   // pi: pi = pi_b + 2dt (wt_pi - sum dsig dbar)
-  const cx<T> bpi = a.ab2[i2];
+  const cx<T> bpi = a.traj ? zero<T>() : a.ab2[i2];
   const cx<T> pin_a = a.aa2[i2] + spi + cn * bpi;
   a.aa2[i2] = ca * bpi;
   cx<T> pib_a = cb * bpi + pin_a;
@@ -577,7 +624,10 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     wdpix_a = wdpix_a + (ft * (rg * tref[l] * vc[vo.cpg() + l])) * y[l];
   }
   pib_a = pib_a + wdpix_a;
-  a.ab2[i2] = pib_a;
+  if (a.traj)
+    at2[E] = pib_a;  // pi_prev*: to the grid adjoint of ps'_b (k_tend_adj)
+  else
+    a.ab2[i2] = pib_a;
   *at2 = wtp + dt * wdpix_a;
   // phi_l = sum_{k<l} R dlog3_k w_k + R alpha_l w_l,  w = (1 + dt ktt) T_b + dt wt_t
   {

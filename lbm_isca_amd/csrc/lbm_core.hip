@@ -294,6 +294,20 @@ struct lbm_plan {
   // step entry points, which must leave the run's state untouched
   void *x_sa3 = nullptr, *x_sa2 = nullptr, *x_sb3 = nullptr, *x_sb2 = nullptr;
   void *x_psa = nullptr, *x_psb = nullptr, *x_raw = nullptr;
+  // checkpointed trajectory of the default model for its adjoint: the
+  // whole state every ckpt_every executed steps of a run (ckpt), the run it
+  // belongs to (ckpt_len, ckpt_total; ckpt_ok once that run has ended), the
+  // per-step trajectory of one recomputed segment (traj, ckpt_every slots)
+  // and the slot the next tendency launch writes or the next transposed
+  // launch reads (traj_cur, traj_step), the carried grid adjoints of ps'
+  // (psa_a, psb_a) and the forward state kept aside during the sweep (keep)
+  int ckpt_every = 0, ckpt_total = 0, ckpt_count = 0;
+  bool ckpt_ok = false, ckpt_raw = false;
+  std::vector<int> ckpt_len;
+  void *ckpt = nullptr, *traj = nullptr, *keep = nullptr, *psa_a = nullptr, *psb_a = nullptr;
+  size_t ckpt_cap = 0, traj_cap = 0;
+  void *traj_cur = nullptr;
+  int traj_step = -1;
   // raw grid (u, v, t, q) of the previous level after a partial update
   // (lbm_set_state_keep) that left the two levels different
   void *rawb = nullptr;
@@ -1294,6 +1308,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   ta.robert = (T)p->robert;
   ta.am = am;
   ta.h = hist_args<T>(p);
+  ta.traj = (T *)p->traj_cur;
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
   CHK(join(F_SURF));
@@ -1900,6 +1915,7 @@ int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
   a.ecnt = ecnt;
+  a.traj = p->traj_cur != nullptr;
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step_adj
   hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64), 2), dim3(64), 0, st, a);
@@ -1927,9 +1943,12 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   }
   CHK(mark(p, 6, st));
   {
+    // the default model also carries pi_prev* (tendency field 1): both
+    // surface fields as 2E columns, as the forward analysis runs them
     OutSlots sl{{9 * EL}};
-    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, E, Co, sl, st, 2, t.Pa, t.Ha, t.nn1a2,
-                                     am)));
+    const bool traj = p->traj_cur != nullptr;
+    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, traj ? 2 * E : E, Co, sl, st,
+                                     traj ? 1 : 2, t.Pa, t.Ha, t.nn1a2, am)));
   }
   CHK(mark(p, 4, st));
   CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st, 1, am));
@@ -1948,6 +1967,14 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   ta.L = L;
   ta.nbasic = p->nb;
   ta.am = am;
+  ta.traj = (const T *)p->traj_cur;
+  ta.psa_a = (T *)p->psa_a;
+  ta.psb_a = (T *)p->psb_a;
+  ta.nsteps = p->d_nsteps;
+  ta.inistp = p->inistp;
+  ta.step = p->traj_step;
+  ta.full_ke = !(p->strictly_linear || p->linear);
+  ta.robert = (T)p->robert;
   hipLaunchKernelGGL((k_tend_adj<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
@@ -2000,6 +2027,9 @@ int adjoint_finish(lbm_plan *p, double *gvor, double *gdiv, double *gt, double *
                      (const cx<T> *)p->ab3, (cx<T> *)p->aa3, n3);
   hipLaunchKernelGGL((k_add_cx<T>), dim3(nblk(n2)), dim3(256), 0, st, (const cx<T> *)p->aa2,
                      (const cx<T> *)p->ab2, (cx<T> *)p->aa2, n2);
+  // the default model drops the global means of the initial vorticity and
+  // divergence on entry (coefficient s = 0 of fields 0, 1): no gradient
+  if (!p->linear) HIPCHK(hipMemsetAsync(p->aa3, 0, 2 * EL * sizeof(cx<T>), st));
   double *dst[4] = {gvor, gdiv, gt, gq};
   const size_t tot = (size_t)E * M * M * L;
   for (int f = 0; f < 4; ++f)
@@ -2146,6 +2176,169 @@ int set_steady_t(lbm_plan *p, const double *vor, const double *div, const double
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+
+// One copy of the whole model state (both spectral levels, pi and the
+// column sums, ps' of both levels, the pending ps' update, loop control).
+struct StateParts {
+  size_t off[7], size[7], total;
+};
+StateParts state_parts(const lbm_plan *p) {
+  const size_t cs = 2 * p->rsize(), EL = (size_t)p->E * p->L;
+  const size_t sz[7] = {cs * p->nspec * 4 * EL, cs * p->nspec * 4 * EL, cs * p->nspec * 3 * (size_t)p->E,
+                        p->rsize() * (size_t)p->E * p->J * p->I, p->rsize() * (size_t)p->E * p->J * p->I,
+                        sizeof(int) * (size_t)p->E, sizeof(Ctl)};
+  StateParts sp;
+  size_t o = 0;
+  for (int k = 0; k < 7; ++k) {
+    sp.off[k] = o;
+    sp.size[k] = sz[k];
+    o += (sz[k] + 255) / 256 * 256;
+  }
+  sp.total = o;
+  return sp;
+}
+int copy_state(lbm_plan *p, void *buf, bool save, hipStream_t st) {
+  const StateParts sp = state_parts(p);
+  void *parts[7] = {p->sa3, p->sb3, p->sa2, p->psa, p->psb, p->pend, p->ctl};
+  for (int k = 0; k < 7; ++k) {
+    char *b = (char *)buf + sp.off[k];
+    if (save)
+      HIPCHK(hipMemcpyAsync(b, parts[k], sp.size[k], hipMemcpyDeviceToDevice, st));
+    else
+      HIPCHK(hipMemcpyAsync(parts[k], b, sp.size[k], hipMemcpyDeviceToDevice, st));
+  }
+  return LBM_OK;
+}
+
+int grow(void **buf, size_t *cap, size_t bytes, hipStream_t st) {
+  if (bytes <= *cap) return LBM_OK;
+  if (*buf) {
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+  }
+  HIPCHK(hipMalloc(buf, bytes));
+  *cap = bytes;
+  return LBM_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// The grid field (ps_bar + ps'_0) (ps'_a* + ps'_b*) of the initial state:
+// ps' of both levels was ps_bar expm1(pi_0) on the grid, so this is the
+// grid adjoint of pi_0 whose synthesis transpose joins the pi gradient.
+template <class T>
+__global__ void k_ps0_adj(const T *__restrict__ traj0, const T *__restrict__ bsg,
+                          const T *__restrict__ psa_a, const T *__restrict__ psb_a,
+                          double *__restrict__ out, int E, int J, int I, int L, int nbasic) {
+  const size_t tot = (size_t)E * J * I;
+  const BRow br{L};
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % I);
+    const size_t ej = q / I;
+    const int j = (int)(ej % J), e = (int)(ej / J);
+    const int eb = nbasic > 1 ? e : 0;
+    const T pb = bsg[(((size_t)eb * J + j) * br.rows() + br.psbar()) * I + i];
+    const T ps0 = traj0[(ej * (2 * L + 2) + 2 * L) * I + i];
+    out[q] = (double)((pb + ps0) * (psa_a[q] + psb_a[q]));
+  }
+}
+
+__global__ void k_add_f64(const double *__restrict__ x, double *__restrict__ y, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    y[q] += x[q];
+}
+
+// Adjoint of the default model (not tangent-linear): backward over the
+// checkpointed segments of the last run; each segment is recomputed from
+// its checkpoint storing the per-step trajectory, then swept backward with
+// the transposed kernels about that trajectory.
+template <class T>
+int adjoint_traj_t(lbm_plan *p, const std::vector<int> &len, double *const out[6],
+                   hipStream_t st) {
+  const int E = p->E, L = p->L, J = p->J, I = p->I, K = p->ckpt_every;
+  const int total = p->ckpt_total;
+  const size_t slot = p->rsize() * (size_t)E * J * (2 * L + 2) * I;
+  const size_t nps = p->rsize() * (size_t)E * J * I;
+  CHK(grow(&p->traj, &p->traj_cap, (size_t)K * slot, st));
+  size_t keep_cap = p->keep ? state_parts(p).total : 0;
+  CHK(grow(&p->keep, &keep_cap, state_parts(p).total, st));
+  for (void **b : {&p->psa_a, &p->psb_a}) {
+    if (!*b) HIPCHK(hipMalloc(b, nps));
+    HIPCHK(hipMemsetAsync(*b, 0, nps, st));
+  }
+  CHK(copy_state(p, p->keep, true, st));
+  // the recomputed steps add no response, history or diagnostics
+  const bool resp = p->resp_set, keep_last = p->keep_last;
+  const int hist_mode = p->hist_mode;
+  p->resp_set = false;
+  p->hist_mode = 0;
+  p->keep_last = false;
+  int rc = LBM_OK;
+  for (int g = p->ckpt_count - 1; g >= 0 && rc == LBM_OK; --g) {
+    const int k0 = g * K, k1 = std::min(total, k0 + K);
+    rc = copy_state(p, (char *)p->ckpt + (size_t)g * state_parts(p).total, false, st);
+    for (int k = k0; k < k1 && rc == LBM_OK; ++k) {
+      p->traj_cur = (char *)p->traj + (size_t)(k - k0) * slot;
+      rc = step_dispatch(p, st, all_live(p, len, k) ? 0 : live_members(p, len, k, true));
+    }
+    if (rc != LBM_OK) break;
+    p->resp_set = resp;
+    static Ctl cb;  // host source of the copy below, alive until it is done
+    cb = Ctl{k1 - 1, 0, INT_MAX, 0};
+    if (hipMemcpyAsync(p->ctl, &cb, sizeof(Ctl), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = fail(LBM_ERR_RUNTIME, "control block upload failed");
+      break;
+    }
+    for (int k = k1 - 1; k >= k0 && rc == LBM_OK; --k) {
+      p->traj_cur = (char *)p->traj + (size_t)(k - k0) * slot;
+      p->traj_step = k;
+      rc = adjoint_dispatch(p, st, all_live(p, len, k) ? 0 : live_members(p, len, k, false));
+    }
+    p->resp_set = false;
+  }
+  p->traj_cur = nullptr;
+  p->traj_step = -1;
+  p->resp_set = resp;
+  p->hist_mode = hist_mode;
+  p->keep_last = keep_last;
+  if (rc == LBM_OK)
+    rc = adjoint_finish<T>(p, out[0], out[1], out[2], out[3], out[4], out[5], st);
+  if (rc == LBM_OK && out[4]) {
+    // pi_0 also set ps' of both levels on the grid (segment 0's trajectory
+    // slot 0 still holds the initial ps')
+    const size_t n2 = (size_t)E * J * I, nw = (size_t)E * p->M * p->M * 2;
+    double *g = nullptr, *w = nullptr;
+    if (hipMalloc(&g, n2 * sizeof(double)) != hipSuccess || hipMalloc(&w, nw * sizeof(double)) != hipSuccess)
+      rc = fail(LBM_ERR_RUNTIME, "allocation failed");
+    if (rc == LBM_OK) {
+      hipLaunchKernelGGL((k_ps0_adj<T>), dim3(nblk(n2)), dim3(256), 0, st, (const T *)p->traj,
+                         (const T *)p->bsg, (const T *)p->psa_a, (const T *)p->psb_a, g, E, J, I, L,
+                         p->nb);
+      rc = spectral_op<T>(p, OP_W2G_ADJ, g, nullptr, w, nullptr, E, 1, st);
+    }
+    if (rc == LBM_OK) {
+      hipLaunchKernelGGL(k_add_f64, dim3(nblk(nw)), dim3(256), 0, st, (const double *)w, out[4], nw);
+      if (hipGetLastError() != hipSuccess) rc = fail(LBM_ERR_RUNTIME, "launch failed");
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "sync failed");
+    if (g) (void)hipFree(g);
+    if (w) (void)hipFree(w);
+  }
+  // the forward state is left as the run left it
+  if (copy_state(p, p->keep, false, st) != LBM_OK && rc == LBM_OK) rc = LBM_ERR_RUNTIME;
+  return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -2167,7 +2360,8 @@ int lbm_plan_destroy(lbm_plan *p) {
                   p->ab2, p->s3a, p->frc_a, p->obs, p->wlev, p->wstep, p->jacc,
                   p->hspec[0], p->hspec[1], p->hgrid[0], p->hgrid[1], p->hlast_spec,
                   p->hlast_grid, p->mjo, p->sfv, p->sfp, p->x_sa3, p->x_sa2, p->x_sb3,
-                  p->x_sb2, p->x_psa, p->x_psb, p->x_raw, p->x_pend, p->x_ctl, p->rawb};
+                  p->x_sb2, p->x_psa, p->x_psb, p->x_raw, p->x_pend, p->x_ctl, p->rawb, p->ckpt, p->traj, p->keep, p->psa_a,
+                  p->psb_a};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &e : p->ev)
@@ -2631,6 +2825,24 @@ int lbm_get_tuning(const lbm_plan *p, int32_t *h_choice, int n) {
   return lbm_plan::NTUNE;
 }
 
+int lbm_set_checkpoints(lbm_plan *p, int every) {
+  if (!p) return fail(LBM_ERR_VALUE, "null plan");
+  if (every < 0) return fail(LBM_ERR_VALUE, "every must be >= 0");
+  if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+  p->ckpt_every = every;
+  p->ckpt_ok = false;
+  if (every == 0) {
+    void **bufs[] = {&p->ckpt, &p->traj, &p->keep, &p->psa_a, &p->psb_a};
+    HIPCHK(hipDeviceSynchronize());
+    for (void **b : bufs) {
+      if (*b) HIPCHK(hipFree(*b));
+      *b = nullptr;
+    }
+    p->ckpt_cap = p->traj_cap = 0;
+  }
+  return LBM_OK;
+}
+
 int lbm_run_abort(lbm_plan *p) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   p->in_run = false;
@@ -2687,13 +2899,29 @@ int run_core(lbm_plan *p, int nsteps, int stop, int count, int use_graph, int ch
     p->in_run = true;
     p->run_len = len;
     p->run_done = 0;
+    // checkpoints of this run for the adjoint of the default model
+    p->ckpt_ok = false;
+    if (p->ckpt_every > 0 && !p->linear) {
+      p->ckpt_count = (total + p->ckpt_every - 1) / p->ckpt_every;
+      CHK(grow(&p->ckpt, &p->ckpt_cap, (size_t)p->ckpt_count * state_parts(p).total, st));
+      p->ckpt_len = len;
+      p->ckpt_total = total;
+      p->ckpt_raw = p->raw_pending;
+    }
   }
+  const bool ckpt_on = p->ckpt_every > 0 && !p->linear;
+  auto ckpt_at = [&](int step) -> int {
+    if (ckpt_on && step % p->ckpt_every == 0)
+      CHK(copy_state(p, (char *)p->ckpt + (size_t)(step / p->ckpt_every) * state_parts(p).total, true, st));
+    return LBM_OK;
+  };
   const int target = count > 0 ? std::min(total, p->run_done + count) : target_stop;
   const int chunk = 8;
   int done = p->run_done;
   int rc = LBM_OK;
   if (done == 0 && p->raw_pending && done < target) {
     // the first step after a raw initial state reads the raw grid (unfused)
+    CHK(ckpt_at(0));
     CHK(step_dispatch(p, st, all_live(p, len, 0) ? 0 : live_members(p, len, 0, true), true));
     p->raw_pending = false;
     done = 1;
@@ -2701,6 +2929,10 @@ int run_core(lbm_plan *p, int nsteps, int stop, int count, int use_graph, int ch
   while (done < target) {
     int todo = target - done;
     if (check_every > 0) todo = std::min(todo, check_every - (done % check_every));
+    if (ckpt_on) {
+      CHK(ckpt_at(done));
+      todo = std::min(todo, p->ckpt_every - done % p->ckpt_every);
+    }
     const int n = (use_graph && todo < chunk) ? 1 : std::min(todo, chunk);
     // the members still live at the first step of this stretch cover it;
     // no mask while every member advances through it
@@ -2717,6 +2949,7 @@ int run_core(lbm_plan *p, int nsteps, int stop, int count, int use_graph, int ch
   }
   p->steps_total += done - p->run_done;
   p->run_done = done;
+  if (ckpt_on && rc == LBM_OK && done == total) p->ckpt_ok = true;
   if (rc != LBM_OK || done == total) p->in_run = false;
   if (rc != LBM_OK) return rc;
   if (done == total) CHK(p->prec == LBM_FP64 ? final_check<double>(p, st) : final_check<float>(p, st));
@@ -2851,14 +3084,46 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
                     double *gpi, double *gfrc, int use_graph, void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   if (!p->basic_set) return fail(LBM_ERR_STATE, "lbm_adjoint_run before lbm_set_basic_state");
-  if (!p->linear) return fail(LBM_ERR_STATE, "the adjoint needs a plan with tangent_linear = 1");
   if (!p->resp_set) return fail(LBM_ERR_STATE, "the adjoint needs a response (lbm_set_response)");
   if (p->mjo)
     return fail(LBM_ERR_STATE, "the adjoint is built for constant forcings (an MJO forcing is set)");
-  if (p->sfv || p->sfp)
+  if (p->linear && (p->sfv || p->sfp))
     return fail(LBM_ERR_STATE,
                 "the adjoint is built for the linear model (steady vorticity, divergence or ln ps "
                 "forcing is set)");
+  if (!p->linear) {
+    // the default model: about the checkpointed trajectory of the last run
+    if (p->ckpt_every <= 0 || !p->ckpt_ok)
+      return fail(LBM_ERR_STATE, "the adjoint of the default model needs the checkpoints of a "
+                  "finished run (lbm_set_checkpoints before lbm_run)");
+    if (p->ckpt_raw)
+      return fail(LBM_ERR_STATE, "the checkpointed run must start from lbm_set_spectral_state");
+    if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
+    std::vector<int> len(p->E);
+    for (int e = 0; e < p->E; ++e) len[e] = nsteps > 0 ? nsteps : p->nsteps_h[e];
+    if (len != p->ckpt_len)
+      return fail(LBM_ERR_VALUE, "the lengths differ from the checkpointed run's");
+    hipStream_t st = p->strm(stream);
+    const size_t EL = (size_t)p->E * p->L, cs = 2 * p->rsize();
+    struct A {
+      void **ptr;
+      size_t bytes;
+    } bufs[] = {{&p->aa3, cs * p->nspec * 4 * EL}, {&p->ab3, cs * p->nspec * 4 * EL},
+                {&p->aa2, cs * p->nspec * p->E},   {&p->ab2, cs * p->nspec * p->E},
+                {&p->s3a, cs * p->nspec * 3 * p->E},
+                {&p->frc_a, p->rsize() * (size_t)p->E * p->J * p->I * p->L}};
+    for (auto &b : bufs) {
+      if (!*b.ptr) HIPCHK(hipMalloc(b.ptr, b.bytes));
+      HIPCHK(hipMemsetAsync(*b.ptr, 0, b.bytes, st));
+    }
+    HIPCHK(hipMemcpyAsync(p->d_nsteps, len.data(), sizeof(int) * p->E, hipMemcpyHostToDevice, st));
+    double *out[6] = {gvor, gdiv, gt, gq, gpi, gfrc};
+    (void)use_graph;  // the sweep follows each recomputed segment eagerly
+    CHK(p->prec == LBM_FP64 ? adjoint_traj_t<double>(p, len, out, st)
+                            : adjoint_traj_t<float>(p, len, out, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return LBM_OK;
+  }
   p->in_run = false;
   hipStream_t st = use_graph ? p->own : p->strm(stream);
   if (use_graph && stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));

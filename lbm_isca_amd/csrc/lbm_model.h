@@ -282,6 +282,10 @@ struct TendArgs {
   T robert;
   ActiveMask am;
   HistArgs<T> h;
+  // trajectory of this step for the adjoint of the default model, or null:
+  // [e][j][2L+2][I] = u, v of every level, ps' of both time levels after
+  // the deferred update
+  T *traj;
 };
 
 // Tendencies on grid rows, one thread per column, one pass over the levels
@@ -360,6 +364,12 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0,
                       gpi, pib, cs.chk, pst);
   }
+  T *trj = nullptr;
+  if (a.traj) {
+    trj = a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i;
+    trj[(size_t)(2 * L) * I] = a.linear ? pst : a.psa[pidx];
+    trj[(size_t)(2 * L + 1) * I] = a.linear ? (T)0 : a.psb[pidx];
+  }
   // history samples of this top-of-step state (uniform per block)
   int hslot[2];
   T hw[2];
@@ -399,6 +409,10 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     const T fkn = top ? (T)0 : FR(k + 1);
     const BLev<T> bn = top ? BLev<T>{} : blev(k + 1);
     T out[9], omega;
+    if (trj) {
+      trj[(size_t)k * I] = up;
+      trj[(size_t)(L + k) * I] = vp;
+    }
     column_level<T>(k, L, a.vc, tref, bc, bn, fk, cc, cs, up, vp, vor, dv, tp, qp, upn, vpn, tpn,
                     qpn, out, psbar, pst, omega);
     vor = vorn;

@@ -70,6 +70,10 @@ def zero_damping(nmax, nlev):
 class Plan:
     """One model/transform plan: grid, tables, operators, state, work buffers."""
 
+    # bumped by every change of what a run computes (response, lengths,
+    # forcings, basic state): the autograd Function checks it
+    generation = 0
+
     def __init__(self, nlon=IMAX, nlat=JMAX, nmax=NMAX, radius=ER, sigm=SIGM, nmem=1,
                  nbasic=1, precision="fp64", damping=None, vdif=(None, None, None, None),
                  delt=1200.0, robert=0.04, inistp=2, strictly_linear=False,
@@ -301,6 +305,7 @@ class Plan:
     # -- model -------------------------------------------------------------
     def set_basic_state(self, u, v, t, q, ps):
         """nbasic basic states: u, v, t, q (nbasic, J, I, L), ps (nbasic, J, I) hPa."""
+        self.generation += 1
         nb = (self.nbasic,)
         for k, x in (("u", u), ("v", v), ("t", t), ("q", q)):
             self._grid_check(x, f"basic state {k}", self.nlev, nb)
@@ -312,6 +317,7 @@ class Plan:
         _lib.check(self._lib.lbm_set_basic_state(self._h, *[_lib.ptr(x) for x in f], _stream()))
 
     def set_forcing(self, frc):
+        self.generation += 1
         torch = _torch()
         if frc is None:
             _lib.check(self._lib.lbm_set_forcing(self._h, None, 0, _stream()))
@@ -322,6 +328,7 @@ class Plan:
     def set_steady_forcing(self, vor=None, div=None, ps=None):
         """Steady vorticity / divergence (J, I, L) and ln ps (J, I) forcings,
         or with a leading member axis; all None removes them."""
+        self.generation += 1
         torch = _torch()
         E, J, I, L = self.nmem, self.nlat, self.nlon, self.nlev
         given, nf = [], None
@@ -377,6 +384,7 @@ class Plan:
     def set_response(self, wgrid, wlev=None, wstep=None):
         """J_e = sum_k wstep[k] sum_l wlev[l] sum_ji wgrid[j,i] psi_l(j,i) over the
         states after steps k of the next run; wgrid None removes it."""
+        self.generation += 1
         torch = _torch()
         torch.cuda.current_stream().synchronize()
         if wgrid is None:
@@ -413,7 +421,14 @@ class Plan:
                                              _stream()))
         return g3 + [gpi], gf
 
+    def set_checkpoints(self, every):
+        """Keep the model state every ``every`` executed steps of the next runs
+        (0: off), for the adjoint of the default model."""
+        _torch().cuda.synchronize()
+        _lib.check(self._lib.lbm_set_checkpoints(self._h, int(every)))
+
     def set_lengths(self, nsteps):
+        self.generation += 1
         if nsteps is None:
             _lib.check(self._lib.lbm_set_lengths(self._h, None, None))
             return
@@ -466,6 +481,7 @@ class Plan:
         """Daily temperature forcing fields (days, J, I, L) shared by every
         member or (E, days, J, I, L) per member, interpolated on the device
         (with the float32 slope of the reference when ``gtf`` is float32)."""
+        self.generation += 1
         torch = _torch()
         single = (gtf.dtype == torch.float32) if isinstance(gtf, torch.Tensor) else \
             (np.asarray(gtf).dtype == np.float32)
