@@ -75,6 +75,11 @@ def load():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
             "g.build()'` (hipcc --offload-arch=gfx950)")
+    # torch first: the library then binds to the HIP runtime torch has
+    # already loaded.  Loaded the other way round the process holds two HIP
+    # runtimes, and device pointers and streams of one mean nothing to the
+    # other.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     P = _vp
     sig = {
