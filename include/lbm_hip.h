@@ -233,6 +233,18 @@ int lbm_robert_asselin(int64_t n, double *d_prev, const double *d_cur, const dou
 int lbm_lnps(int64_t n, int64_t nper, const double *d_in, const double *d_psbar, double *d_out,
              int inverse, void *stream);
 int lbm_grid_stats(int64_t n, const double *d_x, double *h_out, void *stream);
+
+/* lbm_grid_stats_noalloc: the same h_out as lbm_grid_stats, bit for bit,
+ * without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving and monitoring
+ * loops, harnesses that forbid hipMalloc).  Its partial results live in a
+ * static device array of 64 KiB + 256 bytes per GPU, held until the process
+ * ends: at most 4096 block pairs, and the launch is capped at that many
+ * blocks (grid-stride), so every n >= 0 is accepted.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two host threads,
+ * two streams) race and must be serialised by the caller.  Synchronises the
+ * stream. */
+int lbm_grid_stats_noalloc(int64_t n, const double *d_x, double *h_out, void *stream);
 int lbm_spec_scale(lbm_plan *plan, const double *d_in, double *d_out, int B, int nk, int kind,
                    void *stream);
 int lbm_hydrostatic(const double *d_in, double *d_out, int64_t npts, int nlev, int inner,

@@ -38,6 +38,7 @@ EXPORTS = (
     "lbm_set_mjo_forcing", "lbm_run_segment_done", "lbm_spec_resize", "lbm_abi_version",
     "lbm_set_steady_forcing", "lbm_set_state_keep", "lbm_get_prev_state", "lbm_tendencies",
     "lbm_prepare_dt", "lbm_step", "lbm_robert_asselin", "lbm_lnps", "lbm_grid_stats",
+    "lbm_grid_stats_noalloc",
     "lbm_spec_scale", "lbm_hydrostatic", "lbm_run_steps", "lbm_run_abort",
     "lbm_autotune", "lbm_set_tuning", "lbm_get_tuning", "lbm_set_checkpoints",
 )
@@ -130,6 +131,7 @@ def load():
         "lbm_robert_asselin": ([ctypes.c_int64, P, P, P, ctypes.c_double, P], ctypes.c_int),
         "lbm_lnps": ([ctypes.c_int64, ctypes.c_int64, P, P, P, ctypes.c_int, P], ctypes.c_int),
         "lbm_grid_stats": ([ctypes.c_int64, P, _dp, P], ctypes.c_int),
+        "lbm_grid_stats_noalloc": ([ctypes.c_int64, P, _dp, P], ctypes.c_int),
         "lbm_spec_scale": ([P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P], ctypes.c_int),
         "lbm_hydrostatic": ([P, P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, P, P, P],
                             ctypes.c_int),

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -2339,6 +2340,42 @@ int adjoint_traj_t(lbm_plan *p, const std::vector<int> &len, double *const out[6
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Device scratch of lbm_grid_stats_noalloc, one copy per GPU for the life of
+// the process: 256 bytes for the result pair, then kStatsSlots block pairs
+// (64 KiB + 256 bytes).  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kStatsSlots = 4096;  // block (max, count) pairs
+constexpr int kStatsOff = 32;      // the pairs start 256 bytes in
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[kStatsOff + 2 * kStatsSlots];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch[kMaxDevices];  // scratch's address on each device
+
+// The current device's copy of scratch, looked up on the first call there.
+int device_scratch(unsigned long long **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(LBM_ERR_RUNTIME, "device %d is beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(LBM_ERR_RUNTIME, "no device address for the statistics scratch");
+    g_scratch[dev] = (unsigned long long *)ptr;
+  }
+  *out = g_scratch[dev];
+  return LBM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -2764,6 +2801,31 @@ int lbm_grid_stats(int64_t n, const double *x, double *h_out, void *stream) {
     rc = fail(LBM_ERR_RUNTIME, "statistics copy failed");
   (void)hipFree(d);
   if (rc != LBM_OK) return rc;
+  double mx;
+  std::memcpy(&mx, &h[0], sizeof(double));
+  h_out[0] = mx;
+  h_out[1] = (double)h[1];
+  return LBM_OK;
+}
+
+int lbm_grid_stats_noalloc(int64_t n, const double *x, double *h_out, void *stream) {
+  if (!h_out || n < 0 || (n > 0 && !x)) return fail(LBM_ERR_VALUE, "bad arguments");
+  unsigned long long h[2] = {0, 0};
+  if (n > 0) {
+    unsigned long long *s = nullptr;
+    CHK(device_scratch(&s));
+    // the grid-stride launch never has more blocks than scratch has pairs,
+    // so every n fits; each block writes its own pair, nothing is cleared
+    const size_t nb = std::min<size_t>(((size_t)n + 255) / 256, kStatsSlots);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_grid_stats_part, dim3((unsigned)nb), dim3(256), 0, st, x, (size_t)n,
+                       s + kStatsOff);
+    hipLaunchKernelGGL(k_grid_stats_final, dim3(1), dim3(256), 0, st,
+                       (const unsigned long long *)(s + kStatsOff), (int)nb, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, s, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   double mx;
   std::memcpy(&mx, &h[0], sizeof(double));
   h_out[0] = mx;

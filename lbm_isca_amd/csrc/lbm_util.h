@@ -31,11 +31,12 @@ __global__ void k_lnps(const double *__restrict__ in, const double *__restrict__
   }
 }
 
-// stats[0] = bits of max |x| over finite x (non-negative doubles order like
-// their bit patterns), stats[1] = number of non-finite x.  Block reduction in
-// LDS, one atomic per block and statistic.
-__global__ __launch_bounds__(256) void k_grid_stats(const double *__restrict__ x, size_t n,
-                                                    unsigned long long *stats) {
+// Max of the bit patterns of |x| over finite x (non-negative doubles order
+// like their bit patterns) and the number of non-finite x, over this
+// block's grid-stride share of x[0, n); reduced in LDS, valid in thread 0.
+__device__ __forceinline__ void block_stats(const double *__restrict__ x, size_t n,
+                                            unsigned long long &mx_out,
+                                            unsigned long long &bad_out) {
   __shared__ unsigned long long smax[256];
   __shared__ unsigned long long sbad[256];
   unsigned long long mx = 0, bad = 0;
@@ -60,9 +61,60 @@ __global__ __launch_bounds__(256) void k_grid_stats(const double *__restrict__ x
     }
     __syncthreads();
   }
+  mx_out = smax[0];
+  bad_out = sbad[0];
+}
+
+// stats[0] = bits of max |x| over finite x, stats[1] = number of non-finite
+// x; one atomic per block and statistic (stats starts at zero).
+__global__ __launch_bounds__(256) void k_grid_stats(const double *__restrict__ x, size_t n,
+                                                    unsigned long long *stats) {
+  unsigned long long mx, bad;
+  block_stats(x, n, mx, bad);
   if (threadIdx.x == 0) {
-    atomicMax(&stats[0], smax[0]);
-    atomicAdd(&stats[1], sbad[0]);
+    atomicMax(&stats[0], mx);
+    atomicAdd(&stats[1], bad);
+  }
+}
+
+// The same statistics without atomics: block b writes its pair to
+// part[2b], part[2b+1]; k_grid_stats_final reduces the gridDim.x pairs.
+// Nothing has to start at zero.
+__global__ __launch_bounds__(256) void k_grid_stats_part(const double *__restrict__ x, size_t n,
+                                                         unsigned long long *__restrict__ part) {
+  unsigned long long mx, bad;
+  block_stats(x, n, mx, bad);
+  if (threadIdx.x == 0) {
+    part[2 * (size_t)blockIdx.x] = mx;
+    part[2 * (size_t)blockIdx.x + 1] = bad;
+  }
+}
+
+// One block of 256: stats[0..1] from the nb pairs of part.
+__global__ __launch_bounds__(256) void k_grid_stats_final(const unsigned long long *__restrict__ part,
+                                                          int nb, unsigned long long *__restrict__ stats) {
+  __shared__ unsigned long long smax[256];
+  __shared__ unsigned long long sbad[256];
+  unsigned long long mx = 0, bad = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const unsigned long long m = part[2 * b];
+    mx = m > mx ? m : mx;
+    bad += part[2 * b + 1];
+  }
+  smax[threadIdx.x] = mx;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const unsigned long long o = smax[threadIdx.x + w];
+      smax[threadIdx.x] = o > smax[threadIdx.x] ? o : smax[threadIdx.x];
+      sbad[threadIdx.x] += sbad[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    stats[0] = smax[0];
+    stats[1] = sbad[0];
   }
 }
 
