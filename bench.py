@@ -262,9 +262,11 @@ def measure_fp32_vs_fp64(E=256, max_days=None):
             "max_rel_l2": float(f"{max(err.values()):.3e}")}
 
 
-def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0)):
+def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0), every=48):
     """Adjoint sweep time over forward run time for the streamfunction
-    response of the uneven ensemble's first E members."""
+    response of the uneven ensemble's first E members: the tangent-linear
+    model (no trajectory) and the default model about its trajectory
+    (checkpoints every ``every`` steps, segments recomputed in the sweep)."""
     import numpy as np
     import torch
     import lbm_isca_amd as L
@@ -272,47 +274,81 @@ def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0)):
     bs, frc, lengths, sigm = uneven_ensemble(E, max_days=max_days)
     nmax_steps = int(lengths.max())
     res = {}
-    for prec in ("fp64", "fp32"):
-        cfg = L.Config(precision=prec, tangent_linear=True)
-        m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
-        plan = m.plan
-        lat = np.degrees(plan.get("lat", plan.nlat))
-        lon = np.degrees(plan.get("lon", plan.nlon))
-        adjoint.StreamfunctionResponse(plan, lat, lon, plan.get("gw", plan.nlat),
-                                       plan.get("sig", plan.nlev), 0.2, (30.0, 60.0),
-                                       (150.0, 210.0), window[0], window[1], nmax_steps, 72,
-                                       cfg.inistp)
-        plan.set_forcing(frc)
-        plan.set_lengths(lengths)
-        plan.set_spectral_state()
-        plan.run(0, use_graph=True)
-        plan.adjoint_run(0)
-        torch.cuda.synchronize()
-        plan.set_spectral_state()
-        t0 = time.perf_counter()
-        plan.run(0, use_graph=True)
-        torch.cuda.synchronize()
-        tf = time.perf_counter() - t0
-        J = plan.get_response()
-        t0 = time.perf_counter()
-        gx, gf = plan.adjoint_run(0)
-        torch.cuda.synchronize()
-        ta = time.perf_counter() - t0
-        # from a zero initial state J_e = <dJ_e/df_e, f_e> exactly (linearity)
-        pred = (gf.cpu().numpy() * frc).reshape(E, -1).sum(1)
-        sel = np.abs(J) > 0
-        lin = float(np.max(np.abs(pred[sel] - J[sel]) / np.abs(J[sel]))) if sel.any() else 0.0
-        res[prec] = {"forward_s": round(tf, 4), "adjoint_s": round(ta, 4),
-                     "adjoint_over_forward": round(ta / tf, 3),
-                     "members_with_window": int(sel.sum()),
-                     "max_rel_err_J_vs_grad_dot_forcing": float(f"{lin:.2e}")}
-        log(f"adjoint {prec}: {res[prec]}")
-        del m, plan
-        torch.cuda.empty_cache()
-    return {"workload": f"T42L20 {E} members of the uneven ensemble, tangent-linear model, "
-                        "J = area mean of psi(sigma=0.2) over 30-60N 150-210E, days "
+    for model in ("tangent_linear", "default"):
+        for prec in ("fp64", "fp32"):
+            cfg = L.Config(precision=prec, tangent_linear=model == "tangent_linear")
+            m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True)
+            plan = m.plan
+            lat = np.degrees(plan.get("lat", plan.nlat))
+            lon = np.degrees(plan.get("lon", plan.nlon))
+            adjoint.StreamfunctionResponse(plan, lat, lon, plan.get("gw", plan.nlat),
+                                           plan.get("sig", plan.nlev), 0.2, (30.0, 60.0),
+                                           (150.0, 210.0), window[0], window[1], nmax_steps, 72,
+                                           cfg.inistp)
+            plan.set_forcing(frc)
+            plan.set_lengths(lengths)
+            rec = {}
+            if model == "default":
+                # the forward without checkpoints, for their cost
+                plan.set_checkpoints(0)
+                plan.set_spectral_state()
+                plan.run(0, use_graph=True)
+                torch.cuda.synchronize()
+                plan.set_spectral_state()
+                t0 = time.perf_counter()
+                plan.run(0, use_graph=True)
+                torch.cuda.synchronize()
+                rec["forward_without_checkpoints_s"] = round(time.perf_counter() - t0, 4)
+                plan.set_checkpoints(every)
+            plan.set_spectral_state()
+            plan.run(0, use_graph=True)
+            plan.adjoint_run(0)
+            torch.cuda.synchronize()
+            plan.set_spectral_state()
+            t0 = time.perf_counter()
+            plan.run(0, use_graph=True)
+            torch.cuda.synchronize()
+            tf = time.perf_counter() - t0
+            J = plan.get_response()
+            t0 = time.perf_counter()
+            gx, gf = plan.adjoint_run(0)
+            torch.cuda.synchronize()
+            ta = time.perf_counter() - t0
+            gf = gf.cpu().numpy()
+            sel = np.abs(J) > 0
+            rec.update({"forward_s": round(tf, 4), "adjoint_s": round(ta, 4),
+                        "adjoint_over_forward": round(ta / tf, 3),
+                        "members_with_window": int(sel.sum())})
+            if model == "tangent_linear":
+                # from a zero initial state J_e = <dJ_e/df_e, f_e> exactly (linearity)
+                pred = (gf * frc).reshape(E, -1).sum(1)
+                lin = float(np.max(np.abs(pred[sel] - J[sel]) / np.abs(J[sel]))) if sel.any() else 0.0
+                rec["max_rel_err_J_vs_grad_dot_forcing"] = float(f"{lin:.2e}")
+            else:
+                rec["checkpoint_every"] = every
+                # central difference of J along a forcing direction
+                h = 1e-3
+                dfrc = 0.5 * np.roll(frc, 17, axis=2)
+                out = []
+                for sgn in (1.0, -1.0):
+                    plan.set_forcing(frc + sgn * h * dfrc)
+                    plan.set_spectral_state()
+                    plan.run(0, use_graph=True)
+                    out.append(plan.get_response())
+                fd = (out[0] - out[1]) / (2 * h)
+                pred = (gf * dfrc).reshape(E, -1).sum(1)
+                ok = np.abs(fd) > 0
+                err = float(np.max(np.abs(pred[ok] - fd[ok]) / np.abs(fd[ok]))) if ok.any() else 0.0
+                rec["max_rel_err_grad_vs_central_difference"] = float(f"{err:.2e}")
+            res[f"{model}_{prec}"] = rec
+            log(f"adjoint {model} {prec}: {rec}")
+            del m, plan
+            torch.cuda.empty_cache()
+    return {"workload": f"T42L20 {E} members of the uneven ensemble, J = area mean of "
+                        "psi(sigma=0.2) over 30-60N 150-210E, days "
                         f"{window[0]:g}-{window[1]:g}; gradients w.r.t. forcing and spectral "
-                        "initial state",
+                        "initial state, of the tangent-linear model and of the default model "
+                        "about its trajectory",
             "members": E, **res}
 
 
