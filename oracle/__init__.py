@@ -15,10 +15,13 @@ the reference checkout, e.g. ``pylbm_isca/spectral.py:93-154``).
 
 Pinning (what checks the restatement itself, see ``tests/test_oracle_*.py``):
 
-* ``sht``      -- Gauss-Legendre closed forms, orthonormality, brute-force
-  closed-form P_n^m at small n, DFT on tiny inputs, the reference's analytic
-  transform tests (round trip, Laplacian eigen, gradients, solid body,
-  div(grad)=lap), and reference-generated vectors (``tests/golden``).
+* ``sht``      -- quadrature exactness on monomials, discrete
+  orthonormality, closed-form P_n^m / H_n^m at n <= 7, P_n^m at every n of
+  T42 and T127 against scipy's spherical harmonics and H_n^m against the
+  textbook derivative identity, brute-force DFT of a few rows
+  (``tests/test_oracle_sht_pins.py``); the reference's analytic transform
+  tests (round trip, Laplacian eigen, gradients, solid body, div(grad)=lap)
+  and reference-generated vectors (``tests/golden``).
 * ``vgrid``    -- closed forms and the hydrostatic recurrence of the
   reference tests, plus reference-generated operator values.
 * ``dyn``      -- exact zero on a resting isothermal state, tangent-linear
