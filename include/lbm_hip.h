@@ -59,7 +59,12 @@ typedef enum { LBM_FP64 = 0, LBM_FP32 = 1, LBM_BF16X3 = 2 } lbm_precision;
  * (model.py:34-74); unit conversions (hours/days -> s^-1) are done by the
  * caller as in integrator.py:41-118, 243-253. */
 typedef struct {
-  int32_t nlon, nlat, nmax, nlev;   /* 128, 64, 42, 20 at T42 L20           */
+  int32_t nlon, nlat, nmax, nlev;   /* 128, 64, 42, 20 at T42 L20.  nlon: 128
+                                       or 384 (specialised FFTs) or a multiple
+                                       of 64 up to 1024 with nlon/2 a product
+                                       of 2, 3 and 5 (T21 64, T63 192, T85 256,
+                                       T106 320 ...); nlat a multiple of 8;
+                                       nmax + 1 <= nlon/2; nlev 20 or 40      */
   int32_t nmem;                     /* ensemble members E >= 1               */
   int32_t nbasic;                   /* 1 = one shared basic state, or nmem   */
   int32_t precision;                /* lbm_precision                         */

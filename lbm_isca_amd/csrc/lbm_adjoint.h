@@ -183,9 +183,9 @@ struct TendAdjArgs {
 
 template <class T>
 __global__ __launch_bounds__(128) void k_tend_adj(TendAdjArgs<T> a) {
-  const int nch = a.I / 128;
+  const int nch = a.I / (int)blockDim.x;  // blocks of 128 or 64 longitudes
   const int ej = blockIdx.x / nch;
-  const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
+  const int i = (blockIdx.x - ej * nch) * (int)blockDim.x + threadIdx.x;
   const int e = ej / a.J, j = ej - e * a.J;
   if (!a.am(e)) return;  // a finished member contributes nothing
   const int L = a.L, I = a.I, J = a.J;

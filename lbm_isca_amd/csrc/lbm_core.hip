@@ -182,6 +182,7 @@ struct DevTabs {
   T *tref = nullptr;                                             // [nb][L]
   T *minv = nullptr;                                             // [nset][nb][M][L][L]
   T *vinv = nullptr;                                             // [nset][4][L][L]
+  T *twa = nullptr;  // [I] complex exp(-2 pi i k / I): FFTs of the Stockham path
   // split-BF16x3 plans: Ps, Hs, Pa, Ha as BF16 high / low parts
   short *split_hi[4] = {}, *split_lo[4] = {};
 };
@@ -212,7 +213,7 @@ int upload(T **dst, const std::vector<double> &src) {
 template <class T>
 void free_tabs(DevTabs<T> &t) {
   T *ps[] = {t.Ps, t.Hs, t.Pa, t.Ha, t.inv_lap, t.nn1a2, t.lap, t.rcos, t.vc,
-             t.kv, t.kd, t.ktt, t.kq, t.tref, t.minv, t.vinv};
+             t.kv, t.kd, t.ktt, t.kq, t.tref, t.minv, t.vinv, t.twa};
   for (T *p : ps)
     if (p) (void)hipFree(p);
   for (int k = 0; k < 4; ++k) {
@@ -235,6 +236,9 @@ struct lbm_plan {
   int J = 0, I = 0, N = 0, M = 0, L = 0, E = 0, nb = 0, prec = 0, nspec = 0, Jh = 0, nset = 0;
   int inistp = 2, strictly_linear = 0, skip_low_n = 1, vmask = 0, linear = 0;
   bool split = false;  // LBM_BF16X3: split-BF16x3 Legendre contractions
+  // FFT lengths other than the specialised 128 and 384: the radices of the
+  // Stockham passes (ffac.np = 0 for the specialised lengths)
+  FftFactors ffac{};
   double a = 0, delt = 0, robert = 0;
   // host FP64 setup
   std::vector<double> mu, gw, coslat, lat, lon;
@@ -393,6 +397,10 @@ long fft_rows(long J, int C, const ActiveMask &am) {
   return am.reduced() ? J * (C / am.E) * am.ecnt : J * fpad(C);
 }
 
+// Longitudes per block of the grid-point kernels (one thread each): 128, or
+// 64 where the row length is not a multiple of 128.
+int grid_chunk(int I) { return I % 128 == 0 ? 128 : 64; }
+
 // Tables of one precision from the FP64 host setup.
 template <class T>
 int build_tabs(lbm_plan *p) {
@@ -416,6 +424,15 @@ int build_tabs(lbm_plan *p) {
   CHK(upload(&t.Hs, Hs));
   CHK(upload(&t.Pa, Pa));
   CHK(upload(&t.Ha, Ha));
+  if (p->ffac.np > 0) {
+    std::vector<double> tw(2 * (size_t)p->I);
+    for (int k = 0; k < p->I; ++k) {
+      const long double ang = -2.0L * (long double)M_PI * k / p->I;
+      tw[2 * k] = (double)std::cos(ang);
+      tw[2 * k + 1] = (double)std::sin(ang);
+    }
+    CHK(upload(&t.twa, tw));
+  }
   if (std::is_same<T, float>::value && p->split) {
     const std::vector<double> *tabs4[4] = {&Ps, &Hs, &Pa, &Ha};
     for (int k = 0; k < 4; ++k) {
@@ -652,9 +669,32 @@ int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st,
   return fft_rows_launch<T, NI, 0>(p, G, F, cm, st, mode, am);
 }
 
+// Rows per block of the Stockham FFTs (32 threads and 2 NC complex of LDS
+// per row).
+int fft_any_rows(int NC) { return NC <= 128 ? 8 : NC <= 256 ? 4 : 2; }
+
+template <class T, bool INV>
+int launch_fft_any(lbm_plan *p, const void *in, void *out, ColMap cm, hipStream_t st, int mode,
+                   ActiveMask am) {
+  const int C = cm.ncols(), rb = fft_any_rows(p->ffac.NC);
+  const long rows = fft_rows(p->J, C, am);
+  const dim3 grid((unsigned)((rows + rb - 1) / rb)), block(rb * FFT_ANY_TPR);
+  const size_t lds = (size_t)rb * 2 * p->ffac.NC * sizeof(cx<T>);
+  const cx<T> *tw = (const cx<T> *)p->tabs<T>().twa;
+  if (INV)
+    hipLaunchKernelGGL((k_ifft_any<T>), grid, block, lds, st, (const cx<T> *)in, (T *)out, C, cm, p->J,
+                       p->M, mode, am, p->ffac, tw);
+  else
+    hipLaunchKernelGGL((k_fft_any<T>), grid, block, lds, st, (const T *)in, (cx<T> *)out, C, cm, p->J,
+                       p->M, mode, am, p->ffac, tw);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
 template <class T>
 int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, int mode = 0,
                 ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  if (p->ffac.np > 0) return launch_fft_any<T, true>(p, F, G, cm, st, mode, am);
   return p->I == 128 ? launch_ifft_n<T, 128>(p, F, G, cm, st, mode, am)
                      : launch_ifft_n<T, 384>(p, F, G, cm, st, mode, am);
 }
@@ -662,6 +702,7 @@ int launch_ifft(lbm_plan *p, const void *F, void *G, ColMap cm, hipStream_t st, 
 template <class T>
 int launch_fft(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st, int mode = 0,
                ActiveMask am = ActiveMask{nullptr, nullptr, nullptr, 0, 1, 0}) {
+  if (p->ffac.np > 0) return launch_fft_any<T, false>(p, G, F, cm, st, mode, am);
   return p->I == 128 ? launch_fft_n<T, 128>(p, G, F, cm, st, mode, am)
                      : launch_fft_n<T, 384>(p, G, F, cm, st, mode, am);
 }
@@ -990,7 +1031,7 @@ int set_basic(lbm_plan *p, const double *u, const double *v, const double *t, co
     CHK((launch_syn<double, SynGrad>(p, Sln, F, nb, 2 * nb, s2, st)));
     CHK(launch_ifft<double>(p, F, gg, ColMap{nb, 1, 0, 2}, st));
   }
-  hipLaunchKernelGGL(k_basic_terms, dim3(nb * J * (I / 128)), dim3(128), 0, st, u, v, t, q, ps, gs, gg,
+  hipLaunchKernelGGL(k_basic_terms, dim3(nb * J * (I / grid_chunk(I))), dim3(grid_chunk(I)), 0, st, u, v, t, q, ps, gs, gg,
                      p->d64.vc, p->d_mu, bsg64, J, I, L);
   hipLaunchKernelGGL(k_tbar, dim3(nb * L), dim3(256), 0, st, bsg64, tb, J, I, L);
   HIPCHK(hipGetLastError());
@@ -1318,7 +1359,8 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   if (!fork && p->mjo) CHK(launch_mjo());
   if (p->mjo) CHK(join(F_MJO));
   CHK(mark(p, 3, st));
-  hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / grid_chunk(p->I))), dim3(grid_chunk(p->I)), 0,
+                     st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 4, st));
   CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
@@ -1466,6 +1508,14 @@ int autotune_t(lbm_plan *p, int reps, std::string &rep) {
   const char *names[lbm_plan::NTUNE] = {"ifft_rows", "fft_rows", "syn_waves", "ana_waves"};
   char buf[256];
   for (int knob = 0; knob < lbm_plan::NTUNE; ++knob) {
+    if (knob < 2 && p->ffac.np > 0) {
+      // one Stockham shape per length: nothing to choose
+      p->tune[knob] = 0;
+      std::snprintf(buf, sizeof(buf), "%s%s: stockham %d", knob ? "; " : "", names[knob],
+                    fft_any_rows(p->ffac.NC));
+      rep += buf;
+      continue;
+    }
     double ms[3];
     for (int v = 0; v < 3; ++v) {
       p->tune[knob] = v;
@@ -1976,7 +2026,8 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   ta.step = p->traj_step;
   ta.full_ke = !(p->strictly_linear || p->linear);
   ta.robert = (T)p->robert;
-  hipLaunchKernelGGL((k_tend_adj<T>), dim3(ne * p->J * (p->I / 128)), dim3(128), 0, st, ta);
+  hipLaunchKernelGGL((k_tend_adj<T>), dim3(ne * p->J * (p->I / grid_chunk(p->I))),
+                     dim3(grid_chunk(p->I)), 0, st, ta);
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
   CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2, am));
@@ -2414,16 +2465,28 @@ int lbm_plan_destroy(lbm_plan *p) {
 int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   if (!d || !out) return fail(LBM_ERR_VALUE, "null descriptor or output");
   *out = nullptr;
-  if (d->nlon != 128 && d->nlon != 384)
-    return fail(LBM_ERR_VALUE, "nlon=%d: 128 or 384 longitudes are built", d->nlon);
-  if (d->nlat <= 0 || d->nlat % 2 || d->nlat > 64 * 32)
-    return fail(LBM_ERR_VALUE, "nlat must be even and positive");
+  FftFactors ffac{};
+  if (d->nlon != 128 && d->nlon != 384) {
+    // the Stockham path: radices 4, 2, 3, 5 of nlon / 2, at most 512
+    bool ok = d->nlon >= 64 && d->nlon % 64 == 0 && d->nlon / 2 <= 512;
+    int n = ok ? d->nlon / 2 : 1;
+    for (int f : {4, 2, 3, 5})
+      while (n % f == 0 && ffac.np < 12) {
+        ffac.r[ffac.np++] = f;
+        n /= f;
+      }
+    if (!ok || n != 1)
+      return fail(LBM_ERR_VALUE, "nlon=%d: longitudes must be a multiple of 64, at most 1024, "
+                  "with nlon/2 a product of 2, 3 and 5", d->nlon);
+    ffac.NI = d->nlon;
+    ffac.NC = d->nlon / 2;
+  }
+  if (d->nlat <= 0 || d->nlat % 8 || d->nlat > 64 * 32)
+    return fail(LBM_ERR_VALUE, "nlat=%d: a positive multiple of 8 is required", d->nlat);
   if (d->nmax < 1 || d->nmax + 1 > d->nlon / 2 || d->nmax >= d->nlat)
     return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=%d, nlat=%d", d->nmax, d->nlon,
                 d->nlat);
   if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
-  if (d->precision == LBM_BF16X3 && d->nlat % 8)
-    return fail(LBM_ERR_VALUE, "split-BF16x3 needs nlat a multiple of 8 (got %d)", d->nlat);
   if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
   if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");
   if (d->precision != LBM_FP64 && d->precision != LBM_FP32 && d->precision != LBM_BF16X3)
@@ -2451,6 +2514,7 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->nb = d->nbasic;
   p->prec = d->precision;
   p->split = d->precision == LBM_BF16X3;
+  p->ffac = ffac;
   p->nspec = p->M * (p->M + 1) / 2;
   p->Jh = p->J / 2;
   p->inistp = d->inistp;

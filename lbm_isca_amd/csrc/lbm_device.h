@@ -1,5 +1,6 @@
 // Device code of the gfx950 time-stepping core: complex helpers, the
-// real FFT pair (128 or 384 longitudes), the per-m Legendre synthesis/analysis with
+// real FFT pair (128 and 384 longitudes specialised, other lengths of
+// radices 2, 3, 4, 5 through a Stockham pass loop), the per-m Legendre synthesis/analysis with
 // north/south folding, the fused grid-point tendency kernel and the fused
 // spectral semi-implicit leapfrog + vertical diffusion + Robert-Asselin
 // kernel.  Everything is templated on the arithmetic type T (double/float).
@@ -643,6 +644,179 @@ __global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(c
   }
   __syncthreads();
   if (live) fft_c<T, NI>(sh[rl], F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, tt, mode);
+}
+
+// ---------------------------------------------------------------------------
+// Real FFTs of the other grids: NI = 2 NC longitudes with NC a product of
+// the radices 2, 3, 4 and 5, NC <= 512 (T21 64, T63 192, T85 256, T106 320
+// ...).  Same row addressing, real split and scale modes as k_ifft / k_fft;
+// the complex DFT of length NC is a Stockham autosort in LDS, one pass per
+// radix, ping-ponging between two buffers of the row: pass q (radix r,
+// Ns = product of the earlier radices) maps
+//   v_t = a[j + t NC/r] exp(SIGN 2 pi i t (j mod Ns) / (Ns r)),  t < r,
+//   b[(j / Ns) Ns r + (j mod Ns) + t Ns] = (radix-r DFT of v)_t
+// for j < NC/r, and leaves the transform in natural order.  Twiddles
+// exp(-2 pi i k / NI) come from a table in global memory.
+// ---------------------------------------------------------------------------
+struct FftFactors {
+  int NI, NC, np;  // longitudes, NC = NI / 2, passes
+  int r[12];       // radix of each pass
+};
+constexpr int FFT_ANY_TPR = 32;  // threads per row
+
+template <int SIGN, class T>
+__device__ __forceinline__ void dft5(cx<T> *v) {
+  // exp(SIGN 2 pi i k / 5) = c_k + SIGN i s_k
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  const cx<T> t1 = v[1] + v[4], t2 = v[2] + v[3], t3 = v[1] - v[4], t4 = v[2] - v[3];
+  const cx<T> a1 = v[0] + c1 * t1 + c2 * t2, a2 = v[0] + c2 * t1 + c1 * t2;
+  const cx<T> b1 = (T)SIGN * (s1 * t3 + s2 * t4), b2 = (T)SIGN * (s2 * t3 - s1 * t4);
+  v[0] = v[0] + t1 + t2;
+  v[1] = a1 + mk(-b1.y, b1.x);
+  v[4] = a1 - mk(-b1.y, b1.x);
+  v[2] = a2 + mk(-b2.y, b2.x);
+  v[3] = a2 - mk(-b2.y, b2.x);
+}
+
+template <int SIGN, class T>
+__device__ __forceinline__ void dft_radix(cx<T> *v, int r) {
+  if (r == 4) {
+    dft4<SIGN>(v[0], v[1], v[2], v[3]);
+  } else if (r == 2) {
+    const cx<T> a = v[0];
+    v[0] = a + v[1];
+    v[1] = a - v[1];
+  } else if (r == 3) {
+    dftR<3, SIGN>(v);
+  } else {
+    dft5<SIGN>(v);
+  }
+}
+
+// One row's complex DFT of length ff.NC (sign SIGN, unnormalised) from a
+// into LDS buffers a / b; returns the buffer that holds the result.  Every
+// thread of the block calls it (the passes end in block barriers).
+template <int SIGN, class T>
+__device__ cx<T> *stockham_row(cx<T> *a, cx<T> *b, const FftFactors &ff, const cx<T> *__restrict__ tw,
+                               int tt, bool live) {
+  const int NC = ff.NC, NI = ff.NI;
+  int Ns = 1;
+  for (int q = 0; q < ff.np; ++q) {
+    const int r = ff.r[q], nr = NC / r, step = NI / (Ns * r);
+    if (live) {
+      for (int j = tt; j < nr; j += FFT_ANY_TPR) {
+        const int k = j % Ns;
+        cx<T> v[5];
+        v[0] = a[j];
+#pragma unroll
+        for (int t = 1; t < 5; ++t) {
+          v[t] = zero<T>();
+          if (t < r) {
+            const cx<T> w = tw[(k * t * step) % NI];
+            v[t] = cmul(a[j + t * nr], SIGN < 0 ? w : conj(w));
+          }
+        }
+        dft_radix<SIGN>(v, r);
+        const int base = (j / Ns) * Ns * r + k;
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+          if (t < r) b[base + t * Ns] = v[t];
+      }
+    }
+    __syncthreads();
+    cx<T> *c = a;
+    a = b;
+    b = c;
+    Ns *= r;
+  }
+  return a;
+}
+
+template <class T>
+__device__ __forceinline__ T fourier_scale_any(int mode, int m, int NI) {
+  if (mode == 1) return (m == 0 ? (T)1 : (T)0.5) / (T)NI;
+  if (mode == 2) return m == 0 ? (T)1 : (T)2;
+  return (T)1;
+}
+
+// Fourier (m <= M-1) -> grid for any FftFactors length; blockDim.x / 32
+// rows per block, 2 NC complex of dynamic LDS per row.
+template <class T>
+__global__ __launch_bounds__(256) void k_ifft_any(const cx<T> *__restrict__ F, T *__restrict__ G, int C,
+                                                  ColMap cm, int J, int M, int mode, ActiveMask am,
+                                                  FftFactors ff, const cx<T> *__restrict__ tw) {
+  extern __shared__ unsigned char fft_any_lds[];
+  const int NC = ff.NC, NI = ff.NI;
+  const int rl = threadIdx.x / FFT_ANY_TPR, tt = threadIdx.x - rl * FFT_ANY_TPR;
+  cx<T> *a = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)rl * 2 * NC, *b = a + NC;
+  const long row = (long)blockIdx.x * (blockDim.x / FFT_ANY_TPR) + rl;
+  int j = 0, col = 0, e = 0, r = 0;
+  bool live = am.row_col(row, J, C, j, col);
+  if (live) {
+    cm.map(col, e, r);
+    live = am(e);
+  }
+  if (live) {
+    const cx<T> *Fr = F + fidx(0, j, col, J, C);
+    const size_t ms = (size_t)J * fpad(C);
+    // Z_k = (A + B) + i e^{+2 pi i k / NI} (A - B), A = X_k, B = conj X_{NC-k}
+    // (spectral.py:166-172: g = X0.re + 2 sum_m Re(X_m e^{i m lam}))
+    for (int k = tt; k < NC; k += FFT_ANY_TPR) {
+      cx<T> A = zero<T>(), B = zero<T>();
+      if (k < M) {
+        A = fourier_scale_any<T>(mode, k, NI) * Fr[(size_t)k * ms];
+        if (k == 0) A.y = 0;
+      }
+      const int kb = NC - k;
+      if (kb < M) B = conj(fourier_scale_any<T>(mode, kb, NI) * Fr[(size_t)kb * ms]);
+      a[k] = (A + B) + cmul(mk((T)0, (T)1), cmul(conj(tw[k]), A - B));
+    }
+  }
+  __syncthreads();
+  const cx<T> *z = stockham_row<+1, T>(a, b, ff, tw, tt, live);
+  if (live) {
+    cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+    for (int n = tt; n < NC; n += FFT_ANY_TPR) st_nt(out + n, z[n]);
+  }
+}
+
+// grid -> Fourier (m <= M-1) for any FftFactors length.
+template <class T>
+__global__ __launch_bounds__(256) void k_fft_any(const T *__restrict__ G, cx<T> *__restrict__ F, int C,
+                                                 ColMap cm, int J, int M, int mode, ActiveMask am,
+                                                 FftFactors ff, const cx<T> *__restrict__ tw) {
+  extern __shared__ unsigned char fft_any_lds[];
+  const int NC = ff.NC, NI = ff.NI;
+  const int rl = threadIdx.x / FFT_ANY_TPR, tt = threadIdx.x - rl * FFT_ANY_TPR;
+  cx<T> *a = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)rl * 2 * NC, *b = a + NC;
+  const long row = (long)blockIdx.x * (blockDim.x / FFT_ANY_TPR) + rl;
+  int j = 0, col = 0, e = 0, r = 0;
+  bool live = am.row_col(row, J, C, j, col);
+  if (live) {
+    cm.map(col, e, r);
+    live = am(e);
+  }
+  if (live) {
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+    for (int n = tt; n < NC; n += FFT_ANY_TPR) a[n] = ld_nt(in + n);
+  }
+  __syncthreads();
+  const cx<T> *Z = stockham_row<-1, T>(a, b, ff, tw, tt, live);
+  if (live) {
+    cx<T> *Fr = F + fidx(0, j, col, J, C);
+    const size_t ms = (size_t)J * fpad(C);
+    const T half = (T)0.5;
+    // X_k = (s - i w d) / 2 / NI, s = Z_k + conj Z_{NC-k}, d = Z_k - conj Z_{NC-k},
+    // w = e^{-2 pi i k / NI} (spectral.py:159-164)
+    for (int k = tt; k < M; k += FFT_ANY_TPR) {
+      const cx<T> zk = Z[k], zc = conj(Z[(NC - k) % NC]);
+      const cx<T> s = zk + zc, d = zk - zc;
+      const cx<T> wd = cmul(tw[k], d);
+      const T sc = mode == 2 ? fourier_scale_any<T>(2, k, NI) : (T)1 / (T)NI;
+      Fr[(size_t)k * ms] = sc * mk(half * (s.x + wd.y), half * (s.y - wd.x));
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -296,10 +296,11 @@ struct TendArgs {
 // d/dx pi, d/dy pi, pi, sum u, sum v, sum D.
 template <class T>
 __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
-  // blocks of 128 longitudes: blockIdx = (e * J + j) * (I / 128) + chunk
-  const int nch = a.I / 128;
+  // blocks of blockDim.x (128 or 64) longitudes:
+  // blockIdx = (e * J + j) * (I / blockDim.x) + chunk
+  const int nch = a.I / (int)blockDim.x;
   const int ej = blockIdx.x / nch;
-  const int i = (blockIdx.x - ej * nch) * 128 + threadIdx.x;
+  const int i = (blockIdx.x - ej * nch) * (int)blockDim.x + threadIdx.x;
   const int e = ej / a.J, j = ej - e * a.J;
   if (!a.am(e)) return;  // a finished member: its tendencies are not used
   const int L = a.L, I = a.I, J = a.J;
@@ -889,9 +890,9 @@ __global__ __launch_bounds__(128) void k_basic_terms(
     const double *__restrict__ q, const double *__restrict__ ps,  // reference layouts
     const double *__restrict__ gsyn, const double *__restrict__ ggrad, const double *__restrict__ vc,
     const double *__restrict__ mu, double *__restrict__ bsg, int J, int I, int L) {
-  const int nch = I / 128;
+  const int nch = I / (int)blockDim.x;  // blocks of 128 or 64 longitudes
   const int bj = blockIdx.x / nch;
-  const int i = (blockIdx.x - bj * nch) * 128 + threadIdx.x;
+  const int i = (blockIdx.x - bj * nch) * (int)blockDim.x + threadIdx.x;
   const int b = bj / J, j = bj - b * J;
   const VOff vo{L};
   const BRow br{L};

@@ -70,3 +70,14 @@ CASES = [
 def test_wrong_shape_raises_value_error(plan, name, call):
     with pytest.raises(ValueError):
         call(plan)
+
+
+@pytest.mark.parametrize("nlon,nlat,msg", [(100, 64, "nlon=100"), (448, 224, "nlon=448"),
+                                           (2048, 64, "nlon=2048"), (128, 60, "nlat=60")])
+def test_unsupported_grids_refused_before_any_device_call(nlon, nlat, msg):
+    """Grids whose FFT length has a radix other than 2, 3, 5 (or exceeds
+    1024 longitudes), or whose latitudes are not a multiple of 8, are
+    refused by lbm_plan_create before it touches the device."""
+    from lbm_isca_amd.plan import Plan
+    with pytest.raises(ValueError, match=msg):
+        Plan(nlon=nlon, nlat=nlat, nmax=21)
