@@ -35,6 +35,11 @@
  * against lbm_abi_version() of the library it loads. */
 #define LBM_ABI_VERSION 5
 
+/* Levels a plan accepts (1 .. LBM_MAX_LEVELS; 20 and 40 run kernels
+ * specialised for them, other counts the next register capacity of 16,
+ * 32, 48, 64 levels with the count at run time). */
+#define LBM_MAX_LEVELS 64
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -64,7 +69,8 @@ typedef struct {
                                        of 64 up to 1024 with nlon/2 a product
                                        of 2, 3 and 5 (T21 64, T63 192, T85 256,
                                        T106 320 ...); nlat a multiple of 8;
-                                       nmax + 1 <= nlon/2; nlev 20 or 40      */
+                                       nmax + 1 <= nlon/2; nlev 1 ..
+                                       LBM_MAX_LEVELS                         */
   int32_t nmem;                     /* ensemble members E >= 1               */
   int32_t nbasic;                   /* 1 = one shared basic state, or nmem   */
   int32_t precision;                /* lbm_precision                         */

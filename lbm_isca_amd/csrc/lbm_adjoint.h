@@ -424,6 +424,7 @@ struct StepAdjArgs {
   const Ctl *ctl;
   const int *nsteps;
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  int L;     // levels
   int ecnt;  // members covered by the launch (<= E)
   T delt, robert;
   // 1: the default model (pi at t-dt is the analysed ps'_b: its adjoint goes
@@ -431,19 +432,21 @@ struct StepAdjArgs {
   int traj;
 };
 
-template <class T, int L, class P>
-__device__ inline void matvec_t(P A, const cx<T> *x, cx<T> *y) {
+template <class T, int LT, bool EXACT, class P>
+__device__ inline void matvec_t(P A, const cx<T> *x, cx<T> *y, int L) {
 #pragma unroll
-  for (int c = 0; c < L; ++c) {
+  for (int c = 0; c < LT; ++c) if (EXACT || c < L) {
     cx<T> acc = zero<T>();
 #pragma unroll
-    for (int r = 0; r < L; ++r) fma_acc(acc, A[r * L + c], x[r]);
+    for (int r = 0; r < LT; ++r) if (EXACT || r < L) fma_acc(acc, A[r * L + c], x[r]);
     y[c] = acc;
   }
 }
 
-template <class T, int L>
+// LT, EXACT: as in k_step.
+template <class T, int LT, bool EXACT = true>
 __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
+  const int L = EXACT ? LT : a.L;
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
   const int s = idx / a.ecnt, e = idx - s * a.ecnt;
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
     // a member past its length: the step is the identity
     if (role == 1) {
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
+      for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
         bA.st(voff, lo(0, l), bA.ld(voff, lo(0, l), tag) + vc[vo.dsig() + l] * svor);
         bT.st(voff, lo(0, l), zero<T>());
         bT.st(voff, lo(3, l), zero<T>());
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
       return;
     }
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
       bA.st(voff, lo(1, l), bA.ld(voff, lo(1, l), tag) + vc[vo.dsig() + l] * sdiv);
       bT.st(voff, lo(1, l), zero<T>());
       bT.st(voff, lo(2, l), zero<T>());
@@ -517,22 +520,22 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const T ca = oinit ? (T)1 : (T)1 - (T)2 * gam;
   const T cb = oinit ? (T)0 : gam;
   const T cn = oinit ? (T)0 : gam;
-  cx<T> x[L], y[L], dbar[L];
+  cx<T> x[LT], y[LT], dbar[LT];
 
   if (role == 1) {
   // vorticity: vor = Vv (vor_b + 2dt wt_vor) / (1 + 2dt kv)
 #pragma unroll
-  for (int l = 0; l < L; ++l)
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L)
     x[l] = bA.ld(voff, lo(0, l), tag) + vc[vo.dsig() + l] * svor + (resp ? wr * a.wlev[l] : (T)0) * ob +
            cn * bB.ld(voff, lo(0, l), tag);
   if ((a.vmask & 1) && vd_low) {
-    matvec_t<T, L>(vinv, x, y);
+    matvec_t<T, LT, EXACT>(vinv, x, y, L);
   } else {
 #pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = x[l];
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) y[l] = x[l];
   }
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const cx<T> yl = ((T)1 / ((T)1 + dt2 * kv[l])) * y[l];
     const cx<T> bb = bB.ld(voff, lo(0, l), tag);
     bB.st(voff, lo(0, l), cb * bb + yl);
@@ -541,15 +544,15 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   }
   // humidity
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(3, l), tag) + cn * bB.ld(voff, lo(3, l), tag);
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = bA.ld(voff, lo(3, l), tag) + cn * bB.ld(voff, lo(3, l), tag);
   if (a.vmask & 8) {
-    matvec_t<T, L>(vinv + 3 * L * L, x, y);
+    matvec_t<T, LT, EXACT>(vinv + 3 * L * L, x, y, L);
   } else {
 #pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = x[l];
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) y[l] = x[l];
   }
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const cx<T> yl = ((T)1 / ((T)1 + dt2 * kq[l])) * y[l];
     const cx<T> bb = bB.ld(voff, lo(3, l), tag);
     bB.st(voff, lo(3, l), cb * bb + yl);
@@ -565,15 +568,15 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   cx<T> pib_a = cb * bpi + pin_a;
   // divergence: D = Vd (2 dbar - D_b)
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(1, l), tag) + vc[vo.dsig() + l] * sdiv + cn * bB.ld(voff, lo(1, l), tag);
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = bA.ld(voff, lo(1, l), tag) + vc[vo.dsig() + l] * sdiv + cn * bB.ld(voff, lo(1, l), tag);
   if ((a.vmask & 2) && vd_low) {
-    matvec_t<T, L>(vinv + 1 * L * L, x, y);
+    matvec_t<T, LT, EXACT>(vinv + 1 * L * L, x, y, L);
   } else {
 #pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = x[l];
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) y[l] = x[l];
   }
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const cx<T> bb = bB.ld(voff, lo(1, l), tag);
     bA.st(voff, lo(1, l), ca * bb);
     bB.st(voff, lo(1, l), cb * bb - y[l]);
@@ -581,15 +584,15 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   }
   // temperature: T = Vt (T_b + 2dt (wt_t - tmtx dbar)) / ft
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = bA.ld(voff, lo(2, l), tag) + cn * bB.ld(voff, lo(2, l), tag);
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = bA.ld(voff, lo(2, l), tag) + cn * bB.ld(voff, lo(2, l), tag);
   if (a.vmask & 4) {
-    matvec_t<T, L>(vinv + 2 * L * L, x, y);
+    matvec_t<T, LT, EXACT>(vinv + 2 * L * L, x, y, L);
   } else {
 #pragma unroll
-    for (int l = 0; l < L; ++l) y[l] = x[l];
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) y[l] = x[l];
   }
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const T ft = (T)1 + dt2 * ktt[l];
     const cx<T> yl = ((T)1 / ft) * y[l];
     const cx<T> bb = bB.ld(voff, lo(2, l), tag);
@@ -602,7 +605,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   {
     cx<T> pre = zero<T>();
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
       const cx<T> tt = (kap * tref[l] * vc[vo.alpha() + l]) * y[l] + vc[vo.dsig() + l] * pre;
       pre = pre + (kap * tref[l] * vc[vo.w3() + l]) * y[l];
       dbar[l] = dbar[l] - tt;
@@ -610,13 +613,13 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   }
   const cx<T> wtp = dt2 * pin_a;
 #pragma unroll
-  for (int l = 0; l < L; ++l) dbar[l] = dbar[l] - vc[vo.dsig() + l] * wtp;
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) dbar[l] = dbar[l] - vc[vo.dsig() + l] * wtp;
   // dbar = M(n)^-1 rhs,  rhs = (1 + dt kd) ft D_b + dt (ft wt_d - lap phy),
   // phy_l = phi_l + ft_l R Tr_l cpg_l (pi_b + dt wt_pi)
-  matvec_t<T, L>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, dbar, x);
+  matvec_t<T, LT, EXACT>(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L, dbar, x, L);
   cx<T> wdpix_a = zero<T>();
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const T ft = (T)1 + dt2 * ktt[l];
     bB.st(voff, lo(1, l), bB.ld(voff, lo(1, l), tag) + (((T)1 + dt * kd[l]) * ft) * x[l]);
     bT.st(voff, lo(1, l), (dt * ft) * x[l]);
@@ -633,7 +636,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   {
     cx<T> suf = zero<T>();
 #pragma unroll
-    for (int l = L - 1; l >= 0; --l) {
+    for (int l = LT - 1; l >= 0; --l) if (EXACT || l < L) {
       const cx<T> w_a = (rg * vc[vo.alpha() + l]) * y[l] + (rg * vc[vo.dlog3() + l]) * suf;
       suf = suf + y[l];
       bB.st(voff, lo(2, l), bB.ld(voff, lo(2, l), tag) + ((T)1 + dt * ktt[l]) * w_a);
@@ -714,6 +717,6 @@ __global__ void k_add_cx(const cx<T> *__restrict__ a, const cx<T> *__restrict__ 
     out[q] = a[q] + b[q];
 }
 
-__global__ void k_tick_back(Ctl *ctl) { ctl->step -= 1; }
+static __global__ void k_tick_back(Ctl *ctl) { ctl->step -= 1; }
 
 }  // namespace lbm

@@ -21,6 +21,7 @@
 #include "../../include/lbm_hip.h"
 #include "lbm_adjoint.h"
 #include "lbm_history.h"
+#include "lbm_levels.h"
 #include "lbm_util.h"
 
 using namespace lbm;
@@ -1229,22 +1230,35 @@ StepArgs<T> step_args(lbm_plan *p, int ecnt) {
   a.skip_low_n = p->skip_low_n;
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
+  a.L = p->L;
   a.ecnt = ecnt;
   a.fixed_set = -1;
   a.fixed_dt = 0;
   return a;
 }
 
-template <class T, int L>
+// 20 and 40 levels run step kernels specialised for them; other counts
+// the wide launchers of lbm_levels.hip (level count at run time).
+template <class T>
 int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   const StepArgs<T> a = step_args<T>(p, ecnt);
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step
   const dim3 grid((unsigned)((nthr + 63) / 64), 2);
-  if (ecnt % 64 == 0 && p->nb == 1)
-    hipLaunchKernelGGL((k_step<T, L, true>), grid, dim3(64), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_step<T, L, false>), grid, dim3(64), 0, st, a);
+  const bool uni = ecnt % 64 == 0 && p->nb == 1;
+  if (p->L == 20) {
+    if (uni)
+      hipLaunchKernelGGL((k_step<T, 20, true>), grid, dim3(64), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_step<T, 20, false>), grid, dim3(64), 0, st, a);
+  } else if (p->L == 40) {
+    if (uni)
+      hipLaunchKernelGGL((k_step<T, 40, true>), grid, dim3(64), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
+  } else {
+    HIPCHK(launch_step_wide<T>(a, grid, st));
+  }
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -1393,12 +1407,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   if (!fork && hist) CHK(launch_hist());
   if (hist) CHK(join(F_HIST));
   CHK(mark(p, 7, st));
-  if (L == 20)
-    CHK((launch_step_kernel<T, 20>(p, st, ne)));
-  else if (L == 40)
-    CHK((launch_step_kernel<T, 40>(p, st, ne)));
-  else
-    return fail(LBM_ERR_VALUE, "nlev=%d is not built (20 or 40)", L);
+  CHK(launch_step_kernel<T>(p, st, ne));
   CHK(mark(p, 8, st));
   if (p->resp_set)
     hipLaunchKernelGGL((k_response<T>), dim3(ne), dim3(256), 0, st, (const cx<T> *)p->sa3,
@@ -1704,8 +1713,10 @@ int step_t(lbm_plan *p, double dt, const double *const prev[5], const double *co
   const dim3 grid((unsigned)(((size_t)p->nspec * p->E + 63) / 64), 2);
   if (p->L == 20)
     hipLaunchKernelGGL((k_step<T, 20, false>), grid, dim3(64), 0, st, a);
-  else
+  else if (p->L == 40)
     hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
+  else
+    HIPCHK(launch_step_wide<T>(a, grid, st));
   HIPCHK(hipGetLastError());
   return unpack_spec5<T>(p, p->x_sa3, 4, p->x_sa2, 3, 0, out, st);
 }
@@ -1929,7 +1940,7 @@ int set_response_t(lbm_plan *p, const double *wgrid, const double *wlev, const d
 // kernels with the analysis tables), inverse FFT, transposed tendencies,
 // forward FFT, transposed synthesis (analysis kernels with the synthesis
 // tables), then the step counter back by one.
-template <class T, int L>
+template <class T>
 int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
   StepAdjArgs<T> a;
@@ -1965,11 +1976,18 @@ int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   a.skip_low_n = p->skip_low_n;
   a.delt = (T)p->delt;
   a.robert = (T)p->robert;
+  a.L = p->L;
   a.ecnt = ecnt;
   a.traj = p->traj_cur != nullptr;
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step_adj
-  hipLaunchKernelGGL((k_step_adj<T, L>), dim3((unsigned)((nthr + 63) / 64), 2), dim3(64), 0, st, a);
+  const dim3 grid((unsigned)((nthr + 63) / 64), 2);
+  if (p->L == 20)
+    hipLaunchKernelGGL((k_step_adj<T, 20>), grid, dim3(64), 0, st, a);
+  else if (p->L == 40)
+    hipLaunchKernelGGL((k_step_adj<T, 40>), grid, dim3(64), 0, st, a);
+  else
+    HIPCHK(launch_step_adj_wide<T>(a, grid, st));
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -1981,10 +1999,7 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   const ActiveMask am = loop_mask(p, false, ecnt);
   const int ne = ecnt > 0 ? ecnt : E;
   CHK(mark(p, 7, st));
-  if (L == 20)
-    CHK((launch_step_adj<T, 20>(p, st, ne)));
-  else
-    CHK((launch_step_adj<T, 40>(p, st, ne)));
+  CHK(launch_step_adj<T>(p, st, ne));
   const ColMap cmo{E, L, 9, 2}, cmi{E, L, 6, 6};
   const int Co = cmo.ncols(), Ci = cmi.ncols();
   CHK(mark(p, 5, st));
@@ -2486,7 +2501,8 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   if (d->nmax < 1 || d->nmax + 1 > d->nlon / 2 || d->nmax >= d->nlat)
     return fail(LBM_ERR_VALUE, "nmax=%d unsupported for nlon=%d, nlat=%d", d->nmax, d->nlon,
                 d->nlat);
-  if (d->nlev != 20 && d->nlev != 40) return fail(LBM_ERR_VALUE, "nlev=%d: 20 or 40 supported", d->nlev);
+  if (d->nlev < 1 || d->nlev > LBM_MAX_LEVELS)
+    return fail(LBM_ERR_VALUE, "nlev=%d: 1 to %d levels are supported", d->nlev, LBM_MAX_LEVELS);
   if (d->nmem < 1) return fail(LBM_ERR_VALUE, "nmem must be >= 1");
   if (d->nbasic != 1 && d->nbasic != d->nmem) return fail(LBM_ERR_VALUE, "nbasic must be 1 or nmem");
   if (d->precision != LBM_FP64 && d->precision != LBM_FP32 && d->precision != LBM_BF16X3)

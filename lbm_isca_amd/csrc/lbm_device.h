@@ -303,10 +303,11 @@ struct FftShape {
 };
 
 // Twiddles exp(-2 pi i k / NI), k = 0..NI-1, filled by the host.
-__constant__ cx<double> c_tw128_d[128];
-__constant__ cx<float> c_tw128_f[128];
-__constant__ cx<double> c_tw384_d[384];
-__constant__ cx<float> c_tw384_f[384];
+// (static: each translation unit that includes this header has its own)
+static __constant__ cx<double> c_tw128_d[128];
+static __constant__ cx<float> c_tw128_f[128];
+static __constant__ cx<double> c_tw384_d[384];
+static __constant__ cx<float> c_tw384_f[384];
 template <class T, int NI>
 __device__ inline const cx<T> *twiddles();
 template <>

@@ -55,7 +55,7 @@ struct Ctl {
   int pad;
 };
 
-__global__ void k_tick(Ctl *ctl) {
+static __global__ void k_tick(Ctl *ctl) {
   ctl->step += 1;
   ctl->raw = 0;
 }
@@ -452,6 +452,7 @@ struct StepArgs {
   int *pend;                     // [E]
   const int *nsteps;             // [E] full steps per member
   int E, M, nspec, nbasic, inistp, vmask, skip_low_n;
+  int L;                         // levels
   int ecnt;                      // members covered by the launch (<= E)
   T delt, robert;
   // >= 0: one Integrator.step of the given dt with matrix set fixed_set on
@@ -474,13 +475,13 @@ struct RoPtr<T, true> {
   using type = const T __attribute__((address_space(4))) *;
 };
 
-template <class T, int L, class P>
-__device__ inline void matvec(P A, const cx<T> *x, cx<T> *y) {
+template <class T, int LT, bool EXACT, class P>
+__device__ inline void matvec(P A, const cx<T> *x, cx<T> *y, int L) {
 #pragma unroll
-  for (int r = 0; r < L; ++r) {
+  for (int r = 0; r < LT; ++r) if (EXACT || r < L) {
     cx<T> acc = zero<T>();
 #pragma unroll
-    for (int c = 0; c < L; ++c) fma_acc(acc, A[r * L + c], x[c]);
+    for (int c = 0; c < LT; ++c) if (EXACT || c < L) fma_acc(acc, A[r * L + c], x[c]);
     y[r] = acc;
   }
 }
@@ -491,9 +492,14 @@ __device__ inline void matvec(P A, const cx<T> *x, cx<T> *y) {
 // width a multiple of 64, nbasic = 1), so the degree, the damping rows and
 // the L x L matrices are wave-uniform and come through the scalar cache
 // instead of one vector load per element.
-template <class T, int L, bool UNI = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(L <= 20 ? 2 : 1)))
+//
+// LT levels in registers.  EXACT: the plan has LT levels (compile-time
+// loops); otherwise it has L = a.L <= LT, the loops run to LT and skip the
+// levels past L, whose registers are never read.
+template <class T, int LT, bool UNI = false, bool EXACT = true>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LT <= 20 ? 2 : 1)))
 void k_step(StepArgs<T> a) {
+  const int L = EXACT ? LT : a.L;
   const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
   const int s = UNI ? __builtin_amdgcn_readfirstlane(idx / a.ecnt) : idx / a.ecnt;
@@ -552,8 +558,8 @@ void k_step(StepArgs<T> a) {
   // (FP32, 40 levels, measured faster there only): dbar goes row by row into
   // the divergence-tendency slot (its last read is the rhs) and is read back
   // where used, one level array fewer for a round trip through the cache.
-  constexpr bool DB_MEM = sizeof(T) == 4 && L == 40;
-  cx<T> x[L], dbar[DB_MEM ? 1 : L];
+  constexpr bool DB_MEM = sizeof(T) == 4 && LT == 40 && EXACT;
+  cx<T> x[LT], dbar[DB_MEM ? 1 : LT];
   const RU vinv = (RU)(a.vinv + (size_t)set * 4 * L * L);
   const bool vd_low = !(a.skip_low_n && n <= 1);
   // The reference keeps winds, not vorticity and divergence, on its grid:
@@ -565,12 +571,12 @@ void k_step(StepArgs<T> a) {
   auto update = [&](int f, RU V, bool apply, cx<T> *sum, bool drop) __attribute__((always_inline)) {
     cx<T> acc = zero<T>();
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
+    for (int r = 0; r < LT; ++r) if (EXACT || r < L) {
       cx<T> y = x[r];
       if (apply) {
         y = zero<T>();
 #pragma unroll
-        for (int c = 0; c < L; ++c) fma_acc(y, V[r * L + c], x[c]);
+        for (int c = 0; c < LT; ++c) if (EXACT || c < L) fma_acc(y, V[r * L + c], x[c]);
       }
       if (drop) y = zero<T>();
       if (!pure) {
@@ -586,22 +592,22 @@ void k_step(StepArgs<T> a) {
   if (role == 1) {
     // vorticity
 #pragma unroll
-    for (int l = 0; l < L; ++l) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (bsb.ld(voff, lo(0, l), tag) + dt2 * btn.ld_nt(voff, lo(0, l), tag));
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = (T)1 / ((T)1 + dt2 * kv[l]) * (bsb.ld(voff, lo(0, l), tag) + dt2 * btn.ld_nt(voff, lo(0, l), tag));
     update(0, vinv, (a.vmask & 1) && vd_low, s2 + E, drop_mean);
     // humidity
 #pragma unroll
-    for (int l = 0; l < L; ++l)
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L)
       x[l] = (T)1 / ((T)1 + dt2 * kq[l]) * (bsb.ld(voff, lo(3, l), tag) + dt2 * btn.ld_nt(voff, lo(3, l), tag));
     update(3, vinv + 3 * L * L, (a.vmask & 8) != 0, nullptr, false);
     return;
   }
   // temperature estimate and its geopotential (integrator.py:287-292)
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = ((T)1 + dt * ktt[l]) * bsb.ld(voff, lo(2, l), tag) + dt * btn.ld(voff, lo(2, l), tag);
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = ((T)1 + dt * ktt[l]) * bsb.ld(voff, lo(2, l), tag) + dt * btn.ld(voff, lo(2, l), tag);
   {
     cx<T> acc = zero<T>();
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
       const cx<T> wl = x[l];
       x[l] = acc + (rg * vc[vo.alpha() + l]) * wl;
       acc = acc + (rg * vc[vo.dlog3() + l]) * wl;
@@ -609,7 +615,7 @@ void k_step(StepArgs<T> a) {
   }
   const cx<T> wdpix = pi_prev + dt * wt_pi;
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) {
     const T ft = (T)1 + dt2 * ktt[l];
     const T gg = rg * tref[l] * vc[vo.cpg() + l];
     const cx<T> phy = x[l] + (ft * gg) * wdpix;
@@ -620,10 +626,10 @@ void k_step(StepArgs<T> a) {
   {
     const RP mi = (RP)(a.minv + (((size_t)set * a.nbasic + eb) * a.M + n) * L * L);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
+    for (int r = 0; r < LT; ++r) if (EXACT || r < L) {
       cx<T> acc = zero<T>();
 #pragma unroll
-      for (int c = 0; c < L; ++c) fma_acc(acc, mi[r * L + c], x[c]);
+      for (int c = 0; c < LT; ++c) if (EXACT || c < L) fma_acc(acc, mi[r * L + c], x[c]);
       if constexpr (DB_MEM)
         btn.st(voff, lo(1, r), acc);
       else
@@ -649,13 +655,13 @@ void k_step(StepArgs<T> a) {
   }
   // divergence
 #pragma unroll
-  for (int l = 0; l < L; ++l) x[l] = (T)2 * DB(l) - bsb.ld(voff, lo(1, l), tag);
+  for (int l = 0; l < LT; ++l) if (EXACT || l < L) x[l] = (T)2 * DB(l) - bsb.ld(voff, lo(1, l), tag);
   update(1, vinv + 1 * L * L, (a.vmask & 2) && vd_low, s2 + 2 * E, drop_mean);
   // temperature: restore -tmtx dbar (integrator.py:300), damped update
   {
     cx<T> suf = zero<T>();
 #pragma unroll
-    for (int l = L - 1; l >= 0; --l) {
+    for (int l = LT - 1; l >= 0; --l) if (EXACT || l < L) {
       const T ft = (T)1 + dt2 * ktt[l];
       const cx<T> db = DB(l);
       const cx<T> tmd = (kap * tref[l]) * (vc[vo.alpha() + l] * db + vc[vo.w3() + l] * suf);
@@ -885,7 +891,7 @@ __global__ void k_ps_from_pi(const T *__restrict__ g, int R, int r0, const T *__
 // Basic-state terms (dynamics.py:54-93), FP64.  gsyn rows: vor (L), div (L);
 // ggrad rows: d/dx ln ps, d/dy ln ps.  Writes the BRow block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_basic_terms(
+static __global__ __launch_bounds__(128) void k_basic_terms(
     const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ t,
     const double *__restrict__ q, const double *__restrict__ ps,  // reference layouts
     const double *__restrict__ gsyn, const double *__restrict__ ggrad, const double *__restrict__ vc,
@@ -942,7 +948,7 @@ __global__ __launch_bounds__(128) void k_basic_terms(
 }
 
 // Plain grid mean of basic Tv per level (vertical.py:206-216), fixed order.
-__global__ __launch_bounds__(256) void k_tbar(const double *__restrict__ bsg, double *__restrict__ tbar,
+static __global__ __launch_bounds__(256) void k_tbar(const double *__restrict__ bsg, double *__restrict__ tbar,
                                               int J, int I, int L) {
   __shared__ double red[256];
   const int b = blockIdx.x / L, k = blockIdx.x - b * L;
