@@ -41,6 +41,7 @@ import json
 import os
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -352,6 +353,59 @@ def measure_adjoint(E=32, max_days=None, window=(20.0, 30.0), every=48):
             "members": E, **res}
 
 
+RESOLUTIONS = [("T21", 21, 64, 32), ("T42", 42, 128, 64), ("T63", 63, 192, 96),
+               ("T85", 85, 256, 128), ("T106", 106, 320, 160)]
+
+
+def measure_resolutions(E=64, single_steps=108, ens_steps=72):
+    """The resolution sweep of the reference's perf benchmark
+    (analysis/benchmark_perf_isca.py:43-45, 78-106): a single member about a
+    resting isothermal (250 K) basic state, default Config (L20, dt 1200 s),
+    ms per step over 1.5 days after a warm-up; plus the E-member ensemble's
+    member-days/s on the same grid."""
+    import numpy as np
+    import torch
+    import lbm_isca_amd as L
+    out = {}
+    for name, nmax, nlon, nlat in RESOLUTIONS:
+        sp = L.BatchedSpectral(nlon, nlat, nmax)
+        vg = L.VerticalGrid()
+        shp = (nlat, nlon, vg.kmax)
+        rest = types.SimpleNamespace(u=np.zeros(shp), v=np.zeros(shp), t=np.full(shp, 250.0),
+                                     q=np.zeros(shp), ps=np.full((nlat, nlon), 1000.0))
+        rec = {"nlon": nlon, "nlat": nlat, "ndof": (nmax + 1) * (nmax + 2) // 2}
+        for nmem, steps in ((1, single_steps), (E, ens_steps)):
+            m, _, _ = L.make_ensemble_model(rest, nmem, L.Config(), spectral=sp)
+            plan = m.plan
+            plan.set_forcing(cases_heating(sp, vg, nmem))
+            plan.run(22, use_graph=True)
+            plan.run_steps(steps, m.cfg.inistp, use_graph=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.run_to(steps, 0, use_graph=True)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if nmem == 1:
+                rec["single_member_ms_per_step"] = round(dt * 1e3 / steps, 4)
+            else:
+                rec[f"members_{nmem}_member_days_per_s"] = round(nmem * steps / 72.0 / dt, 1)
+                rec[f"members_{nmem}_ms_per_step"] = round(dt * 1e3 / steps, 4)
+            del m, plan
+            torch.cuda.empty_cache()
+        log(f"{name}: {rec}")
+        out[name] = rec
+    return {"workload": "resting isothermal 250 K basic state, L20, dt 1200 s, 2 K/day heating at "
+                        "0N 180E; single member (the reference sweep's case) and a "
+                        f"{E}-member ensemble; full steps only",
+            "members": E, **out}
+
+
+def cases_heating(sp, vg, nmem):
+    from lbm_isca_amd import cases
+    return np.broadcast_to(cases.heating(sp.lat, sp.lon, vg.sig, 0.0, 180.0, 2.0)[None],
+                           (nmem, sp.nlat, sp.nlon, vg.kmax))
+
+
 def legendre_flops(E, L, nspec, Jh):
     """Useful flops of the level Legendre kernels of one step: each real
     contraction is 2 nspec Jh (2 E L); synthesis has 8 of them (P for the six
@@ -455,7 +509,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP32-vs-FP64 uneven ensemble and the adjoint timing")
-    ap.add_argument("--only", choices=("uneven", "adjoint", "t127"), default=None,
+    ap.add_argument("--only", choices=("uneven", "adjoint", "t127", "resolutions"), default=None,
                     help="run only one of the extra measurements (profiling)")
     ap.add_argument("--uneven-members", type=int, default=256)
     ap.add_argument("--adjoint-members", type=int, default=32)
@@ -499,6 +553,8 @@ def main():
             res = measure_fp32_vs_fp64(args.uneven_members, args.max_days)
         elif args.only == "t127":
             res = measure_t127(args.t127_members, args.t127_days)
+        elif args.only == "resolutions":
+            res = measure_resolutions()
         else:
             res = measure_adjoint(args.adjoint_members, args.max_days, tuple(args.window))
         print(json.dumps({args.only: res}), flush=True)
@@ -548,7 +604,8 @@ def main():
                          lambda: measure_fp32_vs_fp64(args.uneven_members, args.max_days)),
                         ("adjoint", lambda: measure_adjoint(args.adjoint_members, args.max_days,
                                                             tuple(args.window))),
-                        ("t127l40", lambda: measure_t127(args.t127_members, args.t127_days))):
+                        ("t127l40", lambda: measure_t127(args.t127_members, args.t127_days)),
+                        ("resolutions", measure_resolutions)):
             log(f"{key} ...")
             try:
                 extras[key] = fn()
