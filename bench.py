@@ -565,6 +565,10 @@ def main():
     log(f"warmup {args.warmup} steps ...")
     plan.run(args.warmup, use_graph=True)
     torch.cuda.synchronize()
+    # the step's launch variants timed on this ensemble (the state is kept)
+    from lbm_isca_amd import tuning
+    tuned = tuning.autotune(plan, True)
+    log(f"autotuned: {tuned['report']}")
     # a run's two start-up half steps go before the timed region, which then
     # holds exactly --steps full steps (model.py:211-251)
     plan.run_steps(args.steps, m.cfg.inistp, use_graph=True)
@@ -625,7 +629,8 @@ def main():
         "config": {"workload": f"T42L20 {E}-member heating ensemble, solid-body basic state, "
                                f"{args.precision.upper()}, 20 equal sigma levels",
                    "members": E, "resolution": "T42L20", "dt_s": 1200.0,
-                   "graph": True, "startup_steps_in_timed_region": 0},
+                   "graph": True, "startup_steps_in_timed_region": 0,
+                   "autotuned": dict(zip(plan.TUNED, tuned["choice"]))},
         "roofline": rl, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(out), flush=True)

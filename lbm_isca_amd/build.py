@@ -16,6 +16,10 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 arithmetic is kept off (v_pk_*_f32 not emitted)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b+c is fused within a source expression only, never across
+    # statements: the same expression rounds the same in every kernel it is
+    # inlined into (fused and separate launch variants are bit-identical)
+    "-ffp-contract=on",
 ]
 
 

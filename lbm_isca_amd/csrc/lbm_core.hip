@@ -332,8 +332,8 @@ struct lbm_plan {
   // launch variants chosen by lbm_autotune (0 = the built-in choice):
   // inverse-FFT rows per block, forward-FFT rows per block, occupancy target
   // of the level synthesis and of the level tendency analyses
-  static constexpr int NTUNE = 4;
-  int tune[NTUNE] = {0, 0, 0, 0};
+  static constexpr int NTUNE = 5;
+  int tune[NTUNE] = {0, 0, 0, 0, 0};
   // kernel timing (lbm_profile)
   static constexpr int NPROF = 8;
   bool prof = false;
@@ -1269,6 +1269,69 @@ inline int mark(lbm_plan *p, int k, hipStream_t st) {
   return LBM_OK;
 }
 
+// Arguments of k_tend for the plan's buffers.
+template <class T>
+TendArgs<T> tend_args(lbm_plan *p, bool raw_path, const ActiveMask &am) {
+  DevTabs<T> &t = p->tabs<T>();
+  TendArgs<T> ta;
+  ta.gin = (const T *)p->gin;
+  ta.raw = (raw_path && p->have_raw) ? (const T *)p->raw : nullptr;
+  ta.bsg = (const T *)p->bsg;
+  ta.frc = (const T *)p->frc;
+  ta.gout = (T *)p->gout;
+  ta.psa = (T *)p->psa;
+  ta.psb = (T *)p->psb;
+  ta.pend = p->pend;
+  ta.vc = t.vc;
+  ta.tref = t.tref;
+  ta.rcos = t.rcos;
+  ta.ctl = p->ctl;
+  ta.E = p->E;
+  ta.J = p->J;
+  ta.I = p->I;
+  ta.L = p->L;
+  ta.nbasic = p->nb;
+  ta.nfrc = p->nfrc;
+  ta.strictly_linear = p->strictly_linear || p->linear;
+  ta.linear = p->linear;
+  ta.robert = (T)p->robert;
+  ta.am = am;
+  ta.h = hist_args<T>(p);
+  ta.traj = (T *)p->traj_cur;
+  ta.fout = (cx<T> *)p->fout;
+  ta.C = ColMap{p->E, p->L, 9, 2}.ncols();
+  ta.M = p->M;
+  ta.ne = p->E;
+  return ta;
+}
+
+// The forward FFT of the tendency rows as k_fft after k_tend (tuning
+// choice 4 = 0, built in) or inside k_tend (1, 128 longitudes).
+inline bool tend_fused(const lbm_plan *p) { return p->I == 128 && p->tune[4] == 1; }
+
+// k_tend for ne members, then (unfused) the forward FFT of its rows; the
+// profiling marks 3 and 4 bracket k_tend.
+template <class T>
+int launch_tend(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
+  const ColMap cmo{p->E, p->L, 9, 2};
+  CHK(mark(p, 3, st));
+  if (tend_fused(p)) {
+    // blocks of (MB members, latitude), MB x 128 threads
+    constexpr int MB = tend_fft_members<T>();
+    TendArgs<T> tf = ta;
+    tf.ne = ne;
+    hipLaunchKernelGGL((k_tend<T, true>), dim3((unsigned)((ne + MB - 1) / MB * p->J)), dim3(MB * 128), 0,
+                       st, tf);
+  } else {
+    hipLaunchKernelGGL((k_tend<T, false>), dim3(ne * p->J * (p->I / grid_chunk(p->I))),
+                       dim3(grid_chunk(p->I)), 0, st, ta);
+  }
+  HIPCHK(hipGetLastError());
+  CHK(mark(p, 4, st));
+  if (!tend_fused(p)) CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, ta.am));
+  return LBM_OK;
+}
+
 // One time step: synthesis, inverse FFT, grid tendencies, forward FFT,
 // analysis, spectral step.  raw_path: the first step after a raw initial
 // state (k_tend reads the raw grid).
@@ -1340,31 +1403,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
     if (fork && p->mjo) CHK(launch_mjo());
     if (fork && hist) CHK(launch_hist());
   }
-  TendArgs<T> ta;
-  ta.gin = (const T *)p->gin;
-  ta.raw = (raw_path && p->have_raw) ? (const T *)p->raw : nullptr;
-  ta.bsg = (const T *)p->bsg;
-  ta.frc = (const T *)p->frc;
-  ta.gout = (T *)p->gout;
-  ta.psa = (T *)p->psa;
-  ta.psb = (T *)p->psb;
-  ta.pend = p->pend;
-  ta.vc = t.vc;
-  ta.tref = t.tref;
-  ta.rcos = t.rcos;
-  ta.ctl = p->ctl;
-  ta.E = E;
-  ta.J = p->J;
-  ta.I = p->I;
-  ta.L = L;
-  ta.nbasic = p->nb;
-  ta.nfrc = p->nfrc;
-  ta.strictly_linear = p->strictly_linear || p->linear;
-  ta.linear = p->linear;
-  ta.robert = (T)p->robert;
-  ta.am = am;
-  ta.h = hist_args<T>(p);
-  ta.traj = (T *)p->traj_cur;
+  const TendArgs<T> ta = tend_args<T>(p, raw_path, am);
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
   CHK(join(F_SURF));
@@ -1372,12 +1411,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
   if (!fork && p->mjo) CHK(launch_mjo());
   if (p->mjo) CHK(join(F_MJO));
-  CHK(mark(p, 3, st));
-  hipLaunchKernelGGL((k_tend<T>), dim3(ne * p->J * (p->I / grid_chunk(p->I))), dim3(grid_chunk(p->I)), 0,
-                     st, ta);
-  HIPCHK(hipGetLastError());
-  CHK(mark(p, 4, st));
-  CHK(launch_fft<T>(p, p->gout, p->fout, cmo, st, 0, am));
+  CHK(launch_tend<T>(p, ta, ne, st));
   CHK(fork_at(F_FFT));
   CHK(mark(p, 5, st));
   {
@@ -1490,6 +1524,44 @@ int final_check(lbm_plan *p, hipStream_t st) {
 // variants differ only in block shape and register budget, never in
 // arithmetic, so results are bit-identical whichever is chosen.
 // ---------------------------------------------------------------------------
+size_t state_bytes(const lbm_plan *p);
+int copy_state(lbm_plan *p, void *buf, bool save, hipStream_t st);
+
+// Tuning choice 4: the tendency rows' forward FFT as k_fft after k_tend (0)
+// or inside k_tend (1, 128 longitudes only), timed on the plan's buffers
+// with no history sums or trajectory; the state k_tend updates (deferred
+// ps' updates, the blow-up flag) is saved before and restored after.
+template <class T, class Timed>
+int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
+  if (p->I != 128) {
+    p->tune[4] = 0;
+    std::snprintf(buf, len, "; tend_fft: separate (fused at 128 longitudes only)");
+    return LBM_OK;
+  }
+  hipStream_t st = p->own;
+  void *save = nullptr;
+  HIPCHK(hipMalloc(&save, state_bytes(p)));
+  int rc = copy_state(p, save, true, st);
+  TendArgs<T> ta = tend_args<T>(p, false, loop_mask(p, true, 0));
+  ta.h.mode = 0;
+  ta.h.last_grid = nullptr;
+  ta.h.last_spec = nullptr;
+  ta.traj = nullptr;
+  double ms[2] = {0, 0};
+  for (int v = 0; v < 2 && rc == LBM_OK; ++v) {
+    p->tune[4] = v;
+    rc = timed([&] { return launch_tend<T>(p, ta, p->E, st); }, &ms[v]);
+  }
+  if (copy_state(p, save, false, st) != LBM_OK && rc == LBM_OK) rc = LBM_ERR_RUNTIME;
+  if (hipStreamSynchronize(st) != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "sync failed");
+  (void)hipFree(save);
+  if (rc != LBM_OK) return rc;
+  p->tune[4] = ms[1] < ms[0] ? 1 : 0;
+  std::snprintf(buf, len, "; tend_fft: separate=%.4f fused=%.4f ms -> %s", ms[0], ms[1],
+                p->tune[4] ? "fused" : "separate");
+  return LBM_OK;
+}
+
 template <class T>
 int autotune_t(lbm_plan *p, int reps, std::string &rep) {
   const int E = p->E, L = p->L, EL = E * L;
@@ -1514,9 +1586,14 @@ int autotune_t(lbm_plan *p, int reps, std::string &rep) {
     *ms = best;
     return LBM_OK;
   };
-  const char *names[lbm_plan::NTUNE] = {"ifft_rows", "fft_rows", "syn_waves", "ana_waves"};
+  const char *names[lbm_plan::NTUNE] = {"ifft_rows", "fft_rows", "syn_waves", "ana_waves", "tend_fft"};
   char buf[256];
   for (int knob = 0; knob < lbm_plan::NTUNE; ++knob) {
+    if (knob == 4) {
+      CHK(autotune_tend_fft<T>(p, timed, buf, sizeof(buf)));
+      rep += buf;
+      continue;
+    }
     if (knob < 2 && p->ffac.np > 0) {
       // one Stockham shape per length: nothing to choose
       p->tune[knob] = 0;
@@ -2265,6 +2342,8 @@ StateParts state_parts(const lbm_plan *p) {
   sp.total = o;
   return sp;
 }
+size_t state_bytes(const lbm_plan *p) { return state_parts(p).total; }
+
 int copy_state(lbm_plan *p, void *buf, bool save, hipStream_t st) {
   const StateParts sp = state_parts(p);
   void *parts[7] = {p->sa3, p->sb3, p->sa2, p->psa, p->psb, p->pend, p->ctl};
@@ -2954,7 +3033,8 @@ int lbm_set_tuning(lbm_plan *p, const int32_t *h_choice, int n) {
   if (!p || (n > 0 && !h_choice)) return fail(LBM_ERR_VALUE, "null argument");
   if (n < 0 || n > lbm_plan::NTUNE) return fail(LBM_ERR_VALUE, "at most %d choices", lbm_plan::NTUNE);
   for (int k = 0; k < n; ++k)
-    if (h_choice[k] < 0 || h_choice[k] > 2) return fail(LBM_ERR_VALUE, "choice %d out of range", k);
+    if (h_choice[k] < 0 || h_choice[k] > (k == 4 ? 1 : 2))
+      return fail(LBM_ERR_VALUE, "choice %d out of range", k);
   if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
   destroy_graphs(p);
   for (int k = 0; k < lbm_plan::NTUNE; ++k) p->tune[k] = k < n ? h_choice[k] : 0;

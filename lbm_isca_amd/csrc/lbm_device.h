@@ -450,7 +450,7 @@ __device__ inline void store_row_pairs(const cx<T> *z, cx<T> *out, int tt) {
 // for n_b = t + 8 n2, keep output r, twiddle exp(-2 pi i n_b r / NC), first
 // radix-8 pass; v[k2] (twiddled) goes to scr by fft_a_store.  in: the row in
 // global memory (R = 1) or staged in LDS (R > 1).
-template <class T, int NI>
+template <class T, int NI, bool LDS_IN = false>
 __device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt) {
   constexpr int R = FftShape<T, NI>::R;
   const cx<T> *tw = twiddles<T, NI>();
@@ -459,8 +459,9 @@ __device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt) {
   for (int n2 = 0; n2 < 8; ++n2) {
     const int nb = t + 8 * n2;
     if constexpr (R == 1) {
-      // FP64 rows stream past the caches (measured: FP32 rows lose)
-      if constexpr (sizeof(T) == 8)
+      // FP64 rows stream past the caches (measured: FP32 rows lose); a row
+      // staged in LDS (LDS_IN) is read directly
+      if constexpr (sizeof(T) == 8 && !LDS_IN)
         v[n2] = ld_nt(in + nb);
       else
         v[n2] = in[nb];

@@ -286,7 +286,54 @@ struct TendArgs {
   // [e][j][2L+2][I] = u, v of every level, ps' of both time levels after
   // the deferred update
   T *traj;
+  // k_tend<T, true>: the Fourier columns of the 9L + 2 tendency rows
+  // (ColMap {E, L, 9, 2}, C columns, M coefficients) instead of gout
+  cx<T> *fout;
+  int C, M;
+  int ne;  // members launched (k_tend<T, true>: tiles of 8 cover them)
 };
+
+// Members per block of k_tend<T, true>: 8 (FP32) or 4 (FP64, whose
+// tendency registers do not fit the 128 per lane of 1024-thread blocks).
+template <class T>
+constexpr int tend_fft_members() { return sizeof(T) == 4 ? 8 : 4; }
+
+// The forward FFT of nf fields x MB members of tendency rows of one
+// latitude, held by the MB x 128 threads of a k_tend<T, true> block
+// (thread = (member m, longitude i)), through the LDS rows sh[f * MB + m]:
+// 8 threads per row (k_fft's stages for 128 points), so each store of a
+// coefficient by a wave covers MB consecutive Fourier columns of a field;
+// rows of inactive members are skipped.  Every thread of the block calls it.
+template <class T, int MB>
+__device__ inline void fft_tend_rows(cx<T> (*sh)[72], const T (&vals)[9], int nf, int m, int i,
+                                     bool act, unsigned live8, cx<T> *__restrict__ fout,
+                                     const int (&col0)[9], int J, int j, int C, int M) {
+  __syncthreads();  // the previous rows' last stage has read sh
+#pragma unroll
+  for (int f = 0; f < 9; ++f)
+    if (f < nf && act) reinterpret_cast<T *>(sh[f * MB + m])[i] = vals[f];
+  __syncthreads();
+  const int rl = threadIdx.x >> 3, tt = threadIdx.x & 7;
+  const int rf = rl / MB, rm = rl - rf * MB;
+  const bool live = rf < nf && ((live8 >> rm) & 1u);
+  cx<T> v[8];
+  if (live) fft_a<T, 128, true>(sh[rl], v, tt);
+  __syncthreads();  // the staged row is overwritten by the first pass
+  if (live) fft_a_store<T, 128>(v, sh[rl], tt);
+  __syncthreads();
+  if (live) fft_b<T, 128>(sh[rl], v, tt);
+  __syncthreads();  // the scratch takes Z in natural order
+  if (live) {
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) sh[rl][tt + 8 * k1] = v[k1];
+  }
+  __syncthreads();
+  int c = 0;
+#pragma unroll
+  for (int f = 0; f < 9; ++f)
+    if (f == rf) c = col0[f];
+  if (live) fft_c<T, 128>(sh[rl], fout + fidx(0, j, c + rm, J, C), (size_t)J * fpad(C), M, tt, 0);
+}
 
 // Tendencies on grid rows, one thread per column, one pass over the levels
 // with S_0 and sum dsig D from the synthesised column sums.  The first
@@ -294,15 +341,42 @@ struct TendArgs {
 // u, v, T, q and forms S_0 in a first pass over the column, as the
 // reference does on its grid.  gin rows: U V vor div T Q (L each), then
 // d/dx pi, d/dy pi, pi, sum u, sum v, sum D.
-template <class T>
-__global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
-  // blocks of blockDim.x (128 or 64) longitudes:
-  // blockIdx = (e * J + j) * (I / blockDim.x) + chunk
-  const int nch = a.I / (int)blockDim.x;
-  const int ej = blockIdx.x / nch;
-  const int i = (blockIdx.x - ej * nch) * (int)blockDim.x + threadIdx.x;
-  const int e = ej / a.J, j = ej - e * a.J;
-  if (!a.am(e)) return;  // a finished member: its tendencies are not used
+// FFT (128 longitudes): one block per (group of MB members, latitude),
+// MB x 128 threads; each level's nine tendency rows of the MB members (and
+// first the two surface rows) go through the forward FFT in the block to
+// the Fourier columns of a.fout, the values and arithmetic of k_fft; the
+// grid rows are never stored.  Members past the launch or finished compute
+// on clamped addresses and store nothing.
+template <class T, bool FFT = false, int MB = (FFT ? tend_fft_members<T>() : 1)>
+__global__ __launch_bounds__(FFT ? MB * 128 : 128) void k_tend(TendArgs<T> a) {
+  __shared__ cx<T> sh[FFT ? 9 * MB : 1][72];
+  int e, j, i, m = 0, e0 = 0;
+  bool act = true;
+  unsigned live8 = 0;  // FFT: bit m set for the block's active members
+  if constexpr (FFT) {
+    const int tile = blockIdx.x / a.J;
+    j = blockIdx.x - tile * a.J;
+    m = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);  // wave-uniform member
+    i = threadIdx.x & 127;
+    e0 = tile * MB;  // the block's first member: its Fourier columns start there
+    e = e0 + m;
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+      const int eq = tile * MB + q;
+      if (eq < a.ne && a.am(eq)) live8 |= 1u << q;
+    }
+    act = (live8 >> m) & 1u;
+    if (e >= a.E) e = a.E - 1;
+  } else {
+    // blocks of blockDim.x (128 or 64) longitudes:
+    // blockIdx = (e * J + j) * (I / blockDim.x) + chunk
+    const int nch = a.I / (int)blockDim.x;
+    const int ej = blockIdx.x / nch;
+    i = (blockIdx.x - ej * nch) * (int)blockDim.x + threadIdx.x;
+    e = ej / a.J;
+    j = ej - e * a.J;
+    if (!a.am(e)) return;  // a finished member: its tendencies are not used
+  }
   const int L = a.L, I = a.I, J = a.J;
   const int eb = a.nbasic > 1 ? e : 0;
   const VOff vo{L};
@@ -362,11 +436,12 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     pib = 0;
     pst = psbar * pig;
   } else {
-    column_surface<T>(a.psa + pidx, a.psb + pidx, a.pend[e], pig, psbar, a.robert, cc.S0, cc.D0,
-                      gpi, pib, cs.chk, pst);
+    // an inactive member (FFT blocks) leaves its ps' as it is
+    column_surface<T>(a.psa + pidx, a.psb + pidx, act ? a.pend[e] : 0, pig, psbar, a.robert, cc.S0,
+                      cc.D0, gpi, pib, cs.chk, pst);
   }
   T *trj = nullptr;
-  if (a.traj) {
+  if (a.traj && act) {
     trj = a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i;
     trj[(size_t)(2 * L) * I] = a.linear ? pst : a.psa[pidx];
     trj[(size_t)(2 * L + 1) * I] = a.linear ? (T)0 : a.psb[pidx];
@@ -376,12 +451,20 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
   T hw[2];
   const int top_a = a.ctl->step - a.h.inistp + 1;
   const int nst = a.h.nsteps ? a.h.nsteps[e] : 0;
-  const int hn = hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw);
-  const bool hlast = a.h.last_grid != nullptr && top_a == nst;
+  const int hn = act ? hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw) : 0;
+  const bool hlast = act && a.h.last_grid != nullptr && top_a == nst;
   const size_t hrow = (size_t)(e * J + j) * (L + 1) * I + i;
   for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
-  go.st(voff, (unsigned)(9 * L) * rowb, gpi);
-  go.st(voff, (unsigned)(9 * L + 1) * rowb, pib);
+  const int EL = a.E * L;
+  if constexpr (FFT) {
+    const T sv[9] = {gpi, pib, 0, 0, 0, 0, 0, 0, 0};
+    const int s0 = e0 + 9 * EL;  // first column of the block, surface fields 0 and 1
+    const int c0[9] = {s0, s0 + a.E, 0, 0, 0, 0, 0, 0, 0};
+    fft_tend_rows<T, MB>(sh, sv, 2, m, i, act, live8, a.fout, c0, J, j, a.C, a.M);
+  } else {
+    go.st(voff, (unsigned)(9 * L) * rowb, gpi);
+    go.st(voff, (unsigned)(9 * L + 1) * rowb, pib);
+  }
   const T *tref = a.tref + (size_t)eb * L;
   // the state, basic state and forcing of level k+1 are in flight while
   // level k is formed
@@ -420,8 +503,16 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     dv = dvn;
     fk = fkn;
     bc = bn;
+    if constexpr (FFT) {
+      // first columns (f, k, tile) of the level fields
+      int c0[9];
 #pragma unroll
-    for (int f = 0; f < 9; ++f) go.template st<2>(voff, (unsigned)(f * L + k) * rowb, out[f]);
+      for (int f = 0; f < 9; ++f) c0[f] = f * EL + k * a.E + e0;
+      fft_tend_rows<T, MB>(sh, out, 9, m, i, act, live8, a.fout, c0, J, j, a.C, a.M);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 9; ++f) go.template st<2>(voff, (unsigned)(f * L + k) * rowb, out[f]);
+    }
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
     if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
     up = upn;
@@ -429,7 +520,7 @@ __global__ __launch_bounds__(128) void k_tend(TendArgs<T> a) {
     tp = tpn;
     qp = qpn;
   }
-  if (!isfinite(cs.chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
+  if (act && !isfinite(cs.chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
 }
 
 // ---------------------------------------------------------------------------

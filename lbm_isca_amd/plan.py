@@ -618,7 +618,7 @@ class Plan:
         return out
 
     # -- autotuner -----------------------------------------------------------
-    TUNED = ("ifft_rows", "fft_rows", "syn_waves", "ana_waves")
+    TUNED = ("ifft_rows", "fft_rows", "syn_waves", "ana_waves", "tend_fft")
 
     def autotune(self, reps=10):
         """Time the variants of the step's tuned launches on this plan and

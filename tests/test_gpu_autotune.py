@@ -1,7 +1,8 @@
 """The autotuner of the step's launches: it times every variant on the
 plan, keeps the fastest, and every variant computes the same numbers (the
-variants differ in FFT rows per block and in the register budget of the
-Legendre kernels only), so a run under any tuning is bit-identical."""
+variants differ in FFT rows per block, in the register budget of the
+Legendre kernels and in whether the tendency rows' forward FFT runs inside
+the tendency kernel), so a run under any tuning is bit-identical."""
 
 import numpy as np
 import pytest
@@ -20,16 +21,24 @@ def test_every_tuning_gives_identical_results(precision, tmp_path, monkeypatch):
     bs = cases.jet_basic_state(sht.lat, sht.lon, vg.sig)
     heat = np.stack([cases.heating(sht.lat, sht.lon, vg.sig, lon0=30.0 * e) for e in range(16)])
     m, _, _ = L.make_ensemble_model(bs, 16, L.Config(ndays=6 / 72.0, precision=precision))
+    # the autotuner leaves the state as it found it (the tendency kernel it
+    # times updates the pending surface pressure)
+    m.plan.set_forcing(heat)
+    m.plan.run(3)
+    before = [x.cpu().numpy() for x in m.plan.get_state() + m.plan.get_prev_state()]
     res = m.plan.autotune(reps=3)
+    after = [x.cpu().numpy() for x in m.plan.get_state() + m.plan.get_prev_state()]
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))
     print(precision, res["report"])
     assert all(c in (0, 1, 2) for c in res["choice"]) and "ifft_rows" in res["report"]
+    assert "tend_fft" in res["report"] and res["choice"][4] in (0, 1)
     out = {}
-    for choice in ([0, 0, 0, 0], [1, 1, 1, 1], [2, 2, 2, 2], res["choice"]):
+    for choice in ([0, 0, 0, 0, 0], [1, 1, 1, 1, 1], [2, 2, 2, 2, 1], res["choice"]):
         m.plan.set_tuning(choice)
         m.plan.set_state()                      # every run from rest
         m.run(mjo_forcing=L.EnsembleForcing(heat), verbose=False)
         out[tuple(choice)] = [x.cpu().numpy() for x in m.plan.get_state()]
-    ref = out[(0, 0, 0, 0)]
+    ref = out[(0, 0, 0, 0, 0)]
     assert np.abs(ref[2]).max() > 0
     for k, v in out.items():
         for a, b in zip(v, ref):
@@ -38,7 +47,32 @@ def test_every_tuning_gives_identical_results(precision, tmp_path, monkeypatch):
     monkeypatch.setenv("LBM_TUNING_CACHE", str(tmp_path / "tune.json"))
     first = tuning.autotune(m.plan, "cache", reps=2)
     assert first["source"] == "timed"
-    m.plan.set_tuning([0, 0, 0, 0])
+    m.plan.set_tuning([0, 0, 0, 0, 0])
     again = tuning.autotune(m.plan, "cache")
     assert again["source"].endswith("tune.json") and again["choice"] == first["choice"]
     assert m.plan.get_tuning() == first["choice"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_tendency_fft_on_partial_member_groups(precision):
+    """The tendency kernel with the forward FFT inside (one block per group
+    of 4 (FP64) or 8 (FP32) members) on 5 members of uneven lengths: the
+    last group is partial and members finish at different steps; the
+    result is bit-identical to the separate FFT kernel's."""
+    import lbm_isca_amd as L
+    sht, vg = SHT(), VerticalOps()
+    bs = cases.jet_basic_state(sht.lat, sht.lon, vg.sig)
+    E = 5
+    heat = np.stack([cases.heating(sht.lat, sht.lon, vg.sig, lon0=50.0 * e) for e in range(E)])
+    m, _, _ = L.make_ensemble_model(bs, E, L.Config(precision=precision))
+    m.plan.set_lengths([8, 3, 8, 6, 1])
+    out = {}
+    for fused in (0, 1):
+        m.plan.set_tuning([0, 0, 0, 0, fused])
+        m.plan.set_forcing(heat)
+        m.plan.set_state()
+        m.plan.run(0)
+        out[fused] = [x.cpu().numpy() for x in m.plan.get_state()]
+    assert np.abs(out[0][2]).max() > 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b), precision
