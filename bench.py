@@ -59,30 +59,35 @@ def log(*a):
 # kernels of each profiling slot of one step, with their launches per step
 SLOT_KERNEL = {"syn_level": {"k_syn_mfma<double, lbm::SynLevel": 1},
                "syn_surf": {"k_syn_mfma<double, lbm::SynSurf": 1},
-               "ifft": {"k_ifft<double, 128>": 1}, "tend": {"k_tend<double>": 1},
-               "fft": {"k_fft<double, 128>": 1},
+               "ifft": {"k_ifft<double, 128": 1}, "tend": {"k_tend<double": 1},
+               "fft": {"k_fft<double, 128": 1},
                "ana_level": {"k_ana_mfma<double, lbm::AnaTendVD": 1,
                              "k_ana_mfma<double, lbm::AnaFluxDiv": 2},
                "ana_surf": {"k_ana_mfma<double, lbm::AnaScalar": 1, "k_hist_acc<double>": 1},
                "step": {"k_step<double, 20": 1}}
 
 
-def measured_traffic(slot, E, precision):
+def measured_traffic(slot, E, precision, fused_tend_fft=False):
     """HBM bytes of a slot's kernels per step from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload, or None."""
+    FETCH_SIZE / WRITE_SIZE summary of this workload (of each kernel name,
+    the variant with the most launches; the tendency kernel with or without
+    the forward FFT inside as the run chose), or None."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_T42L20_E{E}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        table = json.load(fh)["bytes_per_launch"]
+        table = json.load(fh)["kernels"]
     total = 0
     for name, launches in SLOT_KERNEL[slot].items():
+        if slot == "tend":
+            name += ", true" if fused_tend_fft else ", false"
         if precision == "fp32":
             name = name.replace("double", "float")
         hits = [v for k, v in table.items() if name in k]
         if not hits:
             return None
-        total += launches * int(hits[0])
+        best = max(hits, key=lambda v: v["launches"])
+        total += launches * int(best["bytes_per_launch"])
     return total
 
 
@@ -520,6 +525,8 @@ def main():
     ap.add_argument("--t127-members", type=int, default=32)
     ap.add_argument("--t127-days", type=float, default=20.0)
     ap.add_argument("--cpu-worker", type=int, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--tuning", default=None, metavar="C0,C1,C2,C3,C4",
+                    help="apply these launch variants instead of autotuning (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=None, metavar="MIB",
                     help="only copy a float64 buffer of MIB mebibytes three times (the "
                          "reference kernel that scales the HBM byte counters) and exit")
@@ -567,8 +574,12 @@ def main():
     torch.cuda.synchronize()
     # the step's launch variants timed on this ensemble (the state is kept)
     from lbm_isca_amd import tuning
-    tuned = tuning.autotune(plan, True)
-    log(f"autotuned: {tuned['report']}")
+    if args.tuning is not None:
+        plan.set_tuning([int(c) for c in args.tuning.split(",")])
+        tuned = {"choice": plan.get_tuning(), "report": "given"}
+    else:
+        tuned = tuning.autotune(plan, True)
+    log(f"autotuned: {tuned['report']} -> {tuned['choice']}")
     # a run's two start-up half steps go before the timed region, which then
     # holds exactly --steps full steps (model.py:211-251)
     plan.run_steps(args.steps, m.cfg.inistp, use_graph=True)
@@ -588,7 +599,8 @@ def main():
     prof = plan.profile_read()
     plan.profile(False)
     s = 8 if args.precision == "fp64" else 4
-    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s, nfrc=E)
+    fused = tuned["choice"][4] != 0
+    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s, nfrc=E, fused_tend_fft=fused)
     avg = {k: ms / n for k, (ms, n) in prof.items() if n and ms > 0}
     dom = max(avg, key=lambda k: prof[k][0])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
@@ -596,7 +608,7 @@ def main():
     rl = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
           "peak": roofline.MI355X_HBM_GBS, "unit": "GB/s",
           "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
-          "traffic": measured_traffic(dom, E, args.precision),
+          "traffic": measured_traffic(dom, E, args.precision, fused),
           "algorithmic_bytes": kb[dom], "avg_ms": round(avg[dom], 5),
           "kernels_ms": {k: round(v, 5) for k, v in avg.items()}}
 

@@ -1306,8 +1306,9 @@ TendArgs<T> tend_args(lbm_plan *p, bool raw_path, const ActiveMask &am) {
 }
 
 // The forward FFT of the tendency rows as k_fft after k_tend (tuning
-// choice 4 = 0, built in) or inside k_tend (1, 128 longitudes).
-inline bool tend_fused(const lbm_plan *p) { return p->I == 128 && p->tune[4] == 1; }
+// choice 4 = 0, built in) or inside k_tend (1 and 2: two group sizes, 128
+// longitudes only).
+inline bool tend_fused(const lbm_plan *p) { return p->I == 128 && p->tune[4] != 0; }
 
 // k_tend for ne members, then (unfused) the forward FFT of its rows; the
 // profiling marks 3 and 4 bracket k_tend.
@@ -1317,11 +1318,17 @@ int launch_tend(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
   CHK(mark(p, 3, st));
   if (tend_fused(p)) {
     // blocks of (MB members, latitude), MB x 128 threads
-    constexpr int MB = tend_fft_members<T>();
     TendArgs<T> tf = ta;
     tf.ne = ne;
-    hipLaunchKernelGGL((k_tend<T, true>), dim3((unsigned)((ne + MB - 1) / MB * p->J)), dim3(MB * 128), 0,
-                       st, tf);
+    if (p->tune[4] == 1) {
+      constexpr int MB = tend_fft_members<T>(1);
+      hipLaunchKernelGGL((k_tend<T, true, MB>), dim3((unsigned)((ne + MB - 1) / MB * p->J)), dim3(MB * 128),
+                         0, st, tf);
+    } else {
+      constexpr int MB = tend_fft_members<T>(2);
+      hipLaunchKernelGGL((k_tend<T, true, MB>), dim3((unsigned)((ne + MB - 1) / MB * p->J)), dim3(MB * 128),
+                         0, st, tf);
+    }
   } else {
     hipLaunchKernelGGL((k_tend<T, false>), dim3(ne * p->J * (p->I / grid_chunk(p->I))),
                        dim3(grid_chunk(p->I)), 0, st, ta);
@@ -1547,8 +1554,8 @@ int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   ta.h.last_grid = nullptr;
   ta.h.last_spec = nullptr;
   ta.traj = nullptr;
-  double ms[2] = {0, 0};
-  for (int v = 0; v < 2 && rc == LBM_OK; ++v) {
+  double ms[3] = {0, 0, 0};
+  for (int v = 0; v < 3 && rc == LBM_OK; ++v) {
     p->tune[4] = v;
     rc = timed([&] { return launch_tend<T>(p, ta, p->E, st); }, &ms[v]);
   }
@@ -1556,9 +1563,13 @@ int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   if (hipStreamSynchronize(st) != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "sync failed");
   (void)hipFree(save);
   if (rc != LBM_OK) return rc;
-  p->tune[4] = ms[1] < ms[0] ? 1 : 0;
-  std::snprintf(buf, len, "; tend_fft: separate=%.4f fused=%.4f ms -> %s", ms[0], ms[1],
-                p->tune[4] ? "fused" : "separate");
+  int best = 0;
+  for (int v = 1; v < 3; ++v)
+    if (ms[v] < ms[best]) best = v;
+  p->tune[4] = best;
+  const char *lab[3] = {"separate", "fused", "fused"};
+  std::snprintf(buf, len, "; tend_fft: separate=%.4f fused/%d=%.4f fused/%d=%.4f ms -> %s", ms[0],
+                tend_fft_members<T>(1), ms[1], tend_fft_members<T>(2), ms[2], lab[best]);
   return LBM_OK;
 }
 
@@ -3033,8 +3044,7 @@ int lbm_set_tuning(lbm_plan *p, const int32_t *h_choice, int n) {
   if (!p || (n > 0 && !h_choice)) return fail(LBM_ERR_VALUE, "null argument");
   if (n < 0 || n > lbm_plan::NTUNE) return fail(LBM_ERR_VALUE, "at most %d choices", lbm_plan::NTUNE);
   for (int k = 0; k < n; ++k)
-    if (h_choice[k] < 0 || h_choice[k] > (k == 4 ? 1 : 2))
-      return fail(LBM_ERR_VALUE, "choice %d out of range", k);
+    if (h_choice[k] < 0 || h_choice[k] > 2) return fail(LBM_ERR_VALUE, "choice %d out of range", k);
   if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
   destroy_graphs(p);
   for (int k = 0; k < lbm_plan::NTUNE; ++k) p->tune[k] = k < n ? h_choice[k] : 0;

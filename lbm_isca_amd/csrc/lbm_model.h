@@ -293,10 +293,11 @@ struct TendArgs {
   int ne;  // members launched (k_tend<T, true>: tiles of 8 cover them)
 };
 
-// Members per block of k_tend<T, true>: 8 (FP32) or 4 (FP64, whose
-// tendency registers do not fit the 128 per lane of 1024-thread blocks).
+// Members per block of k_tend<T, true>, tuning variant v = 1 or 2: 8 or 4
+// (FP32), 4 or 2 (FP64, whose tendency registers do not fit the 128 per
+// lane of 1024-thread blocks).
 template <class T>
-constexpr int tend_fft_members() { return sizeof(T) == 4 ? 8 : 4; }
+constexpr int tend_fft_members(int v) { return (sizeof(T) == 4 ? 8 : 4) / v; }
 
 // The forward FFT of nf fields x MB members of tendency rows of one
 // latitude, held by the MB x 128 threads of a k_tend<T, true> block
@@ -347,7 +348,7 @@ __device__ inline void fft_tend_rows(cx<T> (*sh)[72], const T (&vals)[9], int nf
 // the Fourier columns of a.fout, the values and arithmetic of k_fft; the
 // grid rows are never stored.  Members past the launch or finished compute
 // on clamped addresses and store nothing.
-template <class T, bool FFT = false, int MB = (FFT ? tend_fft_members<T>() : 1)>
+template <class T, bool FFT = false, int MB = 1>
 __global__ __launch_bounds__(FFT ? MB * 128 : 128) void k_tend(TendArgs<T> a) {
   __shared__ cx<T> sh[FFT ? 9 * MB : 1][72];
   int e, j, i, m = 0, e0 = 0;

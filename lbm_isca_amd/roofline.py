@@ -12,7 +12,10 @@ MI355X_FP32_TFLOPS = 157.3
 MI355X_BF16_TFLOPS = 2500.0      # dense matrix cores
 
 
-def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):
+def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1, fused_tend_fft=False):
+    """``fused_tend_fft``: the tendency kernel with the forward FFT inside
+    (its "tend" reads what k_tend reads and writes the Fourier columns k_fft
+    writes; "fft" is then empty)."""
     M = nmax + 1
     nspec = M * (M + 1) // 2
     Jh = J // 2
@@ -22,7 +25,7 @@ def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):
     cin = 6 * EL + 6 * E           # Fourier/grid columns of the synthesised state
     cout = 9 * EL + 2 * E          # columns of the grid tendencies
     rb = 9 * L + 6                 # basic-state rows
-    return {
+    out = {
         "syn_level": (nspec * 4 * EL + M * J * 6 * EL) * c + tables,
         "syn_surf": (nspec * 3 * E + M * J * 6 * E) * c + tables,
         "ifft": M * J * cin * c + J * cin * I * s,
@@ -33,6 +36,10 @@ def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1):
         "ana_surf": M * J * 2 * E * c + nspec * 2 * E * c + tables,
         "step": (5 * nspec * 4 * EL + nspec * 2 * E + nspec * 3 * E) * c + nbasic * M * L * L * s,
     }
+    if fused_tend_fft:
+        out["tend"] += M * J * cout * c - J * cout * I * s
+        out["fft"] = 0
+    return out
 
 
 def compulsory_bytes_per_member_step(L=20, J=64, I=128, s=8, per_member_basic=False, nmax=42):

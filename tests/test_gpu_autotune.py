@@ -31,9 +31,9 @@ def test_every_tuning_gives_identical_results(precision, tmp_path, monkeypatch):
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
     print(precision, res["report"])
     assert all(c in (0, 1, 2) for c in res["choice"]) and "ifft_rows" in res["report"]
-    assert "tend_fft" in res["report"] and res["choice"][4] in (0, 1)
+    assert "tend_fft" in res["report"] and res["choice"][4] in (0, 1, 2)
     out = {}
-    for choice in ([0, 0, 0, 0, 0], [1, 1, 1, 1, 1], [2, 2, 2, 2, 1], res["choice"]):
+    for choice in ([0, 0, 0, 0, 0], [1, 1, 1, 1, 1], [2, 2, 2, 2, 2], res["choice"]):
         m.plan.set_tuning(choice)
         m.plan.set_state()                      # every run from rest
         m.run(mjo_forcing=L.EnsembleForcing(heat), verbose=False)
@@ -56,9 +56,9 @@ def test_every_tuning_gives_identical_results(precision, tmp_path, monkeypatch):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_fused_tendency_fft_on_partial_member_groups(precision):
     """The tendency kernel with the forward FFT inside (one block per group
-    of 4 (FP64) or 8 (FP32) members) on 5 members of uneven lengths: the
-    last group is partial and members finish at different steps; the
-    result is bit-identical to the separate FFT kernel's."""
+    of 4 or 2 (FP64), 8 or 4 (FP32) members) on 5 members of uneven
+    lengths: the last group is partial and members finish at different
+    steps; the result is bit-identical to the separate FFT kernel's."""
     import lbm_isca_amd as L
     sht, vg = SHT(), VerticalOps()
     bs = cases.jet_basic_state(sht.lat, sht.lon, vg.sig)
@@ -67,12 +67,13 @@ def test_fused_tendency_fft_on_partial_member_groups(precision):
     m, _, _ = L.make_ensemble_model(bs, E, L.Config(precision=precision))
     m.plan.set_lengths([8, 3, 8, 6, 1])
     out = {}
-    for fused in (0, 1):
+    for fused in (0, 1, 2):
         m.plan.set_tuning([0, 0, 0, 0, fused])
         m.plan.set_forcing(heat)
         m.plan.set_state()
         m.plan.run(0)
         out[fused] = [x.cpu().numpy() for x in m.plan.get_state()]
     assert np.abs(out[0][2]).max() > 0
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b), precision
+    for v in (1, 2):
+        for a, b in zip(out[0], out[v]):
+            assert np.array_equal(a, b), (precision, v)
