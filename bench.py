@@ -612,6 +612,7 @@ def main():
           "algorithmic_bytes": kb[dom], "avg_ms": round(avg[dom], 5),
           "kernels_ms": {k: round(v, 5) for k, v in avg.items()}}
 
+    autotuned = dict(zip(plan.TUNED, tuned["choice"]))
     extras = {}
     if not args.no_extras:
         del m, plan
@@ -642,7 +643,7 @@ def main():
                                f"{args.precision.upper()}, 20 equal sigma levels",
                    "members": E, "resolution": "T42L20", "dt_s": 1200.0,
                    "graph": True, "startup_steps_in_timed_region": 0,
-                   "autotuned": dict(zip(plan.TUNED, tuned["choice"]))},
+                   "autotuned": autotuned},
         "roofline": rl, "cpu_baseline": cpu, **extras,
     }
     print(json.dumps(out), flush=True)

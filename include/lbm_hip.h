@@ -430,7 +430,9 @@ int lbm_get_response(lbm_plan *plan, double *h_j);
  * kept (lbm_set_checkpoints before it; the run starts from
  * lbm_set_spectral_state): each segment is recomputed from its checkpoint
  * and swept backward with the transposed kernels about it.  The forward
- * state is left as the run left it. */
+ * state is left as the run left it.  The forcing must be constant through
+ * the run (no MJO forcing, no lbm_set_forcing between its steps):
+ * LBM_ERR_STATE otherwise. */
 int lbm_adjoint_run(lbm_plan *plan, int nsteps, double *d_gvor, double *d_gdiv, double *d_gt,
                     double *d_gq, double *d_gpi, double *d_gfrc, int use_graph, void *stream);
 

@@ -284,6 +284,7 @@ struct lbm_plan {
   int nsf = 0;
   // a run in progress (lbm_run_to): per-member lengths and steps done
   bool in_run = false;
+  bool frc_in_run = false;  // the forcing changed between the steps of the last run
   std::vector<int> run_len;
   int run_done = 0;
   Ctl *ctl = nullptr;
@@ -2808,6 +2809,7 @@ int lbm_set_basic_state(lbm_plan *p, const double *u, const double *v, const dou
 int lbm_set_forcing(lbm_plan *p, const double *frc, int nfrc, void *stream) {
   if (!p) return fail(LBM_ERR_VALUE, "null plan");
   hipStream_t st = p->strm(stream);
+  if (p->in_run) p->frc_in_run = true;
   // the captured graphs stay valid while the forcing buffer keeps its
   // address (a new forcing of the same member count between steps)
   if (p->mjo) {
@@ -3129,6 +3131,7 @@ int run_core(lbm_plan *p, int nsteps, int stop, int count, int use_graph, int ch
     h.bad_step = INT_MAX;
     HIPCHK(hipMemcpyAsync(p->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, st));
     p->in_run = true;
+    p->frc_in_run = false;
     p->run_len = len;
     p->run_done = 0;
     // checkpoints of this run for the adjoint of the default model
@@ -3319,6 +3322,9 @@ int lbm_adjoint_run(lbm_plan *p, int nsteps, double *gvor, double *gdiv, double 
   if (!p->resp_set) return fail(LBM_ERR_STATE, "the adjoint needs a response (lbm_set_response)");
   if (p->mjo)
     return fail(LBM_ERR_STATE, "the adjoint is built for constant forcings (an MJO forcing is set)");
+  if (p->frc_in_run)
+    return fail(LBM_ERR_STATE, "the adjoint is built for constant forcings (the forcing changed "
+                "between the steps of the last run)");
   if (p->linear && (p->sfv || p->sfp))
     return fail(LBM_ERR_STATE,
                 "the adjoint is built for the linear model (steady vorticity, divergence or ln ps "

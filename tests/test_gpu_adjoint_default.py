@@ -184,3 +184,19 @@ def test_autograd_default_model(setup):
     plan.set_lengths([NSTEPS, NSTEPS])
     with pytest.raises(RuntimeError, match="changed"):
         J.sum().backward()
+
+
+def test_adjoint_refuses_a_forcing_changed_during_the_run(setup):
+    """A forcing uploaded between the steps of the run (a time-varying
+    .step() forcing) is not the constant forcing the sweep differentiates:
+    refused."""
+    sht, vg, bs, heat, x0 = setup
+    plan = model(bs, 2)
+    plan.set_checkpoints(16)
+    plan.set_forcing(heat)
+    plan.set_spectral_state(*[x0[f] for f in FIELDS])
+    plan.run_steps(8, 2)
+    plan.set_forcing(0.5 * heat)
+    plan.run_to(8, 0)
+    with pytest.raises(RuntimeError, match="forcing changed"):
+        plan.adjoint_run(8)
