@@ -672,8 +672,8 @@ int launch_fft_n(lbm_plan *p, const void *G, void *F, ColMap cm, hipStream_t st,
 }
 
 // Rows per block of the Stockham FFTs (32 threads and 2 NC complex of LDS
-// per row).
-int fft_any_rows(int NC) { return NC <= 128 ? 8 : NC <= 256 ? 4 : 2; }
+// per row): one tile of 8 Fourier columns.
+int fft_any_rows(int) { return FFT_ANY_ROWS; }
 
 template <class T, bool INV>
 int launch_fft_any(lbm_plan *p, const void *in, void *out, ColMap cm, hipStream_t st, int mode,
@@ -681,7 +681,7 @@ int launch_fft_any(lbm_plan *p, const void *in, void *out, ColMap cm, hipStream_
   const int C = cm.ncols(), rb = fft_any_rows(p->ffac.NC);
   const long rows = fft_rows(p->J, C, am);
   const dim3 grid((unsigned)((rows + rb - 1) / rb)), block(rb * FFT_ANY_TPR);
-  const size_t lds = (size_t)rb * 2 * p->ffac.NC * sizeof(cx<T>);
+  const size_t lds = ((size_t)rb * 2 * p->ffac.NC + p->ffac.NI) * sizeof(cx<T>);
   const cx<T> *tw = (const cx<T> *)p->tabs<T>().twa;
   if (INV)
     hipLaunchKernelGGL((k_ifft_any<T>), grid, block, lds, st, (const cx<T> *)in, (T *)out, C, cm, p->J,
@@ -2586,6 +2586,10 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
                   "with nlon/2 a product of 2, 3 and 5", d->nlon);
     ffac.NI = d->nlon;
     ffac.NC = d->nlon / 2;
+    for (int q = 0, ns = 1; q < ffac.np; ns *= ffac.r[q], ++q) {
+      ffac.ns[q] = ns;
+      ffac.dmul[q] = ((1 << 20) + ns - 1) / ns;
+    }
   }
   if (d->nlat <= 0 || d->nlat % 8 || d->nlat > 64 * 32)
     return fail(LBM_ERR_VALUE, "nlat=%d: a positive multiple of 8 is required", d->nlat);

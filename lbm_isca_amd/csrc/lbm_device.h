@@ -663,6 +663,9 @@ __global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(c
 struct FftFactors {
   int NI, NC, np;  // longitudes, NC = NI / 2, passes
   int r[12];       // radix of each pass
+  // Ns of each pass (product of the earlier radices) and ceil(2^20 / Ns):
+  // j / Ns = (j * dmul) >> 20 exactly for j < 512
+  int ns[12], dmul[12];
 };
 constexpr int FFT_ANY_TPR = 32;  // threads per row
 
@@ -703,24 +706,24 @@ template <int SIGN, class T>
 __device__ cx<T> *stockham_row(cx<T> *a, cx<T> *b, const FftFactors &ff, const cx<T> *__restrict__ tw,
                                int tt, bool live) {
   const int NC = ff.NC, NI = ff.NI;
-  int Ns = 1;
   for (int q = 0; q < ff.np; ++q) {
-    const int r = ff.r[q], nr = NC / r, step = NI / (Ns * r);
+    const int r = ff.r[q], Ns = ff.ns[q], nr = NC / r, step = NI / (Ns * r), dm = ff.dmul[q];
     if (live) {
       for (int j = tt; j < nr; j += FFT_ANY_TPR) {
-        const int k = j % Ns;
+        const int g = (j * dm) >> 20, k = j - g * Ns;
         cx<T> v[5];
         v[0] = a[j];
 #pragma unroll
         for (int t = 1; t < 5; ++t) {
           v[t] = zero<T>();
           if (t < r) {
-            const cx<T> w = tw[(k * t * step) % NI];
+            // k t step < Ns r step = NI: no wrap
+            const cx<T> w = tw[k * t * step];
             v[t] = cmul(a[j + t * nr], SIGN < 0 ? w : conj(w));
           }
         }
         dft_radix<SIGN>(v, r);
-        const int base = (j / Ns) * Ns * r + k;
+        const int base = g * Ns * r + k;
 #pragma unroll
         for (int t = 0; t < 5; ++t)
           if (t < r) b[base + t * Ns] = v[t];
@@ -730,7 +733,6 @@ __device__ cx<T> *stockham_row(cx<T> *a, cx<T> *b, const FftFactors &ff, const c
     cx<T> *c = a;
     a = b;
     b = c;
-    Ns *= r;
   }
   return a;
 }
@@ -742,82 +744,123 @@ __device__ __forceinline__ T fourier_scale_any(int mode, int m, int NI) {
   return (T)1;
 }
 
-// Fourier (m <= M-1) -> grid for any FftFactors length; blockDim.x / 32
-// rows per block, 2 NC complex of dynamic LDS per row.
+constexpr int FFT_ANY_ROWS = 8;  // rows per block: one tile of 8 Fourier columns
+
+// The rows of a Stockham FFT block: each row's Fourier column base
+// (fidx(0, j, col)), -1 when the row is past the launch or its member has
+// finished, and its grid row (e, j, r).  Threads 0..7 fill them.
+struct AnyRows {
+  long fbase[FFT_ANY_ROWS];
+  long gbase[FFT_ANY_ROWS];
+};
+__device__ inline void any_rows(AnyRows &rs, int C, const ColMap &cm, int J, int NI,
+                                const ActiveMask &am) {
+  if (threadIdx.x < FFT_ANY_ROWS) {
+    const int q = threadIdx.x;
+    const long row = (long)blockIdx.x * FFT_ANY_ROWS + q;
+    int j = 0, col = 0, e = 0, r = 0;
+    bool live = am.row_col(row, J, C, j, col);
+    if (live) {
+      cm.map(col, e, r);
+      live = am(e);
+    }
+    rs.fbase[q] = live ? (long)fidx(0, j, col, J, C) : -1;
+    rs.gbase[q] = live ? (long)((size_t)(e * J + j) * cm.rows() + r) * NI : -1;
+  }
+}
+
+// Fourier (m <= M-1) -> grid for any FftFactors length: 8 rows (one tile of
+// Fourier columns) per block, 32 threads and 2 NC complex of dynamic LDS per
+// row.  The block stages the 8 columns' coefficients together, m by m, so
+// each coefficient's 8 columns are one contiguous run.
 template <class T>
 __global__ __launch_bounds__(256) void k_ifft_any(const cx<T> *__restrict__ F, T *__restrict__ G, int C,
                                                   ColMap cm, int J, int M, int mode, ActiveMask am,
                                                   FftFactors ff, const cx<T> *__restrict__ tw) {
   extern __shared__ unsigned char fft_any_lds[];
+  __shared__ AnyRows rs;
   const int NC = ff.NC, NI = ff.NI;
   const int rl = threadIdx.x / FFT_ANY_TPR, tt = threadIdx.x - rl * FFT_ANY_TPR;
   cx<T> *a = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)rl * 2 * NC, *b = a + NC;
-  const long row = (long)blockIdx.x * (blockDim.x / FFT_ANY_TPR) + rl;
-  int j = 0, col = 0, e = 0, r = 0;
-  bool live = am.row_col(row, J, C, j, col);
-  if (live) {
-    cm.map(col, e, r);
-    live = am(e);
+  // the twiddle table after the rows
+  cx<T> *twl = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)FFT_ANY_ROWS * 2 * NC;
+  for (int k = threadIdx.x; k < NI; k += blockDim.x) twl[k] = tw[k];
+  any_rows(rs, C, cm, J, NI, am);
+  __syncthreads();
+  tw = twl;
+  const size_t ms = (size_t)J * fpad(C);
+  for (int idx = threadIdx.x; idx < FFT_ANY_ROWS * M; idx += blockDim.x) {
+    const int q = idx & (FFT_ANY_ROWS - 1), m = idx / FFT_ANY_ROWS;
+    if (rs.fbase[q] >= 0)
+      reinterpret_cast<cx<T> *>(fft_any_lds)[(size_t)q * 2 * NC + NC + m] = F[rs.fbase[q] + (size_t)m * ms];
   }
+  __syncthreads();
+  const bool live = rs.fbase[rl] >= 0;
   if (live) {
-    const cx<T> *Fr = F + fidx(0, j, col, J, C);
-    const size_t ms = (size_t)J * fpad(C);
     // Z_k = (A + B) + i e^{+2 pi i k / NI} (A - B), A = X_k, B = conj X_{NC-k}
     // (spectral.py:166-172: g = X0.re + 2 sum_m Re(X_m e^{i m lam}))
     for (int k = tt; k < NC; k += FFT_ANY_TPR) {
       cx<T> A = zero<T>(), B = zero<T>();
       if (k < M) {
-        A = fourier_scale_any<T>(mode, k, NI) * Fr[(size_t)k * ms];
+        A = fourier_scale_any<T>(mode, k, NI) * b[k];
         if (k == 0) A.y = 0;
       }
       const int kb = NC - k;
-      if (kb < M) B = conj(fourier_scale_any<T>(mode, kb, NI) * Fr[(size_t)kb * ms]);
+      if (kb < M) B = conj(fourier_scale_any<T>(mode, kb, NI) * b[kb]);
       a[k] = (A + B) + cmul(mk((T)0, (T)1), cmul(conj(tw[k]), A - B));
     }
   }
   __syncthreads();
   const cx<T> *z = stockham_row<+1, T>(a, b, ff, tw, tt, live);
   if (live) {
-    cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+    cx<T> *out = reinterpret_cast<cx<T> *>(G + rs.gbase[rl]);
     for (int n = tt; n < NC; n += FFT_ANY_TPR) st_nt(out + n, z[n]);
   }
 }
 
-// grid -> Fourier (m <= M-1) for any FftFactors length.
+// grid -> Fourier (m <= M-1) for any FftFactors length; the 8 columns'
+// coefficients leave the block together, m by m, as contiguous runs.
 template <class T>
 __global__ __launch_bounds__(256) void k_fft_any(const T *__restrict__ G, cx<T> *__restrict__ F, int C,
                                                  ColMap cm, int J, int M, int mode, ActiveMask am,
                                                  FftFactors ff, const cx<T> *__restrict__ tw) {
   extern __shared__ unsigned char fft_any_lds[];
+  __shared__ AnyRows rs;
   const int NC = ff.NC, NI = ff.NI;
   const int rl = threadIdx.x / FFT_ANY_TPR, tt = threadIdx.x - rl * FFT_ANY_TPR;
   cx<T> *a = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)rl * 2 * NC, *b = a + NC;
-  const long row = (long)blockIdx.x * (blockDim.x / FFT_ANY_TPR) + rl;
-  int j = 0, col = 0, e = 0, r = 0;
-  bool live = am.row_col(row, J, C, j, col);
+  cx<T> *twl = reinterpret_cast<cx<T> *>(fft_any_lds) + (size_t)FFT_ANY_ROWS * 2 * NC;
+  for (int k = threadIdx.x; k < NI; k += blockDim.x) twl[k] = tw[k];
+  any_rows(rs, C, cm, J, NI, am);
+  __syncthreads();
+  tw = twl;
+  const bool live = rs.fbase[rl] >= 0;
   if (live) {
-    cm.map(col, e, r);
-    live = am(e);
-  }
-  if (live) {
-    const cx<T> *in = reinterpret_cast<const cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
+    const cx<T> *in = reinterpret_cast<const cx<T> *>(G + rs.gbase[rl]);
     for (int n = tt; n < NC; n += FFT_ANY_TPR) a[n] = ld_nt(in + n);
   }
   __syncthreads();
   const cx<T> *Z = stockham_row<-1, T>(a, b, ff, tw, tt, live);
+  // X_k = (s - i w d) / 2 / NI, s = Z_k + conj Z_{NC-k}, d = Z_k - conj Z_{NC-k},
+  // w = e^{-2 pi i k / NI} (spectral.py:159-164), into the other buffer
+  cx<T> *X = Z == a ? b : a;
   if (live) {
-    cx<T> *Fr = F + fidx(0, j, col, J, C);
-    const size_t ms = (size_t)J * fpad(C);
     const T half = (T)0.5;
-    // X_k = (s - i w d) / 2 / NI, s = Z_k + conj Z_{NC-k}, d = Z_k - conj Z_{NC-k},
-    // w = e^{-2 pi i k / NI} (spectral.py:159-164)
     for (int k = tt; k < M; k += FFT_ANY_TPR) {
       const cx<T> zk = Z[k], zc = conj(Z[(NC - k) % NC]);
       const cx<T> s = zk + zc, d = zk - zc;
       const cx<T> wd = cmul(tw[k], d);
       const T sc = mode == 2 ? fourier_scale_any<T>(2, k, NI) : (T)1 / (T)NI;
-      Fr[(size_t)k * ms] = sc * mk(half * (s.x + wd.y), half * (s.y - wd.x));
+      X[k] = sc * mk(half * (s.x + wd.y), half * (s.y - wd.x));
     }
+  }
+  __syncthreads();
+  const size_t ms = (size_t)J * fpad(C);
+  const int xoff = (int)(X - a);  // X is the same buffer (0 or NC) of every row
+  for (int idx = threadIdx.x; idx < FFT_ANY_ROWS * M; idx += blockDim.x) {
+    const int q = idx & (FFT_ANY_ROWS - 1), m = idx / FFT_ANY_ROWS;
+    if (rs.fbase[q] >= 0)
+      F[rs.fbase[q] + (size_t)m * ms] = reinterpret_cast<cx<T> *>(fft_any_lds)[(size_t)q * 2 * NC + xoff + m];
   }
 }
 
