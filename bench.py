@@ -59,7 +59,7 @@ def log(*a):
 # kernels of each profiling slot of one step, with their launches per step
 SLOT_KERNEL = {"syn_level": {"k_syn_mfma<double, lbm::SynLevel": 1},
                "syn_surf": {"k_syn_mfma<double, lbm::SynSurf": 1},
-               "ifft": {"k_ifft<double, 128": 1}, "tend": {"k_tend<double": 1},
+               "ifft": {"k_ifft<double, 128": 1}, "tend": {"k_tend<double": 1, "k_grid<double": 1},
                "fft": {"k_fft<double, 128": 1},
                "ana_level": {"k_ana_mfma<double, lbm::AnaTendVD": 1,
                              "k_ana_mfma<double, lbm::AnaFluxDiv": 2},
@@ -67,11 +67,13 @@ SLOT_KERNEL = {"syn_level": {"k_syn_mfma<double, lbm::SynLevel": 1},
                "step": {"k_step<double, 20": 1}}
 
 
-def measured_traffic(slot, E, precision, fused_tend_fft=False):
+def measured_traffic(slot, E, precision, grid_choice=0):
     """HBM bytes of a slot's kernels per step from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (of each kernel name,
-    the variant with the most launches; the tendency kernel with or without
-    the forward FFT inside as the run chose), or None."""
+    the variant with the most launches; the tendency kernel as the run chose
+    it: separate (grid_choice 0), with the forward FFT inside (1, 2) or the
+    fused grid kernel with 2 or 4 (FP64), 4 or 8 (FP32) members per block
+    (3, 4)), or None."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_T42L20_E{E}_{precision}.json")
     if not os.path.exists(path):
         return None
@@ -80,7 +82,13 @@ def measured_traffic(slot, E, precision, fused_tend_fft=False):
     total = 0
     for name, launches in SLOT_KERNEL[slot].items():
         if slot == "tend":
-            name += ", true" if fused_tend_fft else ", false"
+            if (grid_choice >= 3) != ("k_grid" in name):
+                continue
+            if grid_choice >= 3:
+                mb = (2 if precision == "fp64" else 4) << (grid_choice - 3)
+                name += f", {mb}, true>"
+            else:
+                name += ", true" if grid_choice else ", false"
         if precision == "fp32":
             name = name.replace("double", "float")
         hits = [v for k, v in table.items() if name in k]
@@ -599,16 +607,21 @@ def main():
     prof = plan.profile_read()
     plan.profile(False)
     s = 8 if args.precision == "fp64" else 4
-    fused = tuned["choice"][4] != 0
-    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s, nfrc=E, fused_tend_fft=fused)
+    gchoice = tuned["choice"][4]
+    kb = roofline.kernel_bytes(E, L=vg.kmax, s=s, nfrc=E, fused_tend_fft=gchoice in (1, 2),
+                               grid_fused=gchoice >= 3)
     avg = {k: ms / n for k, (ms, n) in prof.items() if n and ms > 0}
     dom = max(avg, key=lambda k: prof[k][0])
     achieved = kb[dom] / (avg[dom] * 1e-3) / 1e9
     log("kernel ms/launch: " + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()))
-    rl = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    if dom == "tend" and gchoice >= 3:
+        dom_name = "grid (k_grid: inverse FFT, tendencies, forward FFT)"
+    else:
+        dom_name = dom
+    rl = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
           "peak": roofline.MI355X_HBM_GBS, "unit": "GB/s",
           "frac": round(achieved / roofline.MI355X_HBM_GBS, 4),
-          "traffic": measured_traffic(dom, E, args.precision, fused),
+          "traffic": measured_traffic(dom, E, args.precision, gchoice),
           "algorithmic_bytes": kb[dom], "avg_ms": round(avg[dom], 5),
           "kernels_ms": {k: round(v, 5) for k, v in avg.items()}}
 

@@ -1303,20 +1303,63 @@ TendArgs<T> tend_args(lbm_plan *p, bool raw_path, const ActiveMask &am) {
   ta.C = ColMap{p->E, p->L, 9, 2}.ncols();
   ta.M = p->M;
   ta.ne = p->E;
+  ta.fin = (const cx<T> *)p->fin;
   return ta;
 }
 
-// The forward FFT of the tendency rows as k_fft after k_tend (tuning
-// choice 4 = 0, built in) or inside k_tend (1 and 2: two group sizes, 128
-// longitudes only).
-inline bool tend_fused(const lbm_plan *p) { return p->I == 128 && p->tune[4] != 0; }
+// The grid phase (tuning choice 4, 128 longitudes only for 1-4): the
+// inverse FFT, k_tend and the forward FFT as three launches (0, built in);
+// the forward FFT inside k_tend (1 and 2: two group sizes); all three in
+// k_grid (3 and 4: grid_members<T>(v) members per block; nmax + 1 <=
+// GRID_PF_M), except on the raw first step after set_initial_perturbation,
+// which takes choice 0.
+inline bool tend_fused(const lbm_plan *p) {
+  return p->I == 128 && (p->tune[4] == 1 || p->tune[4] == 2);
+}
+inline bool grid_fused(const lbm_plan *p, bool raw) {
+  return p->I == 128 && p->M <= GRID_PF_M && p->tune[4] >= 3 && !raw;
+}
 
-// k_tend for ne members, then (unfused) the forward FFT of its rows; the
-// profiling marks 3 and 4 bracket k_tend.
+template <class T, int MB>
+int launch_grid_mb(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
+  // NSIB blocks per (latitude, 8-member tile), 8 apart: see k_grid
+  constexpr int NSIB = 8 / MB;
+  const long units = (long)((ne + 7) / 8) * p->J;
+  TendArgs<T> tf = ta;
+  tf.ne = ne;
+  const dim3 grid((unsigned)((units + 7) / 8 * 8 * NSIB)), block(MB * 128);
+  if (p->nb == 1)
+    hipLaunchKernelGGL((k_grid<T, MB, true>), grid, block, 0, st, tf);
+  else
+    hipLaunchKernelGGL((k_grid<T, MB, false>), grid, block, 0, st, tf);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+// The inverse FFT of the synthesised columns unless k_grid does it; the
+// profiling marks 2 and 3 bracket it.
+template <class T>
+int launch_grid_in(lbm_plan *p, const TendArgs<T> &ta, const ActiveMask &am, hipStream_t st) {
+  CHK(mark(p, 2, st));
+  if (!grid_fused(p, ta.raw != nullptr))
+    CHK(launch_ifft<T>(p, p->fin, p->gin, ColMap{p->E, p->L, 6, 6}, st, 0, am));
+  return LBM_OK;
+}
+
+// k_tend for ne members, then (unfused) the forward FFT of its rows; or
+// k_grid.  The profiling marks 3 and 4 bracket k_tend or k_grid.
 template <class T>
 int launch_tend(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
   const ColMap cmo{p->E, p->L, 9, 2};
   CHK(mark(p, 3, st));
+  if (grid_fused(p, ta.raw != nullptr)) {
+    if (p->tune[4] == 3)
+      CHK((launch_grid_mb<T, grid_members<T>(3)>(p, ta, ne, st)));
+    else
+      CHK((launch_grid_mb<T, grid_members<T>(4)>(p, ta, ne, st)));
+    CHK(mark(p, 4, st));
+    return LBM_OK;
+  }
   if (tend_fused(p)) {
     // blocks of (MB members, latitude), MB x 128 threads
     TendArgs<T> tf = ta;
@@ -1415,8 +1458,7 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   const ColMap cmo{E, L, 9, 2};
   const int Co = cmo.ncols();
   CHK(join(F_SURF));
-  CHK(mark(p, 2, st));
-  CHK(launch_ifft<T>(p, p->fin, p->gin, cmi, st, 0, am));
+  CHK(launch_grid_in<T>(p, ta, am, st));
   if (!fork && p->mjo) CHK(launch_mjo());
   if (p->mjo) CHK(join(F_MJO));
   CHK(launch_tend<T>(p, ta, ne, st));
@@ -1535,10 +1577,11 @@ int final_check(lbm_plan *p, hipStream_t st) {
 size_t state_bytes(const lbm_plan *p);
 int copy_state(lbm_plan *p, void *buf, bool save, hipStream_t st);
 
-// Tuning choice 4: the tendency rows' forward FFT as k_fft after k_tend (0)
-// or inside k_tend (1, 128 longitudes only), timed on the plan's buffers
-// with no history sums or trajectory; the state k_tend updates (deferred
-// ps' updates, the blow-up flag) is saved before and restored after.
+// Tuning choice 4: the grid phase as three launches (0), with the forward
+// FFT inside k_tend (1, 2) or in one k_grid launch (3, 4), 128 longitudes
+// only, timed on the plan's buffers with no history sums or trajectory;
+// the state the tendencies update (deferred ps' updates, the blow-up flag)
+// is saved before and restored after.
 template <class T, class Timed>
 int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   if (p->I != 128) {
@@ -1555,22 +1598,31 @@ int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   ta.h.last_grid = nullptr;
   ta.h.last_spec = nullptr;
   ta.traj = nullptr;
-  double ms[3] = {0, 0, 0};
-  for (int v = 0; v < 3 && rc == LBM_OK; ++v) {
+  const ActiveMask am = loop_mask(p, true, 0);
+  constexpr int NV = 5;
+  double ms[NV] = {0, 0, 0, 0, 0};
+  for (int v = 0; v < NV && rc == LBM_OK; ++v) {
     p->tune[4] = v;
-    rc = timed([&] { return launch_tend<T>(p, ta, p->E, st); }, &ms[v]);
+    rc = timed([&] {
+      CHK(launch_grid_in<T>(p, ta, am, st));
+      return launch_tend<T>(p, ta, p->E, st);
+    }, &ms[v]);
   }
   if (copy_state(p, save, false, st) != LBM_OK && rc == LBM_OK) rc = LBM_ERR_RUNTIME;
   if (hipStreamSynchronize(st) != hipSuccess && rc == LBM_OK) rc = fail(LBM_ERR_RUNTIME, "sync failed");
   (void)hipFree(save);
   if (rc != LBM_OK) return rc;
   int best = 0;
-  for (int v = 1; v < 3; ++v)
+  for (int v = 1; v < NV; ++v)
     if (ms[v] < ms[best]) best = v;
   p->tune[4] = best;
-  const char *lab[3] = {"separate", "fused", "fused"};
-  std::snprintf(buf, len, "; tend_fft: separate=%.4f fused/%d=%.4f fused/%d=%.4f ms -> %s", ms[0],
-                tend_fft_members<T>(1), ms[1], tend_fft_members<T>(2), ms[2], lab[best]);
+  const char *lab[NV] = {"separate", "tend+fft", "tend+fft", "grid", "grid"};
+  const int mb[NV] = {0, tend_fft_members<T>(1), tend_fft_members<T>(2), grid_members<T>(3),
+                      grid_members<T>(4)};
+  std::snprintf(buf, len,
+                "; tend_fft: separate=%.4f tend+fft/%d=%.4f tend+fft/%d=%.4f grid/%d=%.4f "
+                "grid/%d=%.4f ms -> %d (%s)",
+                ms[0], mb[1], ms[1], mb[2], ms[2], mb[3], ms[3], mb[4], ms[4], best, lab[best]);
   return LBM_OK;
 }
 
@@ -3050,7 +3102,8 @@ int lbm_set_tuning(lbm_plan *p, const int32_t *h_choice, int n) {
   if (!p || (n > 0 && !h_choice)) return fail(LBM_ERR_VALUE, "null argument");
   if (n < 0 || n > lbm_plan::NTUNE) return fail(LBM_ERR_VALUE, "at most %d choices", lbm_plan::NTUNE);
   for (int k = 0; k < n; ++k)
-    if (h_choice[k] < 0 || h_choice[k] > 2) return fail(LBM_ERR_VALUE, "choice %d out of range", k);
+    if (h_choice[k] < 0 || h_choice[k] > (k == 4 ? 4 : 2))
+      return fail(LBM_ERR_VALUE, "choice %d out of range", k);
   if (p->in_run) return fail(LBM_ERR_STATE, "a run is in progress (finish it first)");
   destroy_graphs(p);
   for (int k = 0; k < lbm_plan::NTUNE; ++k) p->tune[k] = k < n ? h_choice[k] : 0;

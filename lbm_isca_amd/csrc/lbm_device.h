@@ -304,6 +304,8 @@ struct FftShape {
 
 // Twiddles exp(-2 pi i k / NI), k = 0..NI-1, filled by the host.
 // (static: each translation unit that includes this header has its own)
+// The stage functions below take the table (this one or a copy in LDS) as
+// tw.
 static __constant__ cx<double> c_tw128_d[128];
 static __constant__ cx<float> c_tw128_f[128];
 static __constant__ cx<double> c_tw384_d[384];
@@ -370,9 +372,8 @@ __device__ inline void fft_a_store(const cx<T> *v, cx<T> *scr, int tt) {
 // g = X0.re + 2 sum_m Re(X_m e^{i m lam}) = irfft(zero-filled * nlon)
 // (spectral.py:166-172); the imaginary part of X0 is discarded.
 template <class T, int NI, class XF>
-__device__ inline void ifft_a(XF X, int M, cx<T> *v, int tt, int mode) {
+__device__ inline void ifft_a(XF X, int M, cx<T> *v, int tt, int mode, const cx<T> *tw) {
   using S = FftShape<T, NI>;
-  const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
@@ -451,9 +452,8 @@ __device__ inline void store_row_pairs(const cx<T> *z, cx<T> *out, int tt) {
 // radix-8 pass; v[k2] (twiddled) goes to scr by fft_a_store.  in: the row in
 // global memory (R = 1) or staged in LDS (R > 1).
 template <class T, int NI, bool LDS_IN = false>
-__device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt) {
+__device__ inline void fft_a(const cx<T> *in, cx<T> *v, int tt, const cx<T> *tw) {
   constexpr int R = FftShape<T, NI>::R;
-  const cx<T> *tw = twiddles<T, NI>();
   const int r = tt >> 3, t = tt & 7;
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) {
@@ -494,9 +494,8 @@ __device__ inline void fft_b(const cx<T> *scr, cx<T> *v, int tt) {
 // (times the mode's scale), written to F with stride ms between m.
 template <class T, int NI>
 __device__ inline void fft_c(const cx<T> *Z, cx<T> *__restrict__ F, size_t ms, int M, int tt,
-                             int mode) {
+                             int mode, const cx<T> *tw) {
   using S = FftShape<T, NI>;
-  const cx<T> *tw = twiddles<T, NI>();
   const T half = (T)0.5;
   for (int k = tt; k < M; k += S::TPR) {
     const cx<T> zk = Z[k], zc = conj(Z[(S::NC - k) % S::NC]);
@@ -537,10 +536,10 @@ __global__ __launch_bounds__((FftShape<T, NI, true, RW>::THREADS)) void k_ifft(c
       for (int m = tt; m < M; m += S::TPR) sh[rl][m] = Fr[(size_t)m * ms];
     __syncthreads();
     const cx<T> *X = sh[rl];
-    if (live) ifft_a<T, NI>([&](int k) { return X[k]; }, M, v, tt, mode);
+    if (live) ifft_a<T, NI>([&](int k) { return X[k]; }, M, v, tt, mode, twiddles<T, NI>());
     __syncthreads();  // the staged coefficients are overwritten by the first pass
   } else {
-    if (live) ifft_a<T, NI>([&](int k) { return Fr[(size_t)k * ms]; }, M, v, tt, mode);
+    if (live) ifft_a<T, NI>([&](int k) { return Fr[(size_t)k * ms]; }, M, v, tt, mode, twiddles<T, NI>());
   }
   if (live) fft_a_store<T, NI>(v, sh[rl], tt);
   __syncthreads();
@@ -608,7 +607,7 @@ __global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(c
   cx<T> v[8];
   if constexpr (S::R == 1 && sizeof(T) == 8) {
     if (live) {
-      fft_a<T, NI>(row_in, v, tt);
+      fft_a<T, NI>(row_in, v, tt, twiddles<T, NI>());
       fft_a_store<T, NI>(v, sh[rl], tt);
     }
   } else {
@@ -632,7 +631,7 @@ __global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(c
       }
     }
     __syncthreads();
-    if (live) fft_a<T, NI>(sh[rl], v, tt);
+    if (live) fft_a<T, NI>(sh[rl], v, tt, twiddles<T, NI>());
     __syncthreads();  // the staged row is overwritten by the first pass
     if (live) fft_a_store<T, NI>(v, sh[rl], tt);
   }
@@ -645,7 +644,8 @@ __global__ __launch_bounds__((FftShape<T, NI, false, RW>::THREADS)) void k_fft(c
     for (int k1 = 0; k1 < 8; ++k1) sh[rl][rr + S::R * (tq + 8 * k1)] = v[k1];
   }
   __syncthreads();
-  if (live) fft_c<T, NI>(sh[rl], F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, tt, mode);
+  if (live) fft_c<T, NI>(sh[rl], F + fidx(0, j, col, J, C), (size_t)J * fpad(C), M, tt, mode,
+                         twiddles<T, NI>());
 }
 
 // ---------------------------------------------------------------------------

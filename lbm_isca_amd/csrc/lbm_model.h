@@ -291,6 +291,9 @@ struct TendArgs {
   cx<T> *fout;
   int C, M;
   int ne;  // members launched (k_tend<T, true>: tiles of 8 cover them)
+  // k_grid: the synthesised Fourier columns (ColMap {E, L, 6, 6}) instead
+  // of gin
+  const cx<T> *fin;
 };
 
 // Members per block of k_tend<T, true>, tuning variant v = 1 or 2: 8 or 4
@@ -318,7 +321,7 @@ __device__ inline void fft_tend_rows(cx<T> (*sh)[72], const T (&vals)[9], int nf
   const int rf = rl / MB, rm = rl - rf * MB;
   const bool live = rf < nf && ((live8 >> rm) & 1u);
   cx<T> v[8];
-  if (live) fft_a<T, 128, true>(sh[rl], v, tt);
+  if (live) fft_a<T, 128, true>(sh[rl], v, tt, twiddles<T, 128>());
   __syncthreads();  // the staged row is overwritten by the first pass
   if (live) fft_a_store<T, 128>(v, sh[rl], tt);
   __syncthreads();
@@ -333,7 +336,9 @@ __device__ inline void fft_tend_rows(cx<T> (*sh)[72], const T (&vals)[9], int nf
 #pragma unroll
   for (int f = 0; f < 9; ++f)
     if (f == rf) c = col0[f];
-  if (live) fft_c<T, 128>(sh[rl], fout + fidx(0, j, c + rm, J, C), (size_t)J * fpad(C), M, tt, 0);
+  if (live)
+    fft_c<T, 128>(sh[rl], fout + fidx(0, j, c + rm, J, C), (size_t)J * fpad(C), M, tt, 0,
+                  twiddles<T, 128>());
 }
 
 // Tendencies on grid rows, one thread per column, one pass over the levels
@@ -520,6 +525,362 @@ __global__ __launch_bounds__(FFT ? MB * 128 : 128) void k_tend(TendArgs<T> a) {
     vp = vpn;
     tp = tpn;
     qp = qpn;
+  }
+  if (act && !isfinite(cs.chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
+}
+
+// ---------------------------------------------------------------------------
+// The grid phase in one kernel (128 longitudes): inverse FFT of the
+// synthesised Fourier columns, the tendencies of k_tend and the forward FFT
+// of the tendency rows, so that neither grid block (6L + 6 rows in, 9L + 2
+// out per member and latitude) passes through memory.  One block per (group
+// of MB members, latitude), MB x 128 threads in two roles that alternate
+// between block barriers:
+//   column phase: thread (member m, longitude i) forms one level of its
+//     column (column_level) from grid rows in LDS and writes the level's
+//     nine tendency rows to LDS;
+//   row phase: 8 threads per row (k_ifft / k_fft's stages for 128 points,
+//     whose exchanges stay within the row's wave) transform the tendency
+//     rows of the level just formed to their Fourier columns in memory and
+//     the synthesised columns of the level after next to grid rows in LDS.
+// Grid rows of two levels are held (levels k and k+1 feed level k's
+// vertical advection), so each level costs two block barriers.  The blocks
+// of one 8-column Fourier tile and latitude are 8 apart in blockIdx: they
+// run on one XCD together and their partial lines meet in its L2.  The raw
+// first step after set_initial_perturbation takes k_tend.
+// ---------------------------------------------------------------------------
+
+// Orders the LDS accesses of one wave around a point (the LDS serves a
+// wave's accesses in issue order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Members per block of k_grid, tuning variant v = 3 or 4.
+template <class T>
+constexpr int grid_members(int v) { return (sizeof(T) == 4 ? 4 : 2) << (v - 3); }
+
+// LDS row of k_grid: 72 complex (the 8 x 9 transpose scratch of the
+// 128-point stages); a grid row is its first 128 real values.
+template <class T>
+using GridRow = cx<T>[72];
+
+// Inverse FFT of one Fourier column (M coefficients at stride ms from Fr)
+// into the LDS row sc as 128 real samples in natural order: thread tt of
+// the row's 8 consecutive threads (one wave).
+template <class T>
+__device__ inline void grid_ifft_staged(int M, cx<T> *sc, int tt, const cx<T> *tw);
+
+template <class T>
+__device__ inline void grid_ifft_row(const cx<T> *__restrict__ Fr, size_t ms, int M, cx<T> *sc,
+                                     int tt, const cx<T> *tw) {
+  for (int m = tt; m < M; m += 8) sc[m] = Fr[(size_t)m * ms];
+  grid_ifft_staged<T>(M, sc, tt, tw);
+}
+
+// The same, from the M coefficients already staged in sc.
+template <class T>
+__device__ inline void grid_ifft_staged(int M, cx<T> *sc, int tt, const cx<T> *tw) {
+  wave_lds_sync();
+  cx<T> v[8];
+  ifft_a<T, 128>([&](int k) { return sc[k]; }, M, v, tt, 0, tw);
+  wave_lds_sync();
+  fft_a_store<T, 128>(v, sc, tt);
+  wave_lds_sync();
+  ifft_b<T, 128>(sc, v, tt);
+  wave_lds_sync();
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) sc[tt + 8 * k1] = v[k1];
+}
+
+// Forward FFT of the LDS row sc (128 real samples in natural order) to the
+// Fourier column F (M coefficients at stride ms); sc is used as scratch.
+template <class T>
+__device__ inline void grid_fft_row(cx<T> *sc, cx<T> *__restrict__ F, size_t ms, int M, int tt,
+                                    const cx<T> *tw) {
+  cx<T> v[8];
+  fft_a<T, 128, true>(sc, v, tt, tw);
+  wave_lds_sync();
+  fft_a_store<T, 128>(v, sc, tt);
+  wave_lds_sync();
+  fft_b<T, 128>(sc, v, tt);
+  wave_lds_sync();
+#pragma unroll
+  for (int k1 = 0; k1 < 8; ++k1) sc[tt + 8 * k1] = v[k1];
+  wave_lds_sync();
+  fft_c<T, 128>(sc, F, ms, M, tt, 0, tw);
+}
+
+// SBS: one basic state for every member (nbasic = 1), whose level rows the
+// block stages in LDS during the row phase before the level is formed;
+// otherwise each column loads its own.  FP64 with SBS: the inverse rows' Fourier
+// coefficients are loaded one row phase ahead (M <= GRID_PF_M).
+constexpr int GRID_PF_M = 48;
+template <class T, int MB, bool SBS>
+__global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
+  constexpr int NSIB = 8 / MB;  // blocks per 8-column tile and latitude
+  constexpr int NT = MB * 128;
+  constexpr int I = 128;
+  constexpr bool PF = sizeof(T) == 8 && SBS;
+  // basic-state values a thread loads per level: SBS, its share of the
+  // block's nine rows; else the nine values of its column
+  constexpr int NBS = SBS ? (9 * I + NT - 1) / NT : 9;
+  __shared__ GridRow<T> rin[2][6][MB];  // synthesised rows of two levels (first: the surface rows)
+  __shared__ GridRow<T> rout[9][MB];    // tendency rows of one level
+  __shared__ cx<T> twl[128];            // the FFT twiddles
+  __shared__ T bsl[SBS ? 2 : 1][SBS ? 9 : 1][I];  // SBS: basic-state rows of two levels
+  const int J = a.J, L = a.L, E = a.E;
+  // blockIdx -> unit u = (latitude, tile), sibling s < NSIB
+  const int b = blockIdx.x;
+  const int u = (b / (8 * NSIB)) * 8 + (b & 7);
+  const int s = (b >> 3) % NSIB;
+  const int ntile = (a.ne + 7) >> 3;
+  if (u >= ntile * J) return;
+  const int tile = u / J, j = u - tile * J;
+  const int e0 = tile * 8 + s * MB;
+  unsigned live = 0;  // bit q: member e0 + q is launched and active
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int eq = e0 + q;
+    if (eq < a.ne && a.am(eq)) live |= 1u << q;
+  }
+  if (live == 0) return;
+  const int t = (int)threadIdx.x;
+  const int EL = E * L;
+  const int Ci = 6 * EL + 6 * E, Co = a.C;
+  const size_t msi = (size_t)J * fpad(Ci), mso = (size_t)J * fpad(Co);
+  const int M = a.M;
+  // row role: row rl = (group rq, member rm), thread tt of its 8
+  const int rl = t >> 3, tt = t & 7;
+  const int rq = rl / MB, rm = rl - rq * MB;
+  const bool rlive = (live >> rm) & 1u;
+  const int er = e0 + rm;
+  // column role: member m (wave-uniform), longitude i
+  const int m = __builtin_amdgcn_readfirstlane(t >> 7), i = t & 127;
+  const bool act = (live >> m) & 1u;
+  const int e = e0 + m < E ? e0 + m : E - 1;
+  const int eb = a.nbasic > 1 ? e : 0;
+  const BRow br{L};
+  const T *bsj = a.bsg + (size_t)(eb * J + j) * br.rows() * I;
+  // basic-state row of field f (u v t q tv absvor div x5, then sigma-dot of
+  // the half level below) of level lv
+  auto bsrow = [&](int f, int lv) { return f < 8 ? f * L + lv : 8 * L + lv + 1; };
+  auto bs_load = [&](int lv, T (&st)[NBS]) {
+    if (lv >= L) return;
+#pragma unroll
+    for (int q = 0; q < NBS; ++q) {
+      if constexpr (SBS) {
+        const int idx = t + q * NT;
+        if (idx < 9 * I) st[q] = bsj[(size_t)bsrow(idx >> 7, lv) * I + (idx & 127)];
+      } else {
+        st[q] = bsj[(size_t)bsrow(q, lv) * I + i];
+      }
+    }
+  };
+  // SBS: the staged values into the level's slot
+  auto bs_store = [&](int lv, const T (&st)[NBS]) {
+    if constexpr (SBS) {
+      if (lv >= L) return;
+#pragma unroll
+      for (int q = 0; q < NBS; ++q) {
+        const int idx = t + q * NT;
+        if (idx < 9 * I) bsl[lv & 1][idx >> 7][idx & 127] = st[q];
+      }
+    }
+  };
+  const bool has_fr = a.frc != nullptr;
+  const T *frj = has_fr ? a.frc + ((size_t)((a.nfrc > 1 ? e : 0) * J + j) * L) * I + i : nullptr;
+  auto FR = [&](int k) { return has_fr && k < L ? frj[(size_t)k * I] : (T)0; };
+  // row phase: inverse rows (groups 0-5) of level il into its slot, when
+  // il < L; forward rows (groups 6..) of the tendency rows of level fl
+  // (fl < 0: the two surface rows)
+  // PF: this thread's coefficients of its inverse row at level il (group
+  // rq < 6), in flight until the row phase that transforms them
+  // (six named registers, not an array: an array carried around the level
+  // loop went to scratch; the loads read coefficient min(m, M - 1))
+  static_assert(GRID_PF_M == 48, "six coefficients per thread");
+  cx<T> cf0, cf1, cf2, cf3, cf4, cf5;
+#define LBM_GRID_COEF_LOAD(il)                                                      \
+  if (PF && rlive && rq < 6 && (il) < L) {                                         \
+    const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + (il) * E + er, J, Ci);          \
+    auto ld = [&](int m) { return Fr[(size_t)(m < M ? m : M - 1) * msi]; };        \
+    cf0 = ld(tt);                                                                  \
+    cf1 = ld(tt + 8);                                                              \
+    cf2 = ld(tt + 16);                                                             \
+    cf3 = ld(tt + 24);                                                             \
+    cf4 = ld(tt + 32);                                                             \
+    cf5 = ld(tt + 40);                                                             \
+  }
+  auto row_phase = [&](int fl, int il) {
+    if (!rlive) return;
+    if (rq < 6) {
+      if (il < L) {
+        cx<T> *sc = rin[(il + 1) & 1][rq][rm];
+        if constexpr (PF) {
+          if (tt < M) sc[tt] = cf0;
+          if (tt + 8 < M) sc[tt + 8] = cf1;
+          if (tt + 16 < M) sc[tt + 16] = cf2;
+          if (tt + 24 < M) sc[tt + 24] = cf3;
+          if (tt + 32 < M) sc[tt + 32] = cf4;
+          if (tt + 40 < M) sc[tt + 40] = cf5;
+          LBM_GRID_COEF_LOAD(il + 1)
+          grid_ifft_staged<T>(M, sc, tt, twl);
+        } else {
+          grid_ifft_row<T>(a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci), msi, M, sc, tt, twl);
+        }
+      }
+    } else {
+      const int f = rq - 6;
+      if (f < (fl < 0 ? 2 : 9)) {
+        const int col = (fl < 0 ? 9 * EL + f * E : f * EL + fl * E) + er;
+        grid_fft_row<T>(rout[f][rm], a.fout + fidx(0, j, col, J, Co), mso, M, tt, twl);
+      }
+    }
+  };
+
+  // basic state of levels 0 and 1 (SBS: staged; else held per column), the
+  // forcing of level 0, the twiddles; the surface rows (slot 0) and level 0
+  // (slot 1)
+  T bc[NBS], bn[NBS];
+  bs_load(0, bc);
+  bs_load(1, bn);
+  T fk = FR(0);
+  for (int q = t; q < 128; q += NT) twl[q] = twiddles<T, 128>()[q];
+  bs_store(0, bc);
+  bs_store(1, bn);
+  __syncthreads();
+  if (rlive && rq < 12) {
+    const int g = rq < 6 ? rq : rq - 6;
+    const int col = (rq < 6 ? 6 * EL + g * E : g * EL) + er;
+    grid_ifft_row<T>(a.fin + fidx(0, j, col, J, Ci), msi, M, rin[rq < 6 ? 0 : 1][g][rm], tt, twl);
+  }
+  LBM_GRID_COEF_LOAD(1)
+#undef LBM_GRID_COEF_LOAD
+  __syncthreads();
+
+  const VOff vo{L};
+  auto LR = [&](const GridRow<T> &row) { return reinterpret_cast<const T *>(row)[i]; };
+  auto WR = [&](GridRow<T> &row, T v) { reinterpret_cast<T *>(row)[i] = v; };
+  auto BS = [&](int row) { return bsj[(size_t)row * I + i]; };
+  ColConst<T> cc;
+  cc.pix = LR(rin[0][0][m]);
+  cc.piy = LR(rin[0][1][m]);
+  const T pig = LR(rin[0][2][m]);
+  cc.bpix = BS(br.pix());
+  cc.bpiy = BS(br.piy());
+  const T psbar = BS(br.psbar());
+  cc.rc = a.rcos[j];
+  cc.full_ke = !a.strictly_linear;
+  {
+    const T su = LR(rin[0][3][m]), sv = LR(rin[0][4][m]), sd = LR(rin[0][5][m]);
+    cc.D0 = sd;
+    cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * BS(br.sbu()) + cc.piy * BS(br.sbv());
+  }
+  ColState<T> cs;
+  cs.bsd_lo = BS(br.sdot());
+  const size_t pidx = (size_t)(e * J + j) * I + i;
+  T gpi, pib, pst;
+  if (a.linear) {
+    gpi = cc.D0 - cc.S0;
+    pib = 0;
+    pst = psbar * pig;
+  } else {
+    // an inactive member leaves its ps' as it is
+    column_surface<T>(a.psa + pidx, a.psb + pidx, act ? a.pend[e] : 0, pig, psbar, a.robert, cc.S0,
+                      cc.D0, gpi, pib, cs.chk, pst);
+  }
+  T *trj = nullptr;
+  if (a.traj && act) {
+    trj = a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i;
+    trj[(size_t)(2 * L) * I] = a.linear ? pst : a.psa[pidx];
+    trj[(size_t)(2 * L + 1) * I] = a.linear ? (T)0 : a.psb[pidx];
+  }
+  // history samples of this top-of-step state
+  int hslot[2];
+  T hw[2];
+  const int top_a = a.ctl->step - a.h.inistp + 1;
+  const int nst = a.h.nsteps ? a.h.nsteps[e] : 0;
+  const int hn = act ? hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw) : 0;
+  const bool hlast = act && a.h.last_grid != nullptr && top_a == nst;
+  const size_t hrow = (size_t)(e * J + j) * (L + 1) * I + i;
+  for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
+  WR(rout[0][m], gpi);
+  WR(rout[1][m], pib);
+  const T *tref = a.tref + (size_t)eb * L;
+  __syncthreads();
+  row_phase(-1, 1);  // the surface tendency rows out, level 1 in
+  __syncthreads();
+
+  for (int k = 0; k < L; ++k) {
+    const int sk = (k + 1) & 1, sn = k & 1;  // slots of levels k and k+1
+    const bool top = (k + 1 == L);
+    BLev<T> bk, bk1;  // basic state of level k; u, v, T, q of level k+1
+    if constexpr (SBS) {
+      const int b0 = k & 1, b1 = (k + 1) & 1;
+      bk.u = bsl[b0][0][i];
+      bk.v = bsl[b0][1][i];
+      bk.t = bsl[b0][2][i];
+      bk.q = bsl[b0][3][i];
+      bk.tv = bsl[b0][4][i];
+      bk.av = bsl[b0][5][i];
+      bk.dv = bsl[b0][6][i];
+      bk.x5 = bsl[b0][7][i];
+      bk.sd = bsl[b0][8][i];
+      if (!top) {
+        bk1.u = bsl[b1][0][i];
+        bk1.v = bsl[b1][1][i];
+        bk1.t = bsl[b1][2][i];
+        bk1.q = bsl[b1][3][i];
+      }
+    } else {
+      bk = BLev<T>{bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], bc[6], bc[7], bc[8]};
+      if (!top) {
+        bk1.u = bn[0];
+        bk1.v = bn[1];
+        bk1.t = bn[2];
+        bk1.q = bn[3];
+      }
+    }
+    const T up = LR(rin[sk][0][m]), vp = LR(rin[sk][1][m]);
+    const T vor = LR(rin[sk][2][m]), dv = LR(rin[sk][3][m]);
+    const T tp = LR(rin[sk][4][m]), qp = LR(rin[sk][5][m]);
+    T upn = 0, vpn = 0, tpn = 0, qpn = 0;
+    if (!top) {
+      upn = LR(rin[sn][0][m]);
+      vpn = LR(rin[sn][1][m]);
+      tpn = LR(rin[sn][4][m]);
+      qpn = LR(rin[sn][5][m]);
+    }
+    T out[9], omega;
+    if (trj) {
+      trj[(size_t)k * I] = up;
+      trj[(size_t)(L + k) * I] = vp;
+    }
+    column_level<T>(k, L, a.vc, tref, bk, bk1, fk, cc, cs, up, vp, vor, dv, tp, qp, upn, vpn, tpn,
+                    qpn, out, psbar, pst, omega);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) WR(rout[f][m], out[f]);
+    for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
+    if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
+    __syncthreads();
+    // level k's tendency rows out, level k + 2 in; meanwhile the basic
+    // state of level k + 2 and this column's forcing of level k + 1 arrive
+    T st[NBS];
+    bs_load(k + 2, st);
+    const T fkn = FR(k + 1);
+    row_phase(k, k + 2);
+    bs_store(k + 2, st);
+    if constexpr (!SBS) {
+#pragma unroll
+      for (int q = 0; q < NBS; ++q) {
+        bc[q] = bn[q];
+        bn[q] = st[q];
+      }
+    }
+    fk = fkn;
+    __syncthreads();
   }
   if (act && !isfinite(cs.chk)) atomicMin(&a.ctl->bad_step, a.ctl->step);
 }

@@ -12,10 +12,14 @@ MI355X_FP32_TFLOPS = 157.3
 MI355X_BF16_TFLOPS = 2500.0      # dense matrix cores
 
 
-def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1, fused_tend_fft=False):
+def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1, fused_tend_fft=False,
+                 grid_fused=False):
     """``fused_tend_fft``: the tendency kernel with the forward FFT inside
     (its "tend" reads what k_tend reads and writes the Fourier columns k_fft
-    writes; "fft" is then empty)."""
+    writes; "fft" is then empty).  ``grid_fused``: the grid phase in one
+    kernel (k_grid: "tend" reads the synthesised Fourier columns, the basic
+    state, the forcing and ps', and writes ps' and the tendencies' Fourier
+    columns; "ifft" and "fft" are empty)."""
     M = nmax + 1
     nspec = M * (M + 1) // 2
     Jh = J // 2
@@ -39,6 +43,10 @@ def kernel_bytes(E, L=20, J=64, I=128, nmax=42, s=8, nbasic=1, nfrc=1, fused_ten
     if fused_tend_fft:
         out["tend"] += M * J * cout * c - J * cout * I * s
         out["fft"] = 0
+    if grid_fused:
+        out["tend"] = (M * J * cin * c + M * J * cout * c
+                       + (nbasic * J * rb * I + nfrc * J * L * I + 4 * E * J * I) * s)
+        out["ifft"] = out["fft"] = 0
     return out
 
 
