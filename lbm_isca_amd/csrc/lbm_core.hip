@@ -1322,12 +1322,12 @@ inline bool grid_fused(const lbm_plan *p, bool raw) {
 
 template <class T, int MB>
 int launch_grid_mb(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
-  // NSIB blocks per (latitude, 8-member tile), 8 apart: see k_grid
-  constexpr int NSIB = 8 / MB;
-  const long units = (long)((ne + 7) / 8) * p->J;
+  // JP x NSIB blocks per (8-member tile, JP latitudes), 8 apart: see k_grid
+  constexpr int NSIB = 8 / MB, JP = sizeof(T) == 4 ? 2 : 1;
+  const long units = (long)((ne + 7) / 8) * (p->J / JP);
   TendArgs<T> tf = ta;
   tf.ne = ne;
-  const dim3 grid((unsigned)((units + 7) / 8 * 8 * NSIB)), block(MB * 128);
+  const dim3 grid((unsigned)((units + 7) / 8 * 8 * NSIB * JP)), block(MB * 128);
   if (p->nb == 1)
     hipLaunchKernelGGL((k_grid<T, MB, true>), grid, block, 0, st, tf);
   else
@@ -2597,6 +2597,9 @@ int lbm_last_error(char *buf, int64_t len) {
 
 int lbm_plan_destroy(lbm_plan *p) {
   if (!p) return LBM_OK;
+  // nothing of the plan may still run when its graphs, buffers and streams go
+  if (p->own) (void)hipStreamSynchronize(p->own);
+  if (p->side) (void)hipStreamSynchronize(p->side);
   destroy_graphs(p);
   free_tabs(p->d64);
   free_tabs(p->d32);

@@ -615,7 +615,8 @@ __device__ inline void grid_fft_row(cx<T> *sc, cx<T> *__restrict__ F, size_t ms,
 
 // SBS: one basic state for every member (nbasic = 1), whose level rows the
 // block stages in LDS during the row phase before the level is formed;
-// otherwise each column loads its own.  FP64 with SBS: the inverse rows' Fourier
+// otherwise each column loads its own.  With SBS (FP64, and FP32 at 4
+// members per block, whose registers allow it) the inverse rows' Fourier
 // coefficients are loaded one row phase ahead (M <= GRID_PF_M).
 constexpr int GRID_PF_M = 48;
 template <class T, int MB, bool SBS>
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   constexpr int NSIB = 8 / MB;  // blocks per 8-column tile and latitude
   constexpr int NT = MB * 128;
   constexpr int I = 128;
-  constexpr bool PF = sizeof(T) == 8 && SBS;
+  constexpr bool PF = SBS && (sizeof(T) == 8 || MB <= 4);
   // basic-state values a thread loads per level: SBS, its share of the
   // block's nine rows; else the nine values of its column
   constexpr int NBS = SBS ? (9 * I + NT - 1) / NT : 9;
@@ -632,13 +633,19 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   __shared__ cx<T> twl[128];            // the FFT twiddles
   __shared__ T bsl[SBS ? 2 : 1][SBS ? 9 : 1][I];  // SBS: basic-state rows of two levels
   const int J = a.J, L = a.L, E = a.E;
-  // blockIdx -> unit u = (latitude, tile), sibling s < NSIB
+  // blockIdx -> unit u = (tile, group of JP latitudes) and within it the
+  // latitude jj < JP and sibling s < NSIB: the JP x NSIB blocks of a unit
+  // are 8 apart (one XCD).  A 128-byte line of the Fourier columns holds
+  // the 8 columns of a tile at 128 / (8 sizeof(cx<T>)) = JP latitudes.
+  constexpr int JP = sizeof(T) == 4 ? 2 : 1;
+  constexpr int NSUB = JP * NSIB;
   const int b = blockIdx.x;
-  const int u = (b / (8 * NSIB)) * 8 + (b & 7);
-  const int s = (b >> 3) % NSIB;
-  const int ntile = (a.ne + 7) >> 3;
-  if (u >= ntile * J) return;
-  const int tile = u / J, j = u - tile * J;
+  const int u = (b / (8 * NSUB)) * 8 + (b & 7);
+  const int sub = (b >> 3) % NSUB;
+  const int s = sub % NSIB, jj = sub / NSIB;
+  const int ntile = (a.ne + 7) >> 3, nju = J / JP;
+  if (u >= ntile * nju) return;
+  const int tile = u / nju, j = (u - tile * nju) * JP + jj;
   const int e0 = tile * 8 + s * MB;
   unsigned live = 0;  // bit q: member e0 + q is launched and active
 #pragma unroll
