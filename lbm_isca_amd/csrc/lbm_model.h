@@ -570,23 +570,25 @@ using GridRow = cx<T>[72];
 // Inverse FFT of one Fourier column (M coefficients at stride ms from Fr)
 // into the LDS row sc as 128 real samples in natural order: thread tt of
 // the row's 8 consecutive threads (one wave).
-template <class T>
-__device__ inline void grid_ifft_staged(int M, cx<T> *sc, int tt, const cx<T> *tw);
+template <class T, class XF, class F>
+__device__ inline void grid_ifft_from(XF X, int M, cx<T> *sc, int tt, const cx<T> *tw, F then);
 
 template <class T>
 __device__ inline void grid_ifft_row(const cx<T> *__restrict__ Fr, size_t ms, int M, cx<T> *sc,
                                      int tt, const cx<T> *tw) {
   for (int m = tt; m < M; m += 8) sc[m] = Fr[(size_t)m * ms];
-  grid_ifft_staged<T>(M, sc, tt, tw);
+  wave_lds_sync();
+  grid_ifft_from<T>([&](int k) { return sc[k]; }, M, sc, tt, tw, [] {});
 }
 
-// The same, from the M coefficients already staged in sc.
-template <class T>
-__device__ inline void grid_ifft_staged(int M, cx<T> *sc, int tt, const cx<T> *tw) {
-  wave_lds_sync();
+// The same from coefficients X(k), k < M, read elsewhere (sc may hold
+// them); then() runs once every X(k) has been read.
+template <class T, class XF, class F>
+__device__ inline void grid_ifft_from(XF X, int M, cx<T> *sc, int tt, const cx<T> *tw, F then) {
   cx<T> v[8];
-  ifft_a<T, 128>([&](int k) { return sc[k]; }, M, v, tt, 0, tw);
+  ifft_a<T, 128>(X, M, v, tt, 0, tw);
   wave_lds_sync();
+  then();
   fft_a_store<T, 128>(v, sc, tt);
   wave_lds_sync();
   ifft_b<T, 128>(sc, v, tt);
@@ -613,18 +615,53 @@ __device__ inline void grid_fft_row(cx<T> *sc, cx<T> *__restrict__ F, size_t ms,
   fft_c<T, 128>(sc, F, ms, M, tt, 0, tw);
 }
 
+// Loads straight into LDS (global_load_lds): lane l of the wave writes
+// element l of the wave-uniform 64-element array dst, FP32 complex values
+// as two 4-byte halves (the 64 real parts, then the 64 imaginary parts).
+// The data arrives under vmcnt: readers wait with wait_vm0 first.
+template <class T>
+__device__ __forceinline__ void lds_dma(const cx<T> *src, cx<T> *dst);
+template <>
+__device__ __forceinline__ void lds_dma<double>(const cx<double> *src, cx<double> *dst) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                   (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+template <>
+__device__ __forceinline__ void lds_dma<float>(const cx<float> *src, cx<float> *dst) {
+  float *d = reinterpret_cast<float *>(dst);
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)&src->x,
+                                   (__attribute__((address_space(3))) void *)d, 4, 0, 0);
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)&src->y,
+                                   (__attribute__((address_space(3))) void *)(d + 64), 4, 0, 0);
+}
+template <class T>
+__device__ __forceinline__ cx<T> lds_dma_get(const cx<T> *st, int l) {
+  if constexpr (sizeof(T) == 8) {
+    return st[l];
+  } else {
+    const float *f = reinterpret_cast<const float *>(st);
+    return mk(f[l], f[64 + l]);
+  }
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left open)
+__device__ __forceinline__ void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // SBS: one basic state for every member (nbasic = 1), whose level rows the
 // block stages in LDS during the row phase before the level is formed;
-// otherwise each column loads its own.  With SBS (FP64, and FP32 at 4
-// members per block, whose registers allow it) the inverse rows' Fourier
-// coefficients are loaded one row phase ahead (M <= GRID_PF_M).
+// otherwise each column loads its own.  The inverse rows' Fourier
+// coefficients of the next level are requested one row phase ahead:
+// straight to LDS (a 6 x 64 stage per wave that transforms inverse rows),
+// or for FP64 with SBS into registers (M <= GRID_PF_M).
 constexpr int GRID_PF_M = 48;
 template <class T, int MB, bool SBS>
 __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   constexpr int NSIB = 8 / MB;  // blocks per 8-column tile and latitude
   constexpr int NT = MB * 128;
   constexpr int I = 128;
-  constexpr bool PF = SBS && (sizeof(T) == 8 || MB <= 4);
+  // FP64 with SBS keeps the next level's coefficients in registers
+  // (measured faster there); the other variants stage them in LDS
+  constexpr bool REG = SBS && sizeof(T) == 8;
+  constexpr int NSW = REG ? 1 : (6 * MB + 7) / 8;  // waves holding inverse rows
   // basic-state values a thread loads per level: SBS, its share of the
   // block's nine rows; else the nine values of its column
   constexpr int NBS = SBS ? (9 * I + NT - 1) / NT : 9;
@@ -632,6 +669,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   __shared__ GridRow<T> rout[9][MB];    // tendency rows of one level
   __shared__ cx<T> twl[128];            // the FFT twiddles
   __shared__ T bsl[SBS ? 2 : 1][SBS ? 9 : 1][I];  // SBS: basic-state rows of two levels
+  __shared__ cx<T> stg[NSW][GRID_PF_M / 8][64];    // next level's coefficients, per wave
   const int J = a.J, L = a.L, E = a.E;
   // blockIdx -> unit u = (tile, group of JP latitudes) and within it the
   // latitude jj < JP and sibling s < NSIB: the JP x NSIB blocks of a unit
@@ -703,39 +741,56 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   // row phase: inverse rows (groups 0-5) of level il into its slot, when
   // il < L; forward rows (groups 6..) of the tendency rows of level fl
   // (fl < 0: the two surface rows)
-  // PF: this thread's coefficients of its inverse row at level il (group
-  // rq < 6), in flight until the row phase that transforms them
+  // REG: the six coefficients m = tt + 8 q of this thread's inverse row
   // (six named registers, not an array: an array carried around the level
   // loop went to scratch; the loads read coefficient min(m, M - 1))
   static_assert(GRID_PF_M == 48, "six coefficients per thread");
   cx<T> cf0, cf1, cf2, cf3, cf4, cf5;
-#define LBM_GRID_COEF_LOAD(il)                                                      \
-  if (PF && rlive && rq < 6 && (il) < L) {                                         \
-    const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + (il) * E + er, J, Ci);          \
-    auto ld = [&](int m) { return Fr[(size_t)(m < M ? m : M - 1) * msi]; };        \
-    cf0 = ld(tt);                                                                  \
-    cf1 = ld(tt + 8);                                                              \
-    cf2 = ld(tt + 16);                                                             \
-    cf3 = ld(tt + 24);                                                             \
-    cf4 = ld(tt + 32);                                                             \
-    cf5 = ld(tt + 40);                                                             \
-  }
+  // this thread's coefficients of its inverse row (group rq < 6) at level
+  // il: REG into cf0..cf5, else into its wave's stage, coefficient
+  // m = tt + 8 q < M at [q][lane]
+  auto coef_dma = [&](int il) {
+    if (REG && rlive && rq < 6 && il < L) {
+      const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci);
+      auto ld = [&](int m) { return Fr[(size_t)(m < M ? m : M - 1) * msi]; };
+      cf0 = ld(tt);
+      cf1 = ld(tt + 8);
+      cf2 = ld(tt + 16);
+      cf3 = ld(tt + 24);
+      cf4 = ld(tt + 32);
+      cf5 = ld(tt + 40);
+    }
+    if (!REG && rlive && rq < 6 && il < L) {
+      const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci) + (size_t)tt * msi;
+      const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll
+      for (int q = 0; q < GRID_PF_M / 8; ++q)
+        if (tt + 8 * q < M) lds_dma<T>(Fr + (size_t)(8 * q) * msi, stg[w][q]);
+    }
+  };
   auto row_phase = [&](int fl, int il) {
     if (!rlive) return;
     if (rq < 6) {
       if (il < L) {
+        // level il's coefficients landed in the stage before the barrier
+        // (wait_vm0 at the end of the column phase); the next level's are
+        // requested once they are read
         cx<T> *sc = rin[(il + 1) & 1][rq][rm];
-        if constexpr (PF) {
+        if constexpr (REG) {
           if (tt < M) sc[tt] = cf0;
           if (tt + 8 < M) sc[tt + 8] = cf1;
           if (tt + 16 < M) sc[tt + 16] = cf2;
           if (tt + 24 < M) sc[tt + 24] = cf3;
           if (tt + 32 < M) sc[tt + 32] = cf4;
           if (tt + 40 < M) sc[tt + 40] = cf5;
-          LBM_GRID_COEF_LOAD(il + 1)
-          grid_ifft_staged<T>(M, sc, tt, twl);
+          coef_dma(il + 1);
+          wave_lds_sync();
+          grid_ifft_from<T>([&](int k) { return sc[k]; }, M, sc, tt, twl, [] {});
         } else {
-          grid_ifft_row<T>(a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci), msi, M, sc, tt, twl);
+          const int w = __builtin_amdgcn_readfirstlane(t >> 6), rb = t & 56;
+          grid_ifft_from<T>([&](int k) { return lds_dma_get<T>(stg[w][k >> 3], rb + (k & 7)); },
+                            M, sc, tt, twl, [] {});
+          coef_dma(il + 1);
         }
       }
     } else {
@@ -763,8 +818,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
     const int col = (rq < 6 ? 6 * EL + g * E : g * EL) + er;
     grid_ifft_row<T>(a.fin + fidx(0, j, col, J, Ci), msi, M, rin[rq < 6 ? 0 : 1][g][rm], tt, twl);
   }
-  LBM_GRID_COEF_LOAD(1)
-#undef LBM_GRID_COEF_LOAD
+  coef_dma(1);
   __syncthreads();
 
   const VOff vo{L};
@@ -816,6 +870,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   WR(rout[0][m], gpi);
   WR(rout[1][m], pib);
   const T *tref = a.tref + (size_t)eb * L;
+  if constexpr (!REG) wait_vm0();  // the stage of level 1
   __syncthreads();
   row_phase(-1, 1);  // the surface tendency rows out, level 1 in
   __syncthreads();
@@ -871,6 +926,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
     for (int f = 0; f < 9; ++f) WR(rout[f][m], out[f]);
     for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
     if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
+    if constexpr (!REG) wait_vm0();  // the stage of level k + 2 (requested in the last row phase)
     __syncthreads();
     // level k's tendency rows out, level k + 2 in; meanwhile the basic
     // state of level k + 2 and this column's forcing of level k + 1 arrive
