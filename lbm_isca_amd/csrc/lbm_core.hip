@@ -2675,6 +2675,9 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->I = d->nlon;
   p->N = d->nmax;
   p->M = d->nmax + 1;
+  // built-in grid phase: k_grid where it applies (128 longitudes, nmax + 1
+  // <= GRID_PF_M), else the three launches
+  p->tune[4] = (p->I == 128 && p->M <= GRID_PF_M) ? 4 : 0;
   p->L = d->nlev;
   p->E = d->nmem;
   p->nb = d->nbasic;
