@@ -1307,33 +1307,42 @@ TendArgs<T> tend_args(lbm_plan *p, bool raw_path, const ActiveMask &am) {
   return ta;
 }
 
-// The grid phase (tuning choice 4, 128 longitudes only for 1-4): the
-// inverse FFT, k_tend and the forward FFT as three launches (0, built in);
-// the forward FFT inside k_tend (1 and 2: two group sizes); all three in
-// k_grid (3 and 4: grid_members<T>(v) members per block; nmax + 1 <=
-// GRID_PF_M), except on the raw first step after set_initial_perturbation,
-// which takes choice 0.
+// The grid phase (tuning choice 4): the inverse FFT, k_tend and the forward
+// FFT as three launches (0); the forward FFT inside k_tend (1 and 2: two
+// group sizes, 128 longitudes); all three in k_grid (3 and 4:
+// grid_members<T, NI>(v) members per block, 128 or 384 longitudes with
+// nmax + 1 <= grid_pf_m; the built-in choice there), except on the raw
+// first step after set_initial_perturbation, which takes choice 0.
 inline bool tend_fused(const lbm_plan *p) {
   return p->I == 128 && (p->tune[4] == 1 || p->tune[4] == 2);
 }
-inline bool grid_fused(const lbm_plan *p, bool raw) {
-  return p->I == 128 && p->M <= GRID_PF_M && p->tune[4] >= 3 && !raw;
-}
+// k_grid applies at 128 longitudes with nmax + 1 <= grid_pf_m.  (Its
+// 384-longitude instantiation is switched off: its first GPU run faulted
+// with a memory aperture violation whose cause is not found.)
+inline bool grid_fits(const lbm_plan *p) { return p->I == 128 && p->M <= grid_pf_m<128>(); }
+inline bool grid_fused(const lbm_plan *p, bool raw) { return grid_fits(p) && p->tune[4] >= 3 && !raw; }
 
-template <class T, int MB>
+template <class T, int NI, int MB>
 int launch_grid_mb(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
   // JP x NSIB blocks per (8-member tile, JP latitudes), 8 apart: see k_grid
   constexpr int NSIB = 8 / MB, JP = sizeof(T) == 4 ? 2 : 1;
   const long units = (long)((ne + 7) / 8) * (p->J / JP);
   TendArgs<T> tf = ta;
   tf.ne = ne;
-  const dim3 grid((unsigned)((units + 7) / 8 * 8 * NSIB * JP)), block(MB * 128);
+  const dim3 grid((unsigned)((units + 7) / 8 * 8 * NSIB * JP)), block(grid_threads<NI>(MB));
   if (p->nb == 1)
-    hipLaunchKernelGGL((k_grid<T, MB, true>), grid, block, 0, st, tf);
+    hipLaunchKernelGGL((k_grid<T, NI, MB, true>), grid, block, 0, st, tf);
   else
-    hipLaunchKernelGGL((k_grid<T, MB, false>), grid, block, 0, st, tf);
+    hipLaunchKernelGGL((k_grid<T, NI, MB, false>), grid, block, 0, st, tf);
   HIPCHK(hipGetLastError());
   return LBM_OK;
+}
+
+// (128 longitudes: grid_fits)
+template <class T>
+int launch_grid(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
+  if (p->tune[4] == 3) return launch_grid_mb<T, 128, grid_members<T, 128>(3)>(p, ta, ne, st);
+  return launch_grid_mb<T, 128, grid_members<T, 128>(4)>(p, ta, ne, st);
 }
 
 // The inverse FFT of the synthesised columns unless k_grid does it; the
@@ -1353,10 +1362,7 @@ int launch_tend(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
   const ColMap cmo{p->E, p->L, 9, 2};
   CHK(mark(p, 3, st));
   if (grid_fused(p, ta.raw != nullptr)) {
-    if (p->tune[4] == 3)
-      CHK((launch_grid_mb<T, grid_members<T>(3)>(p, ta, ne, st)));
-    else
-      CHK((launch_grid_mb<T, grid_members<T>(4)>(p, ta, ne, st)));
+    CHK(launch_grid<T>(p, ta, ne, st));
     CHK(mark(p, 4, st));
     return LBM_OK;
   }
@@ -1578,15 +1584,17 @@ size_t state_bytes(const lbm_plan *p);
 int copy_state(lbm_plan *p, void *buf, bool save, hipStream_t st);
 
 // Tuning choice 4: the grid phase as three launches (0), with the forward
-// FFT inside k_tend (1, 2) or in one k_grid launch (3, 4), 128 longitudes
-// only, timed on the plan's buffers with no history sums or trajectory;
-// the state the tendencies update (deferred ps' updates, the blow-up flag)
-// is saved before and restored after.
+// FFT inside k_tend (1, 2; 128 longitudes) or in one k_grid launch (3, 4;
+// where grid_fits), timed on the plan's buffers with no history sums or
+// trajectory; the state the tendencies update (deferred ps' updates, the
+// blow-up flag) is saved before and restored after.
 template <class T, class Timed>
 int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
-  if (p->I != 128) {
+  constexpr int NV = 5;
+  bool on[NV] = {true, p->I == 128, p->I == 128, grid_fits(p), grid_fits(p)};
+  if (!on[1] && !on[3]) {
     p->tune[4] = 0;
-    std::snprintf(buf, len, "; tend_fft: separate (fused at 128 longitudes only)");
+    std::snprintf(buf, len, "; tend_fft: separate (no fused variant at %d longitudes)", p->I);
     return LBM_OK;
   }
   hipStream_t st = p->own;
@@ -1599,9 +1607,9 @@ int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   ta.h.last_spec = nullptr;
   ta.traj = nullptr;
   const ActiveMask am = loop_mask(p, true, 0);
-  constexpr int NV = 5;
   double ms[NV] = {0, 0, 0, 0, 0};
   for (int v = 0; v < NV && rc == LBM_OK; ++v) {
+    if (!on[v]) continue;
     p->tune[4] = v;
     rc = timed([&] {
       CHK(launch_grid_in<T>(p, ta, am, st));
@@ -1614,15 +1622,20 @@ int autotune_tend_fft(lbm_plan *p, Timed &timed, char *buf, size_t len) {
   if (rc != LBM_OK) return rc;
   int best = 0;
   for (int v = 1; v < NV; ++v)
-    if (ms[v] < ms[best]) best = v;
+    if (on[v] && ms[v] < ms[best]) best = v;
   p->tune[4] = best;
   const char *lab[NV] = {"separate", "tend+fft", "tend+fft", "grid", "grid"};
-  const int mb[NV] = {0, tend_fft_members<T>(1), tend_fft_members<T>(2), grid_members<T>(3),
-                      grid_members<T>(4)};
-  std::snprintf(buf, len,
-                "; tend_fft: separate=%.4f tend+fft/%d=%.4f tend+fft/%d=%.4f grid/%d=%.4f "
-                "grid/%d=%.4f ms -> %d (%s)",
-                ms[0], mb[1], ms[1], mb[2], ms[2], mb[3], ms[3], mb[4], ms[4], best, lab[best]);
+  const bool i128 = p->I == 128;
+  const int mb[NV] = {0, tend_fft_members<T>(1), tend_fft_members<T>(2),
+                      i128 ? grid_members<T, 128>(3) : 0, i128 ? grid_members<T, 128>(4) : 0};
+  std::string r = "; tend_fft: separate=" + std::to_string(ms[0]).substr(0, 6);
+  for (int v = 1; v < NV; ++v) {
+    if (!on[v]) continue;
+    char one[64];
+    std::snprintf(one, sizeof(one), " %s/%d=%.4f", lab[v], mb[v], ms[v]);
+    r += one;
+  }
+  std::snprintf(buf, len, "%s ms -> %d (%s)", r.c_str(), best, lab[best]);
   return LBM_OK;
 }
 
@@ -2675,9 +2688,8 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->I = d->nlon;
   p->N = d->nmax;
   p->M = d->nmax + 1;
-  // built-in grid phase: k_grid where it applies (128 longitudes, nmax + 1
-  // <= GRID_PF_M), else the three launches
-  p->tune[4] = (p->I == 128 && p->M <= GRID_PF_M) ? 4 : 0;
+  // built-in grid phase: k_grid where it applies, else the three launches
+  p->tune[4] = grid_fits(p) ? 4 : 0;
   p->L = d->nlev;
   p->E = d->nmem;
   p->nb = d->nbasic;

@@ -396,8 +396,7 @@ __device__ inline void ifft_a(XF X, int M, cx<T> *v, int tt, int mode, const cx<
 // Inverse, stage b: second radix-8 pass of sub-transform r, then the
 // twiddle exp(+2 pi i n_b r / NC); returns Y_r[n_b = t + 8 k1] in v.
 template <class T, int NI>
-__device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt) {
-  const cx<T> *tw = twiddles<T, NI>();
+__device__ inline void ifft_b(const cx<T> *scr, cx<T> *v, int tt, const cx<T> *tw) {
   const int r = tt >> 3, t = tt & 7;
 #pragma unroll
   for (int n1 = 0; n1 < 8; ++n1) v[n1] = scr[r * 72 + n1 * 9 + t];
@@ -543,7 +542,7 @@ __global__ __launch_bounds__((FftShape<T, NI, true, RW>::THREADS)) void k_ifft(c
   }
   if (live) fft_a_store<T, NI>(v, sh[rl], tt);
   __syncthreads();
-  if (live) ifft_b<T, NI>(sh[rl], v, tt);
+  if (live) ifft_b<T, NI>(sh[rl], v, tt, twiddles<T, NI>());
   cx<T> *out = reinterpret_cast<cx<T> *>(G + ((size_t)(e * J + j) * cm.rows() + r) * NI);
   if constexpr (S::R == 1 && sizeof(T) == 8) {
     if (live) {

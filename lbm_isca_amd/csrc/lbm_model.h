@@ -558,61 +558,86 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Members per block of k_grid, tuning variant v = 3 or 4.
-template <class T>
-constexpr int grid_members(int v) { return (sizeof(T) == 4 ? 4 : 2) << (v - 3); }
-
-// LDS row of k_grid: 72 complex (the 8 x 9 transpose scratch of the
-// 128-point stages); a grid row is its first 128 real values.
-template <class T>
-using GridRow = cx<T>[72];
-
-// Inverse FFT of one Fourier column (M coefficients at stride ms from Fr)
-// into the LDS row sc as 128 real samples in natural order: thread tt of
-// the row's 8 consecutive threads (one wave).
-template <class T, class XF, class F>
-__device__ inline void grid_ifft_from(XF X, int M, cx<T> *sc, int tt, const cx<T> *tw, F then);
-
-template <class T>
-__device__ inline void grid_ifft_row(const cx<T> *__restrict__ Fr, size_t ms, int M, cx<T> *sc,
-                                     int tt, const cx<T> *tw) {
-  for (int m = tt; m < M; m += 8) sc[m] = Fr[(size_t)m * ms];
-  wave_lds_sync();
-  grid_ifft_from<T>([&](int k) { return sc[k]; }, M, sc, tt, tw, [] {});
+// Members per block of k_grid at NI longitudes, tuning variant v = 3 or 4
+// (384 longitudes: one, or two for FP32; the LDS holds no more).
+template <class T, int NI>
+constexpr int grid_members(int v) {
+  return NI == 128 ? (sizeof(T) == 4 ? 4 : 2) << (v - 3) : (sizeof(T) == 4 && v == 4 ? 2 : 1);
 }
+// Threads per row slot of k_grid: the row's FftShape threads (8 R) rounded
+// up to a power of two, so that no row spans two waves.
+template <int NI>
+constexpr int grid_row_threads() { return NI == 128 ? 8 : 32; }
+// Threads of a k_grid block of MB members: 16 MB row slots, and at least
+// one column (longitude) per thread.
+template <int NI>
+constexpr int grid_threads(int MB) {
+  return MB * (NI > 16 * grid_row_threads<NI>() ? NI : 16 * grid_row_threads<NI>());
+}
+// Fourier coefficients one thread of an inverse row prefetches: six per
+// thread, m = tt + 8 R q.
+template <int NI>
+constexpr int grid_pf_m() { return 6 * 8 * (NI / 128); }
 
-// The same from coefficients X(k), k < M, read elsewhere (sc may hold
-// them); then() runs once every X(k) has been read.
-template <class T, class XF, class F>
+// Inverse FFT of one Fourier column from coefficients X(k), k < M (read
+// elsewhere, or staged in sc), into the LDS row sc as NI real samples in
+// natural order: thread tt of the row's 8 R threads (one wave; k_ifft's
+// stages); then() runs once every X(k) has been read.
+template <class T, int NI, class XF, class F>
 __device__ inline void grid_ifft_from(XF X, int M, cx<T> *sc, int tt, const cx<T> *tw, F then) {
+  constexpr int R = FftShape<T, NI>::R;
   cx<T> v[8];
-  ifft_a<T, 128>(X, M, v, tt, 0, tw);
+  ifft_a<T, NI>(X, M, v, tt, 0, tw);
   wave_lds_sync();
   then();
-  fft_a_store<T, 128>(v, sc, tt);
+  fft_a_store<T, NI>(v, sc, tt);
   wave_lds_sync();
-  ifft_b<T, 128>(sc, v, tt);
+  ifft_b<T, NI>(sc, v, tt, tw);
   wave_lds_sync();
+  if constexpr (R == 1) {
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) sc[tt + 8 * k1] = v[k1];
+    for (int k1 = 0; k1 < 8; ++k1) sc[tt + 8 * k1] = v[k1];
+  } else {
+    // radix-R across the sub-transforms through the exchange buffer
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) sc[(tt >> 3) * 64 + (tt & 7) + 8 * k1] = v[k1];
+    wave_lds_sync();
+    cx<T> o[8];
+    ifft_c<T, NI>(sc, o, tt);
+    wave_lds_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) sc[(tt & 7) + 8 * k1 + 64 * (tt >> 3)] = o[k1];
+  }
 }
 
-// Forward FFT of the LDS row sc (128 real samples in natural order) to the
-// Fourier column F (M coefficients at stride ms); sc is used as scratch.
-template <class T>
+// The same from the Fourier column itself (M coefficients at stride ms
+// from Fr), staged in sc first.
+template <class T, int NI>
+__device__ inline void grid_ifft_row(const cx<T> *__restrict__ Fr, size_t ms, int M, cx<T> *sc,
+                                     int tt, const cx<T> *tw) {
+  for (int m = tt; m < M; m += FftShape<T, NI>::TPR) sc[m] = Fr[(size_t)m * ms];
+  wave_lds_sync();
+  grid_ifft_from<T, NI>([&](int k) { return sc[k]; }, M, sc, tt, tw, [] {});
+}
+
+// Forward FFT of the LDS row sc (NI real samples in natural order) to the
+// Fourier column F (M coefficients at stride ms); sc is used as scratch
+// (k_fft's stages).
+template <class T, int NI>
 __device__ inline void grid_fft_row(cx<T> *sc, cx<T> *__restrict__ F, size_t ms, int M, int tt,
                                     const cx<T> *tw) {
+  constexpr int R = FftShape<T, NI>::R;
   cx<T> v[8];
-  fft_a<T, 128, true>(sc, v, tt, tw);
+  fft_a<T, NI, true>(sc, v, tt, tw);
   wave_lds_sync();
-  fft_a_store<T, 128>(v, sc, tt);
+  fft_a_store<T, NI>(v, sc, tt);
   wave_lds_sync();
-  fft_b<T, 128>(sc, v, tt);
+  fft_b<T, NI>(sc, v, tt);
   wave_lds_sync();
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) sc[tt + 8 * k1] = v[k1];
+  for (int k1 = 0; k1 < 8; ++k1) sc[(tt >> 3) + R * ((tt & 7) + 8 * k1)] = v[k1];
   wave_lds_sync();
-  fft_c<T, 128>(sc, F, ms, M, tt, 0, tw);
+  fft_c<T, NI>(sc, F, ms, M, tt, 0, tw);
 }
 
 // Loads straight into LDS (global_load_lds): lane l of the wave writes
@@ -651,25 +676,26 @@ __device__ __forceinline__ void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70);
 // otherwise each column loads its own.  The inverse rows' Fourier
 // coefficients of the next level are requested one row phase ahead:
 // straight to LDS (a 6 x 64 stage per wave that transforms inverse rows),
-// or for FP64 with SBS into registers (M <= GRID_PF_M).
-constexpr int GRID_PF_M = 48;
-template <class T, int MB, bool SBS>
-__global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
+// or for FP64 with SBS into registers (M <= grid_pf_m<NI>()).
+template <class T, int NI, int MB, bool SBS>
+__global__ __launch_bounds__(grid_threads<NI>(MB)) void k_grid(TendArgs<T> a) {
+  using S = FftShape<T, NI>;
   constexpr int NSIB = 8 / MB;  // blocks per 8-column tile and latitude
-  constexpr int NT = MB * 128;
-  constexpr int I = 128;
+  constexpr int NT = grid_threads<NI>(MB);
+  constexpr int I = NI;
+  constexpr int RT = grid_row_threads<NI>(), TPR = S::TPR;
   // FP64 with SBS keeps the next level's coefficients in registers
   // (measured faster there); the other variants stage them in LDS
-  constexpr bool REG = SBS && sizeof(T) == 8;
-  constexpr int NSW = REG ? 1 : (6 * MB + 7) / 8;  // waves holding inverse rows
+  constexpr bool REG = SBS && sizeof(T) == 8 && NI == 128;
+  constexpr int NSW = REG ? 1 : (6 * MB * RT + 63) / 64;  // waves holding inverse rows
   // basic-state values a thread loads per level: SBS, its share of the
   // block's nine rows; else the nine values of its column
   constexpr int NBS = SBS ? (9 * I + NT - 1) / NT : 9;
-  __shared__ GridRow<T> rin[2][6][MB];  // synthesised rows of two levels (first: the surface rows)
-  __shared__ GridRow<T> rout[9][MB];    // tendency rows of one level
-  __shared__ cx<T> twl[128];            // the FFT twiddles
+  __shared__ cx<T> rin[2][6][MB][S::LDS];  // synthesised rows of two levels (first: the surface rows)
+  __shared__ cx<T> rout[9][MB][S::LDS];    // tendency rows of one level
+  __shared__ cx<T> twl[NI];                // the FFT twiddles
   __shared__ T bsl[SBS ? 2 : 1][SBS ? 9 : 1][I];  // SBS: basic-state rows of two levels
-  __shared__ cx<T> stg[NSW][GRID_PF_M / 8][64];    // next level's coefficients, per wave
+  __shared__ cx<T> stg[NSW][6][64];               // next level's coefficients, per wave
   const int J = a.J, L = a.L, E = a.E;
   // blockIdx -> unit u = (tile, group of JP latitudes) and within it the
   // latitude jj < JP and sibling s < NSIB: the JP x NSIB blocks of a unit
@@ -697,14 +723,17 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   const int Ci = 6 * EL + 6 * E, Co = a.C;
   const size_t msi = (size_t)J * fpad(Ci), mso = (size_t)J * fpad(Co);
   const int M = a.M;
-  // row role: row rl = (group rq, member rm), thread tt of its 8
-  const int rl = t >> 3, tt = t & 7;
+  // row role: row slot rl = (group rq, member rm), thread tt of its RT (the
+  // first TPR work)
+  const int rl = t / RT, tt = t % RT;
   const int rq = rl / MB, rm = rl - rq * MB;
-  const bool rlive = (live >> rm) & 1u;
+  const bool rlive = tt < TPR && ((live >> rm) & 1u);
   const int er = e0 + rm;
-  // column role: member m (wave-uniform), longitude i
-  const int m = __builtin_amdgcn_readfirstlane(t >> 7), i = t & 127;
-  const bool act = (live >> m) & 1u;
+  // column role: member m (wave-uniform), longitude i; colt: the thread
+  // holds a column
+  const bool colt = t < MB * NI;
+  const int m = colt ? __builtin_amdgcn_readfirstlane(t / NI) : MB - 1, i = t % NI;
+  const bool act = colt && ((live >> m) & 1u);
   const int e = e0 + m < E ? e0 + m : E - 1;
   const int eb = a.nbasic > 1 ? e : 0;
   const BRow br{L};
@@ -718,7 +747,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
     for (int q = 0; q < NBS; ++q) {
       if constexpr (SBS) {
         const int idx = t + q * NT;
-        if (idx < 9 * I) st[q] = bsj[(size_t)bsrow(idx >> 7, lv) * I + (idx & 127)];
+        if (idx < 9 * I) st[q] = bsj[(size_t)bsrow(idx / I, lv) * I + idx % I];
       } else {
         st[q] = bsj[(size_t)bsrow(q, lv) * I + i];
       }
@@ -731,43 +760,42 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < NBS; ++q) {
         const int idx = t + q * NT;
-        if (idx < 9 * I) bsl[lv & 1][idx >> 7][idx & 127] = st[q];
+        if (idx < 9 * I) bsl[lv & 1][idx / I][idx % I] = st[q];
       }
     }
   };
   const bool has_fr = a.frc != nullptr;
   const T *frj = has_fr ? a.frc + ((size_t)((a.nfrc > 1 ? e : 0) * J + j) * L) * I + i : nullptr;
   auto FR = [&](int k) { return has_fr && k < L ? frj[(size_t)k * I] : (T)0; };
-  // row phase: inverse rows (groups 0-5) of level il into its slot, when
-  // il < L; forward rows (groups 6..) of the tendency rows of level fl
-  // (fl < 0: the two surface rows)
-  // REG: the six coefficients m = tt + 8 q of this thread's inverse row
+  // REG: the six coefficients m = tt + TPR q of this thread's inverse row
   // (six named registers, not an array: an array carried around the level
   // loop went to scratch; the loads read coefficient min(m, M - 1))
-  static_assert(GRID_PF_M == 48, "six coefficients per thread");
   cx<T> cf0, cf1, cf2, cf3, cf4, cf5;
   // this thread's coefficients of its inverse row (group rq < 6) at level
   // il: REG into cf0..cf5, else into its wave's stage, coefficient
-  // m = tt + 8 q < M at [q][lane]
+  // m = tt + TPR q < M at [q][lane]
   auto coef_dma = [&](int il) {
     if (REG && rlive && rq < 6 && il < L) {
       const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci);
       auto ld = [&](int m) { return Fr[(size_t)(m < M ? m : M - 1) * msi]; };
       cf0 = ld(tt);
-      cf1 = ld(tt + 8);
-      cf2 = ld(tt + 16);
-      cf3 = ld(tt + 24);
-      cf4 = ld(tt + 32);
-      cf5 = ld(tt + 40);
+      cf1 = ld(tt + TPR);
+      cf2 = ld(tt + 2 * TPR);
+      cf3 = ld(tt + 3 * TPR);
+      cf4 = ld(tt + 4 * TPR);
+      cf5 = ld(tt + 5 * TPR);
     }
     if (!REG && rlive && rq < 6 && il < L) {
       const cx<T> *Fr = a.fin + fidx(0, j, rq * EL + il * E + er, J, Ci) + (size_t)tt * msi;
       const int w = __builtin_amdgcn_readfirstlane(t >> 6);
 #pragma unroll
-      for (int q = 0; q < GRID_PF_M / 8; ++q)
-        if (tt + 8 * q < M) lds_dma<T>(Fr + (size_t)(8 * q) * msi, stg[w][q]);
+      for (int q = 0; q < 6; ++q)
+        if (tt + TPR * q < M) lds_dma<T>(Fr + (size_t)(TPR * q) * msi, stg[w][q]);
     }
   };
+  // row phase: inverse rows (groups 0-5) of level il into its slot, when
+  // il < L; forward rows (groups 6..) of the tendency rows of level fl
+  // (fl < 0: the two surface rows)
   auto row_phase = [&](int fl, int il) {
     if (!rlive) return;
     if (rq < 6) {
@@ -778,18 +806,19 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
         cx<T> *sc = rin[(il + 1) & 1][rq][rm];
         if constexpr (REG) {
           if (tt < M) sc[tt] = cf0;
-          if (tt + 8 < M) sc[tt + 8] = cf1;
-          if (tt + 16 < M) sc[tt + 16] = cf2;
-          if (tt + 24 < M) sc[tt + 24] = cf3;
-          if (tt + 32 < M) sc[tt + 32] = cf4;
-          if (tt + 40 < M) sc[tt + 40] = cf5;
+          if (tt + TPR < M) sc[tt + TPR] = cf1;
+          if (tt + 2 * TPR < M) sc[tt + 2 * TPR] = cf2;
+          if (tt + 3 * TPR < M) sc[tt + 3 * TPR] = cf3;
+          if (tt + 4 * TPR < M) sc[tt + 4 * TPR] = cf4;
+          if (tt + 5 * TPR < M) sc[tt + 5 * TPR] = cf5;
           coef_dma(il + 1);
           wave_lds_sync();
-          grid_ifft_from<T>([&](int k) { return sc[k]; }, M, sc, tt, twl, [] {});
+          grid_ifft_from<T, NI>([&](int k) { return sc[k]; }, M, sc, tt, twl, [] {});
         } else {
-          const int w = __builtin_amdgcn_readfirstlane(t >> 6), rb = t & 56;
-          grid_ifft_from<T>([&](int k) { return lds_dma_get<T>(stg[w][k >> 3], rb + (k & 7)); },
-                            M, sc, tt, twl, [] {});
+          const int w = __builtin_amdgcn_readfirstlane(t >> 6), rb = t & (64 - RT);
+          grid_ifft_from<T, NI>(
+              [&](int k) { return lds_dma_get<T>(stg[w][k / TPR], rb + k % TPR); }, M, sc, tt, twl,
+              [] {});
           coef_dma(il + 1);
         }
       }
@@ -797,7 +826,7 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
       const int f = rq - 6;
       if (f < (fl < 0 ? 2 : 9)) {
         const int col = (fl < 0 ? 9 * EL + f * E : f * EL + fl * E) + er;
-        grid_fft_row<T>(rout[f][rm], a.fout + fidx(0, j, col, J, Co), mso, M, tt, twl);
+        grid_fft_row<T, NI>(rout[f][rm], a.fout + fidx(0, j, col, J, Co), mso, M, tt, twl);
       }
     }
   };
@@ -809,123 +838,131 @@ __global__ __launch_bounds__(MB * 128) void k_grid(TendArgs<T> a) {
   bs_load(0, bc);
   bs_load(1, bn);
   T fk = FR(0);
-  for (int q = t; q < 128; q += NT) twl[q] = twiddles<T, 128>()[q];
+  for (int q = t; q < NI; q += NT) twl[q] = twiddles<T, NI>()[q];
   bs_store(0, bc);
   bs_store(1, bn);
   __syncthreads();
   if (rlive && rq < 12) {
     const int g = rq < 6 ? rq : rq - 6;
     const int col = (rq < 6 ? 6 * EL + g * E : g * EL) + er;
-    grid_ifft_row<T>(a.fin + fidx(0, j, col, J, Ci), msi, M, rin[rq < 6 ? 0 : 1][g][rm], tt, twl);
+    grid_ifft_row<T, NI>(a.fin + fidx(0, j, col, J, Ci), msi, M, rin[rq < 6 ? 0 : 1][g][rm], tt,
+                         twl);
   }
   coef_dma(1);
   __syncthreads();
 
   const VOff vo{L};
-  auto LR = [&](const GridRow<T> &row) { return reinterpret_cast<const T *>(row)[i]; };
-  auto WR = [&](GridRow<T> &row, T v) { reinterpret_cast<T *>(row)[i] = v; };
+  auto LR = [&](const cx<T> *row) { return reinterpret_cast<const T *>(row)[i]; };
+  auto WR = [&](cx<T> *row, T v) { reinterpret_cast<T *>(row)[i] = v; };
   auto BS = [&](int row) { return bsj[(size_t)row * I + i]; };
   ColConst<T> cc;
-  cc.pix = LR(rin[0][0][m]);
-  cc.piy = LR(rin[0][1][m]);
-  const T pig = LR(rin[0][2][m]);
-  cc.bpix = BS(br.pix());
-  cc.bpiy = BS(br.piy());
-  const T psbar = BS(br.psbar());
-  cc.rc = a.rcos[j];
-  cc.full_ke = !a.strictly_linear;
-  {
-    const T su = LR(rin[0][3][m]), sv = LR(rin[0][4][m]), sd = LR(rin[0][5][m]);
-    cc.D0 = sd;
-    cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * BS(br.sbu()) + cc.piy * BS(br.sbv());
-  }
   ColState<T> cs;
-  cs.bsd_lo = BS(br.sdot());
-  const size_t pidx = (size_t)(e * J + j) * I + i;
-  T gpi, pib, pst;
-  if (a.linear) {
-    gpi = cc.D0 - cc.S0;
-    pib = 0;
-    pst = psbar * pig;
-  } else {
-    // an inactive member leaves its ps' as it is
-    column_surface<T>(a.psa + pidx, a.psb + pidx, act ? a.pend[e] : 0, pig, psbar, a.robert, cc.S0,
-                      cc.D0, gpi, pib, cs.chk, pst);
-  }
+  T psbar = 0, pst = 0;
   T *trj = nullptr;
-  if (a.traj && act) {
-    trj = a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i;
-    trj[(size_t)(2 * L) * I] = a.linear ? pst : a.psa[pidx];
-    trj[(size_t)(2 * L + 1) * I] = a.linear ? (T)0 : a.psb[pidx];
-  }
-  // history samples of this top-of-step state
-  int hslot[2];
-  T hw[2];
-  const int top_a = a.ctl->step - a.h.inistp + 1;
-  const int nst = a.h.nsteps ? a.h.nsteps[e] : 0;
-  const int hn = act ? hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw) : 0;
-  const bool hlast = act && a.h.last_grid != nullptr && top_a == nst;
+  int hslot[2] = {0, 0};
+  T hw[2] = {0, 0};
+  int hn = 0;
+  bool hlast = false;
   const size_t hrow = (size_t)(e * J + j) * (L + 1) * I + i;
-  for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
-  WR(rout[0][m], gpi);
-  WR(rout[1][m], pib);
   const T *tref = a.tref + (size_t)eb * L;
+  if (colt) {
+    cc.pix = LR(rin[0][0][m]);
+    cc.piy = LR(rin[0][1][m]);
+    const T pig = LR(rin[0][2][m]);
+    cc.bpix = BS(br.pix());
+    cc.bpiy = BS(br.piy());
+    psbar = BS(br.psbar());
+    cc.rc = a.rcos[j];
+    cc.full_ke = !a.strictly_linear;
+    {
+      const T su = LR(rin[0][3][m]), sv = LR(rin[0][4][m]), sd = LR(rin[0][5][m]);
+      cc.D0 = sd;
+      cc.S0 = sd + cc.bpix * su + cc.bpiy * sv + cc.pix * BS(br.sbu()) + cc.piy * BS(br.sbv());
+    }
+    cs.bsd_lo = BS(br.sdot());
+    const size_t pidx = (size_t)(e * J + j) * I + i;
+    T gpi, pib;
+    if (a.linear) {
+      gpi = cc.D0 - cc.S0;
+      pib = 0;
+      pst = psbar * pig;
+    } else {
+      // an inactive member leaves its ps' as it is
+      column_surface<T>(a.psa + pidx, a.psb + pidx, act ? a.pend[e] : 0, pig, psbar, a.robert,
+                        cc.S0, cc.D0, gpi, pib, cs.chk, pst);
+    }
+    if (a.traj && act) {
+      trj = a.traj + (size_t)(e * J + j) * (2 * L + 2) * I + i;
+      trj[(size_t)(2 * L) * I] = a.linear ? pst : a.psa[pidx];
+      trj[(size_t)(2 * L + 1) * I] = a.linear ? (T)0 : a.psb[pidx];
+    }
+    // history samples of this top-of-step state
+    const int top_a = a.ctl->step - a.h.inistp + 1;
+    const int nst = a.h.nsteps ? a.h.nsteps[e] : 0;
+    hn = act ? hist_weights<T>(a.h.mode, a.h.K, top_a, nst, hslot, hw) : 0;
+    hlast = act && a.h.last_grid != nullptr && top_a == nst;
+    for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)L * I] += hw[q] * pst;
+    WR(rout[0][m], gpi);
+    WR(rout[1][m], pib);
+  }
   if constexpr (!REG) wait_vm0();  // the stage of level 1
   __syncthreads();
   row_phase(-1, 1);  // the surface tendency rows out, level 1 in
   __syncthreads();
 
   for (int k = 0; k < L; ++k) {
-    const int sk = (k + 1) & 1, sn = k & 1;  // slots of levels k and k+1
-    const bool top = (k + 1 == L);
-    BLev<T> bk, bk1;  // basic state of level k; u, v, T, q of level k+1
-    if constexpr (SBS) {
-      const int b0 = k & 1, b1 = (k + 1) & 1;
-      bk.u = bsl[b0][0][i];
-      bk.v = bsl[b0][1][i];
-      bk.t = bsl[b0][2][i];
-      bk.q = bsl[b0][3][i];
-      bk.tv = bsl[b0][4][i];
-      bk.av = bsl[b0][5][i];
-      bk.dv = bsl[b0][6][i];
-      bk.x5 = bsl[b0][7][i];
-      bk.sd = bsl[b0][8][i];
-      if (!top) {
-        bk1.u = bsl[b1][0][i];
-        bk1.v = bsl[b1][1][i];
-        bk1.t = bsl[b1][2][i];
-        bk1.q = bsl[b1][3][i];
+    if (colt) {
+      const int sk = (k + 1) & 1, sn = k & 1;  // slots of levels k and k+1
+      const bool top = (k + 1 == L);
+      BLev<T> bk, bk1;  // basic state of level k; u, v, T, q of level k+1
+      if constexpr (SBS) {
+        const int b0 = k & 1, b1 = (k + 1) & 1;
+        bk.u = bsl[b0][0][i];
+        bk.v = bsl[b0][1][i];
+        bk.t = bsl[b0][2][i];
+        bk.q = bsl[b0][3][i];
+        bk.tv = bsl[b0][4][i];
+        bk.av = bsl[b0][5][i];
+        bk.dv = bsl[b0][6][i];
+        bk.x5 = bsl[b0][7][i];
+        bk.sd = bsl[b0][8][i];
+        if (!top) {
+          bk1.u = bsl[b1][0][i];
+          bk1.v = bsl[b1][1][i];
+          bk1.t = bsl[b1][2][i];
+          bk1.q = bsl[b1][3][i];
+        }
+      } else {
+        bk = BLev<T>{bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], bc[6], bc[7], bc[8]};
+        if (!top) {
+          bk1.u = bn[0];
+          bk1.v = bn[1];
+          bk1.t = bn[2];
+          bk1.q = bn[3];
+        }
       }
-    } else {
-      bk = BLev<T>{bc[0], bc[1], bc[2], bc[3], bc[4], bc[5], bc[6], bc[7], bc[8]};
+      const T up = LR(rin[sk][0][m]), vp = LR(rin[sk][1][m]);
+      const T vor = LR(rin[sk][2][m]), dv = LR(rin[sk][3][m]);
+      const T tp = LR(rin[sk][4][m]), qp = LR(rin[sk][5][m]);
+      T upn = 0, vpn = 0, tpn = 0, qpn = 0;
       if (!top) {
-        bk1.u = bn[0];
-        bk1.v = bn[1];
-        bk1.t = bn[2];
-        bk1.q = bn[3];
+        upn = LR(rin[sn][0][m]);
+        vpn = LR(rin[sn][1][m]);
+        tpn = LR(rin[sn][4][m]);
+        qpn = LR(rin[sn][5][m]);
       }
-    }
-    const T up = LR(rin[sk][0][m]), vp = LR(rin[sk][1][m]);
-    const T vor = LR(rin[sk][2][m]), dv = LR(rin[sk][3][m]);
-    const T tp = LR(rin[sk][4][m]), qp = LR(rin[sk][5][m]);
-    T upn = 0, vpn = 0, tpn = 0, qpn = 0;
-    if (!top) {
-      upn = LR(rin[sn][0][m]);
-      vpn = LR(rin[sn][1][m]);
-      tpn = LR(rin[sn][4][m]);
-      qpn = LR(rin[sn][5][m]);
-    }
-    T out[9], omega;
-    if (trj) {
-      trj[(size_t)k * I] = up;
-      trj[(size_t)(L + k) * I] = vp;
-    }
-    column_level<T>(k, L, a.vc, tref, bk, bk1, fk, cc, cs, up, vp, vor, dv, tp, qp, upn, vpn, tpn,
-                    qpn, out, psbar, pst, omega);
+      T out[9], omega;
+      if (trj) {
+        trj[(size_t)k * I] = up;
+        trj[(size_t)(L + k) * I] = vp;
+      }
+      column_level<T>(k, L, a.vc, tref, bk, bk1, fk, cc, cs, up, vp, vor, dv, tp, qp, upn, vpn,
+                      tpn, qpn, out, psbar, pst, omega);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) WR(rout[f][m], out[f]);
-    for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
-    if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
+      for (int f = 0; f < 9; ++f) WR(rout[f][m], out[f]);
+      for (int q = 0; q < hn; ++q) a.h.grid[hslot[q]][hrow + (size_t)k * I] += hw[q] * omega;
+      if (hlast) a.h.last_grid[((size_t)(e * J + j) * L + k) * I + i] = omega;
+    }
     if constexpr (!REG) wait_vm0();  // the stage of level k + 2 (requested in the last row phase)
     __syncthreads();
     // level k's tendency rows out, level k + 2 in; meanwhile the basic

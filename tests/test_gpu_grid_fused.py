@@ -135,3 +135,34 @@ def test_adjoint_of_the_default_model(env):
         assert np.array_equal(J, J0), choice
         for a, b in zip(gx + [gf], gx0 + [gf0]):
             assert np.array_equal(a, b), choice
+
+
+# Skipped: the 384-longitude k_grid (now switched off in grid_fits) faulted
+# the GPU on its first run here (FP64, memory aperture violation), cause not
+# found; T127 takes the three-launch grid phase.
+@pytest.mark.skip(reason="384-longitude k_grid switched off after an unexplained GPU fault")
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_t127l40_members(precision):
+    """384 longitudes (T127 L40, 3 members on their own balanced jets, a
+    vorticity and temperature perturbation): k_grid with one or two members
+    per block against the three launches, bit for bit."""
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases
+    E = 3
+    sp = L.BatchedSpectral(384, 192, 127)
+    vg = L.VerticalGrid(cases.EQUAL_SIGM_L40)
+    bs, _ = cases.jet_states(sp, vg.sig, E, seed=11)
+    m, _, _ = L.make_ensemble_model(bs, E, L.Config(**cases.t127l40_config(precision=precision)),
+                                    sigm=cases.EQUAL_SIGM_L40, per_member_basic=True, spectral=sp)
+    vor = cases.vorticity_perturbation(sp, vg.kmax, E, seed=4)
+    m.plan.set_lengths([6, 4, 6])
+    out = {}
+    for choice in (0, 3, 4):
+        m.plan.set_tuning([0, 0, 0, 0, choice])
+        m.plan.set_spectral_state(vor=vor, t=1.0e3 * vor)
+        m.plan.run(0)
+        out[choice] = [x.cpu().numpy() for x in m.plan.get_state()]
+    assert np.abs(out[0][2]).max() > 0
+    for choice in (3, 4):
+        for a, b in zip(out[0], out[choice]):
+            assert np.array_equal(a, b), (precision, choice)
