@@ -731,7 +731,7 @@ __global__ __launch_bounds__(grid_threads<NI>(MB)) void k_grid(TendArgs<T> a) {
   const int er = e0 + rm;
   // column role: member m (wave-uniform), longitude i; colt: the thread
   // holds a column
-  const bool colt = t < MB * NI;
+  const bool colt = NT == MB * NI || t < MB * NI;
   const int m = colt ? __builtin_amdgcn_readfirstlane(t / NI) : MB - 1, i = t % NI;
   const bool act = colt && ((live >> m) & 1u);
   const int e = e0 + m < E ? e0 + m : E - 1;
