@@ -344,9 +344,9 @@ int lbm_run_steps(lbm_plan *plan, int nsteps, int count, int use_graph, int chec
  * the grid phase (inverse FFT, tendencies, forward FFT) as three launches
  * (0), with the forward FFT inside the tendency kernel with 8 or 4 (FP32),
  * 4 or 2 (FP64) members per block (1, 2), or in one kernel with 4 or 8
- * (FP32), 2 or 4 (FP64) members per block (3, 4; the built-in choice
- * where it applies: 128 longitudes and nmax + 1 <= 48; the raw first step
- * after lbm_set_state takes 0), timed on the plan's own
+ * (FP32), 2 or 4 (FP64) members per block (3, 4; where it applies: 128
+ * longitudes and nmax + 1 <= 48; the raw first step after lbm_set_state
+ * takes 0), timed on the plan's own
  * buffers and shapes (best of three trials of reps launches; the state the
  * tendency kernel touches is saved and restored) after
  * lbm_set_basic_state; the fastest of each is kept and the graphs are
@@ -358,8 +358,10 @@ int lbm_autotune(lbm_plan *plan, int reps, char *report, int64_t len);
 /* Apply (lbm_set_tuning, n <= 5; choices past n are 0) or read
  * (lbm_get_tuning, n >= 5) the variant index of each tuned launch in the
  * order above (0..2; 0..4 for the grid phase), e.g. to reuse a cached
- * autotuner result; a new plan holds the built-in choices (0, and 4 for
- * the grid phase where k_grid applies); lbm_get_tuning returns the count. */
+ * autotuner result; a new plan holds the built-in choices (0; for the
+ * grid phase where k_grid applies, 4 from 5 members (FP64) or 17 (FP32,
+ * 3 from 9), else 0: with fewer members k_grid leaves the GPU part
+ * empty); lbm_get_tuning returns the count. */
 int lbm_set_tuning(lbm_plan *plan, const int32_t *h_choice, int n);
 int lbm_get_tuning(const lbm_plan *plan, int32_t *h_choice, int n);
 

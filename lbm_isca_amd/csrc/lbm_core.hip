@@ -1321,6 +1321,20 @@ inline bool tend_fused(const lbm_plan *p) {
 // with a memory aperture violation whose cause is not found.)
 inline bool grid_fits(const lbm_plan *p) { return p->I == 128 && p->M <= grid_pf_m<128>(); }
 inline bool grid_fused(const lbm_plan *p, bool raw) { return grid_fits(p) && p->tune[4] >= 3 && !raw; }
+// Built-in grid phase of a plan of E members: k_grid where it applies and
+// the ensemble gives it enough blocks, else the three launches.  k_grid
+// runs one block per latitude and member group, each through every level
+// in turn, so with a few member groups per latitude the GPU is not filled
+// and the three launches finish first.  Measured at T42L20 (ms per step,
+// three launches / k_grid): FP64 1 member 0.174 / 0.217, 4: 0.192 / 0.216,
+// 8: 0.257 / 0.238 (4 per block); FP32 4 members 0.160 / 0.177 (4 per
+// block), 8: 0.189 / 0.189, 16: 0.261 / 0.223 (4 per block) and 0.247 (8
+// per block), 24: 0.318 / 0.306 (4) and 0.265 (8).
+inline int builtin_grid_choice(const lbm_plan *p) {
+  if (!grid_fits(p)) return 0;
+  if (p->prec == LBM_FP64) return p->E > 4 ? 4 : 0;
+  return p->E > 16 ? 4 : p->E > 8 ? 3 : 0;
+}
 
 template <class T, int NI, int MB>
 int launch_grid_mb(lbm_plan *p, const TendArgs<T> &ta, int ne, hipStream_t st) {
@@ -2688,12 +2702,11 @@ int lbm_plan_create(const lbm_desc *d, lbm_plan **out) {
   p->I = d->nlon;
   p->N = d->nmax;
   p->M = d->nmax + 1;
-  // built-in grid phase: k_grid where it applies, else the three launches
-  p->tune[4] = grid_fits(p) ? 4 : 0;
   p->L = d->nlev;
   p->E = d->nmem;
   p->nb = d->nbasic;
   p->prec = d->precision;
+  p->tune[4] = builtin_grid_choice(p);
   p->split = d->precision == LBM_BF16X3;
   p->ffac = ffac;
   p->nspec = p->M * (p->M + 1) / 2;
