@@ -1240,21 +1240,44 @@ StepArgs<T> step_args(lbm_plan *p, int ecnt) {
 
 // 20 and 40 levels run step kernels specialised for them; other counts
 // the wide launchers of lbm_levels.hip (level count at run time).
+// k_step's lane mapping: one coefficient and 64 members per wave (UNI,
+// member counts that are multiples of 64), the pairs (m, member) of one
+// total wavenumber per wave (NG, any other count; its buffer offsets span
+// all coefficients and must stay below 2^32 bytes), both with one basic
+// state; else consecutive (coefficient, member) pairs.
+enum StepMap { STEP_LANES, STEP_UNI, STEP_NG };
+inline StepMap step_map(const lbm_plan *p, int ecnt) {
+  if (p->nb != 1) return STEP_LANES;
+  if (ecnt % 64 == 0) return STEP_UNI;
+  const size_t span = (size_t)p->nspec * 4 * p->E * p->L * (p->prec == LBM_FP64 ? 16 : 8);
+  return span < ((size_t)1 << 32) ? STEP_NG : STEP_LANES;
+}
+// waves of the NG mapping: ceil((n + 1) ecnt / 64) for each n < M
+inline unsigned step_ng_waves(int M, int ecnt) {
+  unsigned w = 0;
+  for (int n = 0; n < M; ++n) w += (unsigned)(((n + 1) * ecnt + 63) / 64);
+  return w;
+}
+
 template <class T>
 int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   const StepArgs<T> a = step_args<T>(p, ecnt);
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step
-  const dim3 grid((unsigned)((nthr + 63) / 64), 2);
-  const bool uni = ecnt % 64 == 0 && p->nb == 1;
+  const StepMap map = p->L == 20 || p->L == 40 ? step_map(p, ecnt) : STEP_LANES;
+  const dim3 grid(map == STEP_NG ? step_ng_waves(p->M, ecnt) : (unsigned)((nthr + 63) / 64), 2);
   if (p->L == 20) {
-    if (uni)
+    if (map == STEP_UNI)
       hipLaunchKernelGGL((k_step<T, 20, true>), grid, dim3(64), 0, st, a);
+    else if (map == STEP_NG)
+      hipLaunchKernelGGL((k_step<T, 20, false, true, true>), grid, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((k_step<T, 20, false>), grid, dim3(64), 0, st, a);
   } else if (p->L == 40) {
-    if (uni)
+    if (map == STEP_UNI)
       hipLaunchKernelGGL((k_step<T, 40, true>), grid, dim3(64), 0, st, a);
+    else if (map == STEP_NG)
+      hipLaunchKernelGGL((k_step<T, 40, false, true, true>), grid, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
   } else {
