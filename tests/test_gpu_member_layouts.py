@@ -1,0 +1,45 @@
+"""Members are independent: a member's numbers do not depend on how many
+members share its plan.  Five members with one basic state and five
+distinct heatings run alone and as the first five of a 64-member ensemble
+(the rest other heatings) must end bit for bit equal.  The two plans take
+different launch shapes for the same arithmetic:
+
+* the spectral step maps one coefficient and 64 members to a wave at 64
+  members and the (m, member) pairs of one total wavenumber at 5;
+* the grid phase is k_grid at 64 members and the three launches at 5
+  members in FP32 (the built-in choice by ensemble size);
+* the Legendre and FFT kernels cover 64 or 8 (padded) member columns.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_five_members_alone_and_in_64(precision):
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases
+    sigm = cases.EQUAL_SIGM_L20
+    sp = L.BatchedSpectral()
+    vg = L.VerticalGrid(sigm)
+    bs = cases.solid_body_state(sp.lat, sp.lon, vg.sig)
+    frc = cases.random_heatings(sp.lat, sp.lon, vg.sig, 64, seed=3)
+    nsteps = 30
+    out = {}
+    for E in (64, 5):
+        m, _, _ = L.make_ensemble_model(bs, E, L.Config(ndays=nsteps / 72.0, precision=precision),
+                                        sigm=sigm)
+        m.plan.set_forcing(frc[:E])
+        m.plan.set_state()
+        m.plan.run(nsteps)
+        out[E] = ([x.cpu().numpy()[:5] for x in m.plan.get_state()],
+                  [x.cpu().numpy()[:5] for x in m.plan.get_spectral(0)],
+                  m.plan.get_tuning())
+    big, small = out[64], out[5]
+    print(precision, "tunings", big[2], small[2])
+    assert np.abs(small[0][2]).max() > 0
+    for a, b in zip(big[0] + big[1], small[0] + small[1]):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), (precision, float(np.abs(a - b).max()))
