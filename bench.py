@@ -433,7 +433,7 @@ def measure_t127(E=32, days=20.0):
     import numpy as np
     import torch
     import lbm_isca_amd as L
-    from lbm_isca_amd import cases, roofline
+    from lbm_isca_amd import cases, roofline, tuning
     sigm = cases.EQUAL_SIGM_L40
     sp = L.BatchedSpectral(384, 192, 127)
     vg = L.VerticalGrid(sigm)
@@ -454,6 +454,8 @@ def measure_t127(E=32, days=20.0):
         cfg = L.Config(**cases.t127l40_config(precision=prec))
         m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True, spectral=sp)
         plan = m.plan
+        plan.set_spectral_state(vor=vor)
+        tuned = tuning.autotune(plan, True, reps=3)   # bit-identical variants, timed on the plan
         plan.set_spectral_state(vor=vor)
         plan.run(8, use_graph=True)              # capture + warm caches
         plan.set_spectral_state(vor=vor)
@@ -476,7 +478,8 @@ def measure_t127(E=32, days=20.0):
                        "frac_of_peak": round(executed * tf / peaks[prec], 4)}
         res[prec] = {"member_days_per_s": round(E * nsteps / spd / dt, 3),
                      "ms_per_step": round(dt * 1e3 / (nsteps + cfg.inistp), 3),
-                     "legendre": mfma, "peak_tflops": peaks[prec]}
+                     "legendre": mfma, "peak_tflops": peaks[prec],
+                     "autotuned": dict(zip(plan.TUNED, tuned["choice"]))}
         log(f"t127l40 {prec}: {res[prec]}")
         del m, plan
         torch.cuda.empty_cache()
