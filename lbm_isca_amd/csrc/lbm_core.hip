@@ -1446,8 +1446,11 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
   // beside the level kernels (none reads what the other side writes), each
   // joined before its first consumer: the inverse FFT, k_tend, k_step and
   // the steady-forcing / step kernels.  Profiling keeps one stream, so its
-  // per-slot events time each launch.
-  const bool fork = !p->prof && p->side != nullptr;
+  // per-slot events time each launch; so do launches of at most 8 members,
+  // where the fork and join cost more than the overlap gains (measured at
+  // T42L20: 1 member 0.175 -> 0.171 ms per step FP64, 2 members 0.147 ->
+  // 0.140 FP32; 16 members lose without the fork).
+  const bool fork = !p->prof && p->side != nullptr && ne > 8;
   hipStream_t sd = fork ? p->side : st;
   enum { F_FFT, F_SURF, F_MJO, F_HIST, F_ANAS, F_START };
   auto fork_at = [&](int k) -> int {
