@@ -81,17 +81,19 @@ def measured_traffic(slot, E, precision, grid_choice=0):
         table = json.load(fh)["kernels"]
     total = 0
     for name, launches in SLOT_KERNEL[slot].items():
+        tail = None
         if slot == "tend":
             if (grid_choice >= 3) != ("k_grid" in name):
                 continue
             if grid_choice >= 3:
+                # k_grid<T, 128, members per block, one basic state>
                 mb = (2 if precision == "fp64" else 4) << (grid_choice - 3)
-                name += f", {mb}, true>"
+                tail = f", {mb}, true>"
             else:
                 name += ", true" if grid_choice else ", false"
         if precision == "fp32":
             name = name.replace("double", "float")
-        hits = [v for k, v in table.items() if name in k]
+        hits = [v for k, v in table.items() if name in k and (tail is None or k.endswith(tail))]
         if not hits:
             return None
         best = max(hits, key=lambda v: v["launches"])
