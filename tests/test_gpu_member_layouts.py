@@ -9,6 +9,8 @@ different launch shapes for the same arithmetic:
 * the grid phase is k_grid at 64 members and the three launches at 5
   members in FP32 (the built-in choice by ensemble size);
 * the Legendre and FFT kernels cover 64 or 8 (padded) member columns.
+
+And the built-in grid phase follows the ensemble size.
 """
 
 import numpy as np
@@ -43,3 +45,20 @@ def test_five_members_alone_and_in_64(precision):
     for a, b in zip(big[0] + big[1], small[0] + small[1]):
         assert a.shape == b.shape
         assert np.array_equal(a, b), (precision, float(np.abs(a - b).max()))
+
+
+def test_builtin_grid_phase_by_ensemble_size():
+    """The built-in grid phase (tuning knob 4): the three launches (0) for
+    up to 4 FP64 or 8 FP32 members, k_grid with 4 FP32 members per block (3)
+    for 9-16, with the larger member group (4) above; every other knob at 0."""
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases
+    sp = L.BatchedSpectral()
+    vg = L.VerticalGrid()
+    bs = cases.solid_body_state(sp.lat, sp.lon, vg.sig)
+    want = {("fp64", 1): 0, ("fp64", 4): 0, ("fp64", 5): 4, ("fp64", 64): 4,
+            ("fp32", 8): 0, ("fp32", 9): 3, ("fp32", 16): 3, ("fp32", 17): 4}
+    for (precision, E), choice in want.items():
+        m, _, _ = L.make_ensemble_model(bs, E, L.Config(precision=precision))
+        assert m.plan.get_tuning() == [0, 0, 0, 0, choice], (precision, E, m.plan.get_tuning())
+        del m
