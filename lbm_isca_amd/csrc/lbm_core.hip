@@ -1517,7 +1517,8 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
                                   nullptr, am)));
     CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, ft, st, 4, 2, nullptr, nullptr,
                                    nullptr, am)));
-    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, fq, st, 4, 3, nullptr, nullptr,
+    // humidity beside them on the second stream (its own output slots)
+    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, fq, sd, 4, 3, nullptr, nullptr,
                                    nullptr, am)));
   }
   InSlots s2{{9 * EL}};
