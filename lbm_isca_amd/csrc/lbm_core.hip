@@ -1515,17 +1515,18 @@ int one_step(lbm_plan *p, hipStream_t st, bool raw_path, int ecnt, bool tend_onl
     InSlots vd{{0, EL, 2 * EL}}, ft{{3 * EL, 4 * EL, 5 * EL}}, fq{{6 * EL, 7 * EL, 8 * EL}};
     CHK((launch_ana<T, AnaTendVD>(p, p->fout, p->tend3, EL, Co, vd, st, 4, 0, nullptr, nullptr,
                                   nullptr, am)));
-    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, ft, st, 4, 2, nullptr, nullptr,
+    // temperature and humidity beside it on the second stream (their own
+    // output slots), the surface analysis after it
+    CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, ft, sd, 4, 2, nullptr, nullptr,
                                    nullptr, am)));
-    // humidity beside them on the second stream (its own output slots)
     CHK((launch_ana<T, AnaFluxDiv>(p, p->fout, p->tend3, EL, Co, fq, sd, 4, 3, nullptr, nullptr,
                                    nullptr, am)));
   }
+  CHK(side_done(F_ANAS));
   InSlots s2{{9 * EL}};
   CHK(mark(p, 6, st));
-  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, sd, 1, 0, nullptr, nullptr,
+  CHK((launch_ana<T, AnaScalar>(p, p->fout, p->tend2, 2 * E, Co, s2, st, 1, 0, nullptr, nullptr,
                                 nullptr, am)));
-  CHK(side_done(F_ANAS));
   CHK(join(F_ANAS));
   if (p->sfv || p->sfp) {
     const size_t nthr = (size_t)p->nspec * ne;
