@@ -2194,10 +2194,31 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   DevTabs<T> &t = p->tabs<T>();
   const ActiveMask am = loop_mask(p, false, ecnt);
   const int ne = ecnt > 0 ? ecnt : E;
+  // The surface synthesis and analysis run on the second stream beside the
+  // level ones (disjoint outputs), joined before their first consumers (the
+  // inverse FFT, the end of the step), as in the forward step: launches of
+  // more than 8 members, never while profiling.
+  const bool fork = !p->prof && p->side != nullptr && ne > 8;
+  hipStream_t sd = fork ? p->side : st;
+  auto fork_at = [&](int k) -> int {
+    if (fork) {
+      HIPCHK(hipEventRecord(p->fj[k], st));
+      HIPCHK(hipStreamWaitEvent(sd, p->fj[k], 0));
+    }
+    return LBM_OK;
+  };
+  auto join_side = [&](int k) -> int {
+    if (fork) {
+      HIPCHK(hipEventRecord(p->fj[k], sd));
+      HIPCHK(hipStreamWaitEvent(st, p->fj[k], 0));
+    }
+    return LBM_OK;
+  };
   CHK(mark(p, 7, st));
   CHK(launch_step_adj<T>(p, st, ne));
   const ColMap cmo{E, L, 9, 2}, cmi{E, L, 6, 6};
   const int Co = cmo.ncols(), Ci = cmi.ncols();
+  CHK(fork_at(0));
   CHK(mark(p, 5, st));
   {
     OutSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL, 6 * EL, 7 * EL, 8 * EL}};
@@ -2209,9 +2230,10 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
     // surface fields as 2E columns, as the forward analysis runs them
     OutSlots sl{{9 * EL}};
     const bool traj = p->traj_cur != nullptr;
-    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, traj ? 2 * E : E, Co, sl, st,
+    CHK((launch_syn<T, AdjScalarFft>(p, p->tend2, p->fout, traj ? 2 * E : E, Co, sl, sd,
                                      traj ? 1 : 2, t.Pa, t.Ha, t.nn1a2, am)));
   }
+  CHK(join_side(1));
   CHK(mark(p, 4, st));
   CHK(launch_ifft<T>(p, p->fout, p->gout, cmo, st, 1, am));
   CHK(mark(p, 3, st));
@@ -2242,6 +2264,7 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   HIPCHK(hipGetLastError());
   CHK(mark(p, 2, st));
   CHK(launch_fft<T>(p, p->gin, p->fin, cmi, st, 2, am));
+  CHK(fork_at(2));
   CHK(mark(p, 0, st));
   {
     InSlots sl{{0, EL, 2 * EL, 3 * EL, 4 * EL, 5 * EL}};
@@ -2251,9 +2274,10 @@ int adjoint_step(lbm_plan *p, hipStream_t st, int ecnt) {
   CHK(mark(p, 1, st));
   {
     InSlots sl{{6 * EL, 6 * EL + E, 6 * EL + 2 * E, 6 * EL + 3 * E, 6 * EL + 4 * E, 6 * EL + 5 * E}};
-    CHK((launch_ana<T, AdjSynSurf>(p, p->fin, p->s3a, E, Ci, sl, st, 3, 0, t.Ps, t.Hs, t.inv_lap,
+    CHK((launch_ana<T, AdjSynSurf>(p, p->fin, p->s3a, E, Ci, sl, sd, 3, 0, t.Ps, t.Hs, t.inv_lap,
                                    am)));
   }
+  CHK(join_side(3));
   CHK(mark(p, 8, st));
   hipLaunchKernelGGL(k_tick_back, dim3(1), dim3(1), 0, st, p->ctl);
   HIPCHK(hipGetLastError());
