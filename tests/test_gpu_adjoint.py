@@ -128,12 +128,13 @@ def test_response_matches_oracle(env):
     assert rel_err(J, ref) < 1e-9
 
 
+@pytest.mark.parametrize("E", [3, 12])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_dot_product_identity(env, use_graph):
+def test_dot_product_identity(env, use_graph, E):
     """J_e(x0, f) = <g_x, x0_e> + <g_f, f_e> for every member, the imaginary
-    parts of m = 0 included (carried by the step, invisible to the grid)."""
+    parts of m = 0 included (carried by the step, invisible to the grid).
+    12 members: the forward and adjoint steps fork their second stream."""
     sht, vg, bs = env
-    E = 3
     m = tl_model(bs, nmem=E)
     plan = m.plan
     response(plan, NSTEPS)
