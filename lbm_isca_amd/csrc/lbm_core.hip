@@ -1242,15 +1242,22 @@ StepArgs<T> step_args(lbm_plan *p, int ecnt) {
 // the wide launchers of lbm_levels.hip (level count at run time).
 // k_step's lane mapping: one coefficient and 64 members per wave (UNI,
 // member counts that are multiples of 64), the pairs (m, member) of one
-// total wavenumber per wave (NG, any other count; its buffer offsets span
-// all coefficients and must stay below 2^32 bytes), both with one basic
-// state; else consecutive (coefficient, member) pairs.
-enum StepMap { STEP_LANES, STEP_UNI, STEP_NG };
+// total wavenumber per wave (NG, any other count), both with one basic
+// state; with a basic state per member the pairs of one total wavenumber in
+// groups of 128 bytes of members (NGM: 8 FP64, 16 FP32; member counts that
+// are multiples of that; measured: 64 members FP64 0.920 -> 0.895 ms per
+// T42L20 step, groups of 8 FP32 members lost); else consecutive
+// (coefficient, member) pairs.  NG and NGM buffer offsets span all
+// coefficients and must stay below 2^32 bytes.
+enum StepMap { STEP_LANES, STEP_UNI, STEP_NG, STEP_NGM };
 inline StepMap step_map(const lbm_plan *p, int ecnt) {
-  if (p->nb != 1) return STEP_LANES;
+  const size_t cs = p->prec == LBM_FP64 ? 16 : 8;
+  const size_t span = (size_t)p->nspec * 4 * p->E * p->L * cs;
+  const bool fits = span < ((size_t)1 << 32);
+  const int g = (int)(128 / cs);
+  if (p->nb != 1) return p->E % g == 0 && ecnt % g == 0 && fits ? STEP_NGM : STEP_LANES;
   if (ecnt % 64 == 0) return STEP_UNI;
-  const size_t span = (size_t)p->nspec * 4 * p->E * p->L * (p->prec == LBM_FP64 ? 16 : 8);
-  return span < ((size_t)1 << 32) ? STEP_NG : STEP_LANES;
+  return fits ? STEP_NG : STEP_LANES;
 }
 // waves of the NG mapping: ceil((n + 1) ecnt / 64) for each n < M
 inline unsigned step_ng_waves(int M, int ecnt) {
@@ -1265,19 +1272,24 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
   const size_t nthr = (size_t)p->nspec * ecnt;
   // two roles per (s, member): see k_step
   const StepMap map = p->L == 20 || p->L == 40 ? step_map(p, ecnt) : STEP_LANES;
-  const dim3 grid(map == STEP_NG ? step_ng_waves(p->M, ecnt) : (unsigned)((nthr + 63) / 64), 2);
+  const bool ng = map == STEP_NG || map == STEP_NGM;
+  const dim3 grid(ng ? step_ng_waves(p->M, ecnt) : (unsigned)((nthr + 63) / 64), 2);
   if (p->L == 20) {
     if (map == STEP_UNI)
       hipLaunchKernelGGL((k_step<T, 20, true>), grid, dim3(64), 0, st, a);
     else if (map == STEP_NG)
-      hipLaunchKernelGGL((k_step<T, 20, false, true, true>), grid, dim3(64), 0, st, a);
+      hipLaunchKernelGGL((k_step<T, 20, false, true, 1>), grid, dim3(64), 0, st, a);
+    else if (map == STEP_NGM)
+      hipLaunchKernelGGL((k_step<T, 20, false, true, 128 / (2 * sizeof(T))>), grid, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((k_step<T, 20, false>), grid, dim3(64), 0, st, a);
   } else if (p->L == 40) {
     if (map == STEP_UNI)
       hipLaunchKernelGGL((k_step<T, 40, true>), grid, dim3(64), 0, st, a);
     else if (map == STEP_NG)
-      hipLaunchKernelGGL((k_step<T, 40, false, true, true>), grid, dim3(64), 0, st, a);
+      hipLaunchKernelGGL((k_step<T, 40, false, true, 1>), grid, dim3(64), 0, st, a);
+    else if (map == STEP_NGM)
+      hipLaunchKernelGGL((k_step<T, 40, false, true, 128 / (2 * sizeof(T))>), grid, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
   } else {

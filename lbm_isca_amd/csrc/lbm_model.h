@@ -1046,23 +1046,28 @@ __device__ inline void matvec(P A, const cx<T> *x, cx<T> *y, int L) {
 // the L x L matrices are wave-uniform and come through the scalar cache
 // instead of one vector load per element.
 //
-// NG (one basic state, any member count): wave w holds the (m, member)
+// NG = 1 (one basic state, any member count): wave w holds the (m, member)
 // pairs of one total wavenumber n, m fastest-but-one and members fastest,
 // ceil((n + 1) ecnt / 64) waves per n in increasing n; the per-n matrix,
 // damping rows and degree are wave-uniform as under UNI.  (In the order of
 // the coefficients, m-major, consecutive lanes would each hold another n
-// and every matrix load would gather 64 rows.)
+// and every matrix load would gather 64 rows.)  NG = G > 1 (a basic state
+// per member, member counts that are multiples of G = 128 bytes / complex
+// value: 8 FP64, 16 FP32): the same waves with the pairs ordered (group of
+// G members, m, member in the group), so a wave spans G members' matrices
+// instead of 64 (each load of one matrix element touches G rows, not 64)
+// and its state accesses stay whole 128-byte runs.
 //
 // LT levels in registers.  EXACT: the plan has LT levels (compile-time
 // loops); otherwise it has L = a.L <= LT, the loops run to LT and skip the
 // levels past L, whose registers are never read.
-template <class T, int LT, bool UNI = false, bool EXACT = true, bool NG = false>
+template <class T, int LT, bool UNI = false, bool EXACT = true, int NG = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LT <= 20 ? 2 : 1)))
 void k_step(StepArgs<T> a) {
   const int L = EXACT ? LT : a.L;
   const int E = a.E;
   int s, e, ng_n = 0;
-  if constexpr (NG) {
+  if constexpr (NG > 0) {
     // this wave's n: the waves of n' < n come first
     const int w = blockIdx.x;
     int wb = 0;
@@ -1074,8 +1079,15 @@ void k_step(StepArgs<T> a) {
     }
     const int pr = (w - wb) * 64 + (int)threadIdx.x;  // pair (m, member) of n
     if (pr >= (ng_n + 1) * a.ecnt) return;
-    const int mm = pr / a.ecnt;
-    e = pr - mm * a.ecnt;
+    int mm;
+    if constexpr (NG == 1) {
+      mm = pr / a.ecnt;
+      e = pr - mm * a.ecnt;
+    } else {
+      const int g = pr / NG;  // (group of NG members, m)
+      mm = g % (ng_n + 1);
+      e = (g / (ng_n + 1)) * NG + pr % NG;
+    }
     s = spec_off(mm, a.M) + ng_n - mm;
   } else {
     const int idx = blockIdx.x * 64 + threadIdx.x;
@@ -1096,12 +1108,12 @@ void k_step(StepArgs<T> a) {
   const T dt = pure ? a.fixed_dt : oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
   const int set = pure ? a.fixed_set : oinit ? st : a.inistp;
   const T dt2 = (T)2 * dt;
-  const int n = NG ? __builtin_amdgcn_readfirstlane(ng_n)
-                   : UNI ? __builtin_amdgcn_readfirstlane(a.n_of_s[s]) : a.n_of_s[s];
-  const int eb = (!UNI && !NG && a.nbasic > 1) ? e : 0;
+  const int n = NG > 0 ? __builtin_amdgcn_readfirstlane(ng_n)
+                       : UNI ? __builtin_amdgcn_readfirstlane(a.n_of_s[s]) : a.n_of_s[s];
+  const int eb = (!UNI && NG != 1 && a.nbasic > 1) ? e : 0;
   const VOff vo{L};
-  using RU = typename RoPtr<T, true>::type;        // uniform in every launch
-  using RP = typename RoPtr<T, UNI || NG>::type;   // uniform when one n per wave
+  using RU = typename RoPtr<T, true>::type;            // uniform in every launch
+  using RP = typename RoPtr<T, UNI || NG == 1>::type;  // uniform: one n, one basic state per wave
   const RU vc = (RU)a.vc;
   const RP tref = (RP)(a.tref + (size_t)eb * L);
   const RP ktt = (RP)(a.ktt + (size_t)n * L), kd = (RP)(a.kd + (size_t)n * L);
@@ -1112,10 +1124,12 @@ void k_step(StepArgs<T> a) {
   // wave, member) and a wave-uniform level offset, since the level stride E
   // is not a compile-time constant (64-bit addresses per level would not
   // fit in registers).  A wave spans at most 64 coefficients, so the offsets
-  // stay below 2^32 bytes; under NG its coefficients (one n, increasing m
-  // across the lanes, the first lane's the lowest) lie up to nspec apart,
-  // and the host takes NG only while nspec 4 E L sizeof(cx<T>) < 2^32.
-  const int s_w = __builtin_amdgcn_readfirstlane(s);
+  // stay below 2^32 bytes; under NG its coefficients (one n) lie up to
+  // nspec apart, and the host takes NG only while nspec 4 E L sizeof(cx<T>)
+  // < 2^32.  The base is the wave's lowest coefficient: the first lane's
+  // (m increasing across the lanes) or, NG > 1 (m restarting with each
+  // member group), that of m = 0, s = n.
+  const int s_w = NG > 1 ? n : __builtin_amdgcn_readfirstlane(s);
   const size_t wbase = (size_t)s_w * 4 * EL;
   const unsigned voff = (unsigned)(((size_t)(s - s_w) * 4 * EL + e) * sizeof(cx<T>));
   const size_t wbytes = (size_t)(a.nspec - s_w) * 4 * EL * sizeof(cx<T>);

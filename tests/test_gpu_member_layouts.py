@@ -13,6 +13,8 @@ different launch shapes for the same arithmetic:
 And the built-in grid phase follows the ensemble size.
 """
 
+import types
+
 import numpy as np
 import pytest
 
@@ -62,3 +64,38 @@ def test_builtin_grid_phase_by_ensemble_size():
         m, _, _ = L.make_ensemble_model(bs, E, L.Config(precision=precision))
         assert m.plan.get_tuning() == [0, 0, 0, 0, choice], (precision, E, m.plan.get_tuning())
         del m
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_members_with_own_basic_states_alone_and_together(precision):
+    """Members on their own balanced jets in one plan (8 FP64, 16 FP32: the
+    spectral step's lanes take one total wavenumber and that many members'
+    matrices per wave; the grid phase is k_grid) end bit for bit equal to
+    each member run alone in a one-member plan (one basic state, the three
+    launches)."""
+    import lbm_isca_amd as L
+    from lbm_isca_amd import cases
+    sigm = cases.EQUAL_SIGM_L20
+    sp = L.BatchedSpectral()
+    vg = L.VerticalGrid(sigm)
+    E, nsteps = (8 if precision == "fp64" else 16), 24
+    bs, _ = cases.jet_states(sp, vg.sig, E, seed=7)
+    frc = cases.random_heatings(sp.lat, sp.lon, vg.sig, E, seed=5)
+    cfg = L.Config(ndays=nsteps / 72.0, precision=precision)
+    m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True, spectral=sp)
+    m.plan.set_forcing(frc)
+    m.plan.set_state()
+    m.plan.run(nsteps)
+    together = [x.cpu().numpy() for x in m.plan.get_state() + m.plan.get_spectral(0)]
+    assert np.abs(together[2]).max() > 0
+    for e in range(E):
+        one = types.SimpleNamespace(**{k: np.ascontiguousarray(getattr(bs, k)[e])
+                                       for k in ("u", "v", "t", "q", "ps")})
+        m1, _, _ = L.make_ensemble_model(one, 1, cfg, sigm=sigm, spectral=sp)
+        m1.plan.set_forcing(frc[e:e + 1])
+        m1.plan.set_state()
+        m1.plan.run(nsteps)
+        alone = [x.cpu().numpy() for x in m1.plan.get_state() + m1.plan.get_spectral(0)]
+        for a, b in zip(together, alone):
+            assert np.array_equal(a[e], b[0]), (precision, e, float(np.abs(a[e] - b[0]).max()))
+        del m1
