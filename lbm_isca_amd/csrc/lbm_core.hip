@@ -1244,18 +1244,19 @@ StepArgs<T> step_args(lbm_plan *p, int ecnt) {
 // member counts that are multiples of 64), the pairs (m, member) of one
 // total wavenumber per wave (NG, any other count), both with one basic
 // state; with a basic state per member the pairs of one total wavenumber in
-// groups of 128 bytes of members (NGM: 8 FP64, 16 FP32; member counts that
-// are multiples of that; measured: 64 members FP64 0.920 -> 0.895 ms per
-// T42L20 step, groups of 8 FP32 members lost); else consecutive
-// (coefficient, member) pairs.  NG and NGM buffer offsets span all
-// coefficients and must stay below 2^32 bytes.
+// groups of 128 bytes of members (NGM: 8 FP64, 16 FP32; 20 levels and
+// member counts that are multiples of the group; measured: 64 members FP64
+// 0.920 -> 0.895 ms per T42L20 step, groups of 8 FP32 members lost; at
+// T127L40 the FP64 step took 2.36 ms against 1.97 with consecutive pairs);
+// else consecutive (coefficient, member) pairs.  NG and NGM buffer offsets
+// span all coefficients and must stay below 2^32 bytes.
 enum StepMap { STEP_LANES, STEP_UNI, STEP_NG, STEP_NGM };
 inline StepMap step_map(const lbm_plan *p, int ecnt) {
   const size_t cs = p->prec == LBM_FP64 ? 16 : 8;
   const size_t span = (size_t)p->nspec * 4 * p->E * p->L * cs;
   const bool fits = span < ((size_t)1 << 32);
   const int g = (int)(128 / cs);
-  if (p->nb != 1) return p->E % g == 0 && ecnt % g == 0 && fits ? STEP_NGM : STEP_LANES;
+  if (p->nb != 1) return p->L == 20 && p->E % g == 0 && ecnt % g == 0 && fits ? STEP_NGM : STEP_LANES;
   if (ecnt % 64 == 0) return STEP_UNI;
   return fits ? STEP_NG : STEP_LANES;
 }
@@ -1288,8 +1289,6 @@ int launch_step_kernel(lbm_plan *p, hipStream_t st, int ecnt) {
       hipLaunchKernelGGL((k_step<T, 40, true>), grid, dim3(64), 0, st, a);
     else if (map == STEP_NG)
       hipLaunchKernelGGL((k_step<T, 40, false, true, 1>), grid, dim3(64), 0, st, a);
-    else if (map == STEP_NGM)
-      hipLaunchKernelGGL((k_step<T, 40, false, true, 128 / (2 * sizeof(T))>), grid, dim3(64), 0, st, a);
     else
       hipLaunchKernelGGL((k_step<T, 40, false>), grid, dim3(64), 0, st, a);
   } else {
