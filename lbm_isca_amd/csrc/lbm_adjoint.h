@@ -443,19 +443,25 @@ __device__ inline void matvec_t(P A, const cx<T> *x, cx<T> *y, int L) {
   }
 }
 
-// LT, EXACT: as in k_step.
-template <class T, int LT, bool EXACT = true>
+// LT, EXACT, NG (lane mapping): as in k_step.
+template <class T, int LT, bool EXACT = true, int NG = 0>
 __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const int L = EXACT ? LT : a.L;
-  const int idx = blockIdx.x * 64 + threadIdx.x;
   const int E = a.E;
-  const int s = idx / a.ecnt, e = idx - s * a.ecnt;
+  int s, e, ng_n = 0;
+  if constexpr (NG > 0) {
+    if (!ng_lane<NG>(a.M, a.ecnt, s, e, ng_n)) return;
+  } else {
+    const int idx = blockIdx.x * 64 + threadIdx.x;
+    s = idx / a.ecnt;
+    e = idx - s * a.ecnt;
+  }
   if (s >= a.nspec) return;
   const int st = a.ctl->step;
   const size_t EL = (size_t)E * L;
-  // level arrays through buffer views based at the wave's first coefficient
+  // level arrays through buffer views based at the wave's lowest coefficient
   // (per-lane offset + wave-uniform level offset), as in k_step
-  const int s_w = __builtin_amdgcn_readfirstlane(s);
+  const int s_w = NG > 0 ? __builtin_amdgcn_readfirstlane(ng_n) : __builtin_amdgcn_readfirstlane(s);
   const size_t wbase = (size_t)s_w * 4 * EL;
   const unsigned voff = (unsigned)(((size_t)(s - s_w) * 4 * EL + e) * sizeof(cx<T>));
   const size_t wbytes = (size_t)(a.nspec - s_w) * 4 * EL * sizeof(cx<T>);
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(64) void k_step_adj(StepAdjArgs<T> a) {
   const T dt = oinit ? a.delt / (T)(1 << (a.inistp - st)) : a.delt;
   const int set = oinit ? st : a.inistp;
   const T dt2 = (T)2 * dt;
-  const int n = a.n_of_s[s];
+  const int n = NG > 0 ? __builtin_amdgcn_readfirstlane(ng_n) : a.n_of_s[s];
   const int eb = a.nbasic > 1 ? e : 0;
   const T *tref = a.tref + (size_t)eb * L;
   const T *ktt = a.ktt + (size_t)n * L, *kd = a.kd + (size_t)n * L;

@@ -2187,9 +2187,17 @@ int launch_step_adj(lbm_plan *p, hipStream_t st, int ecnt) {
   a.ecnt = ecnt;
   a.traj = p->traj_cur != nullptr;
   const size_t nthr = (size_t)p->nspec * ecnt;
-  // two roles per (s, member): see k_step_adj
-  const dim3 grid((unsigned)((nthr + 63) / 64), 2);
-  if (p->L == 20)
+  // two roles per (s, member): see k_step_adj; at 20 levels the lane
+  // mappings of the forward step (one total wavenumber per wave) except
+  // where every wave holds one coefficient and one basic state already
+  const StepMap map = p->L == 20 ? step_map(p, ecnt) : STEP_LANES;
+  const bool ng = map == STEP_NG || map == STEP_NGM;
+  const dim3 grid(ng ? step_ng_waves(p->M, ecnt) : (unsigned)((nthr + 63) / 64), 2);
+  if (p->L == 20 && map == STEP_NG)
+    hipLaunchKernelGGL((k_step_adj<T, 20, true, 1>), grid, dim3(64), 0, st, a);
+  else if (p->L == 20 && map == STEP_NGM)
+    hipLaunchKernelGGL((k_step_adj<T, 20, true, 128 / (2 * sizeof(T))>), grid, dim3(64), 0, st, a);
+  else if (p->L == 20)
     hipLaunchKernelGGL((k_step_adj<T, 20>), grid, dim3(64), 0, st, a);
   else if (p->L == 40)
     hipLaunchKernelGGL((k_step_adj<T, 40>), grid, dim3(64), 0, st, a);

@@ -1061,6 +1061,36 @@ __device__ inline void matvec(P A, const cx<T> *x, cx<T> *y, int L) {
 // LT levels in registers.  EXACT: the plan has LT levels (compile-time
 // loops); otherwise it has L = a.L <= LT, the loops run to LT and skip the
 // levels past L, whose registers are never read.
+// Coefficient s, member e and total wavenumber n of this lane under the NG
+// mappings (NG > 0) of k_step and k_step_adj; false for a lane past its
+// wave's pairs.
+template <int NG>
+__device__ __forceinline__ bool ng_lane(int M, int ecnt, int &s, int &e, int &n) {
+  // this wave's n: the waves of n' < n come first
+  const int w = blockIdx.x;
+  int wb = 0;
+  n = 0;
+  while (n < M - 1) {
+    const int nw = ((n + 1) * ecnt + 63) / 64;
+    if (w < wb + nw) break;
+    wb += nw;
+    ++n;
+  }
+  const int pr = (w - wb) * 64 + (int)threadIdx.x;  // pair (m, member) of n
+  if (pr >= (n + 1) * ecnt) return false;
+  int mm;
+  if constexpr (NG == 1) {
+    mm = pr / ecnt;
+    e = pr - mm * ecnt;
+  } else {
+    const int g = pr / NG;  // (group of NG members, m)
+    mm = g % (n + 1);
+    e = (g / (n + 1)) * NG + pr % NG;
+  }
+  s = spec_off(mm, M) + n - mm;
+  return true;
+}
+
 template <class T, int LT, bool UNI = false, bool EXACT = true, int NG = 0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LT <= 20 ? 2 : 1)))
 void k_step(StepArgs<T> a) {
@@ -1068,27 +1098,7 @@ void k_step(StepArgs<T> a) {
   const int E = a.E;
   int s, e, ng_n = 0;
   if constexpr (NG > 0) {
-    // this wave's n: the waves of n' < n come first
-    const int w = blockIdx.x;
-    int wb = 0;
-    while (ng_n < a.M - 1) {
-      const int nw = ((ng_n + 1) * a.ecnt + 63) / 64;
-      if (w < wb + nw) break;
-      wb += nw;
-      ++ng_n;
-    }
-    const int pr = (w - wb) * 64 + (int)threadIdx.x;  // pair (m, member) of n
-    if (pr >= (ng_n + 1) * a.ecnt) return;
-    int mm;
-    if constexpr (NG == 1) {
-      mm = pr / a.ecnt;
-      e = pr - mm * a.ecnt;
-    } else {
-      const int g = pr / NG;  // (group of NG members, m)
-      mm = g % (ng_n + 1);
-      e = (g / (ng_n + 1)) * NG + pr % NG;
-    }
-    s = spec_off(mm, a.M) + ng_n - mm;
+    if (!ng_lane<NG>(a.M, a.ecnt, s, e, ng_n)) return;
   } else {
     const int idx = blockIdx.x * 64 + threadIdx.x;
     s = UNI ? __builtin_amdgcn_readfirstlane(idx / a.ecnt) : idx / a.ecnt;

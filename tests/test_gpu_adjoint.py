@@ -232,6 +232,32 @@ def test_uneven_lengths_per_member_basic_state(env):
         assert rel_err(J[e:e + 1], ref) < 1e-9, (e, J[e], ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_identity_members_with_own_basic_states(env, precision):
+    """Sixteen members on eight jets shifted in latitude, their own basic
+    states (the spectral step and its transpose take one total wavenumber
+    and a group of 8 (FP64) or 16 (FP32) members per wave): the dot-product
+    identity per member."""
+    sht, vg, bs = env
+    E = 16
+    jets = [cases.jet_basic_state(sht.lat + 0.02 * (k % 8 - 4), sht.lon, vg.sig) for k in range(E)]
+    stack = types.SimpleNamespace(**{k: np.stack([getattr(b, k) for b in jets])
+                                     for k in ("u", "v", "t", "q", "ps")})
+    m = tl_model(stack, nmem=E, precision=precision, per_member=True)
+    plan = m.plan
+    response(plan, NSTEPS, steps=(1, 4, 6, 9))
+    spec = spectral_state(sht, E, 61)
+    frc = heatings(sht, vg, E, 6)
+    J = forward_J(plan, NSTEPS, spec, frc)
+    gx, gf = plan.adjoint_run(NSTEPS)
+    terms = [inner(np_(g), spec[f]) for g, f in zip(gx, FIELDS)] + [inner(np_(gf), frc)]
+    pred = np.sum(terms, axis=0)
+    scale = np.sum(np.abs(terms), axis=0)
+    tol = 1e-11 if precision == "fp64" else 1e-4
+    print(precision, "J", J, "pred", pred)
+    assert np.all(np.abs(J - pred) <= tol * np.maximum(scale, 1e-300))
+
+
 def test_fp32_adjoint(env):
     """FP32 gradient against the FP64 gradient and the identity at FP32."""
     sht, vg, bs = env
