@@ -66,26 +66,34 @@ def test_builtin_grid_phase_by_ensemble_size():
         del m
 
 
+@pytest.mark.parametrize("uneven", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_members_with_own_basic_states_alone_and_together(precision):
+def test_members_with_own_basic_states_alone_and_together(precision, uneven):
     """Members on their own balanced jets in one plan (8 FP64, 16 FP32: the
     spectral step's lanes take one total wavenumber and that many members'
     matrices per wave; the grid phase is k_grid) end bit for bit equal to
     each member run alone in a one-member plan (one basic state, the three
-    launches)."""
+    launches).  Uneven: 16 members in four lengths, so the launches shrink
+    to the live members (16, then 8) and change lane mapping within the
+    run."""
     import lbm_isca_amd as L
     from lbm_isca_amd import cases
     sigm = cases.EQUAL_SIGM_L20
     sp = L.BatchedSpectral()
     vg = L.VerticalGrid(sigm)
-    E, nsteps = (8 if precision == "fp64" else 16), 24
+    E, nsteps = (16 if uneven or precision == "fp32" else 8), 24
+    lengths = np.repeat([24, 18, 12, 6], E // 4) if uneven else np.full(E, nsteps)
     bs, _ = cases.jet_states(sp, vg.sig, E, seed=7)
     frc = cases.random_heatings(sp.lat, sp.lon, vg.sig, E, seed=5)
     cfg = L.Config(ndays=nsteps / 72.0, precision=precision)
     m, _, _ = L.make_ensemble_model(bs, E, cfg, sigm=sigm, per_member_basic=True, spectral=sp)
     m.plan.set_forcing(frc)
     m.plan.set_state()
-    m.plan.run(nsteps)
+    if uneven:
+        m.plan.set_lengths(lengths)
+        m.plan.run(0)
+    else:
+        m.plan.run(nsteps)
     together = [x.cpu().numpy() for x in m.plan.get_state() + m.plan.get_spectral(0)]
     assert np.abs(together[2]).max() > 0
     for e in range(E):
@@ -94,7 +102,7 @@ def test_members_with_own_basic_states_alone_and_together(precision):
         m1, _, _ = L.make_ensemble_model(one, 1, cfg, sigm=sigm, spectral=sp)
         m1.plan.set_forcing(frc[e:e + 1])
         m1.plan.set_state()
-        m1.plan.run(nsteps)
+        m1.plan.run(int(lengths[e]))
         alone = [x.cpu().numpy() for x in m1.plan.get_state() + m1.plan.get_spectral(0)]
         for a, b in zip(together, alone):
             assert np.array_equal(a[e], b[0]), (precision, e, float(np.abs(a[e] - b[0]).max()))
